@@ -1,0 +1,340 @@
+/*
+ * mi355sssp -- MI355X (gfx950) SSSP and min-plus engine, C ABI.
+ *
+ * Library: libmi355sssp.so (hand-written wave64 HIP kernels + C++ host
+ * drivers).  Every entry point returns an mi355_status (0 = success); on
+ * failure mi355_last_error() holds a message for the calling thread.  No C++
+ * exception crosses this boundary.  Streams are passed as opaque pointers
+ * (hipStream_t; NULL = the default stream).  Unless stated otherwise a call
+ * is blocking: it synchronises its stream before returning, like the
+ * reference functions it replaces.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository root).  The C++ headers in include/compat/
+ * restate those reference signatures on top of this ABI.
+ */
+#ifndef MI355_SSSP_H
+#define MI355_SSSP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ---------------------------------------------------------- */
+typedef enum {
+  MI355_OK = 0,
+  MI355_EINVAL = -1,     /* bad argument (reference: std::invalid_argument / hipErrorInvalidValue) */
+  MI355_ERANGE = -2,     /* source out of range (reference: std::out_of_range) */
+  MI355_EOVERFLOW = -3,  /* size does not fit the index type (std::overflow_error) */
+  MI355_EHIP = -4,       /* HIP runtime error (std::runtime_error / hipError_t) */
+  MI355_ENOMEM = -5,     /* device allocation failed */
+  MI355_ECAPACITY = -6,  /* a queue / hash / pool capacity was exceeded */
+  MI355_EDEVICE = -7     /* a kernel reported an internal error */
+} mi355_status;
+
+const char* mi355_last_error(void);
+/* The hipError_t of the last MI355_EHIP failure on this thread (0 if none). */
+int mi355_last_hip_error(void);
+
+/* ---- device plumbing (for FFI callers without their own allocator) ---- */
+int mi355_device_count(int* count);
+int mi355_set_device(int device);
+int mi355_device_malloc(void** ptr, size_t bytes);
+int mi355_device_free(void* ptr);
+int mi355_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int mi355_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int mi355_memset_d8(void* dst, int value, size_t bytes, void* stream);
+int mi355_stream_create(void** stream);
+int mi355_stream_destroy(void* stream);
+int mi355_stream_sync(void* stream);
+
+/* ---- CSR descriptors --------------------------------------------------- */
+enum { MI355_W_F32 = 0, MI355_W_U32 = 1 };
+enum { MI355_OUT = 0, MI355_IN = 1 };        /* row u lists u->v / row v lists u->v */
+enum { MI355_HOST = 0, MI355_DEVICE = 1 };   /* where the arrays live */
+
+typedef struct {
+  int64_t rows;
+  int64_t cols;
+  int64_t nnz;
+  const void* rowptr;   /* rows+1 offsets, int64 or 32-bit per offset_bits */
+  const void* col;      /* nnz 32-bit column indices */
+  const void* w;        /* nnz 32-bit weights (fp32 or uint32) */
+  int32_t offset_bits;  /* 64 or 32 */
+  int32_t weight_type;  /* MI355_W_F32 / MI355_W_U32 */
+  int32_t orientation;  /* MI355_OUT / MI355_IN */
+  int32_t memory;       /* MI355_HOST / MI355_DEVICE */
+} mi355_csr_desc;
+
+/* ---- graph handle -------------------------------------------------------
+ * A device-resident graph in the engine's layout: outgoing CSR with int32
+ * offsets and interleaved (dst, weight) 8-byte edge records, plus cached
+ * degree statistics.  Creating one validates the input (range, monotone
+ * offsets; finite non-negative weights unless allow_negative) and, for
+ * incoming input, transposes it on the device.  Replaces the per-call
+ * validation + transpose of DS/src/delta_stepping_hip_CSR.cpp:546-605, 659-667
+ * and the per-call uploads of ONE/sssp_hip.cpp:125-160. */
+typedef struct mi355_graph mi355_graph;
+
+typedef struct {
+  int64_t n;
+  int64_t m;
+  int32_t weight_type;
+  int32_t max_degree;
+  double avg_degree;
+  double avg_weight;        /* mean of positive finite weights */
+  int64_t large_vertices;   /* out-degree above the wave-split threshold */
+  int64_t large_chunks;     /* 2048-edge chunks those vertices split into */
+  int32_t has_negative;     /* any negative weight */
+  int32_t device;
+} mi355_graph_info;
+
+int mi355_graph_create(const mi355_csr_desc* desc, int32_t allow_negative, void* stream,
+                       mi355_graph** out);
+int mi355_graph_destroy(mi355_graph* g);
+int mi355_graph_info_get(const mi355_graph* g, mi355_graph_info* info);
+/* Copy the engine's outgoing CSR back (int64 rowptr, int32 col, 32-bit w). */
+int mi355_graph_download(const mi355_graph* g, int64_t* rowptr, int32_t* col, uint32_t* w);
+
+/* ---- single-source shortest paths on a graph handle -------------------- */
+enum {
+  MI355_ALG_BF = 0,         /* frontier Bellman-Ford, in place (chaotic) */
+  MI355_ALG_BF_JACOBI = 1,  /* frontier Bellman-Ford with exact Jacobi rounds */
+  MI355_ALG_DELTA = 2,      /* bucket-synchronous delta-stepping (light/heavy) */
+  MI355_ALG_ADDS = 3        /* asynchronous dynamic delta-stepping, persistent */
+};
+/* dist output conventions for INF */
+enum { MI355_INF_IEEE = 0, MI355_INF_FLT_MAX = 1 };
+
+typedef struct {
+  int32_t algorithm;
+  int32_t max_iters;        /* <0: algorithm default (BF: n-1; DS/ADDS: unbounded) */
+  int32_t check_interval;   /* BF_JACOBI: test convergence when it % k == 0; 0 = never */
+  float delta;              /* DS / ADDS bucket width; <= 0: heuristic 16*wbar/dbar */
+  int32_t inf_mode;         /* MI355_INF_IEEE or MI355_INF_FLT_MAX (f32 outputs) */
+  int32_t dynamic_delta;    /* ADDS: adapt delta per phase */
+  int32_t use_mlp;          /* ADDS: ask the loaded MLP for delta per phase */
+  int32_t worker_blocks;    /* ADDS: persistent workgroups (0 = auto) */
+  int32_t rounds_per_sync;  /* host sync cadence (0 = adaptive) */
+  int32_t reserved[7];
+} mi355_sssp_options;
+
+typedef struct {
+  int32_t iterations_used;  /* BF: rounds; DS: buckets; ADDS: phases */
+  int32_t converged;
+  int32_t rounds;           /* kernel rounds executed (all algorithms) */
+  int32_t syncs;            /* host synchronisations */
+  int64_t edges_relaxed;    /* edge visits (work, not unique edges) */
+  int64_t vertices_processed;
+  int64_t reached;          /* vertices with finite distance */
+  int64_t reached_edges;    /* sum of out-degrees of reached vertices (m_r) */
+  float initial_delta;
+  float final_delta;
+  float device_ms;          /* hipEvent time of the GPU work */
+  float reserved_f;
+  int32_t has_negative_cycle;
+  int32_t reserved_i;
+} mi355_sssp_stats;
+
+void mi355_sssp_options_default(mi355_sssp_options* opt);
+
+/* Shortest paths from source.  d_dist: device array of n 32-bit distances
+ * (fp32 or uint32 per the graph's weight type); d_pred: device array of n
+ * int32 predecessors or NULL.  Predecessor of v is the smallest-index vertex
+ * p with dist[v] == dist[p] (+) w(p,v); -1 for the source and unreachable
+ * vertices.  per_round_frontier (host, may be NULL, length cap_rounds)
+ * receives the frontier size after each BF round. */
+int mi355_sssp(mi355_graph* g, int32_t source, const mi355_sssp_options* opt, void* d_dist,
+               int32_t* d_pred, mi355_sssp_stats* stats, void* stream);
+
+/* Same, leaving distances as packed 64-bit labels (dist key << 32 | pred) in
+ * the graph's internal buffer (no output conversion); used by timing loops.
+ * mi355_sssp_labels_to_output converts them afterwards. */
+int mi355_sssp_labels(mi355_graph* g, int32_t source, const mi355_sssp_options* opt,
+                      mi355_sssp_stats* stats, void* stream);
+int mi355_sssp_labels_to_output(mi355_graph* g, int32_t source, int32_t inf_mode, void* d_dist,
+                                int32_t* d_pred, void* stream);
+/* Record per-round frontier sizes of the next BF / BF_JACOBI run (host). */
+int mi355_sssp_round_log(mi355_graph* g, int32_t* sizes, int32_t cap);
+
+/* ---- reference entry points ------------------------------------------- */
+
+/* Replaces minplus_gemm_f32 (MP/src/minplus_hip.hpp:24-33, minplus_hip.cpp:122-173):
+ * C[i,j] = min_k A[i,k] + B[k,j], row-major, device pointers, asynchronous.
+ * MI355_EINVAL for null pointers, non-positive sizes or ld < dims. */
+int mi355_minplus_gemm_f32(const float* dA, const float* dB, float* dC, int M, int N, int K,
+                           int lda, int ldb, int ldc, void* stream);
+
+/* Batched variant (no reference counterpart): count independent products
+ * with per-item sizes and device pointers held in device arrays. */
+int mi355_minplus_gemm_f32_batched(int count, const float* const* dA, const float* const* dB,
+                                   float* const* dC, const int* M, const int* N, const int* K,
+                                   const int* lda, const int* ldb, const int* ldc,
+                                   int max_M, int max_N, void* stream);
+
+/* Replaces minplus_sparse::minplus_spgemm_csr_f32 / _large_rows / _tuned<HC,BS>
+ * (MP/src/minplus_sparse_hip.hpp:64-150, minplus_sparse_hip.cpp:410-764).
+ * A, B: device CSR (int64 rowptr, int32 col, fp32 values).  Outputs are
+ * allocated here with hipMalloc and owned by the caller (hipFree); an empty
+ * product gives a zero rowptr and NULL colind/values.  hash_capacity bounds
+ * the unique columns of one output row (MI355_ECAPACITY beyond it, outputs
+ * freed); block_size is the workgroup size hint (64, 128 or 256). */
+int mi355_minplus_spgemm_csr_f32(int64_t a_rows, int64_t a_cols, int64_t a_nnz,
+                                 const int64_t* a_rowptr, const int32_t* a_col,
+                                 const float* a_val, int64_t b_rows, int64_t b_cols,
+                                 int64_t b_nnz, const int64_t* b_rowptr, const int32_t* b_col,
+                                 const float* b_val, int hash_capacity, int block_size,
+                                 int64_t** c_rowptr, int32_t** c_col, float** c_val,
+                                 int64_t* c_nnz, void* stream);
+
+/* Replaces bellman_ford_minplus_hip (BF/src/bellman_ford_hip.hpp:14-59,
+ * bellman_ford_hip.cpp:64-264): dense incoming adjacency A[v*n+u] = w(u->v),
+ * INF = no edge, Jacobi rounds until no change or max_iters (<0: n-1).
+ * adjacency lives in `memory` (MI355_HOST / MI355_DEVICE); dist is host. */
+int mi355_bf_dense_f32(const float* adjacency, int32_t memory, int n, int source,
+                       int max_iters, float* dist, int* iterations_used, int* converged,
+                       void* stream);
+
+/* Replaces bellman_ford_minplus_hip_gauss_seidel (BF/src/bf_gauss_seidel.cpp:76-260):
+ * exactly max_iters (<0: n-1) ordered sweeps; has_negative_cycle is never set. */
+int mi355_bf_gauss_seidel_f32(const float* adjacency, int32_t memory, int n, int source,
+                              int max_iters, float* dist, int* iterations_used, void* stream);
+
+/* Progress record with the layout of BellmanFordCsrProgress
+ * (BF/src/bf_hip_CSR.hpp:9-14). */
+typedef struct {
+  int iteration;
+  int max_iters;
+  unsigned char convergence_checked;
+  unsigned char changed;
+} mi355_bf_progress;
+typedef void (*mi355_bf_progress_cb)(const mi355_bf_progress* progress, void* user_data);
+
+/* Statistics of BellmanFordCsrVarChecksResult (BF/src/bf_var_checks.hpp:9-27). */
+typedef struct {
+  int iterations_used;
+  int converged;
+  int check_interval;
+  int checks_performed;
+  int changed_checks;
+  int unchanged_checks;
+  double initialization_ms;
+  double bellman_ford_loop_ms;
+  double relaxation_ms;
+  double checked_merge_ms;
+  double unchecked_merge_ms;
+  double checked_iteration_ms;
+  double unchecked_iteration_ms;
+  double final_copy_ms;
+  double total_runtime_ms;
+  double estimated_check_overhead_ms;
+} mi355_bf_csr_stats;
+
+/* Replaces bellman_ford_minplus_hip_csr (BF/src/bf_hip_CSR.hpp:41-66, check every
+ * iteration: check_interval = 1), bellman_ford_minplus_hip_csr_no_checks
+ * (BF/src/bf_hip_no_checks_CSR.hpp:17-42: check_interval = 0) and
+ * bellman_ford_minplus_hip_csr_var_checks[_with_stats]
+ * (BF/src/bf_var_checks.hpp:33-71: check_interval = k).  A is the INCOMING CSR
+ * (row v, column u = w(u->v)), host or device.  dist: host array of rows floats.
+ * The progress callback runs on the calling thread, once per iteration, in
+ * order, before the call returns. */
+int mi355_bf_csr_f32(const mi355_csr_desc* A, int source, int max_iters, int check_interval,
+                     float* dist, mi355_bf_csr_stats* stats, mi355_bf_progress_cb cb,
+                     void* user_data, void* stream);
+
+/* Replaces delta_stepping_minplus_hip_csr (DS/src/delta_stepping_hip_CSR.hpp:15-48,
+ * delta_stepping_hip_CSR.cpp:648-815): incoming CSR, finite delta > 0,
+ * non-negative weights; iterations_used = buckets processed, converged when no
+ * bucket remains.  max_iters < 0: unbounded. */
+int mi355_delta_stepping_csr_f32(const mi355_csr_desc* A, int source, float delta,
+                                 int max_iters, float* dist, int* iterations_used,
+                                 int* converged, mi355_bf_progress_cb cb, void* user_data,
+                                 void* stream);
+
+/* Options / result of SSSPOptions / SSSPResult (ONE/sssp_hip.hpp:21-47). */
+typedef struct {
+  int max_iters;
+  int num_buckets;
+  int bucket_capacity_per_bucket;
+  float initial_delta;
+  int dynamic_delta;
+  int target_low_work;
+  int target_high_work;
+} mi355_one_options;
+typedef struct {
+  int iterations_used;
+  int converged;
+  int has_negative_cycle;
+  unsigned long long work_count;
+  float milliseconds;
+  float final_delta;
+} mi355_one_result;
+
+/* Replaces adds_sssp_hip (ONE/sssp_hip.hpp:61-64, sssp_hip.cpp:428-595) and
+ * bellman_ford_sssp_hip (ONE/sssp_hip.hpp:56-59, sssp_hip.cpp:350-426):
+ * outgoing host CSR with int offsets; dist is a host array of n floats.
+ * algorithm: MI355_ALG_ADDS or MI355_ALG_BF (the latter also detects a
+ * negative cycle reachable from the source). */
+int mi355_one_sssp_f32(int32_t algorithm, int n, int m, const int* row_offsets,
+                       const int* col_indices, const float* weights, int source,
+                       const mi355_one_options* options, float* dist, mi355_one_result* result,
+                       void* stream);
+
+/* Options / RunStats of hip_adds (JAS/hip_adds_sssp.hpp:28-87). */
+typedef struct {
+  uint32_t worker_blocks;
+  uint32_t assignment_chunk;
+  uint32_t initial_active_buckets;
+  uint32_t max_active_buckets;
+  double initial_delta;
+  double initial_delta_scale;
+  double min_delta;
+  double delta_growth;
+  double delta_shrink;
+  double clip_threshold;
+  double utilization_low;
+  double utilization_high;
+  uint32_t adjust_period_head_switches;
+  int32_t dynamic_delta;
+  int32_t dynamic_active_buckets;
+  int32_t initialize_distances;
+  uint64_t pool_words;
+  double memory_fraction;
+} mi355_adds_options;
+typedef struct {
+  double initial_delta;
+  double final_delta;
+  double final_bucket_base;
+  uint32_t worker_blocks;
+  uint32_t final_active_buckets;
+  uint32_t pool_pages;
+  uint32_t device_error;
+  uint64_t assigned_items;
+  uint64_t processed_items;
+  uint64_t enqueued_items;
+  uint64_t edge_relaxations;
+  uint64_t successful_relaxations;
+  uint64_t head_switches;
+  uint64_t delta_updates;
+  uint64_t max_inflight_items;
+} mi355_adds_stats;
+
+/* Replaces hip_adds::sssp for float and uint32_t weights
+ * (JAS/hip_adds_sssp.hpp:99-113, hip_adds_sssp.cpp:1138-1301): device CSR with
+ * uint32 offsets/indices, caller-owned device distances (INF = +inf for float,
+ * 0xFFFFFFFF for uint32).  Returns the hip_adds::Status value (0 = Success,
+ * 1 InvalidArgument, 2 HipError, 3 OutOfMemory, 4 PoolExhausted,
+ * 5 BucketOverflow, 6 DeviceError). */
+int mi355_hip_adds_sssp(int32_t weight_type, uint32_t num_vertices, uint32_t num_edges,
+                        const uint32_t* d_row_offsets, const uint32_t* d_col_indices,
+                        const void* d_weights, uint32_t source, void* d_distances,
+                        mi355_adds_stats* stats, const mi355_adds_options* options,
+                        void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

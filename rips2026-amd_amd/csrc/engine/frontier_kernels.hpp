@@ -1,0 +1,193 @@
+// Frontier kernels shared by the Bellman-Ford and delta-stepping drivers.
+#pragma once
+
+#include "graph.hpp"
+#include "relax.hpp"
+
+namespace mi355 {
+
+struct FrontierIn {
+  const int* q;          // frontier vertex ids
+  const uint32_t* keys;  // Jacobi snapshot keys (nullptr: read the live label)
+  const uint32_t* cnt;   // frontier size (device)
+};
+
+struct ChunkQueue {
+  uint4* q;       // {vertex, key, first edge, end edge}
+  uint32_t* cnt;  // appended chunk count (device)
+  uint32_t cap;
+};
+
+// Book-keeping done by thread 0 of block 0 of a round's first kernel.
+struct RoundHousekeeping {
+  uint32_t* zero0;  // counters that the next rounds will append to
+  uint32_t* zero1;
+  uint32_t* log;    // log[0] = frontier size of this round (nullptr: none)
+};
+
+__device__ inline uint32_t atomic_load_key(const unsigned long long* p) {
+  return (uint32_t)(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
+}
+
+// Bellman-Ford sink: improved vertices go to the next frontier, once per round.
+struct BfSink {
+  int* qout;
+  uint32_t* cnt;
+  uint32_t cap;
+  uint32_t* stamp;
+  uint32_t stamp_val;
+  uint32_t* err;
+  __device__ inline void vertex(bool, int, uint32_t) const {}
+  __device__ inline void improved(bool imp, uint32_t v, uint32_t) const {
+    bool push = false;
+    if (imp && stamp[v] != stamp_val) push = atomicExch(stamp + v, stamp_val) != stamp_val;
+    wave_append(push, (int)v, qout, cnt, cap, err);
+  }
+};
+
+// Small/medium-degree frontier vertices: 64 per wave, edges packed across
+// lanes.  Frontier entries [first, n_in) are processed; cls selects all,
+// light or heavy edges.
+template <class W, bool JACOBI, class Sink>
+__device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
+                                           const uint2* __restrict__ edges,
+                                           unsigned long long* __restrict__ labels,
+                                           const int* __restrict__ qin,
+                                           const uint32_t* __restrict__ qkeys, uint32_t first,
+                                           uint32_t n_in, ChunkQueue cq, int cls,
+                                           uint32_t delta_bits, Sink& sink,
+                                           unsigned long long* __restrict__ work) {
+  __shared__ int s_incl[kWavesPerBlock][kWave];
+  __shared__ int s_base[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_key[kWavesPerBlock][kWave];
+  __shared__ int s_u[kWavesPerBlock][kWave];
+
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  const unsigned wv = threadIdx.x / kWave;
+  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  unsigned long long edge_visits = 0;
+
+  for (uint32_t base = first + (blockIdx.x * kWavesPerBlock + wv) * kWave; base < n_in;
+       base += nwaves * kWave) {
+    const uint32_t i = base + lane;
+    int u = 0, beg = 0, deg = 0;
+    uint32_t key = W::kInfKey;
+    if (i < n_in) {
+      u = qin[i];
+      key = JACOBI ? qkeys[i] : atomic_load_key(labels + u);
+      beg = rowptr[u];
+      const int end = rowptr[u + 1];
+      if (key != W::kInfKey) {
+        deg = end - beg;
+        if (deg > kLargeDegree) {
+          const uint32_t nch = (uint32_t)((deg + kChunk - 1) / kChunk);
+          const uint32_t pos = atomicAdd(cq.cnt, nch);
+          for (uint32_t c = 0; c < nch; ++c) {
+            const int a = beg + (int)c * kChunk;
+            const int b = min(end, a + kChunk);
+            if (pos + c < cq.cap) cq.q[pos + c] = make_uint4((uint32_t)u, key, (uint32_t)a, (uint32_t)b);
+          }
+          deg = 0;
+        }
+      }
+    }
+    sink.vertex(i < n_in && key != W::kInfKey, u, key);
+    int incl = deg;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int t = __shfl_up(incl, off, kWave);
+      if ((int)lane >= off) incl += t;
+    }
+    const int total = __shfl(incl, kWave - 1, kWave);
+    s_incl[wv][lane] = incl;
+    s_base[wv][lane] = beg - (incl - deg);
+    s_key[wv][lane] = key;
+    s_u[wv][lane] = u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    edge_visits += (unsigned long long)total;
+
+    for (int t = 0; t < total; t += kWave) {
+      const int j = t + (int)lane;
+      bool imp = false;
+      uint32_t v = 0, ck = 0;
+      if (j < total) {
+        int pos = 0;
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1)
+          if (s_incl[wv][pos + s - 1] <= j) pos += s;
+        const uint2 ed = edges[s_base[wv][pos] + j];
+        v = ed.x;
+        if (edge_selected<W>(cls, ed.y, delta_bits)) {
+          ck = W::relax(s_key[wv][pos], ed.y);
+          imp = relax_edge<W>(labels, v, ck, (uint32_t)s_u[wv][pos]);
+        }
+      }
+      sink.improved(imp, v, ck);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (work && lane == 0 && edge_visits) atomicAdd(work, edge_visits);
+}
+
+// Chunks of high-degree vertices: one chunk per wave, lanes stream its edges.
+template <class W, class Sink>
+__device__ inline void relax_chunks_body(const uint2* __restrict__ edges,
+                                         unsigned long long* __restrict__ labels, ChunkQueue cq,
+                                         int cls, uint32_t delta_bits, Sink& sink,
+                                         unsigned long long* __restrict__ work) {
+  const uint32_t n_ch = min(*cq.cnt, cq.cap);
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  unsigned long long edge_visits = 0;
+  for (uint32_t c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; c < n_ch; c += nwaves) {
+    const uint4 ch = cq.q[c];
+    edge_visits += ch.w - ch.z;
+    for (uint32_t t = ch.z; t < ch.w; t += kWave) {
+      const uint32_t e = t + lane;
+      bool imp = false;
+      uint32_t v = 0, ck = 0;
+      if (e < ch.w) {
+        const uint2 ed = edges[e];
+        v = ed.x;
+        if (edge_selected<W>(cls, ed.y, delta_bits)) {
+          ck = W::relax(ch.y, ed.y);
+          imp = relax_edge<W>(labels, v, ck, ch.x);
+        }
+      }
+      sink.improved(imp, v, ck);
+    }
+  }
+  if (work && lane == 0 && edge_visits) atomicAdd(work, edge_visits);
+}
+
+template <class W, bool JACOBI, class Sink>
+__global__ __launch_bounds__(kBlock) void k_relax_frontier(
+    const int* __restrict__ rowptr, const uint2* __restrict__ edges,
+    unsigned long long* __restrict__ labels, FrontierIn in, ChunkQueue cq, Sink sink,
+    RoundHousekeeping hk, unsigned long long* __restrict__ work) {
+  const uint32_t n_in = *in.cnt;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (hk.zero0) *hk.zero0 = 0;
+    if (hk.zero1) *hk.zero1 = 0;
+    if (hk.log) *hk.log = n_in;
+  }
+  relax_frontier_body<W, JACOBI>(rowptr, edges, labels, in.q, in.keys, 0u, n_in, cq, kAllEdges,
+                                 0u, sink, work);
+}
+
+template <class W, class Sink>
+__global__ __launch_bounds__(kBlock) void k_relax_chunks(const uint2* __restrict__ edges,
+                                                         unsigned long long* __restrict__ labels,
+                                                         ChunkQueue cq, Sink sink,
+                                                         unsigned long long* __restrict__ work) {
+  relax_chunks_body<W>(edges, labels, cq, kAllEdges, 0u, sink, work);
+}
+
+// Jacobi snapshot: keys of the next frontier as they stand after the round.
+__global__ void k_snapshot_keys(const int* __restrict__ q, const uint32_t* __restrict__ cnt,
+                                const unsigned long long* __restrict__ labels,
+                                uint32_t* __restrict__ keys);
+
+}  // namespace mi355

@@ -1,0 +1,76 @@
+// Wave64 frontier relaxation: the hot loop shared by Bellman-Ford and
+// delta-stepping.
+//
+// One wave takes 64 frontier vertices (one per lane), reads their offsets and
+// distance keys, and packs their edge lists across the 64 lanes with a wave
+// prefix sum, so a lane relaxes edge j of the packed range and consecutive
+// lanes stream consecutive 8-byte edge records of the same vertex (coalesced).
+// A lane finds the vertex owning packed edge j by a 6-step binary search over
+// the wave's inclusive degree scan held in LDS.  Vertices with more than
+// kLargeDegree edges are instead cut into kChunk-edge pieces and appended to a
+// chunk queue that whole waves drain (relax_chunks), so R-MAT hubs spread over
+// the grid instead of stalling one wave.
+//
+// Relaxation is a packed 64-bit atomicMin of (dist key << 32 | parent) after
+// a plain load that filters candidates which cannot win.  A lane whose
+// candidate strictly lowers the distance part pushes the target to the output
+// queue once per round (round stamp), and the pushes of a wave are compacted
+// with a ballot + mbcnt into one atomicAdd on the queue counter.
+#pragma once
+
+#include "common.hpp"
+
+namespace mi355 {
+
+__device__ inline unsigned lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ inline unsigned long long ballot64(bool p) { return __ballot(p); }
+
+// Count of set bits of mask below this lane.
+__device__ inline unsigned prefix_count(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wave-aggregated append of `v` for the lanes with push == true.
+__device__ inline void wave_append(bool push, int v, int* __restrict__ q, uint32_t* __restrict__ cnt,
+                                   uint32_t cap, uint32_t* __restrict__ err) {
+  const unsigned long long mask = ballot64(push);
+  if (mask == 0) return;
+  const unsigned lane = lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (push) {
+    const uint32_t pos = base + prefix_count(mask);
+    if (pos < cap) q[pos] = v;
+    else atomicOr(err, 1u);
+  }
+}
+
+// Edge filter for delta-stepping's light/heavy split (w <= delta is light,
+// DS/src/delta_stepping_hip_CSR.cpp:402, :440).
+enum EdgeClass { kAllEdges = 0, kLightEdges = 1, kHeavyEdges = 2 };
+
+template <class W>
+__device__ inline bool edge_selected(int cls, uint32_t wbits, uint32_t delta_bits) {
+  if (cls == kAllEdges) return true;
+  bool light;
+  if (W::kType == MI355_W_F32) light = __uint_as_float(wbits) <= __uint_as_float(delta_bits);
+  else light = (float)wbits <= __uint_as_float(delta_bits);
+  return cls == kLightEdges ? light : !light;
+}
+
+// Relax one edge; returns true when the distance part strictly decreased.
+template <class W>
+__device__ inline bool relax_edge(unsigned long long* __restrict__ labels, uint32_t v,
+                                  uint32_t cand_key, uint32_t parent) {
+  if (cand_key == W::kInfKey) return false;
+  const unsigned long long cand = ((unsigned long long)cand_key << 32) | parent;
+  const unsigned long long cur = labels[v];
+  if (cand >= cur) return false;
+  const unsigned long long old = atomicMin(labels + v, cand);
+  return cand_key < (uint32_t)(old >> 32);
+}
+
+}  // namespace mi355

@@ -1,0 +1,53 @@
+// SSSP drivers on a graph handle.
+#pragma once
+
+#include <vector>
+
+#include "graph.hpp"
+
+namespace mi355 {
+
+struct RunRequest {
+  int algorithm = MI355_ALG_BF;
+  int64_t max_iters = -1;  // <0: default
+  int check_interval = 1;
+  float delta = 0.0f;
+  int rounds_per_sync = 0;
+  bool want_work = false;
+  int dynamic_delta = 1;
+  int use_mlp = 0;
+  int worker_blocks = 0;
+};
+
+struct RunResult {
+  int64_t iterations_used = 0;
+  bool converged = false;
+  int64_t rounds = 0;
+  int syncs = 0;
+  int64_t edges = 0;
+  int64_t vertices = 0;
+  float initial_delta = 0.f;
+  float final_delta = 0.f;
+  float device_ms = 0.f;
+  bool negative_cycle = false;
+  std::vector<uint32_t> frontier_log;  // BF: |F_r| for r = 0..rounds
+};
+
+// Runs SSSP leaving packed labels in g.ws.labels.
+void run_sssp(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
+              hipStream_t stream);
+
+// Converts labels to distances (+ predecessors); counts reached vertices and
+// the sum of their out-degrees when reach != nullptr (device: 2 x u64).
+void labels_to_output(mi355_graph& g, uint32_t source, int inf_mode, void* d_dist, int32_t* d_pred,
+                      unsigned long long* reach, hipStream_t stream);
+
+// Delta-stepping driver (delta.hip).
+void run_delta_stepping(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
+                        hipStream_t stream);
+
+// Heuristic bucket width: 16 * mean positive weight / mean degree
+// (ONE/sssp_hip.cpp:59-81, JAS/hip_adds_sssp.cpp:1121).
+float heuristic_delta(const mi355_graph& g);
+
+}  // namespace mi355

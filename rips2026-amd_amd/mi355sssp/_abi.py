@@ -1,0 +1,182 @@
+"""ctypes declarations of include/mi355sssp.h and include/mi355graph.h."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.normpath(os.path.join(_HERE, "..", "lib"))
+
+ALG_BF, ALG_BF_JACOBI, ALG_DELTA, ALG_ADDS = 0, 1, 2, 3
+INF_IEEE, INF_FLT_MAX = 0, 1
+W_F32, W_U32 = 0, 1
+OUT, IN = 0, 1
+HOST, DEVICE = 0, 1
+
+STATUS_NAMES = {
+    0: "MI355_OK", -1: "MI355_EINVAL", -2: "MI355_ERANGE", -3: "MI355_EOVERFLOW",
+    -4: "MI355_EHIP", -5: "MI355_ENOMEM", -6: "MI355_ECAPACITY", -7: "MI355_EDEVICE",
+}
+
+
+class Mi355Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class CsrDesc(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("nnz", C.c_int64),
+                ("rowptr", C.c_void_p), ("col", C.c_void_p), ("w", C.c_void_p),
+                ("offset_bits", C.c_int32), ("weight_type", C.c_int32),
+                ("orientation", C.c_int32), ("memory", C.c_int32)]
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("m", C.c_int64), ("weight_type", C.c_int32),
+                ("max_degree", C.c_int32), ("avg_degree", C.c_double),
+                ("avg_weight", C.c_double), ("large_vertices", C.c_int64),
+                ("large_chunks", C.c_int64), ("has_negative", C.c_int32), ("device", C.c_int32)]
+
+
+class SsspOptions(C.Structure):
+    _fields_ = [("algorithm", C.c_int32), ("max_iters", C.c_int32),
+                ("check_interval", C.c_int32), ("delta", C.c_float), ("inf_mode", C.c_int32),
+                ("dynamic_delta", C.c_int32), ("use_mlp", C.c_int32),
+                ("worker_blocks", C.c_int32), ("rounds_per_sync", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
+
+
+class SsspStats(C.Structure):
+    _fields_ = [("iterations_used", C.c_int32), ("converged", C.c_int32), ("rounds", C.c_int32),
+                ("syncs", C.c_int32), ("edges_relaxed", C.c_int64),
+                ("vertices_processed", C.c_int64), ("reached", C.c_int64),
+                ("reached_edges", C.c_int64), ("initial_delta", C.c_float),
+                ("final_delta", C.c_float), ("device_ms", C.c_float), ("reserved_f", C.c_float),
+                ("has_negative_cycle", C.c_int32), ("reserved_i", C.c_int32)]
+
+
+class BfProgress(C.Structure):
+    _fields_ = [("iteration", C.c_int), ("max_iters", C.c_int),
+                ("convergence_checked", C.c_ubyte), ("changed", C.c_ubyte)]
+
+
+PROGRESS_CB = C.CFUNCTYPE(None, C.POINTER(BfProgress), C.c_void_p)
+
+
+class BfCsrStats(C.Structure):
+    _fields_ = [("iterations_used", C.c_int), ("converged", C.c_int),
+                ("check_interval", C.c_int), ("checks_performed", C.c_int),
+                ("changed_checks", C.c_int), ("unchanged_checks", C.c_int)] + [
+        (name, C.c_double) for name in (
+            "initialization_ms", "bellman_ford_loop_ms", "relaxation_ms", "checked_merge_ms",
+            "unchecked_merge_ms", "checked_iteration_ms", "unchecked_iteration_ms",
+            "final_copy_ms", "total_runtime_ms", "estimated_check_overhead_ms")]
+
+
+class GView(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("nnz", C.c_int64),
+                ("weight_type", C.c_int32), ("orientation", C.c_int32),
+                ("rowptr", C.POINTER(C.c_int64)), ("col", C.POINTER(C.c_int32)),
+                ("w", C.POINTER(C.c_uint32)), ("declared_edges", C.c_uint64),
+                ("loaded_edges", C.c_uint64)]
+
+
+def _load(name: str) -> C.CDLL:
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() first")
+    return C.CDLL(path)
+
+
+_engine = None
+_graph = None
+
+P = C.c_void_p
+I32, I64, U32, U64 = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64
+
+_ENGINE_SIGS = {
+    "mi355_last_error": (C.c_char_p, []),
+    "mi355_last_hip_error": (C.c_int, []),
+    "mi355_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "mi355_set_device": (C.c_int, [C.c_int]),
+    "mi355_device_malloc": (C.c_int, [C.POINTER(P), C.c_size_t]),
+    "mi355_device_free": (C.c_int, [P]),
+    "mi355_memcpy_h2d": (C.c_int, [P, P, C.c_size_t, P]),
+    "mi355_memcpy_d2h": (C.c_int, [P, P, C.c_size_t, P]),
+    "mi355_memset_d8": (C.c_int, [P, C.c_int, C.c_size_t, P]),
+    "mi355_stream_create": (C.c_int, [C.POINTER(P)]),
+    "mi355_stream_destroy": (C.c_int, [P]),
+    "mi355_stream_sync": (C.c_int, [P]),
+    "mi355_graph_create": (C.c_int, [C.POINTER(CsrDesc), I32, P, C.POINTER(P)]),
+    "mi355_graph_destroy": (C.c_int, [P]),
+    "mi355_graph_info_get": (C.c_int, [P, C.POINTER(GraphInfo)]),
+    "mi355_graph_download": (C.c_int, [P, P, P, P]),
+    "mi355_sssp_options_default": (None, [C.POINTER(SsspOptions)]),
+    "mi355_sssp": (C.c_int, [P, I32, C.POINTER(SsspOptions), P, P, C.POINTER(SsspStats), P]),
+    "mi355_sssp_labels": (C.c_int, [P, I32, C.POINTER(SsspOptions), C.POINTER(SsspStats), P]),
+    "mi355_sssp_labels_to_output": (C.c_int, [P, I32, I32, P, P, P]),
+    "mi355_bf_csr_f32": (C.c_int, [C.POINTER(CsrDesc), C.c_int, C.c_int, C.c_int, P,
+                                   C.POINTER(BfCsrStats), PROGRESS_CB, P, P]),
+    "mi355_delta_stepping_csr_f32": (C.c_int, [C.POINTER(CsrDesc), C.c_int, C.c_float, C.c_int, P,
+                                               C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                               PROGRESS_CB, P, P]),
+}
+
+_GRAPH_SIGS = {
+    "mi355g_last_error": (C.c_char_p, []),
+    "mi355g_view": (C.c_int, [P, C.POINTER(GView)]),
+    "mi355g_free": (None, [P]),
+    "mi355g_from_csr": (C.c_int, [I64, I64, I64, P, P, P, I32, I32, C.POINTER(P)]),
+    "mi355g_from_edges": (C.c_int, [I64, I64, P, P, P, I32, I32, I32, C.POINTER(P)]),
+    "mi355g_transpose": (C.c_int, [P, C.POINTER(P)]),
+    "mi355g_read_ripscsr1": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "mi355g_write_ripscsr1": (C.c_int, [P, C.c_char_p]),
+    "mi355g_read_dimacs": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "mi355g_read_galois_gr": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "mi355g_write_galois_gr": (C.c_int, [P, C.c_char_p]),
+    "mi355g_gen_grid4": (C.c_int, [I32, I32, U32, U32, U64, C.POINTER(P)]),
+    "mi355g_gen_rmat": (C.c_int, [I32, I32, C.c_double, C.c_double, C.c_double, U64, I32,
+                                  C.POINTER(P)]),
+    "mi355g_gen_routing_grid": (C.c_int, [I32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                          U64, C.POINTER(P)]),
+    "mi355g_gen_chain_random": (C.c_int, [I64, I32, C.c_double, C.c_double, I32, U64,
+                                          C.POINTER(P)]),
+    "mi355g_pick_sources": (C.c_int, [P, I32, U64, P]),
+    "mi355g_sha256": (C.c_int, [P, C.c_char_p]),
+    "mi355g_rng_u64": (U64, [U64, U64, U64]),
+    "mi355g_rng_unit_f32": (C.c_float, [U64, U64, U64]),
+    "mi355g_fill_unit_f32": (None, [U64, U64, U64, I64, P]),
+}
+
+
+def _bind(lib: C.CDLL, sigs: dict) -> C.CDLL:
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def engine() -> C.CDLL:
+    global _engine
+    if _engine is None:
+        _engine = _bind(_load("libmi355sssp.so"), _ENGINE_SIGS)
+    return _engine
+
+
+def graphlib() -> C.CDLL:
+    global _graph
+    if _graph is None:
+        _graph = _bind(_load("libmi355graph.so"), _GRAPH_SIGS)
+    return _graph
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise Mi355Error(status, engine().mi355_last_error().decode())
+
+
+def gcheck(status: int) -> None:
+    if status != 0:
+        raise Mi355Error(status, graphlib().mi355g_last_error().decode())

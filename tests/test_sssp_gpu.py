@@ -1,0 +1,221 @@
+"""GPU parity of the SSSP engine against the CPU oracle (bit-exact).
+
+Every case compares the HIP path, called through the C ABI, with the oracle
+on the same seeded inputs: distances bit for bit, predecessors bit for bit
+against the canonical (smallest-index) parent, plus the predecessor
+invariant dist[v] == dist[p] (+) w(p, v).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import mi355sssp as M
+from oracle import orc
+from golden import bf_cases
+
+pytestmark = pytest.mark.gpu
+
+ALGS = [M.ALG_BF, M.ALG_BF_JACOBI, M.ALG_DELTA]
+
+
+def _oracle(g: M.HostGraph, src: int):
+    w = g.weights
+    dist = orc.dijkstra(g.rowptr, g.col, w, src)
+    pred = orc.canonical_pred(g.rowptr, g.col, w, src, dist)
+    return dist, pred
+
+
+def _check(g: M.HostGraph, dg: M.DeviceGraph, src: int, alg: int, delta: float = 0.0):
+    print(f"case n={g.n} m={g.m} src={src} alg={alg} delta={delta}", flush=True)
+    dist, pred, st = dg.sssp(src, alg, delta=delta)
+    want_d, want_p = _oracle(g, src)
+    assert dist.view(np.uint32).tobytes() == want_d.view(np.uint32).tobytes(), \
+        f"dist mismatch at {np.flatnonzero(dist.view(np.uint32) != want_d.view(np.uint32))[:10]}"
+    assert orc.check_pred(g.rowptr, g.col, g.weights, src, dist, pred) == 0
+    assert np.array_equal(pred, want_p)
+    mask = want_p >= 0
+    mask[src] = True
+    assert st.reached == int(mask.sum())
+    assert st.reached_edges == int(np.diff(g.rowptr)[mask].sum())
+    return st
+
+
+@pytest.fixture(scope="module")
+def grid256():
+    g = M.grid4(256, 256, 1, 100, seed=2026)
+    return g, M.DeviceGraph(g)
+
+
+@pytest.mark.parametrize("alg", ALGS)
+def test_grid256_u32(grid256, alg):
+    g, dg = grid256
+    assert g.m == 261120
+    st = _check(g, dg, 0, alg)
+    assert st.converged == 1
+
+
+@pytest.mark.parametrize("alg", ALGS)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_chain_random_f32(alg, seed):
+    # the reference's verify generator shape: chain + 8 random out-edges,
+    # weights uniform in [1, 16] (DS/testing/verify_bf_delta_vs_dijkstra.cpp:144-201)
+    g = M.chain_random(2048, 8, 1.0, 16.0, M.W_F32, seed=seed)
+    dg = M.DeviceGraph(g)
+    for src in (0, 17, 2047):
+        _check(g, dg, src, alg, delta=4.0)
+
+
+@pytest.fixture(scope="module")
+def rmat14():
+    g = M.rmat(14, 16, seed=7)
+    return g, M.DeviceGraph(g)
+
+
+@pytest.mark.parametrize("alg", ALGS)
+def test_rmat14_f32(rmat14, alg):
+    g, dg = rmat14
+    assert dg.info.large_vertices > 0  # exercises the hub chunk path
+    for src in g.pick_sources(4, seed=11):
+        _check(g, dg, int(src), alg)
+
+
+@pytest.mark.parametrize("alg", ALGS)
+def test_routing_grid_u32(alg):
+    g = M.routing_grid(120, seed=5)
+    dg = M.DeviceGraph(g)
+    for src in g.pick_sources(3, seed=9):
+        _check(g, dg, int(src), alg)
+
+
+@pytest.mark.parametrize("delta", [0.05, 0.3, 2.0])
+def test_rmat_delta_sweep(rmat14, delta):
+    g, dg = rmat14
+    src = int(g.pick_sources(1, seed=3)[0])
+    _check(g, dg, src, M.ALG_DELTA, delta=delta)
+
+
+def test_tiny_graph_known_answer():
+    ka = bf_cases.known_answers()["tiny_gr"]
+    rp = [0] + ka["row_ends"]
+    g = M.from_csr(rp, ka["destinations"], np.array(ka["weights"], np.float32))
+    dg = M.DeviceGraph(g)
+    for alg in ALGS:
+        dist, pred, _ = dg.sssp(0, alg, delta=1.0)
+        want = np.array([np.inf if x is None else x for x in ka["expected"]], np.float32)
+        assert dist.tobytes() == want.tobytes()
+        assert list(pred) == [-1, 0, 1, 2, -1]
+
+
+def test_rrg_path_known_answer():
+    ka = bf_cases.known_answers()["rrg"]
+    e = np.array(ka["edges"], dtype=object)
+    g = M.from_edges(ka["n"], e[:, 0].astype(np.int32), e[:, 1].astype(np.int32),
+                     e[:, 2].astype(np.float32))
+    dg = M.DeviceGraph(g)
+    for alg in ALGS:
+        dist, pred, _ = dg.sssp(ka["source"], alg, delta=0.5)
+        path = [ka["target"]]
+        while path[-1] != ka["source"]:
+            path.append(int(pred[path[-1]]))
+        assert path[::-1] == ka["path"]
+        # the reference's hand value 3.9 is the fp32 left fold along that path
+        want = np.float32(np.float32(np.float32(0.7) + np.float32(1.1)) + np.float32(2.1))
+        assert dist[ka["target"]] == want
+        assert math.isclose(float(dist[ka["target"]]), ka["expected_target_approx"], rel_tol=1e-6)
+
+
+def test_single_vertex_and_unreachable():
+    g = M.from_csr([0], [], np.zeros(0, np.float32))
+    dg = M.DeviceGraph(g)
+    for alg in ALGS:
+        dist, pred, st = dg.sssp(0, alg, delta=1.0)
+        assert dist.tolist() == [0.0] and pred.tolist() == [-1]
+    g = M.from_edges(4, [0, 2], [1, 3], np.array([1.0, 1.0], np.float32))
+    dg = M.DeviceGraph(g)
+    for alg in ALGS:
+        dist, pred, _ = dg.sssp(0, alg, delta=1.0)
+        assert dist[0] == 0 and dist[1] == 1 and np.isinf(dist[2]) and np.isinf(dist[3])
+        assert pred.tolist() == [-1, 0, -1, -1]
+
+
+def test_source_out_of_range_and_negative_delta_rejected():
+    g = M.chain_random(64, 2, 1.0, 2.0, M.W_F32, seed=1)
+    dg = M.DeviceGraph(g)
+    with pytest.raises(M.Mi355Error) as e:
+        dg.sssp(64, M.ALG_BF)
+    assert e.value.code == -2
+    neg = M.from_edges(3, [0, 1], [1, 2], np.array([1.0, -1.0], np.float32))
+    with pytest.raises(M.Mi355Error):
+        M.DeviceGraph(neg)  # negative weights need allow_negative
+    dneg = M.DeviceGraph(neg, allow_negative=True)
+    with pytest.raises(M.Mi355Error):
+        dneg.sssp(0, M.ALG_DELTA, delta=1.0)
+    dist, _, _ = dneg.sssp(0, M.ALG_BF_JACOBI)
+    assert dist.tolist() == [0.0, 1.0, 0.0]
+
+
+# ---- reference entry points ---------------------------------------------------
+
+def _in_graph(n, edges):
+    e = np.array(edges, dtype=np.float64).reshape(-1, 3)
+    return M.from_edges(n, e[:, 0].astype(np.int32), e[:, 1].astype(np.int32),
+                        e[:, 2].astype(np.float32), orientation=M.IN, dedup_min=True)
+
+
+@pytest.mark.parametrize("case", bf_cases.all_cases(), ids=lambda c: c[0])
+def test_bf_csr_known_answers(case):
+    name, n, src, edges, expected, _neg = case
+    print(f"bf_csr case {name}", flush=True)
+    g = _in_graph(n, edges)
+    for ci in (1, 0, 3):
+        dist, st, prog = M.bf_csr(g, src, -1, ci)
+        want_d, want_it, want_conv = orc.bf_jacobi_in(g.rowptr, g.col, g.weights, src, -1, ci)
+        assert dist.tobytes() == want_d.tobytes()
+        assert (st.iterations_used, bool(st.converged)) == (want_it, want_conv)
+        assert len(prog) == st.iterations_used
+    if n > 1:
+        exp = np.array(expected, np.float32)
+        dist, st, _ = M.bf_csr(g, src, -1, 1)
+        assert dist.tobytes() == exp.tobytes()
+
+
+def test_bf_csr_path_graph_hits_iteration_cap():
+    # A path of n vertices needs n-1 changing rounds; with the default cap of
+    # n-1 iterations the no-change round never runs, so converged is false.
+    n = 50
+    g = _in_graph(n, [(i, i + 1, 1.0) for i in range(n - 1)])
+    dist, st, prog = M.bf_csr(g, 0, -1, 1)
+    assert dist.tolist() == [float(i) for i in range(n)]
+    assert (st.iterations_used, st.converged) == (n - 1, 0)
+    assert all(p[3] == 1 for p in prog)
+    dist, st, prog = M.bf_csr(g, 0, -1, 7)
+    assert (st.iterations_used, st.converged, st.checks_performed) == (n - 1, 0, 7)
+
+
+@pytest.mark.parametrize("case", [c for c in bf_cases.all_cases() if not c[5]], ids=lambda c: c[0])
+def test_delta_stepping_csr_known_answers(case):
+    name, n, src, edges, expected, _neg = case
+    g = _in_graph(n, edges)
+    for delta in (0.5, 3.0, 1000.0):
+        dist, it, conv, prog = M.delta_stepping_csr(g, src, delta)
+        assert dist.tobytes() == np.array(expected, np.float32).tobytes()
+        assert it == orc.ds_bucket_count(dist, delta)
+        assert conv
+        assert len(prog) == it and prog[-1][3] == 0
+
+
+def test_delta_stepping_csr_iteration_cap():
+    g = M.chain_random(512, 4, 1.0, 16.0, M.W_F32, seed=4)
+    gi = g.transpose()
+    dist, it, conv, _ = M.delta_stepping_csr(gi, 0, 2.0)
+    full = orc.ds_bucket_count(dist, 2.0)
+    assert it == full and conv
+    assert dist.tobytes() == orc.dijkstra(g.rowptr, g.col, g.weights, 0).tobytes()
+    _, it2, conv2, prog = M.delta_stepping_csr(gi, 0, 2.0, max_iters=3)
+    assert (it2, conv2) == (3, False) and prog[-1][3] == 1
+    with pytest.raises(M.Mi355Error):
+        M.delta_stepping_csr(gi, 0, 0.0)
+    with pytest.raises(M.Mi355Error) as e:
+        M.delta_stepping_csr(gi, 512, 1.0)
+    assert e.value.code == -2
