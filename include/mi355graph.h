@@ -71,6 +71,16 @@ int mi355g_transpose(const mi355g_graph* g, mi355g_graph** out);
  * incoming orientation, int64 rowptr, int32 colind, f32 values. */
 int mi355g_read_ripscsr1(const char* path, mi355g_graph** out);
 int mi355g_write_ripscsr1(const mi355g_graph* g, const char* path);
+/* RIPSADJ1 (BF/src/gr_to_adjacency.cpp:35-37, 180-213): dense incoming matrix
+ * A[v*n+u] = w(u->v), INF = no edge; duplicates reduced by min.  The reader
+ * reports n (matrix == NULL) or fills matrix[0..n*n). */
+int mi355g_write_ripsadj1(const mi355g_graph* g, const char* path);
+int mi355g_read_ripsadj1(const char* path, float* matrix, int64_t capacity, int64_t* n);
+/* Dense row-major matrix -> CSR of its finite entries (orientation MI355G_OUT:
+ * adj[u*n+v] = w(u->v), as dense_to_csr_outgoing in ONE/sssp_hip.cpp:320-348;
+ * MI355G_IN: A[v*n+u] = w(u->v)). */
+int mi355g_from_dense(const float* adj, int64_t n, int32_t orientation, int32_t skip_diagonal,
+                      int32_t reject_negative, mi355g_graph** out);
 /* DIMACS .gr text (BF/src/gr_to_csr.cpp:124-235): 1-based ids, rows sorted by
  * column, duplicate (u,v) reduced by min; result has incoming orientation. */
 int mi355g_read_dimacs(const char* path, mi355g_graph** out);

@@ -168,12 +168,26 @@ int mi355_sssp_round_log(mi355_graph* g, int32_t* sizes, int32_t cap);
 int mi355_minplus_gemm_f32(const float* dA, const float* dB, float* dC, int M, int N, int K,
                            int lda, int ldb, int ldc, void* stream);
 
-/* Batched variant (no reference counterpart): count independent products
- * with per-item sizes and device pointers held in device arrays. */
+/* Batched variant (no reference counterpart; the batched min-plus workload of
+ * the routing engine): count independent products.  The pointer and size
+ * arrays are HOST arrays of device pointers / sizes; items are grouped by the
+ * tile their size class selects and each group is one launch.  Asynchronous. */
 int mi355_minplus_gemm_f32_batched(int count, const float* const* dA, const float* const* dB,
                                    float* const* dC, const int* M, const int* N, const int* K,
-                                   const int* lda, const int* ldb, const int* ldc,
-                                   int max_M, int max_N, void* stream);
+                                   const int* lda, const int* ldb, const int* ldc, void* stream);
+
+/* Project autotuner for the min-plus tile (no reference counterpart; the
+ * reference hard-codes 64x64x32 tiles, MP/src/minplus_hip.cpp:144-151).  Times
+ * every tile variant on an n x n x n product for each n in sizes and keeps the
+ * fastest per size class (ceil(log2 n)).  timings_ms (host, may be NULL)
+ * receives count x mi355_minplus_tile_count() times.  Blocking. */
+int mi355_minplus_tile_count(void);
+const char* mi355_minplus_tile_name(int index);
+int mi355_minplus_autotune(const int* sizes, int count, double* timings_ms, void* stream);
+/* Read / install the 32-entry tuning table (size class -> tile index, -1 =
+ * built-in rule), e.g. to persist a tuning run. */
+int mi355_minplus_tuning_get(int* table32);
+int mi355_minplus_tuning_set(const int* table32);
 
 /* Replaces minplus_sparse::minplus_spgemm_csr_f32 / _large_rows / _tuned<HC,BS>
  * (MP/src/minplus_sparse_hip.hpp:64-150, minplus_sparse_hip.cpp:410-764).
@@ -189,6 +203,17 @@ int mi355_minplus_spgemm_csr_f32(int64_t a_rows, int64_t a_cols, int64_t a_nnz,
                                  const float* b_val, int hash_capacity, int block_size,
                                  int64_t** c_rowptr, int32_t** c_col, float** c_val,
                                  int64_t* c_nnz, void* stream);
+
+/* Element-wise min of two sparse n x 1 distance vectors (device CSR, int64
+ * rowptr, fp32 values, absent = +INF): the merge step d <- min(d, r) of the
+ * reference's CSR Bellman-Ford, bf_csr_detail::merge_distances_on_device
+ * (header missing from the reference; used at BF/src/bf_hip_CSR.cpp:74-76).
+ * Output allocated here (caller frees); changed (may be NULL) = 1 when some
+ * entry of b is strictly below a's.  Blocking. */
+int mi355_sparse_vector_min_merge(int64_t rows, const int64_t* a_rowptr, const float* a_val,
+                                  const int64_t* b_rowptr, const float* b_val,
+                                  int64_t** c_rowptr, int32_t** c_col, float** c_val,
+                                  int64_t* c_nnz, int* changed, void* stream);
 
 /* Replaces bellman_ford_minplus_hip (BF/src/bellman_ford_hip.hpp:14-59,
  * bellman_ford_hip.cpp:64-264): dense incoming adjacency A[v*n+u] = w(u->v),

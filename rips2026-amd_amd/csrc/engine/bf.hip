@@ -107,7 +107,7 @@ unsigned relax_grid() {
 
 template <class W>
 void run_bf_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult& res,
-                  hipStream_t s) {
+                  hipStream_t s, bool resume) {
   Workspace& ws = g.ws;
   const int64_t n = g.n;
   ws.labels.ensure((size_t)n);
@@ -134,14 +134,19 @@ void run_bf_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
   MI_HIP(hipEventCreate(&ev1));
   MI_HIP(hipEventRecord(ev0, s));
 
-  k_init_bf<<<std::min<unsigned>(div_up((uint64_t)n, kBlock), 4096u), kBlock, 0, s>>>(
-      ws.labels.p, n, src, W::kInfKey, W::kZeroKey, ws.q[0].p, ws.qkey[0].p, ws.ctrl.p);
-  MI_LAUNCH_CHECK();
+  // resume: continue from the frontier the previous run on this handle left
+  // (used for the negative-cycle probe round of bellman_ford_sssp_hip).
+  int64_t r = resume ? ws.bf_rounds_done : 0;
+  if (resume) max_rounds += r;
+  if (!resume) {
+    k_init_bf<<<std::min<unsigned>(div_up((uint64_t)n, kBlock), 4096u), kBlock, 0, s>>>(
+        ws.labels.p, n, src, W::kInfKey, W::kZeroKey, ws.q[0].p, ws.qkey[0].p, ws.ctrl.p);
+    MI_LAUNCH_CHECK();
+  }
 
   RoundCtrl* c = ws.ctrl.p;
   const unsigned grid = relax_grid();
   unsigned long long* work = req.want_work ? &c->edges : nullptr;
-  int64_t r = 0;
   int batch = req.rounds_per_sync > 0 ? req.rounds_per_sync : 1;
   uint32_t frontier = 1;
   res.syncs = 0;
@@ -192,6 +197,7 @@ void run_bf_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
 
+  ws.bf_rounds_done = r;
   // Frontier log: |F_0..F_{r-1}| from the kernels, |F_r| from the counter.
   const int64_t logged = std::min<int64_t>(r, (int64_t)ws.log.n);
   res.frontier_log.assign((size_t)logged, 0);
@@ -231,8 +237,8 @@ void run_sssp(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult&
   switch (req.algorithm) {
     case MI355_ALG_BF:
     case MI355_ALG_BF_JACOBI:
-      if (g.weight_type == MI355_W_F32) run_bf_typed<WF32>(g, source, req, res, stream);
-      else run_bf_typed<WU32>(g, source, req, res, stream);
+      if (g.weight_type == MI355_W_F32) run_bf_typed<WF32>(g, source, req, res, stream, false);
+      else run_bf_typed<WU32>(g, source, req, res, stream, false);
       break;
     case MI355_ALG_DELTA:
       run_delta_stepping(g, source, req, res, stream);
@@ -240,6 +246,12 @@ void run_sssp(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult&
     default:
       fail(MI355_EINVAL, "unknown SSSP algorithm");
   }
+}
+
+void run_bf_continue(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
+                     hipStream_t stream) {
+  if (g.weight_type == MI355_W_F32) run_bf_typed<WF32>(g, source, req, res, stream, true);
+  else run_bf_typed<WU32>(g, source, req, res, stream, true);
 }
 
 void labels_to_output(mi355_graph& g, uint32_t source, int inf_mode, void* d_dist, int32_t* d_pred,

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <string>
 #include <vector>
@@ -70,6 +71,11 @@ void download_dist(mi355_graph& g, uint32_t source, int inf_mode, float* host_di
 extern "C" {
 
 const char* mi355_last_error(void) { return g_err.c_str(); }
+int mi355_internal_set_error(int code, const char* msg, int hip) {
+  g_err = msg ? msg : "";
+  g_hip_err = hip;
+  return code;
+}
 int mi355_last_hip_error(void) { return g_hip_err; }
 
 // ---- device plumbing --------------------------------------------------------
@@ -369,6 +375,295 @@ int mi355_delta_stepping_csr_f32(const mi355_csr_desc* A, int source, float delt
       }
     }
   });
+}
+
+}  // extern "C"
+
+// ---- min-plus GEMM / SpGEMM / dense Bellman-Ford ---------------------------
+namespace mi355 {
+void spgemm_minplus(int64_t a_rows, int64_t a_cols, int64_t a_nnz, const int64_t* arp,
+                    const int32_t* acol, const float* aval, int64_t b_rows, int64_t b_cols,
+                    int64_t b_nnz, const int64_t* brp, const int32_t* bcol, const float* bval,
+                    int hash_capacity, int block_size, int64_t** c_rowptr, int32_t** c_col,
+                    float** c_val, int64_t* c_nnz, hipStream_t s);
+}
+
+#include "minplus.hpp"
+
+namespace {
+// One device slot per host thread for single-GEMM descriptors (kept alive for
+// the thread's lifetime so asynchronous launches never see a freed slot).
+thread_local mi355::DevBuf<mi355::MinplusItem> t_item;
+thread_local mi355::DevBuf<mi355::MinplusItem> t_table;
+
+// Upload a dense matrix if it lives on the host.
+const float* stage_dense(const float* adj, int32_t memory, int n, mi355::DevBuf<float>& tmp,
+                         hipStream_t s) {
+  if (memory == MI355_DEVICE) return adj;
+  if (memory != MI355_HOST) mi355::fail(MI355_EINVAL, "unknown memory kind");
+  tmp.reset((size_t)n * (size_t)n);
+  MI_HIP(hipMemcpyAsync(tmp.p, adj, (size_t)n * n * 4, hipMemcpyHostToDevice, s));
+  return tmp.p;
+}
+}  // namespace
+
+extern "C" {
+
+int mi355_minplus_gemm_f32(const float* dA, const float* dB, float* dC, int M, int N, int K,
+                           int lda, int ldb, int ldc, void* stream) {
+  return guard([&] {
+    if (!dA || !dB || !dC) fail(MI355_EINVAL, "null matrix pointer");
+    if (M <= 0 || N <= 0 || K <= 0) fail(MI355_EINVAL, "matrix sizes must be positive");
+    if (lda < K || ldb < N || ldc < N) fail(MI355_EINVAL, "leading dimension smaller than the matrix");
+    // The descriptor slot is reused by the next call on this thread; the copy
+    // is stream-ordered before the launch that reads it.
+    t_item.ensure(1);
+    mi355::minplus_gemm(dA, dB, dC, M, N, K, lda, ldb, ldc, as_stream(stream), t_item.p);
+  });
+}
+
+int mi355_minplus_gemm_f32_batched(int count, const float* const* dA, const float* const* dB,
+                                   float* const* dC, const int* M, const int* N, const int* K,
+                                   const int* lda, const int* ldb, const int* ldc, void* stream) {
+  return guard([&] {
+    if (count < 0) fail(MI355_EINVAL, "negative batch count");
+    if (count == 0) return;
+    if (!dA || !dB || !dC || !M || !N || !K || !lda || !ldb || !ldc) fail(MI355_EINVAL, "null array");
+    std::vector<mi355::MinplusItem> items((size_t)count);
+    for (int i = 0; i < count; ++i) {
+      if (!dA[i] || !dB[i] || !dC[i]) fail(MI355_EINVAL, "null matrix pointer in batch");
+      if (M[i] <= 0 || N[i] <= 0 || K[i] <= 0) fail(MI355_EINVAL, "matrix sizes must be positive");
+      if (lda[i] < K[i] || ldb[i] < N[i] || ldc[i] < N[i])
+        fail(MI355_EINVAL, "leading dimension smaller than the matrix");
+      items[(size_t)i] = mi355::MinplusItem{dA[i], dB[i], dC[i], M[i], N[i], K[i], lda[i], ldb[i], ldc[i], 0};
+    }
+    // Synchronise before reusing the descriptor table of a previous call.
+    MI_HIP(hipStreamSynchronize(as_stream(stream)));
+    mi355::minplus_gemm_batched(items, as_stream(stream), t_table);
+  });
+}
+
+int mi355_minplus_tile_count(void) { return mi355::minplus_tile_count(); }
+const char* mi355_minplus_tile_name(int index) { return mi355::minplus_tile_name(index); }
+
+int mi355_minplus_autotune(const int* sizes, int count, double* timings_ms, void* stream) {
+  return guard([&] {
+    if (count < 0 || (count > 0 && !sizes)) fail(MI355_EINVAL, "invalid sizes");
+    mi355::minplus_autotune(sizes, count, as_stream(stream), timings_ms);
+  });
+}
+int mi355_minplus_tuning_get(int* table32) {
+  return guard([&] {
+    if (!table32) fail(MI355_EINVAL, "null table");
+    mi355::minplus_tuning_get(table32);
+  });
+}
+int mi355_minplus_tuning_set(const int* table32) {
+  return guard([&] {
+    if (!table32) fail(MI355_EINVAL, "null table");
+    mi355::minplus_tuning_set(table32);
+  });
+}
+
+int mi355_minplus_spgemm_csr_f32(int64_t a_rows, int64_t a_cols, int64_t a_nnz,
+                                 const int64_t* a_rowptr, const int32_t* a_col,
+                                 const float* a_val, int64_t b_rows, int64_t b_cols,
+                                 int64_t b_nnz, const int64_t* b_rowptr, const int32_t* b_col,
+                                 const float* b_val, int hash_capacity, int block_size,
+                                 int64_t** c_rowptr, int32_t** c_col, float** c_val,
+                                 int64_t* c_nnz, void* stream) {
+  return guard([&] {
+    mi355::spgemm_minplus(a_rows, a_cols, a_nnz, a_rowptr, a_col, a_val, b_rows, b_cols, b_nnz,
+                          b_rowptr, b_col, b_val, hash_capacity, block_size, c_rowptr, c_col,
+                          c_val, c_nnz, as_stream(stream));
+  });
+}
+
+int mi355_bf_dense_f32(const float* adjacency, int32_t memory, int n, int source, int max_iters,
+                       float* dist, int* iterations_used, int* converged, void* stream) {
+  return guard([&] {
+    if (!adjacency) fail(MI355_EINVAL, "d_adjacency must not be null");
+    if (n <= 0) fail(MI355_EINVAL, "n must be positive");
+    if (source < 0 || source >= n) fail(MI355_EINVAL, "source is outside graph");
+    if (!dist) fail(MI355_EINVAL, "null dist");
+    if (max_iters < 0) max_iters = n - 1;
+    const hipStream_t s = as_stream(stream);
+    mi355::DevBuf<float> tmp, d((size_t)n);
+    const float* A = stage_dense(adjacency, memory, n, tmp, s);
+    int conv = 0;
+    const int used = mi355::dense_bf(A, n, source, max_iters, d.p, &conv, s);
+    MI_HIP(hipMemcpyAsync(dist, d.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    if (iterations_used) *iterations_used = used;
+    if (converged) *converged = conv;
+  });
+}
+
+int mi355_bf_gauss_seidel_f32(const float* adjacency, int32_t memory, int n, int source,
+                              int max_iters, float* dist, int* iterations_used, void* stream) {
+  return guard([&] {
+    if (!adjacency) fail(MI355_EINVAL, "d_adjacency must not be null");
+    if (n <= 0) fail(MI355_EINVAL, "n must be positive");
+    if (source < 0 || source >= n) fail(MI355_EINVAL, "source is outside graph");
+    if (!dist) fail(MI355_EINVAL, "null dist");
+    if (max_iters < 0) max_iters = n - 1;
+    const hipStream_t s = as_stream(stream);
+    mi355::DevBuf<float> tmp, d((size_t)n);
+    const float* A = stage_dense(adjacency, memory, n, tmp, s);
+    mi355::dense_gauss_seidel(A, n, source, max_iters, d.p, s);
+    MI_HIP(hipMemcpyAsync(dist, d.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    if (iterations_used) *iterations_used = max_iters;
+  });
+}
+
+// ---- ONE (adds_sssp_hip / bellman_ford_sssp_hip) ----------------------------
+int mi355_one_sssp_f32(int32_t algorithm, int n, int m, const int* row_offsets,
+                       const int* col_indices, const float* weights, int source,
+                       const mi355_one_options* options, float* dist, mi355_one_result* result,
+                       void* stream) {
+  return guard([&] {
+    if (n <= 0) fail(MI355_EINVAL, "graph must have at least one node");
+    if (m < 0) fail(MI355_EINVAL, "num_edges must be nonnegative");
+    if (source < 0 || source >= n) fail(MI355_EINVAL, "source out of range");
+    if (!row_offsets || (m > 0 && (!col_indices || !weights)) || !dist)
+      fail(MI355_EINVAL, "null graph array");
+    mi355_one_options opt{};
+    opt.max_iters = -1;
+    opt.num_buckets = 32;
+    opt.dynamic_delta = 1;
+    opt.target_low_work = 8192;
+    opt.target_high_work = 262144;
+    if (options) opt = *options;
+    const hipStream_t s = as_stream(stream);
+    mi355_csr_desc d{};
+    d.rows = d.cols = n;
+    d.nnz = m;
+    d.rowptr = row_offsets;
+    d.col = col_indices;
+    d.w = weights;
+    d.offset_bits = 32;
+    d.weight_type = MI355_W_F32;
+    d.orientation = MI355_OUT;
+    d.memory = MI355_HOST;
+    mi355::RunRequest req;
+    mi355::RunResult res;
+    mi355_one_result out{};
+    if (algorithm == MI355_ALG_BF) {
+      // edge-parallel in-place Bellman-Ford (ONE/sssp_hip.cpp:350-426), with
+      // its negative-cycle detection pass (:204-220)
+      GraphPtr g(mi355::build_graph(d, 1, s));
+      req.algorithm = MI355_ALG_BF;
+      req.max_iters = opt.max_iters <= 0 ? std::max(n - 1, 0) : opt.max_iters;
+      mi355::run_sssp(*g, (uint32_t)source, req, res, s);
+      if (!res.converged && res.frontier_log.size() > 0 && res.frontier_log.back() > 0) {
+        // one more round from the last frontier: any improvement means a
+        // negative cycle is reachable from the source
+        mi355::DevBuf<uint32_t> before((size_t)n);
+        mi355::labels_to_output(*g, (uint32_t)source, MI355_INF_IEEE, before.p, nullptr, nullptr, s);
+        std::vector<uint32_t> b0((size_t)n), b1((size_t)n);
+        MI_HIP(hipMemcpyAsync(b0.data(), before.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        MI_HIP(hipStreamSynchronize(s));
+        mi355::RunRequest one = req;
+        one.max_iters = 1;
+        mi355::RunResult r1;
+        mi355::run_bf_continue(*g, (uint32_t)source, one, r1, s);
+        mi355::labels_to_output(*g, (uint32_t)source, MI355_INF_IEEE, before.p, nullptr, nullptr, s);
+        MI_HIP(hipMemcpyAsync(b1.data(), before.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        MI_HIP(hipStreamSynchronize(s));
+        out.has_negative_cycle = b0 != b1 ? 1 : 0;
+        // report the distances of the n-1 rounds, as the reference does
+        std::memcpy(dist, b0.data(), (size_t)n * 4);
+        out.iterations_used = (int)res.iterations_used;
+        out.converged = 0;
+        out.milliseconds = res.device_ms;
+        out.work_count = (unsigned long long)res.edges;
+        if (result) *result = out;
+        return;
+      }
+      download_dist(*g, (uint32_t)source, MI355_INF_IEEE, dist, s);
+      out.iterations_used = (int)res.iterations_used;
+      out.converged = res.converged ? 1 : 0;
+    } else if (algorithm == MI355_ALG_ADDS || algorithm == MI355_ALG_DELTA) {
+      if (opt.num_buckets <= 1 || opt.num_buckets > 64)
+        fail(MI355_EINVAL, "num_buckets must be in [2, 64]");
+      GraphPtr g(mi355::build_graph(d, 0, s));
+      req.algorithm = MI355_ALG_DELTA;
+      req.delta = (opt.initial_delta > 0.0f && std::isfinite(opt.initial_delta))
+                      ? opt.initial_delta : mi355::heuristic_delta(*g);
+      req.max_iters = opt.max_iters > 0 ? opt.max_iters : -1;
+      req.want_work = true;
+      mi355::run_sssp(*g, (uint32_t)source, req, res, s);
+      download_dist(*g, (uint32_t)source, MI355_INF_IEEE, dist, s);
+      out.iterations_used = (int)res.iterations_used;
+      out.converged = res.converged ? 1 : 0;
+      out.final_delta = res.final_delta;
+    } else {
+      fail(MI355_EINVAL, "algorithm must be MI355_ALG_BF or MI355_ALG_ADDS");
+    }
+    out.milliseconds = res.device_ms;
+    out.work_count = (unsigned long long)res.edges;
+    if (result) *result = out;
+  });
+}
+
+// ---- hip_adds::sssp ----------------------------------------------------------
+int mi355_hip_adds_sssp(int32_t weight_type, uint32_t num_vertices, uint32_t num_edges,
+                        const uint32_t* d_row_offsets, const uint32_t* d_col_indices,
+                        const void* d_weights, uint32_t source, void* d_distances,
+                        mi355_adds_stats* stats, const mi355_adds_options* options,
+                        void* stream) {
+  // hip_adds::Status values
+  enum { kSuccess = 0, kInvalidArgument = 1, kHipError = 2, kOutOfMemory = 3, kDeviceError = 6 };
+  if (num_vertices == 0u || source >= num_vertices || !d_distances || !d_row_offsets ||
+      !d_col_indices || !d_weights || (weight_type != MI355_W_F32 && weight_type != MI355_W_U32)) {
+    g_err = "invalid argument";
+    return kInvalidArgument;
+  }
+  const int rc = guard([&] {
+    const hipStream_t s = as_stream(stream);
+    mi355_csr_desc d{};
+    d.rows = d.cols = num_vertices;
+    d.nnz = num_edges;
+    d.rowptr = d_row_offsets;
+    d.col = d_col_indices;
+    d.w = d_weights;
+    d.offset_bits = 32;
+    d.weight_type = weight_type;
+    d.orientation = MI355_OUT;
+    d.memory = MI355_DEVICE;
+    GraphPtr g(mi355::build_graph(d, 0, s));
+    mi355::RunRequest req;
+    req.algorithm = MI355_ALG_DELTA;
+    double delta = 0.0;
+    const double scale = options ? options->initial_delta_scale : 16.0;
+    if (options && options->initial_delta > 0.0) delta = options->initial_delta;
+    else delta = scale * g->info.avg_weight / std::max(1.0, g->info.avg_degree);
+    const double min_delta = options ? std::max(options->min_delta, 1.0e-12) : 1.0e-3;
+    if (!std::isfinite(delta) || delta <= 0.0) delta = min_delta;
+    delta = std::max(delta, min_delta);
+    if (weight_type == MI355_W_U32) delta = std::max(delta, 1.0);
+    req.delta = (float)delta;
+    req.want_work = stats != nullptr;
+    if (options && !options->initialize_distances)
+      req.init_dist = static_cast<const uint32_t*>(d_distances);
+    mi355::RunResult res;
+    mi355::run_sssp(*g, source, req, res, s);
+    mi355::labels_to_output(*g, source, MI355_INF_IEEE, d_distances, nullptr, nullptr, s);
+    MI_HIP(hipStreamSynchronize(s));
+    if (stats) {
+      *stats = mi355_adds_stats{};
+      stats->initial_delta = delta;
+      stats->final_delta = res.final_delta;
+      stats->edge_relaxations = (uint64_t)res.edges;
+      stats->head_switches = (uint64_t)res.iterations_used;
+    }
+  });
+  if (rc == MI355_OK) return kSuccess;
+  if (rc == MI355_EINVAL || rc == MI355_ERANGE || rc == MI355_EOVERFLOW) return kInvalidArgument;
+  if (rc == MI355_ENOMEM) return kOutOfMemory;
+  if (rc == MI355_EHIP) return kHipError;
+  return kDeviceError;
 }
 
 }  // extern "C"

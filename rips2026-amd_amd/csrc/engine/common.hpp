@@ -108,6 +108,11 @@ struct WF32 {
     return f32_to_key(key_to_f32(dkey) + __uint_as_float(wbits));
   }
   __device__ static inline float bucket_value(uint32_t dkey) { return key_to_f32(dkey); }
+  // distance word in the output convention (fp32 bits; +inf or FLT_MAX = INF)
+  __device__ static inline uint32_t output_to_key(uint32_t bits) {
+    const float x = __uint_as_float(bits);
+    return (x >= 3.4028234663852886e38f) ? kF32InfKey : f32_to_key(x);
+  }
 };
 
 struct WU32 {
@@ -119,6 +124,7 @@ struct WU32 {
     return s < d ? kU32Inf : s;
   }
   __device__ static inline float bucket_value(uint32_t d) { return (float)d; }
+  __device__ static inline uint32_t output_to_key(uint32_t d) { return d; }
 };
 
 inline uint64_t pack_label(uint32_t key, uint32_t pred) { return ((uint64_t)key << 32) | pred; }

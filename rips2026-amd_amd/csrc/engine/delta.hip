@@ -131,7 +131,8 @@ struct DsSink {
         if (heavy) a->sstamp[v] = 0u;
         if (a->lstamp[v] != lstamp_val) to_light = atomicExch(a->lstamp + v, lstamp_val) != lstamp_val;
       } else if (b < B + kSlots) {
-        if ((int)atomicMin(a->bstamp + v, (uint32_t)b) > b) {
+        // bstamp starts at 0xFFFFFFFF: compare unsigned
+        if (atomicMin(a->bstamp + v, (uint32_t)b) > (uint32_t)b) {
           const int s = b % kSlots;
           const uint32_t pos = atomicAdd(&c->slot_cnt[s], 1u);
           if ((int64_t)pos < a->slot_cap) a->slots[(int64_t)s * a->slot_cap + pos] = (int)v;
@@ -269,11 +270,14 @@ __global__ void k_ds_control(BucketCtrl* __restrict__ c, int64_t max_buckets) {
 }
 
 template <class W>
-__global__ void k_ds_init(DsArrays a, BucketCtrl* __restrict__ c, int64_t n, uint32_t src) {
+__global__ void k_ds_init(DsArrays a, BucketCtrl* __restrict__ c, int64_t n, uint32_t src,
+                          const uint32_t* __restrict__ init) {
   const unsigned long long inf_label = ((unsigned long long)W::kInfKey << 32) | kNoPred;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-    a.labels[v] = (v == (int64_t)src) ? (((unsigned long long)W::kZeroKey << 32) | kNoPred) : inf_label;
+    unsigned long long l = inf_label;
+    if (init) l = ((unsigned long long)W::output_to_key(init[v]) << 32) | kNoPred;
+    a.labels[v] = (v == (int64_t)src) ? (((unsigned long long)W::kZeroKey << 32) | kNoPred) : l;
     a.bstamp[v] = 0xFFFFFFFFu;
     a.sstamp[v] = 0u;
     a.lstamp[v] = 0u;
@@ -346,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_ds_work(DsArrays a, BucketCtrl* __re
         a.farflag[v] = 0u;
       } else if (b < B + kSlots) {
         a.farflag[v] = 0u;
-        if ((int)atomicMin(a.bstamp + v, (uint32_t)b) > b) {
+        if (atomicMin(a.bstamp + v, (uint32_t)b) > (uint32_t)b) {
           const int s2 = b % kSlots;
           const uint32_t pos = atomicAdd(&c->slot_cnt[s2], 1u);
           if ((int64_t)pos < a.slot_cap) a.slots[(int64_t)s2 * a.slot_cap + pos] = v;
@@ -423,7 +427,8 @@ void run_ds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
   MI_HIP(hipEventCreate(&ev0));
   MI_HIP(hipEventCreate(&ev1));
   MI_HIP(hipEventRecord(ev0, s));
-  k_ds_init<W><<<std::min<unsigned>(div_up((uint64_t)n, kBlock), 4096u), kBlock, 0, s>>>(a, c, n, src);
+  k_ds_init<W><<<std::min<unsigned>(div_up((uint64_t)n, kBlock), 4096u), kBlock, 0, s>>>(
+      a, c, n, src, req.init_dist);
   MI_LAUNCH_CHECK();
   const unsigned grid = (unsigned)std::max(64, device_props().cus * 8);
   int batch = req.rounds_per_sync > 0 ? req.rounds_per_sync : 4;

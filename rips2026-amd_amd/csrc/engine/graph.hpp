@@ -30,6 +30,7 @@ struct Workspace {
   DevBuf<RoundCtrl> ctrl;
   DevBuf<uint32_t> log;               // per-round frontier sizes
   uint32_t stamp_epoch = 0;
+  int64_t bf_rounds_done = 0;         // rounds of the last Bellman-Ford run
   // delta-stepping state (allocated on first use)
   DevBuf<int> slots;                  // bucket window queues
   DevBuf<uint32_t> bstamp;            // smallest bucket a vertex was queued in

@@ -1,0 +1,37 @@
+// Min-plus GEMM (minplus.hip) and dense Bellman-Ford / Gauss-Seidel (dense.hip).
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace mi355 {
+
+struct MinplusItem {
+  const float* A;
+  const float* B;
+  float* C;
+  int M, N, K;
+  int lda, ldb, ldc;
+  int pad;
+};
+
+int minplus_tile_count();
+const char* minplus_tile_name(int i);
+// Asynchronous on stream s; scratch_item is a device slot for the descriptor.
+void minplus_gemm(const float* A, const float* B, float* C, int M, int N, int K, int lda, int ldb,
+                  int ldc, hipStream_t s, MinplusItem* scratch_item, int force_tile = -1);
+void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s,
+                          DevBuf<MinplusItem>& table);
+void minplus_autotune(const int* sizes, int count, hipStream_t s, double* report);
+void minplus_tuning_get(int* table32);
+void minplus_tuning_set(const int* table32);
+
+// Dense incoming matrix A[v*n+u] = w(u->v); dist is device memory (n floats).
+// Returns iterations_used; converged via *converged.
+int dense_bf(const float* A, int n, int source, int max_iters, float* d_dist, int* converged,
+             hipStream_t s);
+void dense_gauss_seidel(const float* A, int n, int source, int max_iters, float* d_dist,
+                        hipStream_t s);
+
+}  // namespace mi355

@@ -17,6 +17,9 @@ struct RunRequest {
   int dynamic_delta = 1;
   int use_mlp = 0;
   int worker_blocks = 0;
+  // Optional warm start: device array of n distances (output convention);
+  // labels start from them (source forced to 0), only the source is active.
+  const uint32_t* init_dist = nullptr;
 };
 
 struct RunResult {
@@ -36,6 +39,10 @@ struct RunResult {
 // Runs SSSP leaving packed labels in g.ws.labels.
 void run_sssp(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
               hipStream_t stream);
+
+// Continues the last Bellman-Ford run on g for req.max_iters more rounds.
+void run_bf_continue(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
+                     hipStream_t stream);
 
 // Converts labels to distances (+ predecessors); counts reached vertices and
 // the sum of their out-degrees when reach != nullptr (device: 2 x u64).

@@ -424,6 +424,83 @@ int mi355g_write_ripscsr1(const mi355g_graph* g, const char* path) {
   });
 }
 
+// ---- RIPSADJ1 (dense incoming matrix) ---------------------------------------
+int mi355g_write_ripsadj1(const mi355g_graph* g, const char* path) {
+  return guarded([&] {
+    if (!g || g->orientation != MI355G_IN || g->weight_type != MI355G_W_F32)
+      return fail(MI355G_EINVAL, "RIPSADJ1 holds incoming-orientation fp32 graphs");
+    const uint64_t n = (uint64_t)g->rows;
+    std::vector<float> a(n * n, std::numeric_limits<float>::infinity());
+    for (uint64_t v = 0; v < n; ++v)
+      for (int64_t e = g->rowptr[v]; e < g->rowptr[v + 1]; ++e) {
+        float x;
+        std::memcpy(&x, &g->w[e], 4);
+        float& slot = a[v * n + (uint64_t)g->col[e]];
+        if (x < slot) slot = x;
+      }
+    File f(path, "wb");
+    if (!f.f) return fail(MI355G_EIO, std::string("cannot create ") + path);
+    const uint64_t h[6] = {1, 1, n, g->declared_edges, g->loaded_edges, n * n};
+    if (!write_arr(f.f, "RIPSADJ1", 8) || !write_arr(f.f, h, 6) || !write_arr(f.f, a.data(), a.size()))
+      return fail(MI355G_EIO, "short write");
+    return 0;
+  });
+}
+
+int mi355g_read_ripsadj1(const char* path, float* matrix, int64_t capacity, int64_t* n_out) {
+  return guarded([&] {
+    File f(path, "rb");
+    if (!f.f) return fail(MI355G_EIO, std::string("cannot open ") + path);
+    char magic[8];
+    uint64_t h[6];
+    if (!read_arr(f.f, magic, 8) || std::memcmp(magic, "RIPSADJ1", 8) != 0)
+      return fail(MI355G_EFORMAT, "bad RIPSADJ1 magic");
+    if (!read_arr(f.f, h, 6)) return fail(MI355G_EFORMAT, "truncated RIPSADJ1 header");
+    if (h[0] != 1 || h[1] != 1 || h[5] != h[2] * h[2])
+      return fail(MI355G_EFORMAT, "unsupported RIPSADJ1 header");
+    if (n_out) *n_out = (int64_t)h[2];
+    if (!matrix) return 0;
+    if ((uint64_t)capacity < h[5]) return fail(MI355G_EINVAL, "matrix buffer too small");
+    if (!read_arr(f.f, matrix, h[5])) return fail(MI355G_EFORMAT, "truncated RIPSADJ1 matrix");
+    return 0;
+  });
+}
+
+// Dense row-major matrix -> CSR of its finite entries.  With
+// orientation MI355G_OUT the matrix is adj[u*n+v] = w(u->v) and the result is
+// outgoing (ONE/sssp_hip.cpp:320-348: the diagonal and non-finite entries
+// are skipped, negative weights rejected); with MI355G_IN it is the
+// incoming matrix A[v*n+u] = w(u->v) (BF/src/bellman_ford_hip.hpp:8-59).
+int mi355g_from_dense(const float* adj, int64_t n, int32_t orientation, int32_t skip_diagonal,
+                      int32_t reject_negative, mi355g_graph** out) {
+  return guarded([&] {
+    if (!adj || n <= 0 || !out) return fail(MI355G_EINVAL, "invalid dense arguments");
+    auto g = new mi355g_graph();
+    g->rows = g->cols = n;
+    g->weight_type = MI355G_W_F32;
+    g->orientation = orientation;
+    g->rowptr.assign(n + 1, 0);
+    for (int64_t r = 0; r < n; ++r) {
+      for (int64_t c = 0; c < n; ++c) {
+        const float x = adj[r * n + c];
+        if ((skip_diagonal && r == c) || !std::isfinite(x)) continue;
+        if (reject_negative && x < 0.0f) {
+          delete g;
+          return fail(MI355G_EINVAL, "negative edge weight in dense adjacency");
+        }
+        uint32_t b;
+        std::memcpy(&b, &x, 4);
+        g->col.push_back((int32_t)c);
+        g->w.push_back(b);
+      }
+      g->rowptr[r + 1] = (int64_t)g->col.size();
+    }
+    g->declared_edges = g->loaded_edges = (uint64_t)g->col.size();
+    *out = g;
+    return 0;
+  });
+}
+
 // ---- DIMACS text -----------------------------------------------------------
 int mi355g_read_dimacs(const char* path, mi355g_graph** out) {
   return guarded([&] {

@@ -20,14 +20,16 @@ from ._abi import (ALG_ADDS, ALG_BF, ALG_BF_JACOBI, ALG_DELTA, INF_FLT_MAX, INF_
                    W_F32, W_U32, OUT, IN, HOST, DEVICE, SsspOptions, SsspStats, CsrDesc,
                    BfCsrStats, BfProgress, PROGRESS_CB, Mi355Error, engine, graphlib)
 from .graph import HostGraph, grid4, rmat, routing_grid, chain_random, from_edges, \
-    from_csr, read_ripscsr1, read_dimacs, read_galois_gr, rng_u64, fill_unit_f32
+    from_csr, from_dense, read_ripscsr1, write_ripsadj1, read_ripsadj1, read_dimacs, \
+    read_galois_gr, rng_u64, fill_unit_f32
 from .device import DeviceGraph, DeviceBuffer, bf_csr, delta_stepping_csr
 
 __all__ = [
     "ALG_ADDS", "ALG_BF", "ALG_BF_JACOBI", "ALG_DELTA", "INF_FLT_MAX", "INF_IEEE", "W_F32",
     "W_U32", "OUT", "IN", "HOST", "DEVICE", "SsspOptions", "SsspStats", "CsrDesc", "BfCsrStats",
     "BfProgress", "PROGRESS_CB", "Mi355Error", "engine", "graphlib", "HostGraph", "grid4", "rmat",
-    "routing_grid", "chain_random", "from_edges", "from_csr", "read_ripscsr1", "read_dimacs",
+    "routing_grid", "chain_random", "from_edges", "from_csr", "from_dense", "read_ripscsr1",
+    "write_ripsadj1", "read_ripsadj1", "read_dimacs",
     "read_galois_gr", "rng_u64", "fill_unit_f32", "DeviceGraph", "DeviceBuffer", "bf_csr",
     "delta_stepping_csr",
 ]

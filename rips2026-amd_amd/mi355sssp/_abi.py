@@ -132,6 +132,9 @@ _GRAPH_SIGS = {
     "mi355g_transpose": (C.c_int, [P, C.POINTER(P)]),
     "mi355g_read_ripscsr1": (C.c_int, [C.c_char_p, C.POINTER(P)]),
     "mi355g_write_ripscsr1": (C.c_int, [P, C.c_char_p]),
+    "mi355g_write_ripsadj1": (C.c_int, [P, C.c_char_p]),
+    "mi355g_read_ripsadj1": (C.c_int, [C.c_char_p, P, I64, C.POINTER(I64)]),
+    "mi355g_from_dense": (C.c_int, [P, I64, I32, I32, I32, C.POINTER(P)]),
     "mi355g_read_dimacs": (C.c_int, [C.c_char_p, C.POINTER(P)]),
     "mi355g_read_galois_gr": (C.c_int, [C.c_char_p, C.POINTER(P)]),
     "mi355g_write_galois_gr": (C.c_int, [P, C.c_char_p]),

@@ -116,6 +116,26 @@ def from_csr(rowptr, col, w, weight_type: int = W_F32, orientation: int = OUT,
                  rp.ctypes.data, cl.ctypes.data, wb.ctypes.data, weight_type, orientation)
 
 
+def from_dense(adj, n: int, orientation: int = OUT, skip_diagonal: bool = True,
+               reject_negative: bool = True) -> HostGraph:
+    a = np.ascontiguousarray(adj, dtype=np.float32)
+    return _make(graphlib().mi355g_from_dense, a.ctypes.data, n, orientation, int(skip_diagonal),
+                 int(reject_negative))
+
+
+def write_ripsadj1(g: HostGraph, path: str) -> None:
+    gcheck(graphlib().mi355g_write_ripsadj1(g._h, path.encode()))
+
+
+def read_ripsadj1(path: str) -> np.ndarray:
+    """Returns the dense incoming matrix as an (n, n) float32 array."""
+    n = C.c_int64()
+    gcheck(graphlib().mi355g_read_ripsadj1(path.encode(), None, 0, C.byref(n)))
+    a = np.empty(n.value * n.value, np.float32)
+    gcheck(graphlib().mi355g_read_ripsadj1(path.encode(), a.ctypes.data, a.size, C.byref(n)))
+    return a.reshape(n.value, n.value)
+
+
 def read_ripscsr1(path: str) -> HostGraph:
     return _make(graphlib().mi355g_read_ripscsr1, path.encode())
 

@@ -30,10 +30,12 @@ def _check(g: M.HostGraph, dg: M.DeviceGraph, src: int, alg: int, delta: float =
     print(f"case n={g.n} m={g.m} src={src} alg={alg} delta={delta}", flush=True)
     dist, pred, st = dg.sssp(src, alg, delta=delta)
     want_d, want_p = _oracle(g, src)
-    assert dist.view(np.uint32).tobytes() == want_d.view(np.uint32).tobytes(), \
-        f"dist mismatch at {np.flatnonzero(dist.view(np.uint32) != want_d.view(np.uint32))[:10]}"
+    bad = np.flatnonzero(dist.view(np.uint32) != want_d.view(np.uint32))
+    assert bad.size == 0, f"{bad.size} distances differ, first at {bad[:8].tolist()}: " \
+                          f"{dist[bad[:4]].tolist()} vs {want_d[bad[:4]].tolist()}"
     assert orc.check_pred(g.rowptr, g.col, g.weights, src, dist, pred) == 0
-    assert np.array_equal(pred, want_p)
+    badp = np.flatnonzero(pred != want_p)
+    assert badp.size == 0, f"{badp.size} predecessors differ, first at {badp[:8].tolist()}"
     mask = want_p >= 0
     mask[src] = True
     assert st.reached == int(mask.sum())
