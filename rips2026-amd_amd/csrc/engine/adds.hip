@@ -1,0 +1,613 @@
+// Asynchronous dynamic delta-stepping (ADDS) as one persistent kernel.
+//
+// Design inputs: the paper-faithful ADDS of ADDSF/kernel.hip:66-221 +
+// ADDSF/wl.h:145-1915 (a persistent manager kernel beside persistent worker
+// kernels, 32 bags mapped onto 32 lanes, wave32 only) and the single-kernel
+// variant of JAS/hip_adds_sssp.cpp:396-866 (manager block + worker blocks,
+// paged pool).  Neither runs on gfx950 as written, so the worklist is
+// re-designed for wave64 and has no manager at all:
+//
+//  * Buckets.  Work items (vertex ids) live in kAddsSlots circular bucket
+//    queues (ring buffers with monotone head/tail counters); absolute bucket
+//    b uses slot b % kAddsSlots.  The window [base, base + kAddsSlots) moves
+//    forward one bucket per phase.  A candidate distance d maps to
+//    base + floor((d - T_base) / delta), clamped into the window (far items
+//    are clipped into the last slot and re-queued when they come up) and
+//    never below the bucket of the vertex that produced it.
+//  * Workers.  Every workgroup is a worker: wave 0 claims up to 256 items from
+//    the lowest non-empty slot of the active window [base, base + A) with a
+//    CAS on the slot head; the four waves relax them, 64 items per wave with
+//    their edges packed across the lanes as in the frontier kernels; vertices
+//    above kLargeDegree are cut into 2048-edge hub chunks any workgroup can
+//    claim.  A wave's pushes are grouped per target slot, one atomicAdd each.
+//  * Dedup.  qstamp[v] = smallest bucket v is queued in (atomicMin on push).
+//    A pop exchanges it with "none"; the returned stamp is the bucket the
+//    entry is processed in (if it falls in this slot), so every live entry is
+//    processed exactly once and superseded entries are dropped.  The label
+//    read after the acquire-exchange sees every improvement that did not
+//    re-queue v, so each final distance is relaxed from.
+//  * Phases.  The base advances (one 64-bit CAS of (base, T_base)) only when
+//    slot base is empty and no claim of it is still in flight, so no push can
+//    land behind the window.  The advancing wave evaluates the delta policy:
+//    fixed, the feedback rule of plain ADDS (grow delta when the window holds
+//    too little work for the workers, shrink it when it holds too much; after
+//    JAS/hip_adds_sssp.cpp:683-727), or the learned MLP, which reads 32
+//    features of the phase and returns log2 of the delta scale and the active
+//    bucket count.  Re-mapping changes priorities only; distances remain the
+//    exact fixpoint of the fp32 / uint32 left folds.
+//  * Termination.  `pending` counts items queued or being processed; pushes
+//    add before their producer finishes, finished (or dropped) items
+//    subtract afterwards, so pending == 0 means there is no work anywhere.
+//    Only running workgroups ever hold items, so spinning workgroups need not
+//    be co-resident with the rest for the run to make progress.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "adds.hpp"
+#include "frontier_kernels.hpp"
+#include "sssp.hpp"
+
+namespace mi355 {
+
+namespace {
+
+constexpr uint32_t kNoStamp = 0xFFFFFFFFu;
+constexpr uint32_t kClaim = 256;  // items per workgroup claim
+constexpr int kEmptyEntry = -1;   // ring sentinel: entry reserved, not yet written
+// Bound of every spin (ring entry, hub chunk, idle worker): far above any
+// legitimate wait (about a second); reaching it aborts the run with an error
+// instead of hanging the device.
+constexpr uint32_t kSpinLimit = 1u << 24;
+constexpr uint32_t kErrRing = 1u, kErrHub = 2u, kErrStall = 4u;
+
+__device__ inline int atomic_add_sc(int* p, int v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t cas_sc(uint32_t* p, uint32_t expected, uint32_t desired) {
+  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return expected;
+}
+
+__device__ inline unsigned long long pack_base(uint32_t base, float t) {
+  return ((unsigned long long)__float_as_uint(t) << 32) | base;
+}
+__device__ inline uint32_t base_of(unsigned long long w) { return (uint32_t)w; }
+__device__ inline float tbase_of(unsigned long long w) { return __uint_as_float((uint32_t)(w >> 32)); }
+
+template <class T>
+__device__ inline T ld_acq(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline T ld_rlx(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bucket of distance d under the mapping (base, T_base, 1/delta), clamped to
+// [lo, base + kAddsSlots - 1] (lo is the producer's bucket).
+__device__ inline uint32_t map_bucket(float d, uint32_t base, float tbase, float inv_delta,
+                                      uint32_t lo) {
+  const float rel = (d - tbase) * inv_delta;
+  uint32_t j = 0;
+  if (rel > 0.0f) j = rel >= (float)(kAddsSlots - 1) ? (uint32_t)(kAddsSlots - 1) : (uint32_t)rel;
+  uint32_t b = base + j;
+  if (b < lo) b = lo;
+  const uint32_t hi = base + kAddsSlots - 1;
+  return b > hi ? (lo > hi ? lo : hi) : b;
+}
+
+struct AddsArgs {
+  const int* rowptr;
+  const uint2* edges;
+  unsigned long long* labels;
+  uint32_t* qstamp;
+  int* ring;          // kAddsSlots * (cap_mask + 1)
+  uint32_t cap_mask;
+  HubChunk* hub;      // hub_mask + 1 records
+  uint32_t hub_mask;
+  AddsCtrl* c;
+  const AddsPolicy* policy;
+  uint32_t n;
+};
+
+__device__ inline void ring_store(const AddsArgs& a, uint32_t slot, uint32_t pos, int v) {
+  __hip_atomic_store(a.ring + (size_t)slot * (a.cap_mask + 1u) + (pos & a.cap_mask), v,
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Queue the lanes' improved vertices: dedup by qstamp, then one atomicAdd per
+// distinct target slot present in the wave.
+__device__ inline void push_improved(const AddsArgs& a, bool imp, uint32_t v, uint32_t bkt,
+                                     unsigned long long& pushed) {
+  bool want = false;
+  // release: a popper that exchanges qstamp after this sees the new label
+  if (imp)
+    want = __hip_atomic_fetch_min(a.qstamp + v, bkt, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) > bkt;
+  unsigned long long mask = __ballot(want);
+  const unsigned lane = lane_id();
+  while (mask) {
+    const int leader = __ffsll((long long)mask) - 1;
+    const uint32_t b = __shfl(bkt, leader, 64);
+    const bool mine = want && bkt == b;
+    const unsigned long long m2 = __ballot(mine);
+    uint32_t pos0 = 0;
+    if ((int)lane == leader) {
+      const uint32_t k = (uint32_t)__popcll(m2);
+      const uint32_t s = b % kAddsSlots;
+      atomicAdd(&a.c->pending, (unsigned long long)k);  // before the items become visible
+      pos0 = atomicAdd(&a.c->tail[s], k);
+      const uint32_t h = ld_rlx(&a.c->head[s]);
+      if (pos0 + k - h > a.cap_mask + 1u) atomicOr(&a.c->err, 1u);
+      pushed += k;
+    }
+    pos0 = __shfl(pos0, leader, 64);
+    if (mine) ring_store(a, b % kAddsSlots, pos0 + prefix_count(m2), (int)v);
+    mask &= ~m2;
+  }
+}
+
+// Wave-level phase advance; true when this wave moved the base.
+__device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
+  AddsCtrl* c = a.c;
+  const unsigned lane = lane_id();
+  const uint32_t base = base_of(bw);
+  const uint32_t s = base % kAddsSlots;
+  bool ok = false;
+  if (lane == 0) {
+    // head before inflight: a claimer raises inflight before its head CAS
+    const uint32_t h = __hip_atomic_load(&c->head[s], __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t t = __hip_atomic_load(&c->tail[s], __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+    const int inflight = __hip_atomic_load(&c->inflight[s], __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+    if (h == t && inflight == 0) {
+      const float delta = ld_rlx(&c->delta);
+      ok = atomicCAS(&c->base_word, bw, pack_base(base + 1, tbase_of(bw) + delta)) == bw;
+    }
+  }
+  ok = __shfl(ok, 0, 64);
+  if (!ok) return false;
+  // phase boundary: occupancy of the new window, then the policy
+  const uint32_t nb = base + 1;
+  uint32_t occ = 0;
+  if (lane < (unsigned)kAddsSlots) {
+    const uint32_t ss = (nb + lane) % kAddsSlots;
+    occ = ld_rlx(&c->tail[ss]) - ld_rlx(&c->head[ss]);
+  }
+  uint32_t window_items = occ;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) window_items += __shfl_xor(window_items, off, 64);
+  float delta = ld_rlx(&c->delta);
+  uint32_t active = ld_rlx(&c->active);
+  const int mode = a.policy ? a.policy->mode : kPolicyFixed;
+  float feat = 0.0f;
+  if (mode == kPolicyMlp || (a.policy && a.policy->trace)) {
+    mlp_features(a.policy, c, occ, window_items, delta, lane, &feat);
+  }
+  if (mode == kPolicyHeuristic) {
+    const float capacity = (float)c->workers * (float)kClaim;
+    if ((float)window_items < c->util_low * capacity) delta = fminf(delta * c->growth, c->max_delta);
+    else if ((float)window_items > c->util_high * capacity) delta = fmaxf(delta * c->shrink, c->min_delta);
+  } else if (mode == kPolicyMlp) {
+    float o0 = 0.0f, o1 = 0.0f;
+    mlp_forward_wave(a.policy, feat, lane, &o0, &o1);
+    const float scale = exp2f(fminf(fmaxf(o0, -4.0f), 4.0f));
+    delta = fminf(fmaxf(c->base_delta * scale, c->min_delta), c->max_delta);
+    active = (uint32_t)fminf(fmaxf(o1 + 0.5f, 1.0f), 8.0f);
+  }
+  const uint32_t ph = ld_rlx(&c->phases);
+  if (a.policy && a.policy->trace && ph < a.policy->trace_cap) {
+    float* t = a.policy->trace + (size_t)ph * kTraceStride;
+    if (lane < (unsigned)kMlpIn) t[lane] = feat;
+    if (lane == 0) {
+      t[32] = (float)window_items;
+      t[33] = delta;
+      t[34] = (float)active;
+      t[35] = (float)ld_rlx(&c->edges);
+      t[36] = (float)ld_rlx(&c->pushes);
+    }
+  }
+  if (lane == 0) {
+    __hip_atomic_store(&c->delta, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&c->active, active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicAdd(&c->phases, 1u);
+  }
+  return true;
+}
+
+// Relax the packed edges of up to 64 vertices held one per lane (deg = 0 for
+// lanes without work) and queue the improved targets.  lo = producer bucket.
+template <class W>
+__device__ inline void relax_wave(const AddsArgs& a, int (&s_incl)[kWave], int (&s_base)[kWave],
+                                  uint32_t (&s_key)[kWave], int (&s_u)[kWave], uint32_t (&s_lo)[kWave],
+                                  int u, int beg, int deg, uint32_t key, uint32_t lo, uint32_t base,
+                                  float tbase, float inv, unsigned long long& edges,
+                                  unsigned long long& pushed) {
+  const unsigned lane = lane_id();
+  int incl = deg;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int t = __shfl_up(incl, off, kWave);
+    if ((int)lane >= off) incl += t;
+  }
+  const int total = __shfl(incl, kWave - 1, kWave);
+  s_incl[lane] = incl;
+  s_base[lane] = beg - (incl - deg);
+  s_key[lane] = key;
+  s_u[lane] = u;
+  s_lo[lane] = lo;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // live work counter (one atomic per wave per claim): the phase trace reads it
+  if (lane == 0 && total > 0) atomicAdd(&a.c->edges, (unsigned long long)total);
+  edges += (unsigned long long)total;
+  for (int t = 0; t < total; t += kWave) {
+    const int j = t + (int)lane;
+    bool imp = false;
+    uint32_t v = 0, bk = 0;
+    if (j < total) {
+      int pos = 0;
+#pragma unroll
+      for (int s2 = 32; s2 > 0; s2 >>= 1)
+        if (s_incl[pos + s2 - 1] <= j) pos += s2;
+      const uint2 ed = a.edges[s_base[pos] + j];
+      v = ed.x;
+      const uint32_t ck = W::relax(s_key[pos], ed.y);
+      imp = relax_edge<W>(a.labels, v, ck, (uint32_t)s_u[pos]);
+      if (imp) bk = map_bucket(W::bucket_value(ck), base, tbase, inv, s_lo[pos]);
+    }
+    push_improved(a, imp, v, bk, pushed);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <class W>
+__global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
+  __shared__ uint32_t s_slot, s_begin, s_count, s_hub, s_done, s_base;
+  __shared__ float s_tbase, s_inv;
+  __shared__ uint32_t s_active;
+  __shared__ int s_incl[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_key[kWavesPerBlock][kWave];
+  __shared__ int s_u[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_lo[kWavesPerBlock][kWave];
+  AddsCtrl* c = a.c;
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned wv = threadIdx.x >> 6;
+  unsigned long long edges_local = 0, pushed_local = 0, items_local = 0;
+  uint32_t idle = 0;
+
+  for (;;) {
+    if (wv == 0) {
+      uint32_t got = 0, slot = 0, begin = 0, hubc = 0, done = 0;
+      if (lane == 0) {
+        done = ld_acq(&c->done);
+        if (!done) {
+          // hub chunks first: they finish items already claimed
+          uint32_t hh = ld_acq(&c->hub_head);
+          while (hh < ld_acq(&c->hub_reserve)) {
+            const uint32_t old = atomicCAS(&c->hub_head, hh, hh + 1);
+            if (old == hh) {
+              got = 1;
+              hubc = 1;
+              begin = hh;
+              break;
+            }
+            hh = old;
+          }
+        }
+        if (!done && !got) {
+          const unsigned long long bw = ld_acq(&c->base_word);
+          const uint32_t base = base_of(bw);
+          const uint32_t A = max(1u, min(ld_rlx(&c->active), (uint32_t)kAddsSlots));
+          for (uint32_t j = 0; j < A && !got; ++j) {
+            const uint32_t s = (base + j) % kAddsSlots;
+            uint32_t h = ld_acq(&c->head[s]);
+            for (uint32_t t = ld_acq(&c->tail[s]); h < t; t = ld_acq(&c->tail[s])) {
+              const uint32_t k = min(kClaim, t - h);
+              atomic_add_sc(&c->inflight[s], 1);  // before the claim is visible
+              const uint32_t old = cas_sc(&c->head[s], h, h + k);
+              if (old == h) {
+                got = k;
+                slot = s;
+                begin = h;
+                break;
+              }
+              atomic_add_sc(&c->inflight[s], -1);
+              h = old;
+            }
+          }
+        }
+      }
+      got = __shfl(got, 0, 64);
+      done = __shfl(done, 0, 64);
+      if (got) idle = 0;
+      if (!got && !done) {
+        const unsigned long long pend = __shfl(lane == 0 ? ld_acq(&c->pending) : 0ull, 0, 64);
+        if (pend == 0) {
+          if (lane == 0) atomicExch(&c->done, 1u);
+          done = 1;
+        } else {
+          const unsigned long long bw = __shfl(lane == 0 ? ld_acq(&c->base_word) : 0ull, 0, 64);
+          if (try_advance(a, bw)) {
+            idle = 0;
+          } else {
+            __builtin_amdgcn_s_sleep(2);
+            if (++idle > kSpinLimit) {  // no progress for ~a second: abort, do not hang
+              if (lane == 0) {
+                atomicOr(&c->err, kErrStall);
+                atomicExch(&c->done, 1u);
+              }
+              done = 1;
+            }
+          }
+        }
+      }
+      if (lane == 0) {
+        s_done = done;
+        s_count = got;
+        s_slot = slot;
+        s_begin = begin;
+        s_hub = hubc;
+        const unsigned long long cur = ld_acq(&c->base_word);
+        s_base = base_of(cur);
+        s_tbase = tbase_of(cur);
+        s_inv = 1.0f / ld_rlx(&c->delta);
+        s_active = max(1u, ld_rlx(&c->active));
+      }
+    }
+    __syncthreads();
+    const uint32_t count = s_count;
+    if (s_done) break;
+    if (count == 0) {
+      __syncthreads();
+      continue;
+    }
+    const uint32_t base = s_base;
+    const float tbase = s_tbase, inv = s_inv;
+
+    if (s_hub) {
+      // one hub chunk; wait until its producer has published it
+      HubChunk* hc = a.hub + (s_begin & a.hub_mask);
+      __shared__ uint32_t s_hub_ok;
+      if (threadIdx.x == 0) {
+        uint32_t spins = 0;
+        while (ld_acq(&hc->seq) != s_begin + 1u && ++spins < kSpinLimit) __builtin_amdgcn_s_sleep(1);
+        s_hub_ok = spins < kSpinLimit;
+        if (!s_hub_ok) {
+          atomicOr(&c->err, kErrHub);
+          atomicExch(&c->done, 1u);
+        }
+      }
+      __syncthreads();
+      if (!s_hub_ok) break;
+      HubChunk ch;
+      ch.u = hc->u;
+      ch.key = hc->key;
+      ch.first = hc->first;
+      ch.cnt = min(hc->cnt, (uint32_t)kChunk);
+      ch.slot = hc->slot % kAddsSlots;
+      ch.bucket = hc->bucket;
+      for (uint32_t t0 = wv * kWave; t0 < ch.cnt; t0 += kBlock) {
+        const uint32_t e = ch.first + t0 + lane;
+        bool imp = false;
+        uint32_t v = 0, bk = 0;
+        if (t0 + lane < ch.cnt) {
+          const uint2 ed = a.edges[e];
+          v = ed.x;
+          const uint32_t ck = W::relax(ch.key, ed.y);
+          imp = relax_edge<W>(a.labels, v, ck, ch.u);
+          if (imp) bk = map_bucket(W::bucket_value(ck), base, tbase, inv, ch.bucket);
+        }
+        push_improved(a, imp, v, bk, pushed_local);
+      }
+      if (wv == 0 && lane == 0) atomicAdd(&c->edges, (unsigned long long)ch.cnt);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        atomic_add_sc(&c->inflight[ch.slot], -1);
+        atomicAdd(&c->pending, ~0ull);
+      }
+      continue;
+    }
+
+    // regular items: wave w takes claim entries [w*64, w*64 + 64)
+    const uint32_t slot = s_slot, begin = s_begin;
+    const int* ring = a.ring + (size_t)slot * (a.cap_mask + 1u);
+    const uint32_t i = wv * kWave + lane;
+    int u = 0, beg = 0, deg = 0;
+    uint32_t key = W::kInfKey, bk = 0;
+    bool entry_ok = true;
+    if (i < count) {
+      // The pusher reserved this position before writing it: wait for the
+      // write (bounded), then hand the entry back to the ring as empty.
+      int* ep = const_cast<int*>(ring) + ((begin + i) & a.cap_mask);
+      uint32_t spins = 0;
+      while ((u = __hip_atomic_load(ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == kEmptyEntry &&
+             ++spins < kSpinLimit)
+        __builtin_amdgcn_s_sleep(1);
+      entry_ok = u != kEmptyEntry && u >= 0 && (uint32_t)u < a.n;
+      if (!entry_ok) {
+        atomicOr(&c->err, kErrRing);
+        atomicExch(&c->done, 1u);
+        u = 0;
+      } else {
+        __hip_atomic_store(ep, kEmptyEntry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (i < count && entry_ok) {
+      const uint32_t old = __hip_atomic_exchange(a.qstamp + u, kNoStamp, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+      if (old != kNoStamp && old % kAddsSlots == slot) {
+        bk = old;
+        key = atomic_load_key(a.labels + u);
+        const uint32_t want = map_bucket(W::bucket_value(key), base, tbase, inv, bk);
+        if (want >= bk + s_active) {
+          // clipped or re-mapped far item: queue it where it belongs now
+          key = W::kInfKey;
+          if (atomicMin(a.qstamp + u, want) > want) {
+            atomicAdd(&c->pending, 1ull);
+            const uint32_t pos = atomicAdd(&c->tail[want % kAddsSlots], 1u);
+            ring_store(a, want % kAddsSlots, pos, u);
+          }
+        }
+      } else if (old != kNoStamp) {
+        atomicMin(a.qstamp + u, old);  // queued again elsewhere: restore, drop this entry
+      }
+      if (key != W::kInfKey) {
+        ++items_local;
+        beg = a.rowptr[u];
+        const int end = a.rowptr[u + 1];
+        deg = end - beg;
+        if (deg > kLargeDegree) {
+          // cut into hub chunks; each chunk is pending work held in this slot
+          const uint32_t nch = (uint32_t)((deg + kChunk - 1) / kChunk);
+          atomicAdd(&c->pending, (unsigned long long)nch);
+          atomicAdd(&c->inflight[slot], (int)nch);
+          const uint32_t pos = atomicAdd(&c->hub_reserve, nch);
+          if (pos + nch - ld_rlx(&c->hub_head) > a.hub_mask + 1u) {
+            atomicOr(&c->err, kErrHub);
+            atomicExch(&c->done, 1u);
+          }
+          for (uint32_t q = 0; q < nch; ++q) {
+            HubChunk* hc = a.hub + ((pos + q) & a.hub_mask);
+            const int e0 = beg + (int)q * kChunk;
+            hc->u = (uint32_t)u;
+            hc->key = key;
+            hc->first = (uint32_t)e0;
+            hc->cnt = (uint32_t)(min(end, e0 + kChunk) - e0);
+            hc->slot = slot;
+            hc->bucket = bk;
+            __hip_atomic_store(&hc->seq, pos + q + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          deg = 0;
+        }
+      }
+    }
+    relax_wave<W>(a, s_incl[wv], s_beg[wv], s_key[wv], s_u[wv], s_lo[wv], u, beg, deg, key, bk,
+                  base, tbase, inv, edges_local, pushed_local);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      atomic_add_sc(&c->inflight[slot], -1);
+      atomicAdd(&c->pending, 0ull - (unsigned long long)count);
+    }
+  }
+  if (lane == 0) {
+    (void)edges_local;
+    if (pushed_local) atomicAdd(&c->pushes, pushed_local);
+  }
+  if (lane == 0 && items_local) atomicAdd(&c->items, items_local);
+}
+
+template <class W>
+__global__ void k_adds_init(AddsArgs a, int64_t n, uint32_t src, const uint32_t* __restrict__ init,
+                            AddsCtrl init_ctrl) {
+  const unsigned long long inf_label = ((unsigned long long)W::kInfKey << 32) | kNoPred;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    unsigned long long l = inf_label;
+    if (init) l = ((unsigned long long)W::output_to_key(init[v]) << 32) | kNoPred;
+    a.labels[v] = v == (int64_t)src ? (((unsigned long long)W::kZeroKey << 32) | kNoPred) : l;
+    a.qstamp[v] = v == (int64_t)src ? 0u : kNoStamp;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= (int64_t)a.hub_mask; i += stride)
+    a.hub[i].seq = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *a.c = init_ctrl;
+    a.c->tail[0] = 1;
+    a.c->pending = 1;
+    a.ring[0] = (int)src;
+  }
+}
+
+template <class W>
+void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult& res,
+                    hipStream_t s, const AddsPolicy* d_policy) {
+  Workspace& ws = g.ws;
+  const int64_t n = g.n;
+  const float hdelta = heuristic_delta(g);
+  float delta = req.delta > 0.0f ? req.delta : hdelta;
+  if (!(delta > 0.0f) || !std::isfinite(delta)) fail(MI355_EINVAL, "delta must be a finite positive float");
+  uint32_t cap = 1024;
+  while ((int64_t)cap < n + 1024) cap <<= 1;
+  uint32_t hub_cap = 1024;
+  while ((int64_t)hub_cap < g.info.large_chunks * 2 + 1024) hub_cap <<= 1;
+  ws.labels.ensure((size_t)n);
+  ws.adds_ring.ensure((size_t)kAddsSlots * cap);
+  ws.adds_qstamp.ensure((size_t)n);
+  ws.adds_hub.ensure(hub_cap);
+  ws.adds_ctrl_mem.ensure(sizeof(AddsCtrl));
+  AddsCtrl* c = reinterpret_cast<AddsCtrl*>(ws.adds_ctrl_mem.p);
+  const DeviceProps p = device_props();
+  const int workers = req.worker_blocks > 0 ? req.worker_blocks : p.cus * 2;
+  AddsCtrl h{};
+  h.base_word = 0;  // bucket 0 starts at distance 0.0f
+  h.delta = delta;
+  h.base_delta = delta;
+  h.min_delta = delta / 64.0f;
+  h.max_delta = delta * 64.0f;
+  h.util_low = 0.35f;
+  h.util_high = 0.85f;
+  h.growth = 1.5f;
+  h.shrink = 0.75f;
+  h.active = req.active_buckets > 0 ? (uint32_t)req.active_buckets : 1u;
+  h.workers = (uint32_t)workers;
+  AddsArgs a{};
+  a.rowptr = g.rowptr.p;
+  a.edges = g.edges.p;
+  a.labels = ws.labels.p;
+  a.qstamp = ws.adds_qstamp.p;
+  a.ring = ws.adds_ring.p;
+  a.cap_mask = cap - 1;
+  a.hub = ws.adds_hub.p;
+  a.hub_mask = hub_cap - 1;
+  a.c = c;
+  a.policy = d_policy;
+  a.n = (uint32_t)n;
+  if (!ws.adds_ring_clean) {
+    // every entry starts as the empty sentinel; a completed run leaves the
+    // ring in that state again (all pushed entries are consumed)
+    MI_HIP(hipMemsetAsync(ws.adds_ring.p, 0xFF, ws.adds_ring.n * sizeof(int), s));
+    ws.adds_ring_clean = true;
+  }
+
+  hipEvent_t ev0, ev1;
+  MI_HIP(hipEventCreate(&ev0));
+  MI_HIP(hipEventCreate(&ev1));
+  MI_HIP(hipEventRecord(ev0, s));
+  k_adds_init<W><<<std::min<unsigned>(div_up((uint64_t)std::max<int64_t>(n, hub_cap), kBlock), 4096u),
+                   kBlock, 0, s>>>(a, n, src, req.init_dist, h);
+  MI_LAUNCH_CHECK();
+  k_adds<W><<<workers, kBlock, 0, s>>>(a);
+  MI_LAUNCH_CHECK();
+  MI_HIP(hipEventRecord(ev1, s));
+  MI_HIP(hipEventSynchronize(ev1));
+  MI_HIP(hipEventElapsedTime(&res.device_ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  MI_HIP(hipMemcpyAsync(&h, c, sizeof(AddsCtrl), hipMemcpyDeviceToHost, s));
+  MI_HIP(hipStreamSynchronize(s));
+  if (h.err) ws.adds_ring_clean = false;
+  if (h.err & kErrRing) fail(MI355_ECAPACITY, "ADDS bucket queue overflow");
+  if (h.err & kErrHub) fail(MI355_ECAPACITY, "ADDS hub chunk queue overflow");
+  if (h.err & kErrStall) fail(MI355_EDEVICE, "ADDS workers stalled (aborted by the spin bound)");
+  res.iterations_used = h.phases;
+  res.converged = h.done != 0 && h.pending == 0;
+  res.rounds = h.phases;
+  res.syncs = 1;
+  res.edges = (int64_t)h.edges;
+  res.vertices = (int64_t)h.items;
+  res.initial_delta = delta;
+  res.final_delta = h.delta;
+}
+
+}  // namespace
+
+void run_adds(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
+              hipStream_t stream, const AddsPolicy* d_policy) {
+  if (g.info.has_negative) fail(MI355_EINVAL, "ADDS requires nonnegative edge weights");
+  if (g.weight_type == MI355_W_F32) run_adds_typed<WF32>(g, source, req, res, stream, d_policy);
+  else run_adds_typed<WU32>(g, source, req, res, stream, d_policy);
+}
+
+}  // namespace mi355

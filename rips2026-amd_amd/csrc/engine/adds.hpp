@@ -1,0 +1,102 @@
+// Persistent ADDS worklist (adds.hip) and its phase policy, including the
+// learned delta/priority MLP evaluated on the device (mlp.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace mi355 {
+
+constexpr int kAddsSlots = 32;     // circular bucket window
+constexpr int kMlpIn = 32;         // phase features
+constexpr int kMlpHidden = 64;
+constexpr int kMlpOut = 2;         // log2 delta scale, active buckets
+constexpr int kTraceStride = 40;   // floats per traced phase
+
+enum PolicyMode { kPolicyFixed = 0, kPolicyHeuristic = 1, kPolicyMlp = 2 };
+
+// Device control block of one ADDS run.
+struct AddsCtrl {
+  unsigned long long base_word;  // (T_base float bits << 32) | base bucket
+  float delta;                   // current bucket width
+  float base_delta;              // heuristic delta the policy scales
+  float min_delta, max_delta;
+  float util_low, util_high, growth, shrink;
+  uint32_t active;               // buckets served concurrently (priority window)
+  uint32_t workers;
+  uint32_t head[kAddsSlots];     // pop counters (monotone)
+  uint32_t tail[kAddsSlots];     // push counters (monotone)
+  int inflight[kAddsSlots];      // claims of a slot still being processed
+  unsigned long long pending;    // items queued or being processed (two's-complement adds)
+  uint32_t hub_head, hub_reserve;
+  uint32_t done;
+  uint32_t err;
+  uint32_t phases;
+  uint32_t pad;
+  unsigned long long edges, pushes, items;
+};
+
+// One 2048-edge piece of a high-degree vertex.
+struct HubChunk {
+  uint32_t u, key, first, cnt;
+  uint32_t slot, bucket, seq, pad;
+};
+
+// Policy parameters in device memory (read-only during a run except trace).
+struct AddsPolicy {
+  int mode;
+  uint32_t trace_cap;            // phases the trace can hold
+  float* trace;                  // trace_cap x kTraceStride floats (may be null)
+  float graph_feat[14];          // degree + weight quantile features
+  float feat_mean[kMlpIn], feat_inv_std[kMlpIn];
+  // bf16 weights (stored as raw 16-bit words), fp32 biases
+  uint16_t w1[kMlpHidden * kMlpIn];
+  float b1[kMlpHidden];
+  uint16_t w2[kMlpOut * kMlpHidden];
+  float b2[kMlpOut];
+};
+
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Feature i of the phase, held by lane i (< 32).  occ = items queued in slot
+// (base + lane) for lanes < kAddsSlots.
+__device__ inline void mlp_features(const AddsPolicy* p, const AddsCtrl* c, uint32_t occ,
+                                    uint32_t window_items, float delta, unsigned lane, float* feat) {
+  float x = 0.0f;
+  if (lane < 14) x = p->graph_feat[lane];
+  else if (lane == 14) x = log2f(1.0f + (float)window_items);
+  else if (lane == 15) x = log2f(delta / c->base_delta);
+  const float occ_log = log2f(1.0f + (float)occ);
+  // lanes 16..31 take the occupancy of window offsets 0..15
+  const float shifted = __shfl(occ_log, (int)(lane - 16) & 63, 64);
+  if (lane >= 16 && lane < 32) x = shifted;
+  if (lane < kMlpIn) x = (x - p->feat_mean[lane]) * p->feat_inv_std[lane];
+  *feat = lane < kMlpIn ? x : 0.0f;
+}
+
+// 32 -> 64 (ReLU) -> 2 on one wave: lane j owns hidden unit j.
+__device__ inline void mlp_forward_wave(const AddsPolicy* p, float feat, unsigned lane, float* out0,
+                                        float* out1) {
+  float h = p->b1[lane];
+#pragma unroll
+  for (int i = 0; i < kMlpIn; ++i) h += bf16_to_f32(p->w1[lane * kMlpIn + i]) * __shfl(feat, i, 64);
+  h = fmaxf(h, 0.0f);
+  float o0 = bf16_to_f32(p->w2[lane]) * h;
+  float o1 = bf16_to_f32(p->w2[kMlpHidden + lane]) * h;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    o0 += __shfl_xor(o0, off, 64);
+    o1 += __shfl_xor(o1, off, 64);
+  }
+  *out0 = o0 + p->b2[0];
+  *out1 = o1 + p->b2[1];
+}
+
+}  // namespace mi355
+
+struct mi355_graph;
+namespace mi355 {
+struct RunRequest;
+struct RunResult;
+void run_adds(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
+              hipStream_t stream, const AddsPolicy* d_policy);
+}  // namespace mi355

@@ -243,6 +243,9 @@ void run_sssp(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult&
     case MI355_ALG_DELTA:
       run_delta_stepping(g, source, req, res, stream);
       break;
+    case MI355_ALG_ADDS:
+      run_adds(g, source, req, res, stream, prepare_adds_policy(g, req, stream));
+      break;
     default:
       fail(MI355_EINVAL, "unknown SSSP algorithm");
   }

@@ -588,10 +588,11 @@ int mi355_one_sssp_f32(int32_t algorithm, int n, int m, const int* row_offsets,
       if (opt.num_buckets <= 1 || opt.num_buckets > 64)
         fail(MI355_EINVAL, "num_buckets must be in [2, 64]");
       GraphPtr g(mi355::build_graph(d, 0, s));
-      req.algorithm = MI355_ALG_DELTA;
+      req.algorithm = algorithm == MI355_ALG_DELTA ? MI355_ALG_DELTA : MI355_ALG_ADDS;
       req.delta = (opt.initial_delta > 0.0f && std::isfinite(opt.initial_delta))
                       ? opt.initial_delta : mi355::heuristic_delta(*g);
       req.max_iters = opt.max_iters > 0 ? opt.max_iters : -1;
+      req.dynamic_delta = opt.dynamic_delta;
       req.want_work = true;
       mi355::run_sssp(*g, (uint32_t)source, req, res, s);
       download_dist(*g, (uint32_t)source, MI355_INF_IEEE, dist, s);
@@ -634,7 +635,10 @@ int mi355_hip_adds_sssp(int32_t weight_type, uint32_t num_vertices, uint32_t num
     d.memory = MI355_DEVICE;
     GraphPtr g(mi355::build_graph(d, 0, s));
     mi355::RunRequest req;
-    req.algorithm = MI355_ALG_DELTA;
+    req.algorithm = MI355_ALG_ADDS;
+    req.dynamic_delta = options ? options->dynamic_delta : 1;
+    req.worker_blocks = options ? (int)options->worker_blocks : 0;
+    req.active_buckets = options ? (int)std::min<uint32_t>(std::max<uint32_t>(options->initial_active_buckets, 1u), 32u) : 1;
     double delta = 0.0;
     const double scale = options ? options->initial_delta_scale : 16.0;
     if (options && options->initial_delta > 0.0) delta = options->initial_delta;
@@ -656,7 +660,9 @@ int mi355_hip_adds_sssp(int32_t weight_type, uint32_t num_vertices, uint32_t num
       stats->initial_delta = delta;
       stats->final_delta = res.final_delta;
       stats->edge_relaxations = (uint64_t)res.edges;
+      stats->processed_items = (uint64_t)res.vertices;
       stats->head_switches = (uint64_t)res.iterations_used;
+      stats->worker_blocks = (uint32_t)(req.worker_blocks > 0 ? req.worker_blocks : 0);
     }
   });
   if (rc == MI355_OK) return kSuccess;

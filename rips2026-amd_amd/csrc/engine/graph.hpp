@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "adds.hpp"
 #include "common.hpp"
 
 namespace mi355 {
@@ -38,6 +39,13 @@ struct Workspace {
   DevBuf<int> settled;                // vertices processed in the current bucket
   DevBuf<uint32_t> sstamp;            // settled-list dedup
   DevBuf<unsigned char> bctrl_mem;    // BucketCtrl
+  // ADDS state (allocated on first use)
+  DevBuf<int> adds_ring;              // kAddsSlots bucket rings
+  bool adds_ring_clean = false;       // ring holds only empty sentinels
+  DevBuf<uint32_t> adds_qstamp;
+  DevBuf<HubChunk> adds_hub;
+  DevBuf<unsigned char> adds_ctrl_mem;
+  DevBuf<AddsPolicy> adds_policy;
   int64_t slot_cap = 0;
   int nslots = 0;
 };
@@ -57,6 +65,9 @@ struct mi355_graph {
   // optional per-round log request
   int32_t* log_dst = nullptr;
   int32_t log_cap = 0;
+  // graph features of the learned delta policy (computed on first use)
+  bool graph_feat_ready = false;
+  float graph_feat[14] = {};
 };
 
 namespace mi355 {

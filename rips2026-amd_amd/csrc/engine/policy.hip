@@ -1,0 +1,102 @@
+// ADDS phase policy set-up: graph features for the learned delta model, the
+// process-wide MLP weights, and the device policy object of a run.
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+#include "sssp.hpp"
+
+namespace mi355 {
+
+namespace {
+
+std::mutex g_mlp_mu;
+bool g_mlp_loaded = false;
+AddsPolicy g_mlp{};  // weights + feature normalisation (mode/trace unused)
+
+constexpr int kSamples = 4096;
+
+// Deterministic strided sample of out-degrees and edge weights.
+__global__ void k_sample(const int* __restrict__ rowptr, const uint2* __restrict__ edges, int64_t n,
+                         int64_t m, int32_t wtype, float* __restrict__ deg, float* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kSamples) return;
+  const int64_t v = (int64_t)((double)i * (double)n / kSamples);
+  deg[i] = (float)(rowptr[v + 1] - rowptr[v]);
+  if (m > 0) {
+    const int64_t e = (int64_t)((double)i * (double)m / kSamples);
+    const uint32_t b = edges[e].y;
+    w[i] = wtype == MI355_W_F32 ? __uint_as_float(b) : (float)b;
+  } else {
+    w[i] = 1.0f;
+  }
+}
+
+float quantile(std::vector<float>& v, double q) {
+  const size_t k = std::min(v.size() - 1, (size_t)(q * (double)(v.size() - 1) + 0.5));
+  std::nth_element(v.begin(), v.begin() + (long)k, v.end());
+  return v[k];
+}
+
+}  // namespace
+
+// 14 graph features: log2(1 + degree quantile) at 5/25/50/75/95/99 % and
+// log2(mean degree); weight quantiles at the same points over the mean weight,
+// and log2(n) / 8.
+void graph_features(mi355_graph& g, hipStream_t s, float* out14) {
+  if (!g.graph_feat_ready) {
+    DevBuf<float> d(kSamples), w(kSamples);
+    k_sample<<<kSamples / 256, 256, 0, s>>>(g.rowptr.p, g.edges.p, g.n, g.m, g.weight_type, d.p, w.p);
+    MI_LAUNCH_CHECK();
+    std::vector<float> hd(kSamples), hw(kSamples);
+    MI_HIP(hipMemcpyAsync(hd.data(), d.p, kSamples * 4, hipMemcpyDeviceToHost, s));
+    MI_HIP(hipMemcpyAsync(hw.data(), w.p, kSamples * 4, hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    const double qs[6] = {0.05, 0.25, 0.5, 0.75, 0.95, 0.99};
+    const double wbar = g.info.avg_weight > 0 ? g.info.avg_weight : 1.0;
+    for (int k = 0; k < 6; ++k) {
+      g.graph_feat[k] = std::log2(1.0f + quantile(hd, qs[k]));
+      g.graph_feat[7 + k] = (float)(quantile(hw, qs[k]) / wbar);
+    }
+    g.graph_feat[6] = (float)std::log2(std::max(1e-9, g.info.avg_degree));
+    g.graph_feat[13] = (float)(std::log2((double)std::max<int64_t>(g.n, 1)) / 8.0);
+    g.graph_feat_ready = true;
+  }
+  std::copy(g.graph_feat, g.graph_feat + 14, out14);
+}
+
+void set_mlp(const AddsPolicy& p) {
+  std::lock_guard<std::mutex> lk(g_mlp_mu);
+  g_mlp = p;
+  g_mlp_loaded = true;
+}
+
+bool get_mlp(AddsPolicy* p) {
+  std::lock_guard<std::mutex> lk(g_mlp_mu);
+  if (g_mlp_loaded && p) *p = g_mlp;
+  return g_mlp_loaded;
+}
+
+const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hipStream_t s) {
+  AddsPolicy h{};
+  if (req.use_mlp) {
+    if (!get_mlp(&h)) fail(MI355_EINVAL, "no delta model loaded (mi355_adds_mlp_set)");
+    h.mode = kPolicyMlp;
+  } else {
+    h.mode = req.dynamic_delta ? kPolicyHeuristic : kPolicyFixed;
+    for (int i = 0; i < kMlpIn; ++i) {
+      h.feat_mean[i] = 0.0f;
+      h.feat_inv_std[i] = 1.0f;
+    }
+  }
+  h.trace = req.trace;
+  h.trace_cap = req.trace ? req.trace_cap : 0u;
+  if (h.mode == kPolicyFixed && !h.trace) return nullptr;
+  graph_features(g, s, h.graph_feat);
+  g.ws.adds_policy.ensure(1);
+  MI_HIP(hipMemcpyAsync(g.ws.adds_policy.p, &h, sizeof(AddsPolicy), hipMemcpyHostToDevice, s));
+  return g.ws.adds_policy.p;
+}
+
+}  // namespace mi355

@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "adds.hpp"
 #include "graph.hpp"
 
 namespace mi355 {
@@ -20,6 +21,9 @@ struct RunRequest {
   // Optional warm start: device array of n distances (output convention);
   // labels start from them (source forced to 0), only the source is active.
   const uint32_t* init_dist = nullptr;
+  int active_buckets = 0;  // ADDS priority window (0: 1)
+  float* trace = nullptr;  // ADDS phase trace (device, trace_cap x kTraceStride)
+  uint32_t trace_cap = 0;
 };
 
 struct RunResult {
@@ -52,6 +56,10 @@ void labels_to_output(mi355_graph& g, uint32_t source, int inf_mode, void* d_dis
 // Delta-stepping driver (delta.hip).
 void run_delta_stepping(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
                         hipStream_t stream);
+
+// Device policy object of an ADDS run (adds.hip): fixed, feedback heuristic
+// or learned MLP, plus the optional phase trace.
+const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hipStream_t stream);
 
 // Heuristic bucket width: 16 * mean positive weight / mean degree
 // (ONE/sssp_hip.cpp:59-81, JAS/hip_adds_sssp.cpp:1121).

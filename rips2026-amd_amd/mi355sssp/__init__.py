@@ -33,3 +33,7 @@ __all__ = [
     "read_galois_gr", "rng_u64", "fill_unit_f32", "DeviceGraph", "DeviceBuffer", "bf_csr",
     "delta_stepping_csr",
 ]
+
+from .kernels import (minplus_gemm, minplus_gemm_batched, minplus_autotune, minplus_tuning,  # noqa: E402
+                      spgemm, bf_dense, gauss_seidel, one_sssp, hip_adds)
+from ._abi import OneOptions, OneResult, AddsOptions, AddsStats  # noqa: E402

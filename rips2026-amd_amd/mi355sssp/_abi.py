@@ -183,3 +183,70 @@ def check(status: int) -> None:
 def gcheck(status: int) -> None:
     if status != 0:
         raise Mi355Error(status, graphlib().mi355g_last_error().decode())
+
+
+class OneOptions(C.Structure):
+    _fields_ = [("max_iters", C.c_int), ("num_buckets", C.c_int),
+                ("bucket_capacity_per_bucket", C.c_int), ("initial_delta", C.c_float),
+                ("dynamic_delta", C.c_int), ("target_low_work", C.c_int),
+                ("target_high_work", C.c_int)]
+
+
+class OneResult(C.Structure):
+    _fields_ = [("iterations_used", C.c_int), ("converged", C.c_int),
+                ("has_negative_cycle", C.c_int), ("work_count", C.c_ulonglong),
+                ("milliseconds", C.c_float), ("final_delta", C.c_float)]
+
+
+class AddsOptions(C.Structure):
+    _fields_ = [("worker_blocks", C.c_uint32), ("assignment_chunk", C.c_uint32),
+                ("initial_active_buckets", C.c_uint32), ("max_active_buckets", C.c_uint32),
+                ("initial_delta", C.c_double), ("initial_delta_scale", C.c_double),
+                ("min_delta", C.c_double), ("delta_growth", C.c_double),
+                ("delta_shrink", C.c_double), ("clip_threshold", C.c_double),
+                ("utilization_low", C.c_double), ("utilization_high", C.c_double),
+                ("adjust_period_head_switches", C.c_uint32), ("dynamic_delta", C.c_int32),
+                ("dynamic_active_buckets", C.c_int32), ("initialize_distances", C.c_int32),
+                ("pool_words", C.c_uint64), ("memory_fraction", C.c_double)]
+
+    @classmethod
+    def defaults(cls):
+        o = cls()
+        o.assignment_chunk, o.initial_active_buckets, o.max_active_buckets = 256, 1, 32
+        o.initial_delta_scale, o.min_delta = 16.0, 1e-3
+        o.delta_growth, o.delta_shrink, o.clip_threshold = 1.5, 0.75, 0.65
+        o.utilization_low, o.utilization_high = 0.35, 0.85
+        o.adjust_period_head_switches = 8
+        o.dynamic_delta = o.dynamic_active_buckets = o.initialize_distances = 1
+        o.memory_fraction = 0.45
+        return o
+
+
+class AddsStats(C.Structure):
+    _fields_ = [("initial_delta", C.c_double), ("final_delta", C.c_double),
+                ("final_bucket_base", C.c_double), ("worker_blocks", C.c_uint32),
+                ("final_active_buckets", C.c_uint32), ("pool_pages", C.c_uint32),
+                ("device_error", C.c_uint32)] + [
+        (n, C.c_uint64) for n in ("assigned_items", "processed_items", "enqueued_items",
+                                  "edge_relaxations", "successful_relaxations", "head_switches",
+                                  "delta_updates", "max_inflight_items")]
+
+
+_ENGINE_SIGS.update({
+    "mi355_minplus_gemm_f32": (C.c_int, [P, P, P] + [C.c_int] * 6 + [P]),
+    "mi355_minplus_gemm_f32_batched": (C.c_int, [C.c_int] + [P] * 9 + [P]),
+    "mi355_minplus_tile_count": (C.c_int, []),
+    "mi355_minplus_tile_name": (C.c_char_p, [C.c_int]),
+    "mi355_minplus_autotune": (C.c_int, [P, C.c_int, P, P]),
+    "mi355_minplus_tuning_get": (C.c_int, [P]),
+    "mi355_minplus_tuning_set": (C.c_int, [P]),
+    "mi355_minplus_spgemm_csr_f32": (C.c_int, [I64, I64, I64, P, P, P, I64, I64, I64, P, P, P,
+                                               C.c_int, C.c_int, P, P, P, P, P]),
+    "mi355_sparse_vector_min_merge": (C.c_int, [I64, P, P, P, P, P, P, P, P, P, P]),
+    "mi355_bf_dense_f32": (C.c_int, [P, I32, C.c_int, C.c_int, C.c_int, P, P, P, P]),
+    "mi355_bf_gauss_seidel_f32": (C.c_int, [P, I32, C.c_int, C.c_int, C.c_int, P, P, P]),
+    "mi355_one_sssp_f32": (C.c_int, [I32, C.c_int, C.c_int, P, P, P, C.c_int,
+                                     C.POINTER(OneOptions), P, C.POINTER(OneResult), P]),
+    "mi355_hip_adds_sssp": (C.c_int, [I32, U32, U32, P, P, P, U32, P, C.POINTER(AddsStats),
+                                      C.POINTER(AddsOptions), P]),
+})

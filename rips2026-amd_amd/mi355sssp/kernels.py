@@ -1,0 +1,148 @@
+"""Min-plus products, dense Bellman-Ford and the reference-family SSSP entry
+points (ctypes over the C ABI)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._abi import (AddsOptions, AddsStats, DEVICE, HOST, OneOptions, OneResult, W_F32, W_U32,
+                   check, engine)
+from .device import DeviceBuffer
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def minplus_gemm(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """C = A (min,+) B for 2-D float32 arrays, on the device."""
+    A, B = _f32(A), _f32(B)
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2
+    dA, dB, dC = DeviceBuffer.from_array(A), DeviceBuffer.from_array(B), DeviceBuffer(M * N * 4)
+    check(engine().mi355_minplus_gemm_f32(dA.ptr, dB.ptr, dC.ptr, M, N, K, K, N, N, None))
+    check(engine().mi355_stream_sync(None))
+    return dC.to_array(np.float32, M * N).reshape(M, N)
+
+
+def minplus_gemm_batched(As, Bs):
+    bufs = [(DeviceBuffer.from_array(_f32(a)), DeviceBuffer.from_array(_f32(b)),
+             DeviceBuffer(a.shape[0] * b.shape[1] * 4)) for a, b in zip(As, Bs)]
+    cnt = len(bufs)
+    pa = (C.c_void_p * cnt)(*[x[0].ptr.value for x in bufs])
+    pb = (C.c_void_p * cnt)(*[x[1].ptr.value for x in bufs])
+    pc = (C.c_void_p * cnt)(*[x[2].ptr.value for x in bufs])
+    M = (C.c_int * cnt)(*[a.shape[0] for a in As])
+    N = (C.c_int * cnt)(*[b.shape[1] for b in Bs])
+    K = (C.c_int * cnt)(*[a.shape[1] for a in As])
+    check(engine().mi355_minplus_gemm_f32_batched(cnt, pa, pb, pc, M, N, K, K, N, N, None))
+    check(engine().mi355_stream_sync(None))
+    return [x[2].to_array(np.float32, As[i].shape[0] * Bs[i].shape[1]).reshape(As[i].shape[0], Bs[i].shape[1])
+            for i, x in enumerate(bufs)]
+
+
+def minplus_autotune(sizes):
+    cnt = len(sizes)
+    nt = engine().mi355_minplus_tile_count()
+    sz = (C.c_int * cnt)(*sizes)
+    times = np.zeros(cnt * nt, np.float64)
+    check(engine().mi355_minplus_autotune(sz, cnt, times.ctypes.data, None))
+    names = [engine().mi355_minplus_tile_name(i).decode() for i in range(nt)]
+    return names, times.reshape(cnt, nt)
+
+
+def minplus_tuning(table=None):
+    t = (C.c_int * 32)()
+    if table is not None:
+        for i, x in enumerate(table):
+            t[i] = int(x)
+        check(engine().mi355_minplus_tuning_set(t))
+    check(engine().mi355_minplus_tuning_get(t))
+    return list(t)
+
+
+def spgemm(a_rowptr, a_col, a_val, a_cols, b_rowptr, b_col, b_val, b_cols, hash_capacity=512,
+           block_size=64):
+    """Sparse min-plus product; returns (rowptr, col, val) with rows as the GPU wrote them."""
+    ins = [DeviceBuffer.from_array(np.ascontiguousarray(x, dt)) for x, dt in
+           ((a_rowptr, np.int64), (a_col, np.int32), (a_val, np.float32),
+            (b_rowptr, np.int64), (b_col, np.int32), (b_val, np.float32))]
+    a_rows = len(a_rowptr) - 1
+    b_rows = len(b_rowptr) - 1
+    crp, ccol, cval = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    nnz = C.c_int64()
+    check(engine().mi355_minplus_spgemm_csr_f32(
+        a_rows, a_cols, len(a_col), ins[0].ptr, ins[1].ptr, ins[2].ptr, b_rows, b_cols, len(b_col),
+        ins[3].ptr, ins[4].ptr, ins[5].ptr, hash_capacity, block_size, C.byref(crp),
+        C.byref(ccol), C.byref(cval), C.byref(nnz), None))
+    rp = np.empty(a_rows + 1, np.int64)
+    col = np.empty(nnz.value, np.int32)
+    val = np.empty(nnz.value, np.float32)
+    check(engine().mi355_memcpy_d2h(rp.ctypes.data, crp, rp.nbytes, None))
+    if nnz.value:
+        check(engine().mi355_memcpy_d2h(col.ctypes.data, ccol, col.nbytes, None))
+        check(engine().mi355_memcpy_d2h(val.ctypes.data, cval, val.nbytes, None))
+    for p in (crp, ccol, cval):
+        if p.value:
+            engine().mi355_device_free(p)
+    return rp, col, val, (ccol.value is None, cval.value is None)
+
+
+def bf_dense(A, n, source, max_iters=-1, on_device=False):
+    A = _f32(A)
+    dist = np.empty(n, np.float32)
+    it, conv = C.c_int(), C.c_int()
+    if on_device:
+        dA = DeviceBuffer.from_array(A)
+        check(engine().mi355_bf_dense_f32(dA.ptr, DEVICE, n, source, max_iters, dist.ctypes.data,
+                                          C.byref(it), C.byref(conv), None))
+    else:
+        check(engine().mi355_bf_dense_f32(A.ctypes.data, HOST, n, source, max_iters,
+                                          dist.ctypes.data, C.byref(it), C.byref(conv), None))
+    return dist, it.value, bool(conv.value)
+
+
+def gauss_seidel(A, n, source, max_iters=-1):
+    A = _f32(A)
+    dist = np.empty(n, np.float32)
+    it = C.c_int()
+    check(engine().mi355_bf_gauss_seidel_f32(A.ctypes.data, HOST, n, source, max_iters,
+                                             dist.ctypes.data, C.byref(it), None))
+    return dist, it.value
+
+
+def one_sssp(algorithm, rowptr, col, w, source, **opts):
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    cl = np.ascontiguousarray(col, np.int32)
+    ww = np.ascontiguousarray(w, np.float32)
+    n = len(rp) - 1
+    o = OneOptions(-1, 32, 0, 0.0, 1, 8192, 262144)
+    for k, v in opts.items():
+        setattr(o, k, v)
+    dist = np.empty(n, np.float32)
+    r = OneResult()
+    check(engine().mi355_one_sssp_f32(algorithm, n, len(cl), rp.ctypes.data,
+                                      cl.ctypes.data if len(cl) else None,
+                                      ww.ctypes.data if len(cl) else None, source, C.byref(o),
+                                      dist.ctypes.data, C.byref(r), None))
+    return dist, r
+
+
+def hip_adds(rowptr, col, w, source, weight_type=W_F32, options=None, init=None):
+    """hip_adds::sssp over device uint32 CSR; returns (status, dist, stats)."""
+    rp = DeviceBuffer.from_array(np.ascontiguousarray(rowptr, np.uint32))
+    cl = DeviceBuffer.from_array(np.ascontiguousarray(col, np.uint32))
+    wt = np.float32 if weight_type == W_F32 else np.uint32
+    ww = DeviceBuffer.from_array(np.ascontiguousarray(w, wt))
+    n = len(rowptr) - 1
+    d = DeviceBuffer(4 * n)
+    if init is not None:
+        check(engine().mi355_memcpy_h2d(d.ptr, np.ascontiguousarray(init, wt).ctypes.data, 4 * n, None))
+    o = options or AddsOptions.defaults()
+    st = AddsStats()
+    rc = engine().mi355_hip_adds_sssp(weight_type, n, len(col), rp.ptr, cl.ptr, ww.ptr, source,
+                                      d.ptr, C.byref(st), C.byref(o), None)
+    dist = d.to_array(wt, n) if rc == 0 else None
+    return rc, dist, st

@@ -1,0 +1,188 @@
+"""GPU parity of the min-plus products, dense Bellman-Ford / Gauss-Seidel and
+the reference-family SSSP entry points (ONE, hip_adds) against the oracle."""
+import numpy as np
+import pytest
+
+import mi355sssp as M
+from golden import bf_cases
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (7, 5, 3), (64, 64, 64), (33, 70, 129),
+                                   (200, 130, 257), (512, 512, 512)])
+def test_minplus_gemm_bit_exact(shape):
+    M_, N, K = shape
+    print(f"gemm {shape}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    A = ((rng.integers(0, 1 << 24, (M_, K)) + 1) / 2 ** 24).astype(np.float32)
+    B = ((rng.integers(0, 1 << 24, (K, N)) + 1) / 2 ** 24).astype(np.float32)
+    A[rng.random((M_, K)) < 0.05] = np.inf
+    C = M.minplus_gemm(A, B)
+    want = orc.minplus_gemm(A.ravel(), B.ravel(), M_, N, K).reshape(M_, N)
+    assert C.tobytes() == want.tobytes()
+
+
+def test_minplus_gemm_rejects_bad_arguments():
+    A = M.DeviceBuffer(64)
+    lib = M.engine()
+    assert lib.mi355_minplus_gemm_f32(A.ptr, A.ptr, A.ptr, 2, 2, 2, 1, 2, 2, None) == -1  # lda < K
+    assert lib.mi355_minplus_gemm_f32(None, A.ptr, A.ptr, 2, 2, 2, 2, 2, 2, None) == -1
+    assert lib.mi355_minplus_gemm_f32(A.ptr, A.ptr, A.ptr, 0, 2, 2, 2, 2, 2, None) == -1
+
+
+def test_minplus_gemm_batched_and_autotune():
+    rng = np.random.default_rng(3)
+    sizes = [32, 45, 100, 257, 64, 512, 33]
+    As = [rng.random((n, n), dtype=np.float32) for n in sizes]
+    Bs = [rng.random((n, n), dtype=np.float32) for n in sizes]
+    outs = M.minplus_gemm_batched(As, Bs)
+    for a, b, c in zip(As, Bs, outs):
+        n = a.shape[0]
+        assert c.tobytes() == orc.minplus_gemm(a.ravel(), b.ravel(), n, n, n).reshape(n, n).tobytes()
+    names, times = M.minplus_autotune([64, 256])
+    assert times.shape == (2, len(names)) and np.all(times > 0)
+    table = M.minplus_tuning()
+    assert table[6] >= 0 and table[8] >= 0  # classes of 64 and 256 were tuned
+    outs2 = M.minplus_gemm_batched(As, Bs)  # tuned choices give the same bits
+    assert all(x.tobytes() == y.tobytes() for x, y in zip(outs, outs2))
+
+
+def _rand_csr(rng, rows, cols, p):
+    dense = np.where(rng.random((rows, cols)) < p, rng.random((rows, cols)).astype(np.float32), np.inf)
+    rp, col, val = [0], [], []
+    for i in range(rows):
+        for j in range(cols):
+            if np.isfinite(dense[i, j]):
+                col.append(j)
+                val.append(dense[i, j])
+        rp.append(len(col))
+    return np.array(rp), np.array(col, np.int32), np.array(val, np.float32)
+
+
+@pytest.mark.parametrize("hc,bs", [(512, 64), (256, 64), (1024, 128), (4096, 256)])
+def test_spgemm_matches_oracle(hc, bs):
+    rng = np.random.default_rng(hc + bs)
+    arp, acol, aval = _rand_csr(rng, 120, 90, 0.08)
+    brp, bcol, bval = _rand_csr(rng, 90, 70, 0.1)
+    rp, col, val, _ = M.spgemm(arp, acol, aval, 90, brp, bcol, bval, 70, hc, bs)
+    wrp, wcol, wval = orc.minplus_spgemm(arp, acol, aval, 90, brp, bcol, bval, 70)
+    assert np.array_equal(rp, wrp)
+    for i in range(120):
+        o = np.argsort(col[rp[i]:rp[i + 1]])
+        assert np.array_equal(col[rp[i]:rp[i + 1]][o], wcol[wrp[i]:wrp[i + 1]])
+        assert val[rp[i]:rp[i + 1]][o].tobytes() == wval[wrp[i]:wrp[i + 1]].tobytes()
+
+
+def test_spgemm_empty_overflow_and_duplicates():
+    # empty product: zero rowptr, null colind / values (hpp:51-57)
+    z = np.zeros(4, np.int64)
+    rp, col, val, nulls = M.spgemm(z, np.zeros(0, np.int32), np.zeros(0, np.float32), 3,
+                                   z, np.zeros(0, np.int32), np.zeros(0, np.float32), 3)
+    assert rp.tolist() == [0, 0, 0, 0] and nulls == (True, True)
+    # one row with 300 distinct columns overflows a 256-slot table
+    arp = np.array([0, 1]); acol = np.array([0], np.int32); aval = np.array([1.0], np.float32)
+    brp = np.array([0, 300]); bcol = np.arange(300, dtype=np.int32); bval = np.ones(300, np.float32)
+    with pytest.raises(M.Mi355Error) as e:
+        M.spgemm(arp, acol, aval, 1, brp, bcol, bval, 300, 256, 64)
+    assert e.value.code == -6
+    rp, col, val, _ = M.spgemm(arp, acol, aval, 1, brp, bcol, bval, 300, 512, 64)
+    assert rp.tolist() == [0, 300] and sorted(col.tolist()) == list(range(300))
+    # duplicate columns reduce by min; +INF / NaN entries are not stored
+    arp = np.array([0, 2]); acol = np.array([0, 1], np.int32); aval = np.array([1.0, 0.5], np.float32)
+    brp = np.array([0, 2, 4]); bcol = np.array([0, 1, 0, 1], np.int32)
+    bval = np.array([3.0, np.inf, 2.0, np.nan], np.float32)
+    rp, col, val, _ = M.spgemm(arp, acol, aval, 2, brp, bcol, bval, 2)
+    assert rp.tolist() == [0, 1] and col.tolist() == [0] and val.tolist() == [2.5]
+
+
+def _dense(n, edges):
+    A = np.full(n * n, np.inf, np.float32)
+    for a, b, w in edges:
+        A[b * n + a] = min(A[b * n + a], np.float32(w))
+    return A
+
+
+@pytest.mark.parametrize("case", bf_cases.all_cases(include_large=True), ids=lambda c: c[0])
+def test_dense_bellman_ford_known_answers(case):
+    name, n, src, edges, expected, _ = case
+    A = _dense(n, edges)
+    dist, it, conv = M.bf_dense(A, n, src, on_device=(n % 2 == 0))
+    wd, wit, wconv = orc.bf_dense(A, n, src)
+    assert dist.tobytes() == wd.tobytes()
+    assert (it, conv) == (wit, wconv)
+    if n > 1:
+        assert dist.tobytes() == np.array(expected, np.float32).tobytes()
+
+
+@pytest.mark.parametrize("case", bf_cases.all_cases(include_large=False), ids=lambda c: c[0])
+def test_gauss_seidel_known_answers(case):
+    name, n, src, edges, expected, _ = case
+    A = _dense(n, edges)
+    for iters in (-1, 1, 2):
+        dist, it = M.gauss_seidel(A, n, src, iters)
+        wd, wit = orc.gauss_seidel(A, n, src, iters)
+        assert dist.tobytes() == wd.tobytes() and it == wit
+
+
+def _csr_out(n, edges):
+    g = M.from_edges(n, [e[0] for e in edges], [e[1] for e in edges],
+                     np.array([e[2] for e in edges], np.float32))
+    return g
+
+
+@pytest.mark.parametrize("case", bf_cases.all_cases(include_large=False), ids=lambda c: c[0])
+def test_one_bellman_ford_known_answers(case):
+    name, n, src, edges, expected, neg = case
+    if not edges:
+        g = M.from_csr(np.zeros(n + 1, np.int64), [], np.zeros(0, np.float32))
+    else:
+        g = _csr_out(n, edges)
+    dist, r = M.one_sssp(M.ALG_BF, g.rowptr, g.col, g.weights, src)
+    assert dist.tobytes() == np.array(expected, np.float32).tobytes()
+    assert r.has_negative_cycle == 0
+
+
+def test_one_bellman_ford_detects_negative_cycle():
+    edges = [(0, 1, 1.0), (1, 2, -2.0), (2, 1, 1.0), (2, 3, 1.0)]
+    g = _csr_out(4, edges)
+    dist, r = M.one_sssp(M.ALG_BF, g.rowptr, g.col, g.weights, 0)
+    assert r.has_negative_cycle == 1 and r.converged == 0
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_one_adds_matches_oracle(seed):
+    g = M.chain_random(4096, 8, 1.0, 20.0, M.W_F32, seed=seed)  # ONE/test_sssp_hip.cpp:26-66 shape
+    for src in (0, 100):
+        dist, r = M.one_sssp(M.ALG_ADDS, g.rowptr, g.col, g.weights, src)
+        want = orc.dijkstra(g.rowptr, g.col, g.weights, src)
+        assert dist.tobytes() == want.tobytes()
+        assert r.converged == 1 and r.final_delta > 0
+
+
+@pytest.mark.parametrize("wt", [M.W_F32, M.W_U32])
+def test_hip_adds_matches_oracle(wt):
+    g = M.rmat(12, 8, seed=4) if wt == M.W_F32 else M.routing_grid(64, seed=4)
+    for src in g.pick_sources(2, seed=5):
+        rc, dist, st = M.hip_adds(g.rowptr, g.col, g.wbits.view(np.float32) if wt == M.W_F32 else g.wbits,
+                                  int(src), wt)
+        assert rc == 0
+        want = orc.dijkstra(g.rowptr, g.col, g.weights, int(src))
+        assert dist.tobytes() == want.tobytes()
+        assert st.edge_relaxations > 0
+    rc, _, _ = M.hip_adds(g.rowptr, g.col, g.wbits.view(np.float32) if wt == M.W_F32 else g.wbits,
+                          g.n, wt)
+    assert rc == 1  # Status::InvalidArgument
+
+
+def test_hip_adds_warm_start_keeps_existing_distances():
+    # initialize_distances = false: labels start from the caller's buffer and
+    # only the source is expanded (JAS/hip_adds_sssp.hpp:64-66)
+    g = M.from_edges(4, [0, 1, 2], [1, 2, 3], np.array([5.0, 5.0, 5.0], np.float32))
+    o = M.AddsOptions.defaults()
+    o.initialize_distances = 0
+    init = np.array([np.inf, 1.0, np.inf, np.inf], np.float32)
+    rc, dist, _ = M.hip_adds(g.rowptr, g.col, g.weights, 0, M.W_F32, o, init)
+    assert rc == 0
+    assert dist.tolist() == [0.0, 1.0, np.inf, np.inf]

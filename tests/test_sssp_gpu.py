@@ -128,7 +128,7 @@ def test_rrg_path_known_answer():
 
 
 def test_single_vertex_and_unreachable():
-    g = M.from_csr([0], [], np.zeros(0, np.float32))
+    g = M.from_csr([0, 0], [], np.zeros(0, np.float32))
     dg = M.DeviceGraph(g)
     for alg in ALGS:
         dist, pred, st = dg.sssp(0, alg, delta=1.0)
