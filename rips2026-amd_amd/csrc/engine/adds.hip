@@ -19,13 +19,18 @@
 //    CAS on the slot head; the four waves relax them, 64 items per wave with
 //    their edges packed across the lanes as in the frontier kernels; vertices
 //    above kLargeDegree are cut into 2048-edge hub chunks any workgroup can
-//    claim.  A wave's pushes are grouped per target slot, one atomicAdd each.
+//    claim.
+//  * Pushes are staged per wave in LDS and published in batches: one atomic
+//    on `pending`, then one reservation per distinct target slot (an LDS
+//    histogram gives each entry its offset), then the entries.  A reserved
+//    entry is the sentinel -1 until written, so a claimer that gets there
+//    first waits for the write.
 //  * Dedup.  qstamp[v] = smallest bucket v is queued in (atomicMin on push).
 //    A pop exchanges it with "none"; the returned stamp is the bucket the
 //    entry is processed in (if it falls in this slot), so every live entry is
-//    processed exactly once and superseded entries are dropped.  The label
-//    read after the acquire-exchange sees every improvement that did not
-//    re-queue v, so each final distance is relaxed from.
+//    processed once and superseded entries are dropped.  The label read after
+//    the acquire-exchange sees every improvement that did not re-queue v, so
+//    each final distance is relaxed from.
 //  * Phases.  The base advances (one 64-bit CAS of (base, T_base)) only when
 //    slot base is empty and no claim of it is still in flight, so no push can
 //    land behind the window.  The advancing wave evaluates the delta policy:
@@ -35,11 +40,12 @@
 //    features of the phase and returns log2 of the delta scale and the active
 //    bucket count.  Re-mapping changes priorities only; distances remain the
 //    exact fixpoint of the fp32 / uint32 left folds.
-//  * Termination.  `pending` counts items queued or being processed; pushes
-//    add before their producer finishes, finished (or dropped) items
-//    subtract afterwards, so pending == 0 means there is no work anywhere.
-//    Only running workgroups ever hold items, so spinning workgroups need not
-//    be co-resident with the rest for the run to make progress.
+//  * Termination.  `pending` counts items queued or being processed; a batch
+//    adds its count before its entries become visible and before its producer
+//    finishes, a finished (or dropped) claim subtracts afterwards, so
+//    pending == 0 means there is no work anywhere.  Only running workgroups
+//    ever hold items, so idle workgroups (which back off exponentially) need
+//    not be co-resident for the run to make progress.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -53,13 +59,20 @@ namespace mi355 {
 namespace {
 
 constexpr uint32_t kNoStamp = 0xFFFFFFFFu;
-constexpr uint32_t kClaim = 256;  // items per workgroup claim
-constexpr int kEmptyEntry = -1;   // ring sentinel: entry reserved, not yet written
+constexpr uint32_t kClaim = 256;    // items per workgroup claim
+constexpr int kEmptyEntry = -1;     // ring sentinel: reserved, not yet written
+constexpr uint32_t kPushStage = 512;  // staged pushes per wave
 // Bound of every spin (ring entry, hub chunk, idle worker): far above any
-// legitimate wait (about a second); reaching it aborts the run with an error
-// instead of hanging the device.
-constexpr uint32_t kSpinLimit = 1u << 24;
+// legitimate wait; reaching it aborts the run with an error instead of
+// hanging the device.
+constexpr uint32_t kSpinLimit = 1u << 22;
+// Idle polls (each backs off up to ~16K cycles) before a run counts as stalled.
+constexpr uint32_t kIdleLimit = 1u << 20;
 constexpr uint32_t kErrRing = 1u, kErrHub = 2u, kErrStall = 4u;
+
+#define HEAD(c, s) ((c)->head[(s) * kPad])
+#define TAIL(c, s) ((c)->tail[(s) * kPad])
+#define INFL(c, s) ((c)->inflight[(s) * kPad])
 
 __device__ inline int atomic_add_sc(int* p, int v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
@@ -110,6 +123,10 @@ struct AddsArgs {
   AddsCtrl* c;
   const AddsPolicy* policy;
   uint32_t n;
+  int live_edges;     // keep the live edge counter (phase traces read it)
+  // Upper bound on claims by one workgroup: far above any real run; reaching
+  // it means a livelock and aborts the run with an error.
+  unsigned long long claim_limit;
 };
 
 __device__ inline void ring_store(const AddsArgs& a, uint32_t slot, uint32_t pos, int v) {
@@ -117,35 +134,80 @@ __device__ inline void ring_store(const AddsArgs& a, uint32_t slot, uint32_t pos
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Queue the lanes' improved vertices: dedup by qstamp, then one atomicAdd per
-// distinct target slot present in the wave.
-__device__ inline void push_improved(const AddsArgs& a, bool imp, uint32_t v, uint32_t bkt,
-                                     unsigned long long& pushed) {
+// Per-wave push stage in LDS: vertex and target slot of each queued entry.
+struct PushStage {
+  int* v;          // kPushStage
+  uint32_t* slot;  // kPushStage; reused for the entry offset during a flush
+  uint32_t* hist;  // kAddsSlots counters
+  uint32_t n;      // wave-uniform
+};
+
+__device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
+  if (st.n == 0) return;
+  const unsigned lane = lane_id();
+  AddsCtrl* c = a.c;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // count first: pending covers the entries before anyone can claim them
+  if (lane == 0) atomicAdd(&c->pending, (unsigned long long)st.n);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // pending before the reservations
+  if (lane < (unsigned)kAddsSlots) st.hist[lane] = 0;
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t i = lane; i < st.n; i += kWave) {
+    const uint32_t s = st.slot[i];
+    st.slot[i] = (s << 16) | atomicAdd(st.hist + s, 1u);  // LDS atomic: offset in slot group
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // one reservation per non-empty slot; hist[s] becomes the slot's base
+  if (lane < (unsigned)kAddsSlots) {
+    const uint32_t k = st.hist[lane];
+    if (k) {
+      const uint32_t pos0 = atomicAdd(&TAIL(c, lane), k);
+      const uint32_t h = ld_rlx(&HEAD(c, lane));
+      // live entries after the push; head may already be past reserved but
+      // unwritten positions, so the difference is signed
+      if ((int)(pos0 + k - h) > (int)(a.cap_mask + 1u)) {
+        atomicOr(&c->err, kErrRing);
+        atomicExch(&c->done, 1u);
+      }
+      st.hist[lane] = pos0;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t i = lane; i < st.n; i += kWave) {
+    const uint32_t so = st.slot[i];
+    const uint32_t s = so >> 16;
+    ring_store(a, s, st.hist[s] + (so & 0xFFFFu), st.v[i]);
+  }
+  // entries and counts complete before the producer's claim is released
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_wave_barrier();
+  st.n = 0;
+}
+
+// Stage the lanes' improved vertices (dedup by qstamp).
+__device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp, uint32_t v,
+                                     uint32_t bkt, unsigned long long& pushed) {
   bool want = false;
   // release: a popper that exchanges qstamp after this sees the new label
   if (imp)
     want = __hip_atomic_fetch_min(a.qstamp + v, bkt, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) > bkt;
-  unsigned long long mask = __ballot(want);
-  const unsigned lane = lane_id();
-  while (mask) {
-    const int leader = __ffsll((long long)mask) - 1;
-    const uint32_t b = __shfl(bkt, leader, 64);
-    const bool mine = want && bkt == b;
-    const unsigned long long m2 = __ballot(mine);
-    uint32_t pos0 = 0;
-    if ((int)lane == leader) {
-      const uint32_t k = (uint32_t)__popcll(m2);
-      const uint32_t s = b % kAddsSlots;
-      atomicAdd(&a.c->pending, (unsigned long long)k);  // before the items become visible
-      pos0 = atomicAdd(&a.c->tail[s], k);
-      const uint32_t h = ld_rlx(&a.c->head[s]);
-      if (pos0 + k - h > a.cap_mask + 1u) atomicOr(&a.c->err, 1u);
-      pushed += k;
-    }
-    pos0 = __shfl(pos0, leader, 64);
-    if (mine) ring_store(a, b % kAddsSlots, pos0 + prefix_count(m2), (int)v);
-    mask &= ~m2;
+  const unsigned long long mask = __ballot(want);
+  if (mask == 0) return;
+  const uint32_t k = (uint32_t)__popcll(mask);
+  if (st.n + k > kPushStage) stage_publish(a, st);
+  if (want) {
+    const uint32_t i = st.n + prefix_count(mask);
+    st.v[i] = (int)v;
+    st.slot[i] = bkt % kAddsSlots;
   }
+  st.n += k;
+  pushed += k;
 }
 
 // Wave-level phase advance; true when this wave moved the base.
@@ -156,11 +218,16 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   const uint32_t s = base % kAddsSlots;
   bool ok = false;
   if (lane == 0) {
-    // head before inflight: a claimer raises inflight before its head CAS
-    const uint32_t h = __hip_atomic_load(&c->head[s], __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t t = __hip_atomic_load(&c->tail[s], __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-    const int inflight = __hip_atomic_load(&c->inflight[s], __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-    if (h == t && inflight == 0) {
+    // Only holders of a claim (or hub chunk) of bucket `base` push into it,
+    // and a claimer raises inflight before its head CAS.  head == tail (slot
+    // empty at the tail read), then inflight == 0 (no holder), then an
+    // unchanged tail (nothing pushed in between) leave no entry and no holder
+    // that could still add one.
+    const uint32_t h = __hip_atomic_load(&HEAD(c, s), __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t t = __hip_atomic_load(&TAIL(c, s), __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+    const int inflight = h == t ? __hip_atomic_load(&INFL(c, s), __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT) : 1;
+    const uint32_t t2 = inflight == 0 ? __hip_atomic_load(&TAIL(c, s), __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT) : t + 1;
+    if (h == t && inflight == 0 && t2 == t) {
       const float delta = ld_rlx(&c->delta);
       ok = atomicCAS(&c->base_word, bw, pack_base(base + 1, tbase_of(bw) + delta)) == bw;
     }
@@ -172,7 +239,8 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   uint32_t occ = 0;
   if (lane < (unsigned)kAddsSlots) {
     const uint32_t ss = (nb + lane) % kAddsSlots;
-    occ = ld_rlx(&c->tail[ss]) - ld_rlx(&c->head[ss]);
+    occ = ld_rlx(&TAIL(c, ss)) - ld_rlx(&HEAD(c, ss));
+    if ((int)occ < 0) occ = 0;
   }
   uint32_t window_items = occ;
 #pragma unroll
@@ -181,9 +249,8 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   uint32_t active = ld_rlx(&c->active);
   const int mode = a.policy ? a.policy->mode : kPolicyFixed;
   float feat = 0.0f;
-  if (mode == kPolicyMlp || (a.policy && a.policy->trace)) {
+  if (mode == kPolicyMlp || (a.policy && a.policy->trace))
     mlp_features(a.policy, c, occ, window_items, delta, lane, &feat);
-  }
   if (mode == kPolicyHeuristic) {
     const float capacity = (float)c->workers * (float)kClaim;
     if ((float)window_items < c->util_low * capacity) delta = fminf(delta * c->growth, c->max_delta);
@@ -216,13 +283,13 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
 }
 
 // Relax the packed edges of up to 64 vertices held one per lane (deg = 0 for
-// lanes without work) and queue the improved targets.  lo = producer bucket.
+// lanes without work) and stage the improved targets.  lo = producer bucket.
 template <class W>
-__device__ inline void relax_wave(const AddsArgs& a, int (&s_incl)[kWave], int (&s_base)[kWave],
-                                  uint32_t (&s_key)[kWave], int (&s_u)[kWave], uint32_t (&s_lo)[kWave],
-                                  int u, int beg, int deg, uint32_t key, uint32_t lo, uint32_t base,
-                                  float tbase, float inv, unsigned long long& edges,
-                                  unsigned long long& pushed) {
+__device__ inline void relax_wave(const AddsArgs& a, PushStage& st, int (&s_incl)[kWave],
+                                  int (&s_base)[kWave], uint32_t (&s_key)[kWave], int (&s_u)[kWave],
+                                  uint32_t (&s_lo)[kWave], int u, int beg, int deg, uint32_t key,
+                                  uint32_t lo, uint32_t base, float tbase, float inv,
+                                  unsigned long long& edges, unsigned long long& pushed) {
   const unsigned lane = lane_id();
   int incl = deg;
 #pragma unroll
@@ -239,8 +306,7 @@ __device__ inline void relax_wave(const AddsArgs& a, int (&s_incl)[kWave], int (
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // live work counter (one atomic per wave per claim): the phase trace reads it
-  if (lane == 0 && total > 0) atomicAdd(&a.c->edges, (unsigned long long)total);
+  if (a.live_edges && lane == 0 && total > 0) atomicAdd(&a.c->edges, (unsigned long long)total);
   edges += (unsigned long long)total;
   for (int t = 0; t < total; t += kWave) {
     const int j = t + (int)lane;
@@ -257,26 +323,30 @@ __device__ inline void relax_wave(const AddsArgs& a, int (&s_incl)[kWave], int (
       imp = relax_edge<W>(a.labels, v, ck, (uint32_t)s_u[pos]);
       if (imp) bk = map_bucket(W::bucket_value(ck), base, tbase, inv, s_lo[pos]);
     }
-    push_improved(a, imp, v, bk, pushed);
+    push_improved(a, st, imp, v, bk, pushed);
   }
   __builtin_amdgcn_wave_barrier();
 }
 
 template <class W>
 __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
-  __shared__ uint32_t s_slot, s_begin, s_count, s_hub, s_done, s_base;
+  __shared__ uint32_t s_slot, s_begin, s_count, s_hub, s_done, s_base, s_active, s_hub_ok;
   __shared__ float s_tbase, s_inv;
-  __shared__ uint32_t s_active;
   __shared__ int s_incl[kWavesPerBlock][kWave];
   __shared__ int s_beg[kWavesPerBlock][kWave];
   __shared__ uint32_t s_key[kWavesPerBlock][kWave];
   __shared__ int s_u[kWavesPerBlock][kWave];
   __shared__ uint32_t s_lo[kWavesPerBlock][kWave];
+  __shared__ int s_pv[kWavesPerBlock][kPushStage];
+  __shared__ uint32_t s_ps[kWavesPerBlock][kPushStage];
+  __shared__ uint32_t s_ph[kWavesPerBlock][kAddsSlots];
   AddsCtrl* c = a.c;
   const unsigned lane = threadIdx.x & 63;
   const unsigned wv = threadIdx.x >> 6;
+  PushStage st{s_pv[wv], s_ps[wv], s_ph[wv], 0};
   unsigned long long edges_local = 0, pushed_local = 0, items_local = 0;
-  uint32_t idle = 0;
+  uint32_t idle = 0, backoff = 1;
+  unsigned long long claims = 0;
 
   for (;;) {
     if (wv == 0) {
@@ -303,18 +373,18 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           const uint32_t A = max(1u, min(ld_rlx(&c->active), (uint32_t)kAddsSlots));
           for (uint32_t j = 0; j < A && !got; ++j) {
             const uint32_t s = (base + j) % kAddsSlots;
-            uint32_t h = ld_acq(&c->head[s]);
-            for (uint32_t t = ld_acq(&c->tail[s]); h < t; t = ld_acq(&c->tail[s])) {
+            uint32_t h = ld_acq(&HEAD(c, s));
+            for (uint32_t t = ld_acq(&TAIL(c, s)); (int)(t - h) > 0; t = ld_acq(&TAIL(c, s))) {
               const uint32_t k = min(kClaim, t - h);
-              atomic_add_sc(&c->inflight[s], 1);  // before the claim is visible
-              const uint32_t old = cas_sc(&c->head[s], h, h + k);
+              atomic_add_sc(&INFL(c, s), 1);  // before the claim is visible
+              const uint32_t old = cas_sc(&HEAD(c, s), h, h + k);
               if (old == h) {
                 got = k;
                 slot = s;
                 begin = h;
                 break;
               }
-              atomic_add_sc(&c->inflight[s], -1);
+              atomic_add_sc(&INFL(c, s), -1);
               h = old;
             }
           }
@@ -322,7 +392,17 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       }
       got = __shfl(got, 0, 64);
       done = __shfl(done, 0, 64);
-      if (got) idle = 0;
+      if (got) {
+        idle = 0;
+        backoff = 1;
+        if (++claims > a.claim_limit) {
+          if (lane == 0) {
+            atomicOr(&c->err, kErrStall);
+            atomicExch(&c->done, 1u);
+          }
+          done = 1;
+        }
+      }
       if (!got && !done) {
         const unsigned long long pend = __shfl(lane == 0 ? ld_acq(&c->pending) : 0ull, 0, 64);
         if (pend == 0) {
@@ -332,9 +412,13 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           const unsigned long long bw = __shfl(lane == 0 ? ld_acq(&c->base_word) : 0ull, 0, 64);
           if (try_advance(a, bw)) {
             idle = 0;
+            backoff = 1;
           } else {
-            __builtin_amdgcn_s_sleep(2);
-            if (++idle > kSpinLimit) {  // no progress for ~a second: abort, do not hang
+            // exponential back-off keeps idle workgroups from flooding the
+            // control words that the busy ones update
+            for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(8);
+            if (backoff < 32) backoff <<= 1;
+            if (++idle > kIdleLimit) {  // no progress for seconds: abort, do not hang
               if (lane == 0) {
                 atomicOr(&c->err, kErrStall);
                 atomicExch(&c->done, 1u);
@@ -370,7 +454,6 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
     if (s_hub) {
       // one hub chunk; wait until its producer has published it
       HubChunk* hc = a.hub + (s_begin & a.hub_mask);
-      __shared__ uint32_t s_hub_ok;
       if (threadIdx.x == 0) {
         uint32_t spins = 0;
         while (ld_acq(&hc->seq) != s_begin + 1u && ++spins < kSpinLimit) __builtin_amdgcn_s_sleep(1);
@@ -382,30 +465,28 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       }
       __syncthreads();
       if (!s_hub_ok) break;
-      HubChunk ch;
-      ch.u = hc->u;
-      ch.key = hc->key;
-      ch.first = hc->first;
-      ch.cnt = min(hc->cnt, (uint32_t)kChunk);
-      ch.slot = hc->slot % kAddsSlots;
-      ch.bucket = hc->bucket;
-      for (uint32_t t0 = wv * kWave; t0 < ch.cnt; t0 += kBlock) {
-        const uint32_t e = ch.first + t0 + lane;
+      const uint32_t hu = hc->u, hkey = hc->key, hfirst = hc->first;
+      const uint32_t hcnt = min(hc->cnt, (uint32_t)kChunk);
+      const uint32_t hslot = hc->slot % kAddsSlots, hbucket = hc->bucket;
+      for (uint32_t t0 = wv * kWave; t0 < hcnt; t0 += kBlock) {
+        const uint32_t e = hfirst + t0 + lane;
         bool imp = false;
         uint32_t v = 0, bk = 0;
-        if (t0 + lane < ch.cnt) {
+        if (t0 + lane < hcnt) {
           const uint2 ed = a.edges[e];
           v = ed.x;
-          const uint32_t ck = W::relax(ch.key, ed.y);
-          imp = relax_edge<W>(a.labels, v, ck, ch.u);
-          if (imp) bk = map_bucket(W::bucket_value(ck), base, tbase, inv, ch.bucket);
+          const uint32_t ck = W::relax(hkey, ed.y);
+          imp = relax_edge<W>(a.labels, v, ck, hu);
+          if (imp) bk = map_bucket(W::bucket_value(ck), base, tbase, inv, hbucket);
         }
-        push_improved(a, imp, v, bk, pushed_local);
+        push_improved(a, st, imp, v, bk, pushed_local);
       }
-      if (wv == 0 && lane == 0) atomicAdd(&c->edges, (unsigned long long)ch.cnt);
+      stage_publish(a, st);
+      if (a.live_edges && wv == 0 && lane == 0) atomicAdd(&c->edges, (unsigned long long)hcnt);
+      edges_local += hcnt;
       __syncthreads();
       if (threadIdx.x == 0) {
-        atomic_add_sc(&c->inflight[ch.slot], -1);
+        atomic_add_sc(&INFL(c, hslot), -1);
         atomicAdd(&c->pending, ~0ull);
       }
       continue;
@@ -413,15 +494,17 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
 
     // regular items: wave w takes claim entries [w*64, w*64 + 64)
     const uint32_t slot = s_slot, begin = s_begin;
-    const int* ring = a.ring + (size_t)slot * (a.cap_mask + 1u);
+    int* ring = a.ring + (size_t)slot * (a.cap_mask + 1u);
     const uint32_t i = wv * kWave + lane;
     int u = 0, beg = 0, deg = 0;
     uint32_t key = W::kInfKey, bk = 0;
     bool entry_ok = true;
+    bool requeue = false;
+    uint32_t want = 0;
     if (i < count) {
       // The pusher reserved this position before writing it: wait for the
       // write (bounded), then hand the entry back to the ring as empty.
-      int* ep = const_cast<int*>(ring) + ((begin + i) & a.cap_mask);
+      int* ep = ring + ((begin + i) & a.cap_mask);
       uint32_t spins = 0;
       while ((u = __hip_atomic_load(ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == kEmptyEntry &&
              ++spins < kSpinLimit)
@@ -441,15 +524,11 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       if (old != kNoStamp && old % kAddsSlots == slot) {
         bk = old;
         key = atomic_load_key(a.labels + u);
-        const uint32_t want = map_bucket(W::bucket_value(key), base, tbase, inv, bk);
+        want = map_bucket(W::bucket_value(key), base, tbase, inv, bk);
         if (want >= bk + s_active) {
           // clipped or re-mapped far item: queue it where it belongs now
           key = W::kInfKey;
-          if (atomicMin(a.qstamp + u, want) > want) {
-            atomicAdd(&c->pending, 1ull);
-            const uint32_t pos = atomicAdd(&c->tail[want % kAddsSlots], 1u);
-            ring_store(a, want % kAddsSlots, pos, u);
-          }
+          requeue = true;
         }
       } else if (old != kNoStamp) {
         atomicMin(a.qstamp + u, old);  // queued again elsewhere: restore, drop this entry
@@ -463,9 +542,9 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           // cut into hub chunks; each chunk is pending work held in this slot
           const uint32_t nch = (uint32_t)((deg + kChunk - 1) / kChunk);
           atomicAdd(&c->pending, (unsigned long long)nch);
-          atomicAdd(&c->inflight[slot], (int)nch);
+          atomicAdd(&INFL(c, slot), (int)nch);
           const uint32_t pos = atomicAdd(&c->hub_reserve, nch);
-          if (pos + nch - ld_rlx(&c->hub_head) > a.hub_mask + 1u) {
+          if ((int)(pos + nch - ld_rlx(&c->hub_head)) > (int)(a.hub_mask + 1u)) {
             atomicOr(&c->err, kErrHub);
             atomicExch(&c->done, 1u);
           }
@@ -484,40 +563,41 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         }
       }
     }
-    relax_wave<W>(a, s_incl[wv], s_beg[wv], s_key[wv], s_u[wv], s_lo[wv], u, beg, deg, key, bk,
+    // re-queued items go through the stage like any push
+    push_improved(a, st, requeue, (uint32_t)u, want, pushed_local);
+    relax_wave<W>(a, st, s_incl[wv], s_beg[wv], s_key[wv], s_u[wv], s_lo[wv], u, beg, deg, key, bk,
                   base, tbase, inv, edges_local, pushed_local);
+    stage_publish(a, st);
     __syncthreads();
     if (threadIdx.x == 0) {
-      atomic_add_sc(&c->inflight[slot], -1);
+      atomic_add_sc(&INFL(c, slot), -1);
       atomicAdd(&c->pending, 0ull - (unsigned long long)count);
     }
   }
   if (lane == 0) {
-    (void)edges_local;
+    if (!a.live_edges && edges_local) atomicAdd(&c->edges, edges_local);
     if (pushed_local) atomicAdd(&c->pushes, pushed_local);
+    if (items_local) atomicAdd(&c->items, items_local);
   }
-  if (lane == 0 && items_local) atomicAdd(&c->items, items_local);
 }
 
 template <class W>
 __global__ void k_adds_init(AddsArgs a, int64_t n, uint32_t src, const uint32_t* __restrict__ init,
-                            AddsCtrl init_ctrl) {
+                            const AddsCtrl* __restrict__ init_ctrl) {
   const unsigned long long inf_label = ((unsigned long long)W::kInfKey << 32) | kNoPred;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t v = tid; v < n; v += stride) {
     unsigned long long l = inf_label;
     if (init) l = ((unsigned long long)W::output_to_key(init[v]) << 32) | kNoPred;
     a.labels[v] = v == (int64_t)src ? (((unsigned long long)W::kZeroKey << 32) | kNoPred) : l;
     a.qstamp[v] = v == (int64_t)src ? 0u : kNoStamp;
   }
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= (int64_t)a.hub_mask; i += stride)
-    a.hub[i].seq = 0u;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *a.c = init_ctrl;
-    a.c->tail[0] = 1;
-    a.c->pending = 1;
-    a.ring[0] = (int)src;
-  }
+  for (int64_t i = tid; i <= (int64_t)a.hub_mask; i += stride) a.hub[i].seq = 0u;
+  const uint32_t* src_words = reinterpret_cast<const uint32_t*>(init_ctrl);
+  uint32_t* dst_words = reinterpret_cast<uint32_t*>(a.c);
+  for (int64_t i = tid; i < (int64_t)(sizeof(AddsCtrl) / 4); i += stride) dst_words[i] = src_words[i];
+  if (tid == 0) a.ring[0] = (int)src;
 }
 
 template <class W>
@@ -533,14 +613,19 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   uint32_t hub_cap = 1024;
   while ((int64_t)hub_cap < g.info.large_chunks * 2 + 1024) hub_cap <<= 1;
   ws.labels.ensure((size_t)n);
-  ws.adds_ring.ensure((size_t)kAddsSlots * cap);
+  if (ws.adds_ring.n < (size_t)kAddsSlots * cap) {
+    ws.adds_ring.reset((size_t)kAddsSlots * cap);
+    ws.adds_ring_clean = false;
+  }
   ws.adds_qstamp.ensure((size_t)n);
   ws.adds_hub.ensure(hub_cap);
-  ws.adds_ctrl_mem.ensure(sizeof(AddsCtrl));
+  ws.adds_ctrl_mem.ensure(2 * sizeof(AddsCtrl));
   AddsCtrl* c = reinterpret_cast<AddsCtrl*>(ws.adds_ctrl_mem.p);
+  AddsCtrl* c_init = c + 1;
   const DeviceProps p = device_props();
   const int workers = req.worker_blocks > 0 ? req.worker_blocks : p.cus * 2;
-  AddsCtrl h{};
+  std::vector<unsigned char> hbuf(sizeof(AddsCtrl), 0);
+  AddsCtrl& h = *reinterpret_cast<AddsCtrl*>(hbuf.data());
   h.base_word = 0;  // bucket 0 starts at distance 0.0f
   h.delta = delta;
   h.base_delta = delta;
@@ -552,6 +637,8 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   h.shrink = 0.75f;
   h.active = req.active_buckets > 0 ? (uint32_t)req.active_buckets : 1u;
   h.workers = (uint32_t)workers;
+  h.pending = 1;
+  h.tail[0] = 1;
   AddsArgs a{};
   a.rowptr = g.rowptr.p;
   a.edges = g.edges.p;
@@ -564,6 +651,9 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.c = c;
   a.policy = d_policy;
   a.n = (uint32_t)n;
+  a.live_edges = req.trace ? 1 : 0;
+  a.claim_limit = 16ull * (unsigned long long)(n + g.m) + (1ull << 20);
+  MI_HIP(hipMemcpyAsync(c_init, hbuf.data(), sizeof(AddsCtrl), hipMemcpyHostToDevice, s));
   if (!ws.adds_ring_clean) {
     // every entry starts as the empty sentinel; a completed run leaves the
     // ring in that state again (all pushed entries are consumed)
@@ -576,7 +666,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   MI_HIP(hipEventCreate(&ev1));
   MI_HIP(hipEventRecord(ev0, s));
   k_adds_init<W><<<std::min<unsigned>(div_up((uint64_t)std::max<int64_t>(n, hub_cap), kBlock), 4096u),
-                   kBlock, 0, s>>>(a, n, src, req.init_dist, h);
+                   kBlock, 0, s>>>(a, n, src, req.init_dist, c_init);
   MI_LAUNCH_CHECK();
   k_adds<W><<<workers, kBlock, 0, s>>>(a);
   MI_LAUNCH_CHECK();
@@ -585,7 +675,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   MI_HIP(hipEventElapsedTime(&res.device_ms, ev0, ev1));
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
-  MI_HIP(hipMemcpyAsync(&h, c, sizeof(AddsCtrl), hipMemcpyDeviceToHost, s));
+  MI_HIP(hipMemcpyAsync(hbuf.data(), c, sizeof(AddsCtrl), hipMemcpyDeviceToHost, s));
   MI_HIP(hipStreamSynchronize(s));
   if (h.err) ws.adds_ring_clean = false;
   if (h.err & kErrRing) fail(MI355_ECAPACITY, "ADDS bucket queue overflow");

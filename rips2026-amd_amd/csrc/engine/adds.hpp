@@ -14,25 +14,36 @@ constexpr int kTraceStride = 40;   // floats per traced phase
 
 enum PolicyMode { kPolicyFixed = 0, kPolicyHeuristic = 1, kPolicyMlp = 2 };
 
-// Device control block of one ADDS run.
+// Device control block of one ADDS run.  Every counter that many waves hit
+// with atomics sits on its own 256-byte line (kPad words apart) so unrelated
+// counters do not serialise on one line.
+constexpr int kPad = 64;
 struct AddsCtrl {
-  unsigned long long base_word;  // (T_base float bits << 32) | base bucket
+  // read-mostly parameters
   float delta;                   // current bucket width
   float base_delta;              // heuristic delta the policy scales
   float min_delta, max_delta;
   float util_low, util_high, growth, shrink;
   uint32_t active;               // buckets served concurrently (priority window)
   uint32_t workers;
-  uint32_t head[kAddsSlots];     // pop counters (monotone)
-  uint32_t tail[kAddsSlots];     // push counters (monotone)
-  int inflight[kAddsSlots];      // claims of a slot still being processed
-  unsigned long long pending;    // items queued or being processed (two's-complement adds)
-  uint32_t hub_head, hub_reserve;
-  uint32_t done;
-  uint32_t err;
   uint32_t phases;
-  uint32_t pad;
+  uint32_t err;
+  uint32_t pad0[kPad - 12];
+  unsigned long long base_word;  // (T_base float bits << 32) | base bucket
+  uint32_t pad1[kPad - 2];
+  unsigned long long pending;    // items queued or being processed (two's-complement adds)
+  uint32_t pad2[kPad - 2];
+  uint32_t done;
+  uint32_t pad3[kPad - 1];
+  uint32_t hub_head;
+  uint32_t pad4[kPad - 1];
+  uint32_t hub_reserve;
+  uint32_t pad5[kPad - 1];
   unsigned long long edges, pushes, items;
+  uint32_t pad6[kPad - 6];
+  uint32_t head[kAddsSlots * kPad];  // pop counter of slot s at [s * kPad] (monotone)
+  uint32_t tail[kAddsSlots * kPad];  // push counter of slot s
+  int inflight[kAddsSlots * kPad];   // claims of slot s still being processed
 };
 
 // One 2048-edge piece of a high-degree vertex.

@@ -112,19 +112,24 @@ struct DsSink {
   bool heavy;           // heavy steps: a same-bucket straggler must rejoin it
   uint32_t far_out;
 
-  __device__ inline void vertex(bool valid, int u, uint32_t) const {
+  int* stage_light = nullptr;    // LDS per wave
+  int* stage_settled = nullptr;  // LDS per wave
+  uint32_t n_light = 0, n_settled = 0;
+
+  __device__ inline void vertex(bool valid, int u, uint32_t) {
     bool push = false;
     if (record_settled && valid) {
       const uint32_t tag = (uint32_t)B + 1u;
       if (a->sstamp[u] != tag) push = atomicExch(a->sstamp + u, tag) != tag;
     }
-    wave_append(push, u, a->settled, &c->settled_cnt, a->cap, &c->err);
+    stage_push(push, u, stage_settled, n_settled, a->settled, &c->settled_cnt, a->cap, &c->err);
   }
 
-  __device__ inline void improved(bool imp, uint32_t v, uint32_t key) const {
-    bool to_light = false;
+  __device__ inline void improved(bool imp, uint32_t v, uint32_t key) {
+    bool to_light = false, to_slot = false;
+    int b = 0;
     if (imp) {
-      const int b = bucket_of_key<W>(key, a->inv_delta);
+      b = bucket_of_key<W>(key, a->inv_delta);
       if (b <= B) {
         // floorf(fl(d + w) / delta) can round back into bucket B for a heavy
         // edge; such a vertex is relaxed again and its heavy edges redone.
@@ -132,12 +137,7 @@ struct DsSink {
         if (a->lstamp[v] != lstamp_val) to_light = atomicExch(a->lstamp + v, lstamp_val) != lstamp_val;
       } else if (b < B + kSlots) {
         // bstamp starts at 0xFFFFFFFF: compare unsigned
-        if (atomicMin(a->bstamp + v, (uint32_t)b) > (uint32_t)b) {
-          const int s = b % kSlots;
-          const uint32_t pos = atomicAdd(&c->slot_cnt[s], 1u);
-          if ((int64_t)pos < a->slot_cap) a->slots[(int64_t)s * a->slot_cap + pos] = (int)v;
-          else atomicOr(&c->err, 2u);
-        }
+        to_slot = atomicMin(a->bstamp + v, (uint32_t)b) > (uint32_t)b;
       } else if (b < kNoBucket) {
         if (atomicExch(a->farflag + v, 1u) == 0u) {
           const uint32_t pos = atomicAdd(&c->far_cnt[far_out], 1u);
@@ -146,7 +146,31 @@ struct DsSink {
         }
       }
     }
-    wave_append(to_light, (int)v, lq_out, lq_cnt, a->cap, &c->err);
+    // bucket-queue appends: one atomicAdd per distinct target slot in the wave
+    unsigned long long mask = ballot64(to_slot);
+    const unsigned lane = lane_id();
+    while (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      const int bl = __shfl(b, leader, 64);
+      const bool mine = to_slot && b == bl;
+      const unsigned long long m2 = ballot64(mine);
+      const int s = bl % kSlots;
+      uint32_t pos0 = 0;
+      if ((int)lane == leader) pos0 = atomicAdd(&c->slot_cnt[s], (uint32_t)__popcll(m2));
+      pos0 = __shfl(pos0, leader, 64);
+      if (mine) {
+        const uint32_t pos = pos0 + prefix_count(m2);
+        if ((int64_t)pos < a->slot_cap) a->slots[(int64_t)s * a->slot_cap + pos] = (int)v;
+        else atomicOr(&c->err, 2u);
+      }
+      mask &= ~m2;
+    }
+    stage_push(to_light, (int)v, stage_light, n_light, lq_out, lq_cnt, a->cap, &c->err);
+  }
+
+  __device__ inline void flush() {
+    stage_flush(stage_light, n_light, lq_out, lq_cnt, a->cap, &c->err);
+    stage_flush(stage_settled, n_settled, a->settled, &c->settled_cnt, a->cap, &c->err);
   }
 };
 
@@ -293,20 +317,26 @@ __global__ void k_ds_init(DsArrays a, BucketCtrl* __restrict__ c, int64_t n, uin
 
 template <class W>
 __global__ __launch_bounds__(kBlock) void k_ds_work(DsArrays a, BucketCtrl* __restrict__ c) {
+  __shared__ int s_stage_l[kWavesPerBlock][kStageCap];
+  __shared__ int s_stage_s[kWavesPerBlock][kStageCap];
   const uint32_t kind = c->kind;
   if (kind == kStepDone || kind == kStepInit) return;
   const int B = c->bucket;
+  const unsigned wv = threadIdx.x / kWave;
   unsigned long long* work = a.want_work ? &c->edges : nullptr;
   if (kind == kStepLight || kind == kStepHeavy) {
     const uint32_t in_sel = c->in_sel;
     const uint32_t out_sel = in_sel ^ 1u;
     DsSink<W> sink{&a, c, B, c->lstamp, a.lq[out_sel], &c->light_cnt[out_sel], kind == kStepLight,
                    kind == kStepHeavy, c->far_sel};
+    sink.stage_light = s_stage_l[wv];
+    sink.stage_settled = s_stage_s[wv];
     ChunkQueue cq{a.chunks, &c->chunk_cnt, a.chunk_cap};
     const int* qin = kind == kStepLight ? a.lq[in_sel] : a.settled;
     relax_frontier_body<W, false>(a.rowptr, a.edges, a.labels, qin, nullptr, c->first, c->count, cq,
                                   kind == kStepLight ? kLightEdges : kHeavyEdges, a.delta_bits, sink,
                                   work);
+    sink.flush();
     return;
   }
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -317,6 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_ds_work(DsArrays a, BucketCtrl* __re
     const uint32_t cnt = c->slot_cnt[s];
     const int* slot = a.slots + (int64_t)s * a.slot_cap;
     const uint32_t rounds = div_up(cnt, stride) * stride;
+    uint32_t staged = 0;
     for (uint32_t i = tid; i < rounds; i += stride) {
       bool push = false;
       int v = 0;
@@ -324,8 +355,9 @@ __global__ __launch_bounds__(kBlock) void k_ds_work(DsArrays a, BucketCtrl* __re
         v = slot[i];
         push = bucket_of_key<W>(atomic_load_key(a.labels + v), a.inv_delta) == b;
       }
-      wave_append(push, v, a.lq[0], &c->light_cnt[0], a.cap, &c->err);
+      stage_push(push, v, s_stage_l[wv], staged, a.lq[0], &c->light_cnt[0], a.cap, &c->err);
     }
+    stage_flush(s_stage_l[wv], staged, a.lq[0], &c->light_cnt[0], a.cap, &c->err);
     return;
   }
   if (kind == kStepRefillMin) {
@@ -371,11 +403,16 @@ __global__ __launch_bounds__(kBlock) void k_ds_chunks(DsArrays a, BucketCtrl* __
   if (kind != kStepLight && kind != kStepHeavy) return;
   const uint32_t in_sel = c->in_sel;
   const uint32_t out_sel = in_sel ^ 1u;
+  __shared__ int s_stage_l[kWavesPerBlock][kStageCap];
+  __shared__ int s_stage_s[kWavesPerBlock][kStageCap];
   DsSink<W> sink{&a, c, c->bucket, c->lstamp, a.lq[out_sel], &c->light_cnt[out_sel], false,
                  kind == kStepHeavy, c->far_sel};
+  sink.stage_light = s_stage_l[threadIdx.x / kWave];
+  sink.stage_settled = s_stage_s[threadIdx.x / kWave];
   ChunkQueue cq{a.chunks, &c->chunk_cnt, a.chunk_cap};
   relax_chunks_body<W>(a.edges, a.labels, cq, kind == kStepLight ? kLightEdges : kHeavyEdges,
                        a.delta_bits, sink, a.want_work ? &c->edges : nullptr);
+  sink.flush();
 }
 
 template <class W>
@@ -447,6 +484,9 @@ void run_ds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
     ++res.syncs;
     if (h.err) fail(MI355_ECAPACITY, "delta-stepping queue overflow");
     if (h.done) break;
+    // every step settles or moves work; this bound is never reached by a run
+    // that makes progress
+    if (steps > 64 * (n + 1024)) fail(MI355_EDEVICE, "delta-stepping made no progress");
     if (req.rounds_per_sync <= 0) batch = std::min(batch * 2, 256);
   }
   MI_HIP(hipEventRecord(ev1, s));

@@ -29,7 +29,8 @@ __device__ inline uint32_t atomic_load_key(const unsigned long long* p) {
   return (uint32_t)(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
 }
 
-// Bellman-Ford sink: improved vertices go to the next frontier, once per round.
+// Bellman-Ford sink: improved vertices go to the next frontier, once per round
+// (round stamp), staged per wave in LDS.
 struct BfSink {
   int* qout;
   uint32_t* cnt;
@@ -37,12 +38,15 @@ struct BfSink {
   uint32_t* stamp;
   uint32_t stamp_val;
   uint32_t* err;
-  __device__ inline void vertex(bool, int, uint32_t) const {}
-  __device__ inline void improved(bool imp, uint32_t v, uint32_t) const {
+  int* stage = nullptr;  // LDS, set by the kernel per wave
+  uint32_t staged = 0;
+  __device__ inline void vertex(bool, int, uint32_t) {}
+  __device__ inline void improved(bool imp, uint32_t v, uint32_t) {
     bool push = false;
     if (imp && stamp[v] != stamp_val) push = atomicExch(stamp + v, stamp_val) != stamp_val;
-    wave_append(push, (int)v, qout, cnt, cap, err);
+    stage_push(push, (int)v, stage, staged, qout, cnt, cap, err);
   }
+  __device__ inline void flush() { stage_flush(stage, staged, qout, cnt, cap, err); }
 };
 
 // Small/medium-degree frontier vertices: 64 per wave, edges packed across
@@ -167,14 +171,17 @@ __global__ __launch_bounds__(kBlock) void k_relax_frontier(
     const int* __restrict__ rowptr, const uint2* __restrict__ edges,
     unsigned long long* __restrict__ labels, FrontierIn in, ChunkQueue cq, Sink sink,
     RoundHousekeeping hk, unsigned long long* __restrict__ work) {
+  __shared__ int s_stage[kWavesPerBlock][kStageCap];
   const uint32_t n_in = *in.cnt;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (hk.zero0) *hk.zero0 = 0;
     if (hk.zero1) *hk.zero1 = 0;
     if (hk.log) *hk.log = n_in;
   }
+  sink.stage = s_stage[threadIdx.x / kWave];
   relax_frontier_body<W, JACOBI>(rowptr, edges, labels, in.q, in.keys, 0u, n_in, cq, kAllEdges,
                                  0u, sink, work);
+  sink.flush();
 }
 
 template <class W, class Sink>
@@ -182,7 +189,10 @@ __global__ __launch_bounds__(kBlock) void k_relax_chunks(const uint2* __restrict
                                                          unsigned long long* __restrict__ labels,
                                                          ChunkQueue cq, Sink sink,
                                                          unsigned long long* __restrict__ work) {
+  __shared__ int s_stage[kWavesPerBlock][kStageCap];
+  sink.stage = s_stage[threadIdx.x / kWave];
   relax_chunks_body<W>(edges, labels, cq, kAllEdges, 0u, sink, work);
+  sink.flush();
 }
 
 // Jacobi snapshot: keys of the next frontier as they stand after the round.

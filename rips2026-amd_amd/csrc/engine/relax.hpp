@@ -48,6 +48,43 @@ __device__ inline void wave_append(bool push, int v, int* __restrict__ q, uint32
   }
 }
 
+// Per-wave LDS staging of queue appends.  A global atomicAdd on a queue
+// counter is paid once per kStageCap staged entries (or at the end of the
+// wave's work) instead of once per wave-iteration: queue counters are single
+// addresses that every wave of the grid hits, and same-address atomics
+// serialise.  `staged` is wave-uniform and lives in a register.
+constexpr uint32_t kStageCap = 512;
+
+__device__ inline void stage_flush(int* __restrict__ stage, uint32_t& staged, int* __restrict__ q,
+                                   uint32_t* __restrict__ cnt, uint32_t cap,
+                                   uint32_t* __restrict__ err) {
+  if (staged == 0) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const unsigned lane = lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(cnt, staged);
+  base = __shfl(base, 0, 64);
+  for (uint32_t i = lane; i < staged; i += kWave) {
+    if (base + i < cap) q[base + i] = stage[i];
+    else atomicOr(err, 1u);
+  }
+  __builtin_amdgcn_wave_barrier();
+  staged = 0;
+}
+
+__device__ inline void stage_push(bool push, int v, int* __restrict__ stage, uint32_t& staged,
+                                  int* __restrict__ q, uint32_t* __restrict__ cnt, uint32_t cap,
+                                  uint32_t* __restrict__ err) {
+  const unsigned long long mask = ballot64(push);
+  if (mask == 0) return;
+  const uint32_t k = (uint32_t)__popcll(mask);
+  if (staged + k > kStageCap) stage_flush(stage, staged, q, cnt, cap, err);
+  if (push) stage[staged + prefix_count(mask)] = v;
+  staged += k;
+}
+
 // Edge filter for delta-stepping's light/heavy split (w <= delta is light,
 // DS/src/delta_stepping_hip_CSR.cpp:402, :440).
 enum EdgeClass { kAllEdges = 0, kLightEdges = 1, kHeavyEdges = 2 };
