@@ -15,8 +15,10 @@
 //    are clipped into the last slot and re-queued when they come up) and
 //    never below the bucket of the vertex that produced it.
 //  * Workers.  Every workgroup is a worker: wave 0 claims up to 256 items from
-//    the lowest non-empty slot of the active window [base, base + A) with a
-//    CAS on the slot head; the four waves relax them, 64 items per wave with
+//    the lowest non-empty slot of the active window [base, base + A) through
+//    the slot's counting semaphore `avail` (one fetch-sub, surplus given back,
+//    then a fetch-add on the slot head for the positions; no CAS retry loop
+//    for hundreds of workgroups to serialise on); the four waves relax them, 64 items per wave with
 //    their edges packed across the lanes as in the frontier kernels; vertices
 //    above kLargeDegree are cut into 2048-edge hub chunks any workgroup can
 //    claim.
@@ -48,6 +50,8 @@
 //    not be co-resident for the run to make progress.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "adds.hpp"
@@ -73,14 +77,10 @@ constexpr uint32_t kErrRing = 1u, kErrHub = 2u, kErrStall = 4u;
 #define HEAD(c, s) ((c)->head[(s) * kPad])
 #define TAIL(c, s) ((c)->tail[(s) * kPad])
 #define INFL(c, s) ((c)->inflight[(s) * kPad])
+#define AVAIL(c, s) ((c)->avail[(s) * kPad])
 
 __device__ inline int atomic_add_sc(int* p, int v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline uint32_t cas_sc(uint32_t* p, uint32_t expected, uint32_t desired) {
-  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-  return expected;
 }
 
 __device__ inline unsigned long long pack_base(uint32_t base, float t) {
@@ -162,8 +162,9 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   // one reservation per non-empty slot; hist[s] becomes the slot's base
+  uint32_t k = 0;
   if (lane < (unsigned)kAddsSlots) {
-    const uint32_t k = st.hist[lane];
+    k = st.hist[lane];
     if (k) {
       const uint32_t pos0 = atomicAdd(&TAIL(c, lane), k);
       const uint32_t h = ld_rlx(&HEAD(c, lane));
@@ -184,9 +185,11 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
     const uint32_t s = so >> 16;
     ring_store(a, s, st.hist[s] + (so & 0xFFFFu), st.v[i]);
   }
-  // entries and counts complete before the producer's claim is released
+  // entries written before they are offered to claimers
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __builtin_amdgcn_wave_barrier();
+  if (k) atomicAdd(&AVAIL(c, lane), (int)k);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   st.n = 0;
 }
 
@@ -347,6 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
   unsigned long long edges_local = 0, pushed_local = 0, items_local = 0;
   uint32_t idle = 0, backoff = 1;
   unsigned long long claims = 0;
+  unsigned long long d_cas = 0, d_idle = 0, d_spin = 0, d_hub = 0, d_drop = 0, d_req = 0;
 
   for (;;) {
     if (wv == 0) {
@@ -355,16 +359,17 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         done = ld_acq(&c->done);
         if (!done) {
           // hub chunks first: they finish items already claimed
-          uint32_t hh = ld_acq(&c->hub_head);
-          while (hh < ld_acq(&c->hub_reserve)) {
-            const uint32_t old = atomicCAS(&c->hub_head, hh, hh + 1);
-            if (old == hh) {
+          // counting-semaphore claims: one decrement, no retry loop, so
+          // claimers do not serialise on a CAS of the queue head
+          if (ld_acq(&c->hub_avail) > 0) {
+            if (atomicSub(&c->hub_avail, 1) >= 1) {
               got = 1;
               hubc = 1;
-              begin = hh;
-              break;
+              begin = atomicAdd(&c->hub_head, 1u);
+            } else {
+              atomicAdd(&c->hub_avail, 1);
+              ++d_cas;
             }
-            hh = old;
           }
         }
         if (!done && !got) {
@@ -373,19 +378,18 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           const uint32_t A = max(1u, min(ld_rlx(&c->active), (uint32_t)kAddsSlots));
           for (uint32_t j = 0; j < A && !got; ++j) {
             const uint32_t s = (base + j) % kAddsSlots;
-            uint32_t h = ld_acq(&HEAD(c, s));
-            for (uint32_t t = ld_acq(&TAIL(c, s)); (int)(t - h) > 0; t = ld_acq(&TAIL(c, s))) {
-              const uint32_t k = min(kClaim, t - h);
-              atomic_add_sc(&INFL(c, s), 1);  // before the claim is visible
-              const uint32_t old = cas_sc(&HEAD(c, s), h, h + k);
-              if (old == h) {
-                got = k;
-                slot = s;
-                begin = h;
-                break;
-              }
+            if (ld_acq(&AVAIL(c, s)) <= 0) continue;
+            atomic_add_sc(&INFL(c, s), 1);  // before the claim is visible
+            const int old = atomic_add_sc(&AVAIL(c, s), -(int)kClaim);
+            const int k = min(old, (int)kClaim);
+            if (k < (int)kClaim) atomic_add_sc(&AVAIL(c, s), (int)kClaim - max(k, 0));  // give back
+            if (k > 0) {
+              got = (uint32_t)k;
+              slot = s;
+              begin = atomicAdd(&HEAD(c, s), (uint32_t)k);
+            } else {
               atomic_add_sc(&INFL(c, s), -1);
-              h = old;
+              ++d_cas;
             }
           }
         }
@@ -416,6 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           } else {
             // exponential back-off keeps idle workgroups from flooding the
             // control words that the busy ones update
+            ++d_idle;
             for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(8);
             if (backoff < 32) backoff <<= 1;
             if (++idle > kIdleLimit) {  // no progress for seconds: abort, do not hang
@@ -484,6 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       stage_publish(a, st);
       if (a.live_edges && wv == 0 && lane == 0) atomicAdd(&c->edges, (unsigned long long)hcnt);
       edges_local += hcnt;
+      d_hub += threadIdx.x == 0;
       __syncthreads();
       if (threadIdx.x == 0) {
         atomic_add_sc(&INFL(c, hslot), -1);
@@ -509,6 +515,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       while ((u = __hip_atomic_load(ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == kEmptyEntry &&
              ++spins < kSpinLimit)
         __builtin_amdgcn_s_sleep(1);
+      d_spin += spins;
       entry_ok = u != kEmptyEntry && u >= 0 && (uint32_t)u < a.n;
       if (!entry_ok) {
         atomicOr(&c->err, kErrRing);
@@ -529,9 +536,13 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           // clipped or re-mapped far item: queue it where it belongs now
           key = W::kInfKey;
           requeue = true;
+          ++d_req;
         }
       } else if (old != kNoStamp) {
         atomicMin(a.qstamp + u, old);  // queued again elsewhere: restore, drop this entry
+        ++d_drop;
+      } else {
+        ++d_drop;
       }
       if (key != W::kInfKey) {
         ++items_local;
@@ -559,6 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
             hc->bucket = bk;
             __hip_atomic_store(&hc->seq, pos + q + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
           }
+          __hip_atomic_fetch_add(&c->hub_avail, (int)nch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
           deg = 0;
         }
       }
@@ -574,11 +586,18 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       atomicAdd(&c->pending, 0ull - (unsigned long long)count);
     }
   }
+  if (threadIdx.x == 0 && claims) atomicAdd(&c->diag[0], claims);
+  if (d_cas) atomicAdd(&c->diag[1], d_cas);
+  if (threadIdx.x == 0 && d_idle) atomicAdd(&c->diag[2], d_idle);
+  if (d_spin) atomicAdd(&c->diag[3], d_spin);
+  if (d_hub) atomicAdd(&c->diag[4], d_hub);
+  if (d_drop) atomicAdd(&c->diag[5], d_drop);
+  if (d_req) atomicAdd(&c->diag[6], d_req);
   if (lane == 0) {
     if (!a.live_edges && edges_local) atomicAdd(&c->edges, edges_local);
     if (pushed_local) atomicAdd(&c->pushes, pushed_local);
-    if (items_local) atomicAdd(&c->items, items_local);
   }
+  if (items_local) atomicAdd(&c->items, items_local);
 }
 
 template <class W>
@@ -639,6 +658,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   h.workers = (uint32_t)workers;
   h.pending = 1;
   h.tail[0] = 1;
+  h.avail[0] = 1;
   AddsArgs a{};
   a.rowptr = g.rowptr.p;
   a.edges = g.edges.p;
@@ -678,6 +698,12 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   MI_HIP(hipMemcpyAsync(hbuf.data(), c, sizeof(AddsCtrl), hipMemcpyDeviceToHost, s));
   MI_HIP(hipStreamSynchronize(s));
   if (h.err) ws.adds_ring_clean = false;
+  if (std::getenv("MI355_ADDS_DEBUG"))
+    std::fprintf(stderr,
+                 "adds: %.3f ms phases %u items %llu edges %llu pushes %llu | claims %llu cas_fail %llu "
+                 "idle %llu ring_wait %llu hub %llu dropped %llu requeued %llu\n",
+                 res.device_ms, h.phases, h.items, h.edges, h.pushes, h.diag[0], h.diag[1], h.diag[2],
+                 h.diag[3], h.diag[4], h.diag[5], h.diag[6]);
   if (h.err & kErrRing) fail(MI355_ECAPACITY, "ADDS bucket queue overflow");
   if (h.err & kErrHub) fail(MI355_ECAPACITY, "ADDS hub chunk queue overflow");
   if (h.err & kErrStall) fail(MI355_EDEVICE, "ADDS workers stalled (aborted by the spin bound)");

@@ -39,11 +39,18 @@ struct AddsCtrl {
   uint32_t pad4[kPad - 1];
   uint32_t hub_reserve;
   uint32_t pad5[kPad - 1];
+  int hub_avail;                 // published, unclaimed hub chunks (may dip below 0)
+  uint32_t pad5b[kPad - 1];
   unsigned long long edges, pushes, items;
   uint32_t pad6[kPad - 6];
+  // diagnostics: claims, failed claim CAS, idle polls, ring-entry waits,
+  // hub chunks, stale entries dropped, re-queued entries, spare
+  unsigned long long diag[8];
+  uint32_t pad7[kPad - 16];
   uint32_t head[kAddsSlots * kPad];  // pop counter of slot s at [s * kPad] (monotone)
   uint32_t tail[kAddsSlots * kPad];  // push counter of slot s
   int inflight[kAddsSlots * kPad];   // claims of slot s still being processed
+  int avail[kAddsSlots * kPad];      // written, unclaimed entries of slot s (may dip below 0)
 };
 
 // One 2048-edge piece of a high-degree vertex.

@@ -88,15 +88,12 @@ constexpr uint32_t kF32ZeroKey = 0x80000000u;  // key of +0.0f
 constexpr uint32_t kU32Inf = 0xFFFFFFFFu;
 
 __host__ __device__ inline uint32_t f32_to_key(float x) {
-  uint32_t b;
-  std::memcpy(&b, &x, 4);
+  const uint32_t b = __builtin_bit_cast(uint32_t, x);
   return (b & 0x80000000u) ? ~b : (b ^ 0x80000000u);
 }
 __host__ __device__ inline float key_to_f32(uint32_t k) {
   const uint32_t b = (k & 0x80000000u) ? (k ^ 0x80000000u) : ~k;
-  float x;
-  std::memcpy(&x, &b, 4);
-  return x;
+  return __builtin_bit_cast(float, b);
 }
 
 struct WF32 {

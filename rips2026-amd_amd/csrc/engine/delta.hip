@@ -23,7 +23,8 @@
 
 namespace mi355 {
 
-constexpr int kSlots = 32;
+constexpr int kSlots = kBins;
+constexpr int kSlotStride = 32;  // words between slot counters (one 128-byte line)
 constexpr int kNoBucket = 0x3fffffff;
 
 enum DsStep : uint32_t {
@@ -46,23 +47,26 @@ struct BucketCtrl {
   uint32_t count;       // frontier entries to process
   uint32_t lstamp;      // light-queue dedup value of this step
   uint32_t far_sel;     // far pile read / written
-  // results of work kernels
-  uint32_t light_cnt[2];
-  uint32_t settled_cnt;
+  // results of work kernels; counters that the whole grid appends to sit on
+  // cache lines of their own
+  alignas(128) uint32_t light_cnt[2];
+  alignas(128) uint32_t settled_cnt;
   uint32_t heavy_done;
-  uint32_t chunk_cnt;
-  uint32_t slot_cnt[kSlots];
-  uint32_t far_cnt[2];
+  alignas(128) uint32_t chunk_cnt;
+  alignas(128) uint32_t far_cnt[2];
   int32_t far_min;
   uint32_t err;
+  alignas(128) uint32_t slot_cnt[kSlots * kSlotStride];  // slot s at [s * kSlotStride]
   // bookkeeping
-  uint32_t buckets_done;
+  alignas(128) uint32_t buckets_done;
   uint32_t steps;
   uint32_t done;
   uint32_t converged;
   uint32_t prev_kind;
   unsigned long long edges;
 };
+
+#define SLOT_CNT(c, s) ((c)->slot_cnt[(s) * kSlotStride])
 
 struct DsArrays {
   const int* rowptr;
@@ -83,6 +87,7 @@ struct DsArrays {
   float inv_delta;
   uint32_t delta_bits;
   int want_work;
+  const int* split;    // light edges per vertex (rows are sorted by weight)
 };
 
 __device__ inline int bucket_of(float d, float inv_delta) {
@@ -100,6 +105,15 @@ __device__ inline int bucket_of_key(uint32_t key, float inv_delta) {
   return bucket_of(W::bucket_value(key), inv_delta);
 }
 
+// LDS backing of one wave's BinStage.
+struct DsSlotLds {
+  int v[kStageCap];
+  uint16_t bin[kStageCap];
+  uint16_t off[kStageCap];
+  uint32_t hist[kBins];
+  __device__ BinStage stage() { return BinStage{v, bin, off, hist, 0u}; }
+};
+
 template <class W>
 struct DsSink {
   const DsArrays* a;
@@ -115,6 +129,7 @@ struct DsSink {
   int* stage_light = nullptr;    // LDS per wave
   int* stage_settled = nullptr;  // LDS per wave
   uint32_t n_light = 0, n_settled = 0;
+  BinStage slot_stage{};         // LDS per wave: future-bucket appends
 
   __device__ inline void vertex(bool valid, int u, uint32_t) {
     bool push = false;
@@ -146,31 +161,15 @@ struct DsSink {
         }
       }
     }
-    // bucket-queue appends: one atomicAdd per distinct target slot in the wave
-    unsigned long long mask = ballot64(to_slot);
-    const unsigned lane = lane_id();
-    while (mask) {
-      const int leader = __ffsll((long long)mask) - 1;
-      const int bl = __shfl(b, leader, 64);
-      const bool mine = to_slot && b == bl;
-      const unsigned long long m2 = ballot64(mine);
-      const int s = bl % kSlots;
-      uint32_t pos0 = 0;
-      if ((int)lane == leader) pos0 = atomicAdd(&c->slot_cnt[s], (uint32_t)__popcll(m2));
-      pos0 = __shfl(pos0, leader, 64);
-      if (mine) {
-        const uint32_t pos = pos0 + prefix_count(m2);
-        if ((int64_t)pos < a->slot_cap) a->slots[(int64_t)s * a->slot_cap + pos] = (int)v;
-        else atomicOr(&c->err, 2u);
-      }
-      mask &= ~m2;
-    }
+    bin_push(to_slot, (int)v, (uint32_t)(b % kSlots), slot_stage, c->slot_cnt, kSlotStride, a->slots,
+             a->slot_cap, a->slot_cap, &c->err);
     stage_push(to_light, (int)v, stage_light, n_light, lq_out, lq_cnt, a->cap, &c->err);
   }
 
   __device__ inline void flush() {
     stage_flush(stage_light, n_light, lq_out, lq_cnt, a->cap, &c->err);
     stage_flush(stage_settled, n_settled, a->settled, &c->settled_cnt, a->cap, &c->err);
+    bin_flush(slot_stage, c->slot_cnt, kSlotStride, a->slots, a->slot_cap, a->slot_cap, &c->err);
   }
 };
 
@@ -178,7 +177,7 @@ struct DsSink {
 __device__ inline void ds_advance(BucketCtrl* c) {
   // next bucket with a non-empty window slot after B
   for (int b = c->bucket + 1; b < c->bucket + kSlots; ++b) {
-    if (c->slot_cnt[b % kSlots] > 0) {
+    if (SLOT_CNT(c, b % kSlots) > 0) {
       c->kind = kStepFilter;
       c->target = b;
       c->light_cnt[0] = 0;
@@ -254,7 +253,7 @@ __global__ void k_ds_control(BucketCtrl* __restrict__ c, int64_t max_buckets) {
       return;
     }
     case kStepFilter: {
-      c->slot_cnt[c->target % kSlots] = 0;
+      SLOT_CNT(c, c->target % kSlots) = 0;
       c->bucket = c->target;
       if (c->light_cnt[0] > 0) {
         if ((int64_t)c->buckets_done >= max_buckets && max_buckets >= 0) {
@@ -307,11 +306,14 @@ __global__ void k_ds_init(DsArrays a, BucketCtrl* __restrict__ c, int64_t n, uin
     a.lstamp[v] = 0u;
     a.farflag[v] = 0u;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.lq[0][0] = (int)src;
-    BucketCtrl z{};
-    *c = z;
-    c->kind = kStepInit;
+  if (blockIdx.x == 0) {
+    uint32_t* words = reinterpret_cast<uint32_t*>(c);
+    for (uint32_t i = threadIdx.x; i < sizeof(BucketCtrl) / 4; i += blockDim.x) words[i] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      a.lq[0][0] = (int)src;
+      c->kind = kStepInit;
+    }
   }
 }
 
@@ -319,6 +321,7 @@ template <class W>
 __global__ __launch_bounds__(kBlock) void k_ds_work(DsArrays a, BucketCtrl* __restrict__ c) {
   __shared__ int s_stage_l[kWavesPerBlock][kStageCap];
   __shared__ int s_stage_s[kWavesPerBlock][kStageCap];
+  __shared__ DsSlotLds s_slot[kWavesPerBlock];
   const uint32_t kind = c->kind;
   if (kind == kStepDone || kind == kStepInit) return;
   const int B = c->bucket;
@@ -331,11 +334,12 @@ __global__ __launch_bounds__(kBlock) void k_ds_work(DsArrays a, BucketCtrl* __re
                    kind == kStepHeavy, c->far_sel};
     sink.stage_light = s_stage_l[wv];
     sink.stage_settled = s_stage_s[wv];
+    sink.slot_stage = s_slot[wv].stage();
     ChunkQueue cq{a.chunks, &c->chunk_cnt, a.chunk_cap};
     const int* qin = kind == kStepLight ? a.lq[in_sel] : a.settled;
     relax_frontier_body<W, false>(a.rowptr, a.edges, a.labels, qin, nullptr, c->first, c->count, cq,
                                   kind == kStepLight ? kLightEdges : kHeavyEdges, a.delta_bits, sink,
-                                  work);
+                                  work, a.split);
     sink.flush();
     return;
   }
@@ -344,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_ds_work(DsArrays a, BucketCtrl* __re
   if (kind == kStepFilter) {
     const int b = c->target;
     const int s = b % kSlots;
-    const uint32_t cnt = c->slot_cnt[s];
+    const uint32_t cnt = SLOT_CNT(c, s);
     const int* slot = a.slots + (int64_t)s * a.slot_cap;
     const uint32_t rounds = div_up(cnt, stride) * stride;
     uint32_t staged = 0;
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_ds_work(DsArrays a, BucketCtrl* __re
         a.farflag[v] = 0u;
         if (atomicMin(a.bstamp + v, (uint32_t)b) > (uint32_t)b) {
           const int s2 = b % kSlots;
-          const uint32_t pos = atomicAdd(&c->slot_cnt[s2], 1u);
+          const uint32_t pos = atomicAdd(&SLOT_CNT(c, s2), 1u);
           if ((int64_t)pos < a.slot_cap) a.slots[(int64_t)s2 * a.slot_cap + pos] = v;
           else atomicOr(&c->err, 2u);
         }
@@ -405,14 +409,34 @@ __global__ __launch_bounds__(kBlock) void k_ds_chunks(DsArrays a, BucketCtrl* __
   const uint32_t out_sel = in_sel ^ 1u;
   __shared__ int s_stage_l[kWavesPerBlock][kStageCap];
   __shared__ int s_stage_s[kWavesPerBlock][kStageCap];
+  __shared__ DsSlotLds s_slot[kWavesPerBlock];
   DsSink<W> sink{&a, c, c->bucket, c->lstamp, a.lq[out_sel], &c->light_cnt[out_sel], false,
                  kind == kStepHeavy, c->far_sel};
   sink.stage_light = s_stage_l[threadIdx.x / kWave];
   sink.stage_settled = s_stage_s[threadIdx.x / kWave];
+  sink.slot_stage = s_slot[threadIdx.x / kWave].stage();
   ChunkQueue cq{a.chunks, &c->chunk_cnt, a.chunk_cap};
   relax_chunks_body<W>(a.edges, a.labels, cq, kind == kStepLight ? kLightEdges : kHeavyEdges,
                        a.delta_bits, sink, a.want_work ? &c->edges : nullptr);
   sink.flush();
+}
+
+// Light edges of every vertex: rows are sorted by weight, so they are the
+// prefix with w <= delta (the test of edge_selected); binary search per row.
+template <class W>
+__global__ void k_light_count(const int* __restrict__ rowptr, const uint2* __restrict__ edges,
+                              int64_t n, uint32_t delta_bits, int* __restrict__ split) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const int beg = rowptr[v];
+    int lo = beg, hi = rowptr[v + 1];  // first heavy edge in [lo, hi]
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (edge_selected<W>(kLightEdges, edges[mid].y, delta_bits)) lo = mid + 1;
+      else hi = mid;
+    }
+    split[v] = lo - beg;
+  }
 }
 
 template <class W>
@@ -457,6 +481,15 @@ void run_ds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
   a.inv_delta = 1.0f / delta;
   a.delta_bits = __builtin_bit_cast(uint32_t, delta);
   a.want_work = req.want_work ? 1 : 0;
+  if (!ws.split_valid || ws.split_delta_bits != a.delta_bits) {
+    ws.split.ensure((size_t)n);
+    k_light_count<W><<<std::min<unsigned>(div_up((uint64_t)n, kBlock), 8192u), kBlock, 0, s>>>(
+        g.rowptr.p, g.edges.p, n, a.delta_bits, ws.split.p);
+    MI_LAUNCH_CHECK();
+    ws.split_valid = true;
+    ws.split_delta_bits = a.delta_bits;
+  }
+  a.split = ws.split.p;
 
   const int64_t max_buckets = req.max_iters < 0 ? (int64_t)INT64_MAX : req.max_iters;
 

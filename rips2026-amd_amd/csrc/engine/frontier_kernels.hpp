@@ -51,7 +51,9 @@ struct BfSink {
 
 // Small/medium-degree frontier vertices: 64 per wave, edges packed across
 // lanes.  Frontier entries [first, n_in) are processed; cls selects all,
-// light or heavy edges.
+// light or heavy edges.  With `split` (rows sorted by weight, split[u] = light
+// edges of u) a class is a prefix or suffix of the row and only that range is
+// read.
 template <class W, bool JACOBI, class Sink>
 __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
                                            const uint2* __restrict__ edges,
@@ -60,7 +62,8 @@ __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
                                            const uint32_t* __restrict__ qkeys, uint32_t first,
                                            uint32_t n_in, ChunkQueue cq, int cls,
                                            uint32_t delta_bits, Sink& sink,
-                                           unsigned long long* __restrict__ work) {
+                                           unsigned long long* __restrict__ work,
+                                           const int* __restrict__ split = nullptr) {
   __shared__ int s_incl[kWavesPerBlock][kWave];
   __shared__ int s_base[kWavesPerBlock][kWave];
   __shared__ uint32_t s_key[kWavesPerBlock][kWave];
@@ -80,7 +83,12 @@ __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
       u = qin[i];
       key = JACOBI ? qkeys[i] : atomic_load_key(labels + u);
       beg = rowptr[u];
-      const int end = rowptr[u + 1];
+      int end = rowptr[u + 1];
+      if (split && cls != kAllEdges) {
+        const int l = split[u];
+        if (cls == kLightEdges) end = beg + l;
+        else beg += l;
+      }
       if (key != W::kInfKey) {
         deg = end - beg;
         if (deg > kLargeDegree) {

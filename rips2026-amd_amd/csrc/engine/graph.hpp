@@ -39,6 +39,9 @@ struct Workspace {
   DevBuf<int> settled;                // vertices processed in the current bucket
   DevBuf<uint32_t> sstamp;            // settled-list dedup
   DevBuf<unsigned char> bctrl_mem;    // BucketCtrl
+  DevBuf<int> split;                  // light edges per vertex for split_delta_bits
+  uint32_t split_delta_bits = 0;
+  bool split_valid = false;
   // ADDS state (allocated on first use)
   DevBuf<int> adds_ring;              // kAddsSlots bucket rings
   bool adds_ring_clean = false;       // ring holds only empty sentinels

@@ -27,6 +27,12 @@ DeviceProps device_props() {
 
 namespace {
 
+unsigned grid_for(int64_t items) {
+  const int64_t b = (items + kBlock - 1) / kBlock;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
+
+
 enum : uint32_t {
   kBadRowptr = 1u,
   kBadCol = 2u,
@@ -113,6 +119,73 @@ __global__ void k_offsets_from_sorted(const uint32_t* __restrict__ keys, int* __
   }
 }
 
+// Row of every edge: the last u with rowptr[u] <= e (thread per edge, binary
+// search, so hub rows do not serialise on one thread).
+__global__ void k_rowid_search(const int* __restrict__ rowptr, uint32_t* __restrict__ rowid,
+                               int64_t n, int64_t m) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride) {
+    int64_t lo = 0, hi = n;  // rowptr[lo] <= e < rowptr[hi]
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (rowptr[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    rowid[e] = (uint32_t)lo;
+  }
+}
+
+// Sort key of an edge: row in the high word, order-preserving weight key in
+// the low word (f32: sign-flipped bits; u32: the value itself).
+__global__ void k_row_weight_keys(const uint32_t* __restrict__ rowid, const uint2* __restrict__ edges,
+                                  int32_t wtype, unsigned long long* __restrict__ keys,
+                                  uint32_t* __restrict__ cols, int64_t m) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride) {
+    const uint2 ed = edges[e];
+    const uint32_t wk = wtype == MI355_W_F32 ? f32_to_key(__uint_as_float(ed.y)) : ed.y;
+    keys[e] = ((unsigned long long)rowid[e] << 32) | wk;
+    cols[e] = ed.x;
+  }
+}
+
+__global__ void k_row_weight_gather(const unsigned long long* __restrict__ keys,
+                                    const uint32_t* __restrict__ cols, int32_t wtype,
+                                    uint2* __restrict__ edges, int64_t m) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride) {
+    const uint32_t wk = (uint32_t)keys[e];
+    edges[e] = make_uint2(cols[e], wtype == MI355_W_F32 ? __float_as_uint(key_to_f32(wk)) : wk);
+  }
+}
+
+// Order every row's edges by weight (stable: equal weights keep their column
+// order).  Distances and canonical predecessors do not depend on the order of
+// a row; a weight-sorted row turns delta-stepping's light/heavy edge classes
+// into a prefix and a suffix of the row.
+void sort_rows_by_weight(mi355_graph& g, hipStream_t stream) {
+  const int64_t n = g.n, m = g.m;
+  if (m == 0) return;
+  DevBuf<uint32_t> rowid((size_t)m), cols_in((size_t)m), cols_out((size_t)m);
+  DevBuf<unsigned long long> keys_in((size_t)m), keys_out((size_t)m);
+  k_rowid_search<<<grid_for(m), kBlock, 0, stream>>>(g.rowptr.p, rowid.p, n, m);
+  k_row_weight_keys<<<grid_for(m), kBlock, 0, stream>>>(rowid.p, g.edges.p, g.weight_type, keys_in.p,
+                                                        cols_in.p, m);
+  MI_LAUNCH_CHECK();
+  int bits = 1;
+  while (bits < 32 && ((int64_t)1 << bits) < n) ++bits;
+  size_t temp_bytes = 0;
+  MI_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys_in.p, keys_out.p, cols_in.p, cols_out.p,
+                                   (size_t)m, 0, 32 + bits, stream));
+  DevBuf<unsigned char> temp(temp_bytes);
+  MI_HIP(rocprim::radix_sort_pairs(temp.p, temp_bytes, keys_in.p, keys_out.p, cols_in.p, cols_out.p,
+                                   (size_t)m, 0, 32 + bits, stream));
+  k_row_weight_gather<<<grid_for(m), kBlock, 0, stream>>>(keys_out.p, cols_out.p, g.weight_type,
+                                                          g.edges.p, m);
+  MI_LAUNCH_CHECK();
+  MI_HIP(hipStreamSynchronize(stream));
+}
+
 // Per-block partial statistics: [0] sum of positive finite weights (double),
 // [1] their count, [2] max degree, [3] large vertices, [4] large chunks.
 __global__ void k_stats(const int* __restrict__ rowptr, const uint2* __restrict__ edges, int64_t n,
@@ -160,11 +233,6 @@ __global__ void k_stats(const int* __restrict__ rowptr, const uint2* __restrict_
     part_sum[blockIdx.x] = s_sum[0];
     for (int k = 0; k < 4; ++k) part_cnt[blockIdx.x * 4 + k] = s_c[k][0];
   }
-}
-
-unsigned grid_for(int64_t items) {
-  const int64_t b = (items + kBlock - 1) / kBlock;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
 }
 
 }  // namespace
@@ -275,6 +343,8 @@ mi355_graph* build_graph(const mi355_csr_desc& d, int allow_negative, hipStream_
       MI_LAUNCH_CHECK();
       MI_HIP(hipStreamSynchronize(stream));
     }
+
+    sort_rows_by_weight(*g, stream);
 
     // statistics (deterministic: fixed grid, fixed-order reductions)
     const unsigned sb = 512;

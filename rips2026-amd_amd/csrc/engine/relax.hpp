@@ -85,6 +85,69 @@ __device__ inline void stage_push(bool push, int v, int* __restrict__ stage, uin
   staged += k;
 }
 
+// Per-wave LDS staging of appends into kBins queues (bucket windows).  Each
+// staged entry keeps its vertex and bin; a flush counts the bins in LDS,
+// reserves every non-empty bin's range with one global atomicAdd on that
+// bin's counter (counters are cnt_stride words apart, so they sit on
+// separate cache lines), then writes the entries.
+constexpr int kBins = 32;
+
+struct BinStage {
+  int* v;          // kStageCap vertices
+  uint16_t* bin;   // kStageCap bins
+  uint16_t* off;   // kStageCap offsets within the bin (flush only)
+  uint32_t* hist;  // kBins counts, then bases
+  uint32_t n;      // wave-uniform
+};
+
+__device__ inline void bin_flush(BinStage& st, uint32_t* __restrict__ cnt, int cnt_stride,
+                                 int* __restrict__ q, int64_t q_stride, int64_t cap,
+                                 uint32_t* __restrict__ err) {
+  if (st.n == 0) return;
+  const unsigned lane = lane_id();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < (unsigned)kBins) st.hist[lane] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t i = lane; i < st.n; i += kWave) st.off[i] = (uint16_t)atomicAdd(st.hist + st.bin[i], 1u);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < (unsigned)kBins) {
+    const uint32_t k = st.hist[lane];
+    if (k) st.hist[lane] = atomicAdd(cnt + (int64_t)lane * cnt_stride, k);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t i = lane; i < st.n; i += kWave) {
+    const uint32_t b = st.bin[i];
+    const int64_t pos = (int64_t)st.hist[b] + st.off[i];
+    if (pos < cap) q[(int64_t)b * q_stride + pos] = st.v[i];
+    else atomicOr(err, 2u);
+  }
+  __builtin_amdgcn_wave_barrier();
+  st.n = 0;
+}
+
+__device__ inline void bin_push(bool push, int v, uint32_t bin, BinStage& st,
+                                uint32_t* __restrict__ cnt, int cnt_stride, int* __restrict__ q,
+                                int64_t q_stride, int64_t cap, uint32_t* __restrict__ err) {
+  const unsigned long long mask = ballot64(push);
+  if (mask == 0) return;
+  const uint32_t k = (uint32_t)__popcll(mask);
+  if (st.n + k > kStageCap) bin_flush(st, cnt, cnt_stride, q, q_stride, cap, err);
+  if (push) {
+    const uint32_t i = st.n + prefix_count(mask);
+    st.v[i] = v;
+    st.bin[i] = (uint16_t)bin;
+  }
+  st.n += k;
+}
+
 // Edge filter for delta-stepping's light/heavy split (w <= delta is light,
 // DS/src/delta_stepping_hip_CSR.cpp:402, :440).
 enum EdgeClass { kAllEdges = 0, kLightEdges = 1, kHeavyEdges = 2 };

@@ -221,3 +221,43 @@ def test_delta_stepping_csr_iteration_cap():
     with pytest.raises(M.Mi355Error) as e:
         M.delta_stepping_csr(gi, 512, 1.0)
     assert e.value.code == -2
+
+
+def _weight_keys(wbits, wtype):
+    b = np.asarray(wbits, np.uint32)
+    if wtype != M.W_F32:
+        return b.astype(np.uint64)
+    return np.where(b & 0x80000000, ~b, b ^ 0x80000000).astype(np.uint64)
+
+
+@pytest.mark.parametrize("wtype", [M.W_F32, M.W_U32])
+def test_engine_rows_sorted_by_weight(wtype):
+    """The engine keeps every row ordered by weight, ties in column order
+    (delta-stepping reads light edges as a row prefix), from either input
+    orientation, and SSSP results do not depend on it."""
+    rng = np.random.default_rng(17)
+    n, m = 300, 4000
+    s = rng.integers(0, n, m)
+    d = rng.integers(0, n, m)
+    if wtype == M.W_F32:
+        w = rng.choice(np.array([-1.5, -0.0, 0.0, 0.25, 0.5, 2.0], np.float32), m)
+    else:
+        w = rng.choice(np.array([0, 1, 7, 7, 100], np.uint32), m)
+    for orient in (M.OUT, M.IN):
+        g = M.from_edges(n, s, d, w, wtype, orientation=orient)
+        dg = M.DeviceGraph(g, allow_negative=True)
+        rp, col, wb = dg.download()
+        go = g if orient == M.OUT else g.transpose()
+        assert rp.tolist() == go.rowptr.tolist()
+        rows = np.repeat(np.arange(n, dtype=np.uint64), np.diff(go.rowptr))
+        key_in = (rows << np.uint64(32)) | _weight_keys(go.wbits, wtype)
+        key_out = (rows << np.uint64(32)) | _weight_keys(wb, wtype)
+        assert np.all(key_out[1:] >= key_out[:-1])
+        # same (column, weight) multiset in every row
+        pin = np.lexsort((go.col, key_in))
+        pout = np.lexsort((col, key_out))
+        assert np.array_equal(key_in[pin], key_out[pout])
+        assert np.array_equal(go.col[pin], col[pout])
+        if orient == M.OUT:
+            order = np.argsort(key_in, kind="stable")  # ties keep the input column order
+            assert np.array_equal(col, go.col[order])
