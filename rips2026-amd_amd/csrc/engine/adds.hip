@@ -66,6 +66,7 @@ constexpr uint32_t kNoStamp = 0xFFFFFFFFu;
 constexpr uint32_t kClaim = 256;    // items per workgroup claim
 constexpr int kEmptyEntry = -1;     // ring sentinel: reserved, not yet written
 constexpr uint32_t kPushStage = 512;  // staged pushes per wave
+constexpr uint32_t kHeavyStage = 256; // staged heavy-list entries per wave
 // Bound of every spin (ring entry, hub chunk, idle worker): far above any
 // legitimate wait; reaching it aborts the run with an error instead of
 // hanging the device.
@@ -78,9 +79,23 @@ constexpr uint32_t kErrRing = 1u, kErrHub = 2u, kErrStall = 4u;
 #define TAIL(c, s) ((c)->tail[(s) * kPad])
 #define INFL(c, s) ((c)->inflight[(s) * kPad])
 #define AVAIL(c, s) ((c)->avail[(s) * kPad])
+#define HV_HEAD(c, s) ((c)->hv_head[(s) * kPad])
+#define HV_TAIL(c, s) ((c)->hv_tail[(s) * kPad])
+#define HV_AVAIL(c, s) ((c)->hv_avail[(s) * kPad])
 
-__device__ inline int atomic_add_sc(int* p, int v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+// Wait until this wave's outstanding vector memory operations have completed
+// (s_waitcnt vmcnt(0)).  Device-scope atomics, and loads and stores with the
+// device coherence bit, are performed at the device's coherence point, so
+// completion in issue order is all the ordering the worklist needs.  Release
+// and acquire at agent scope would write back or invalidate the whole L2
+// instead.
+__device__ inline void vm_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Relaxed device-scope add that has completed when it returns.
+__device__ inline int atomic_add_done(int* p, int v) {
+  const int old = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  vm_wait();
+  return old;
 }
 
 __device__ inline unsigned long long pack_base(uint32_t base, float t) {
@@ -90,12 +105,12 @@ __device__ inline uint32_t base_of(unsigned long long w) { return (uint32_t)w; }
 __device__ inline float tbase_of(unsigned long long w) { return __uint_as_float((uint32_t)(w >> 32)); }
 
 template <class T>
-__device__ inline T ld_acq(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class T>
 __device__ inline T ld_rlx(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline void st_rlx(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Bucket of distance d under the mapping (base, T_base, 1/delta), clamped to
@@ -117,7 +132,10 @@ struct AddsArgs {
   unsigned long long* labels;
   uint32_t* qstamp;
   int* ring;          // kAddsSlots * (cap_mask + 1)
+  int* hring;         // heavy lists, same shape
   uint32_t cap_mask;
+  uint32_t* sstamp;   // heavy-list stamp: bucket + 1 of the list holding v, 0 none
+  const int* split;   // light edges per vertex (weight-sorted rows)
   HubChunk* hub;      // hub_mask + 1 records
   uint32_t hub_mask;
   AddsCtrl* c;
@@ -151,7 +169,7 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   // count first: pending covers the entries before anyone can claim them
   if (lane == 0) atomicAdd(&c->pending, (unsigned long long)st.n);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // pending before the reservations
+  vm_wait();  // pending covers the entries before their reservations exist
   if (lane < (unsigned)kAddsSlots) st.hist[lane] = 0;
   __builtin_amdgcn_wave_barrier();
   for (uint32_t i = lane; i < st.n; i += kWave) {
@@ -185,11 +203,12 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
     const uint32_t s = so >> 16;
     ring_store(a, s, st.hist[s] + (so & 0xFFFFu), st.v[i]);
   }
-  // entries written before they are offered to claimers
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  // entries written before they are offered to claimers (a claimer that is
+  // early waits on the sentinel anyway)
+  vm_wait();
   __builtin_amdgcn_wave_barrier();
   if (k) atomicAdd(&AVAIL(c, lane), (int)k);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  vm_wait();  // complete before the producer's claim is released
   st.n = 0;
 }
 
@@ -199,7 +218,7 @@ __device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp,
   bool want = false;
   // release: a popper that exchanges qstamp after this sees the new label
   if (imp)
-    want = __hip_atomic_fetch_min(a.qstamp + v, bkt, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) > bkt;
+    want = __hip_atomic_fetch_min(a.qstamp + v, bkt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > bkt;
   const unsigned long long mask = __ballot(want);
   if (mask == 0) return;
   const uint32_t k = (uint32_t)__popcll(mask);
@@ -211,6 +230,38 @@ __device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp,
   }
   st.n += k;
   pushed += k;
+}
+
+// Publish a wave's staged heavy-list entries (all of one bucket, list slot
+// hs): pending first, then the reservation, the entries, and the count that
+// offers them to claimers.
+__device__ inline void heavy_publish(const AddsArgs& a, int* stage, uint32_t& n, uint32_t hs) {
+  if (n == 0) return;
+  const unsigned lane = lane_id();
+  AddsCtrl* c = a.c;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  uint32_t pos0 = 0;
+  if (lane == 0) {
+    atomicAdd(&c->pending, (unsigned long long)n);
+    vm_wait();
+    pos0 = atomicAdd(&HV_TAIL(c, hs), n);
+    if ((int)(pos0 + n - ld_rlx(&HV_HEAD(c, hs))) > (int)(a.cap_mask + 1u)) {
+      atomicOr(&c->err, kErrRing);
+      atomicExch(&c->done, 1u);
+    }
+  }
+  pos0 = __shfl(pos0, 0, 64);
+  int* ring = a.hring + (size_t)hs * (a.cap_mask + 1u);
+  for (uint32_t i = lane; i < n; i += kWave)
+    __hip_atomic_store(ring + ((pos0 + i) & a.cap_mask), stage[i], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  vm_wait();
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) atomicAdd(&HV_AVAIL(c, hs), (int)n);
+  vm_wait();
+  n = 0;
 }
 
 // Wave-level phase advance; true when this wave moved the base.
@@ -226,11 +277,23 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
     // empty at the tail read), then inflight == 0 (no holder), then an
     // unchanged tail (nothing pushed in between) leave no entry and no holder
     // that could still add one.
-    const uint32_t h = __hip_atomic_load(&HEAD(c, s), __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t t = __hip_atomic_load(&TAIL(c, s), __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
-    const int inflight = h == t ? __hip_atomic_load(&INFL(c, s), __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT) : 1;
-    const uint32_t t2 = inflight == 0 ? __hip_atomic_load(&TAIL(c, s), __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT) : t + 1;
-    if (h == t && inflight == 0 && t2 == t) {
+    // The heavy list of bucket base - 1 is drained the same way: its claims
+    // hold inflight on slot base, and nothing is appended to it any more.
+    const uint32_t hs = (base + kAddsSlots - 1) % kAddsSlots;
+    const uint32_t h = __hip_atomic_load(&HEAD(c, s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t t = __hip_atomic_load(&TAIL(c, s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t hh = 0, ht = 0;
+    if (base > 0) {
+      hh = __hip_atomic_load(&HV_HEAD(c, hs), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ht = __hip_atomic_load(&HV_TAIL(c, hs), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const bool empty = h == t && hh == ht;
+    const int inflight = empty ? __hip_atomic_load(&INFL(c, s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 1;
+    bool same = false;
+    if (inflight == 0)
+      same = __hip_atomic_load(&TAIL(c, s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == t &&
+             (base == 0 || __hip_atomic_load(&HV_TAIL(c, hs), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ht);
+    if (empty && inflight == 0 && same) {
       const float delta = ld_rlx(&c->delta);
       ok = atomicCAS(&c->base_word, bw, pack_base(base + 1, tbase_of(bw) + delta)) == bw;
     }
@@ -331,9 +394,56 @@ __device__ inline void relax_wave(const AddsArgs& a, PushStage& st, int (&s_incl
   __builtin_amdgcn_wave_barrier();
 }
 
+enum ClaimKind : uint32_t { kClaimItems = 0, kClaimHub = 1, kClaimHeavy = 2 };
+
+// Cut the row range [beg, end) of u into hub chunks (pending work held on
+// inflight slot `slot`, pushes clamped to bucket >= lo).
+__device__ inline void make_hub_chunks(const AddsArgs& a, uint32_t u, uint32_t key, int beg, int end,
+                                       uint32_t slot, uint32_t lo) {
+  AddsCtrl* c = a.c;
+  const uint32_t nch = (uint32_t)((end - beg + kChunk - 1) / kChunk);
+  atomicAdd(&c->pending, (unsigned long long)nch);
+  atomicAdd(&INFL(c, slot), (int)nch);
+  const uint32_t pos = atomicAdd(&c->hub_reserve, nch);
+  if ((int)(pos + nch - ld_rlx(&c->hub_head)) > (int)(a.hub_mask + 1u)) {
+    atomicOr(&c->err, kErrHub);
+    atomicExch(&c->done, 1u);
+  }
+  for (uint32_t q = 0; q < nch; ++q) {
+    HubChunk* hc = a.hub + ((pos + q) & a.hub_mask);
+    const int e0 = beg + (int)q * kChunk;
+    st_rlx(&hc->u, u);
+    st_rlx(&hc->key, key);
+    st_rlx(&hc->first, (uint32_t)e0);
+    st_rlx(&hc->cnt, (uint32_t)(min(end, e0 + kChunk) - e0));
+    st_rlx(&hc->slot, slot);
+    st_rlx(&hc->bucket, lo);
+    vm_wait();  // fields before the sequence number that publishes them
+    st_rlx(&hc->seq, pos + q + 1u);
+  }
+  vm_wait();
+  atomicAdd(&c->hub_avail, (int)nch);
+  vm_wait();
+}
+
+// Claim from a (ring, head, avail) set through its counting semaphore while
+// holding inflight on slot `pin`; returns the count and sets *begin.
+__device__ inline uint32_t sem_claim(int* avail, uint32_t* head, int* infl, uint32_t* begin) {
+  atomic_add_done(infl, 1);  // before the claim is visible
+  const int old = atomic_add_done(avail, -(int)kClaim);
+  const int k = min(old, (int)kClaim);
+  if (k < (int)kClaim) atomic_add_done(avail, (int)kClaim - max(k, 0));  // give back the surplus
+  if (k <= 0) {
+    atomic_add_done(infl, -1);
+    return 0;
+  }
+  *begin = atomicAdd(head, (uint32_t)k);
+  return (uint32_t)k;
+}
+
 template <class W>
 __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
-  __shared__ uint32_t s_slot, s_begin, s_count, s_hub, s_done, s_base, s_active, s_hub_ok;
+  __shared__ uint32_t s_slot, s_qslot, s_begin, s_count, s_kind, s_done, s_base, s_active, s_hub_ok;
   __shared__ float s_tbase, s_inv;
   __shared__ int s_incl[kWavesPerBlock][kWave];
   __shared__ int s_beg[kWavesPerBlock][kWave];
@@ -343,52 +453,66 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
   __shared__ int s_pv[kWavesPerBlock][kPushStage];
   __shared__ uint32_t s_ps[kWavesPerBlock][kPushStage];
   __shared__ uint32_t s_ph[kWavesPerBlock][kAddsSlots];
+  __shared__ int s_hv[kWavesPerBlock][kHeavyStage];
   AddsCtrl* c = a.c;
   const unsigned lane = threadIdx.x & 63;
   const unsigned wv = threadIdx.x >> 6;
   PushStage st{s_pv[wv], s_ps[wv], s_ph[wv], 0};
+  uint32_t hv_n = 0;
   unsigned long long edges_local = 0, pushed_local = 0, items_local = 0;
   uint32_t idle = 0, backoff = 1;
   unsigned long long claims = 0;
-  unsigned long long d_cas = 0, d_idle = 0, d_spin = 0, d_hub = 0, d_drop = 0, d_req = 0;
+  unsigned long long d_cas = 0, d_idle = 0, d_spin = 0, d_hub = 0, d_drop = 0, d_req = 0, d_heavy = 0;
+  unsigned long long t_claim = 0, t_idle = 0, t_items = 0, t_hubs = 0, t_pub = 0, t_relax = 0;
 
   for (;;) {
+    const long long t_top = wall_clock64();
     if (wv == 0) {
-      uint32_t got = 0, slot = 0, begin = 0, hubc = 0, done = 0;
+      uint32_t got = 0, slot = 0, qslot = 0, begin = 0, kind = kClaimItems, done = 0;
       if (lane == 0) {
-        done = ld_acq(&c->done);
-        if (!done) {
-          // hub chunks first: they finish items already claimed
-          // counting-semaphore claims: one decrement, no retry loop, so
-          // claimers do not serialise on a CAS of the queue head
-          if (ld_acq(&c->hub_avail) > 0) {
-            if (atomicSub(&c->hub_avail, 1) >= 1) {
-              got = 1;
-              hubc = 1;
-              begin = atomicAdd(&c->hub_head, 1u);
-            } else {
-              atomicAdd(&c->hub_avail, 1);
-              ++d_cas;
-            }
+        done = ld_rlx(&c->done);
+        // hub chunks first (they finish claimed work), then the heavy list of
+        // the bucket just finished, then the active buckets.  Every claim is a
+        // counting-semaphore decrement: no retry loop for hundreds of
+        // workgroups to serialise on.
+        if (!done && ld_rlx(&c->hub_avail) > 0) {
+          if (atomicSub(&c->hub_avail, 1) >= 1) {
+            got = 1;
+            kind = kClaimHub;
+            begin = atomicAdd(&c->hub_head, 1u);
+          } else {
+            atomicAdd(&c->hub_avail, 1);
+            ++d_cas;
           }
         }
         if (!done && !got) {
-          const unsigned long long bw = ld_acq(&c->base_word);
+          const unsigned long long bw = ld_rlx(&c->base_word);
           const uint32_t base = base_of(bw);
+          if (base > 0) {
+            const uint32_t hs = (base - 1) % kAddsSlots;
+            if (ld_rlx(&HV_AVAIL(c, hs)) > 0) {
+              // pinned on slot base: bucket base cannot finish before the heavy
+              // edges of bucket base - 1 are relaxed
+              const uint32_t ps = base % kAddsSlots;
+              got = sem_claim(&HV_AVAIL(c, hs), &HV_HEAD(c, hs), &INFL(c, ps), &begin);
+              if (got) {
+                kind = kClaimHeavy;
+                slot = ps;
+                qslot = hs;
+              } else {
+                ++d_cas;
+              }
+            }
+          }
           const uint32_t A = max(1u, min(ld_rlx(&c->active), (uint32_t)kAddsSlots));
           for (uint32_t j = 0; j < A && !got; ++j) {
             const uint32_t s = (base + j) % kAddsSlots;
-            if (ld_acq(&AVAIL(c, s)) <= 0) continue;
-            atomic_add_sc(&INFL(c, s), 1);  // before the claim is visible
-            const int old = atomic_add_sc(&AVAIL(c, s), -(int)kClaim);
-            const int k = min(old, (int)kClaim);
-            if (k < (int)kClaim) atomic_add_sc(&AVAIL(c, s), (int)kClaim - max(k, 0));  // give back
-            if (k > 0) {
-              got = (uint32_t)k;
+            if (ld_rlx(&AVAIL(c, s)) <= 0) continue;
+            got = sem_claim(&AVAIL(c, s), &HEAD(c, s), &INFL(c, s), &begin);
+            if (got) {
               slot = s;
-              begin = atomicAdd(&HEAD(c, s), (uint32_t)k);
+              qslot = s;
             } else {
-              atomic_add_sc(&INFL(c, s), -1);
               ++d_cas;
             }
           }
@@ -408,12 +532,12 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         }
       }
       if (!got && !done) {
-        const unsigned long long pend = __shfl(lane == 0 ? ld_acq(&c->pending) : 0ull, 0, 64);
+        const unsigned long long pend = __shfl(lane == 0 ? ld_rlx(&c->pending) : 0ull, 0, 64);
         if (pend == 0) {
           if (lane == 0) atomicExch(&c->done, 1u);
           done = 1;
         } else {
-          const unsigned long long bw = __shfl(lane == 0 ? ld_acq(&c->base_word) : 0ull, 0, 64);
+          const unsigned long long bw = __shfl(lane == 0 ? ld_rlx(&c->base_word) : 0ull, 0, 64);
           if (try_advance(a, bw)) {
             idle = 0;
             backoff = 1;
@@ -437,9 +561,10 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         s_done = done;
         s_count = got;
         s_slot = slot;
+        s_qslot = qslot;
         s_begin = begin;
-        s_hub = hubc;
-        const unsigned long long cur = ld_acq(&c->base_word);
+        s_kind = kind;
+        const unsigned long long cur = ld_rlx(&c->base_word);
         s_base = base_of(cur);
         s_tbase = tbase_of(cur);
         s_inv = 1.0f / ld_rlx(&c->delta);
@@ -448,6 +573,9 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
     }
     __syncthreads();
     const uint32_t count = s_count;
+    const long long t_claimed = wall_clock64();
+    if (s_count) t_claim += t_claimed - t_top;
+    else t_idle += t_claimed - t_top;
     if (s_done) break;
     if (count == 0) {
       __syncthreads();
@@ -455,13 +583,14 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
     }
     const uint32_t base = s_base;
     const float tbase = s_tbase, inv = s_inv;
+    const uint32_t kind = s_kind;
 
-    if (s_hub) {
+    if (kind == kClaimHub) {
       // one hub chunk; wait until its producer has published it
       HubChunk* hc = a.hub + (s_begin & a.hub_mask);
       if (threadIdx.x == 0) {
         uint32_t spins = 0;
-        while (ld_acq(&hc->seq) != s_begin + 1u && ++spins < kSpinLimit) __builtin_amdgcn_s_sleep(1);
+        while (ld_rlx(&hc->seq) != s_begin + 1u && ++spins < kSpinLimit) __builtin_amdgcn_s_sleep(1);
         s_hub_ok = spins < kSpinLimit;
         if (!s_hub_ok) {
           atomicOr(&c->err, kErrHub);
@@ -470,9 +599,10 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       }
       __syncthreads();
       if (!s_hub_ok) break;
-      const uint32_t hu = hc->u, hkey = hc->key, hfirst = hc->first;
-      const uint32_t hcnt = min(hc->cnt, (uint32_t)kChunk);
-      const uint32_t hslot = hc->slot % kAddsSlots, hbucket = hc->bucket;
+      // coherent reads: the record slot may hold an older chunk in a cache
+      const uint32_t hu = ld_rlx(&hc->u), hkey = ld_rlx(&hc->key), hfirst = ld_rlx(&hc->first);
+      const uint32_t hcnt = min(ld_rlx(&hc->cnt), (uint32_t)kChunk);
+      const uint32_t hslot = ld_rlx(&hc->slot) % kAddsSlots, hbucket = ld_rlx(&hc->bucket);
       for (uint32_t t0 = wv * kWave; t0 < hcnt; t0 += kBlock) {
         const uint32_t e = hfirst + t0 + lane;
         bool imp = false;
@@ -492,23 +622,26 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       d_hub += threadIdx.x == 0;
       __syncthreads();
       if (threadIdx.x == 0) {
-        atomic_add_sc(&INFL(c, hslot), -1);
+        atomic_add_done(&INFL(c, hslot), -1);
         atomicAdd(&c->pending, ~0ull);
       }
+      t_hubs += wall_clock64() - t_claimed;
       continue;
     }
 
-    // regular items: wave w takes claim entries [w*64, w*64 + 64)
+    // items of a bucket ring or a heavy list: wave w takes claim entries
+    // [w*64, w*64 + 64)
+    const bool heavy = kind == kClaimHeavy;
     const uint32_t slot = s_slot, begin = s_begin;
-    int* ring = a.ring + (size_t)slot * (a.cap_mask + 1u);
+    int* ring = (heavy ? a.hring : a.ring) + (size_t)s_qslot * (a.cap_mask + 1u);
     const uint32_t i = wv * kWave + lane;
     int u = 0, beg = 0, deg = 0;
     uint32_t key = W::kInfKey, bk = 0;
     bool entry_ok = true;
-    bool requeue = false;
+    bool requeue = false, to_heavy = false;
     uint32_t want = 0;
     if (i < count) {
-      // The pusher reserved this position before writing it: wait for the
+      // The producer reserved this position before writing it: wait for the
       // write (bounded), then hand the entry back to the ring as empty.
       int* ep = ring + ((begin + i) & a.cap_mask);
       uint32_t spins = 0;
@@ -526,65 +659,86 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       }
     }
     if (i < count && entry_ok) {
-      const uint32_t old = __hip_atomic_exchange(a.qstamp + u, kNoStamp, __ATOMIC_ACQ_REL,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-      if (old != kNoStamp && old % kAddsSlots == slot) {
-        bk = old;
+      if (heavy) {
+        // Clearing the stamp first means a later light pass of u (after an
+        // improvement) lists u again, or this pass reads the label it set.
+        __hip_atomic_exchange(a.sstamp + u, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         key = atomic_load_key(a.labels + u);
-        want = map_bucket(W::bucket_value(key), base, tbase, inv, bk);
-        if (want >= bk + s_active) {
-          // clipped or re-mapped far item: queue it where it belongs now
-          key = W::kInfKey;
-          requeue = true;
-          ++d_req;
-        }
-      } else if (old != kNoStamp) {
-        atomicMin(a.qstamp + u, old);  // queued again elsewhere: restore, drop this entry
-        ++d_drop;
+        bk = base;
+        ++d_heavy;
       } else {
-        ++d_drop;
+        const uint32_t old = __hip_atomic_exchange(a.qstamp + u, kNoStamp, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+        if (old != kNoStamp && old % kAddsSlots == slot) {
+          bk = old;
+          key = atomic_load_key(a.labels + u);
+          want = map_bucket(W::bucket_value(key), base, tbase, inv, bk);
+          if (want >= bk + s_active) {
+            // clipped or re-mapped far item: queue it where it belongs now
+            key = W::kInfKey;
+            requeue = true;
+            ++d_req;
+          }
+        } else {
+          if (old != kNoStamp) atomicMin(a.qstamp + u, old);  // queued again elsewhere: restore
+          ++d_drop;
+        }
       }
       if (key != W::kInfKey) {
         ++items_local;
         beg = a.rowptr[u];
-        const int end = a.rowptr[u + 1];
+        int end = a.rowptr[u + 1];
+        const int split = beg + a.split[u];
+        if (heavy) {
+          beg = split;
+        } else {
+          // light prefix now; the heavy suffix once bucket bk is finished
+          if (split < end)
+            to_heavy = __hip_atomic_exchange(a.sstamp + u, bk + 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) != bk + 1u;
+          end = split;
+        }
         deg = end - beg;
         if (deg > kLargeDegree) {
-          // cut into hub chunks; each chunk is pending work held in this slot
-          const uint32_t nch = (uint32_t)((deg + kChunk - 1) / kChunk);
-          atomicAdd(&c->pending, (unsigned long long)nch);
-          atomicAdd(&INFL(c, slot), (int)nch);
-          const uint32_t pos = atomicAdd(&c->hub_reserve, nch);
-          if ((int)(pos + nch - ld_rlx(&c->hub_head)) > (int)(a.hub_mask + 1u)) {
-            atomicOr(&c->err, kErrHub);
-            atomicExch(&c->done, 1u);
-          }
-          for (uint32_t q = 0; q < nch; ++q) {
-            HubChunk* hc = a.hub + ((pos + q) & a.hub_mask);
-            const int e0 = beg + (int)q * kChunk;
-            hc->u = (uint32_t)u;
-            hc->key = key;
-            hc->first = (uint32_t)e0;
-            hc->cnt = (uint32_t)(min(end, e0 + kChunk) - e0);
-            hc->slot = slot;
-            hc->bucket = bk;
-            __hip_atomic_store(&hc->seq, pos + q + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-          }
-          __hip_atomic_fetch_add(&c->hub_avail, (int)nch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          make_hub_chunks(a, (uint32_t)u, key, beg, end, slot, bk);
           deg = 0;
         }
       }
     }
+    // heavy-list entries of this claim's bucket (list slot = bucket slot)
+    {
+      const unsigned long long hm = __ballot(to_heavy);
+      if (hm) {
+        const uint32_t k = (uint32_t)__popcll(hm);
+        if (hv_n + k > kHeavyStage) heavy_publish(a, s_hv[wv], hv_n, slot);
+        if (to_heavy) s_hv[wv][hv_n + prefix_count(hm)] = u;
+        hv_n += k;
+      }
+    }
     // re-queued items go through the stage like any push
     push_improved(a, st, requeue, (uint32_t)u, want, pushed_local);
+    const long long t_r0 = wall_clock64();
     relax_wave<W>(a, st, s_incl[wv], s_beg[wv], s_key[wv], s_u[wv], s_lo[wv], u, beg, deg, key, bk,
                   base, tbase, inv, edges_local, pushed_local);
+    const long long t_r1 = wall_clock64();
     stage_publish(a, st);
+    heavy_publish(a, s_hv[wv], hv_n, slot);
+    t_relax += t_r1 - t_r0;
+    t_pub += wall_clock64() - t_r1;
     __syncthreads();
     if (threadIdx.x == 0) {
-      atomic_add_sc(&INFL(c, slot), -1);
+      atomic_add_done(&INFL(c, slot), -1);
       atomicAdd(&c->pending, 0ull - (unsigned long long)count);
     }
+    t_items += wall_clock64() - t_claimed;
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&c->diag[8], t_claim);
+    atomicAdd(&c->diag[9], t_idle);
+    atomicAdd(&c->diag[10], t_items);
+    atomicAdd(&c->diag[11], t_hubs);
+    atomicAdd(&c->diag[12], t_pub);
+    atomicAdd(&c->diag[13], t_relax);
   }
   if (threadIdx.x == 0 && claims) atomicAdd(&c->diag[0], claims);
   if (d_cas) atomicAdd(&c->diag[1], d_cas);
@@ -593,6 +747,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
   if (d_hub) atomicAdd(&c->diag[4], d_hub);
   if (d_drop) atomicAdd(&c->diag[5], d_drop);
   if (d_req) atomicAdd(&c->diag[6], d_req);
+  if (d_heavy) atomicAdd(&c->diag[7], d_heavy);
   if (lane == 0) {
     if (!a.live_edges && edges_local) atomicAdd(&c->edges, edges_local);
     if (pushed_local) atomicAdd(&c->pushes, pushed_local);
@@ -611,6 +766,7 @@ __global__ void k_adds_init(AddsArgs a, int64_t n, uint32_t src, const uint32_t*
     if (init) l = ((unsigned long long)W::output_to_key(init[v]) << 32) | kNoPred;
     a.labels[v] = v == (int64_t)src ? (((unsigned long long)W::kZeroKey << 32) | kNoPred) : l;
     a.qstamp[v] = v == (int64_t)src ? 0u : kNoStamp;
+    a.sstamp[v] = 0u;
   }
   for (int64_t i = tid; i <= (int64_t)a.hub_mask; i += stride) a.hub[i].seq = 0u;
   const uint32_t* src_words = reinterpret_cast<const uint32_t*>(init_ctrl);
@@ -632,11 +788,11 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   uint32_t hub_cap = 1024;
   while ((int64_t)hub_cap < g.info.large_chunks * 2 + 1024) hub_cap <<= 1;
   ws.labels.ensure((size_t)n);
-  if (ws.adds_ring.n < (size_t)kAddsSlots * cap) {
-    ws.adds_ring.reset((size_t)kAddsSlots * cap);
+  if (ws.adds_ring.n < (size_t)2 * kAddsSlots * cap) {
+    ws.adds_ring.reset((size_t)2 * kAddsSlots * cap);
     ws.adds_ring_clean = false;
   }
-  ws.adds_qstamp.ensure((size_t)n);
+  ws.adds_qstamp.ensure((size_t)n * 2);
   ws.adds_hub.ensure(hub_cap);
   ws.adds_ctrl_mem.ensure(2 * sizeof(AddsCtrl));
   AddsCtrl* c = reinterpret_cast<AddsCtrl*>(ws.adds_ctrl_mem.p);
@@ -664,8 +820,12 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.edges = g.edges.p;
   a.labels = ws.labels.p;
   a.qstamp = ws.adds_qstamp.p;
+  a.sstamp = ws.adds_qstamp.p + n;
   a.ring = ws.adds_ring.p;
+  a.hring = ws.adds_ring.p + (size_t)kAddsSlots * cap;
   a.cap_mask = cap - 1;
+  // light/heavy split at the starting bucket width
+  a.split = light_split(g, delta, s);
   a.hub = ws.adds_hub.p;
   a.hub_mask = hub_cap - 1;
   a.c = c;
@@ -701,9 +861,12 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   if (std::getenv("MI355_ADDS_DEBUG"))
     std::fprintf(stderr,
                  "adds: %.3f ms phases %u items %llu edges %llu pushes %llu | claims %llu cas_fail %llu "
-                 "idle %llu ring_wait %llu hub %llu dropped %llu requeued %llu\n",
+                 "idle %llu ring_wait %llu hub %llu dropped %llu requeued %llu heavy %llu\n"
+                 "  per-workgroup ms: claim %.2f idle %.2f items %.2f hubs %.2f (publish %.2f relax %.2f)\n",
                  res.device_ms, h.phases, h.items, h.edges, h.pushes, h.diag[0], h.diag[1], h.diag[2],
-                 h.diag[3], h.diag[4], h.diag[5], h.diag[6]);
+                 h.diag[3], h.diag[4], h.diag[5], h.diag[6], h.diag[7], h.diag[8] * 1e-5 / workers,
+                 h.diag[9] * 1e-5 / workers, h.diag[10] * 1e-5 / workers, h.diag[11] * 1e-5 / workers,
+                 h.diag[12] * 1e-5 / workers, h.diag[13] * 1e-5 / workers);
   if (h.err & kErrRing) fail(MI355_ECAPACITY, "ADDS bucket queue overflow");
   if (h.err & kErrHub) fail(MI355_ECAPACITY, "ADDS hub chunk queue overflow");
   if (h.err & kErrStall) fail(MI355_EDEVICE, "ADDS workers stalled (aborted by the spin bound)");

@@ -43,14 +43,21 @@ struct AddsCtrl {
   uint32_t pad5b[kPad - 1];
   unsigned long long edges, pushes, items;
   uint32_t pad6[kPad - 6];
-  // diagnostics: claims, failed claim CAS, idle polls, ring-entry waits,
-  // hub chunks, stale entries dropped, re-queued entries, spare
-  unsigned long long diag[8];
-  uint32_t pad7[kPad - 16];
+  // diagnostics: claims, failed claims, idle polls, ring-entry waits, hub
+  // chunks, stale entries dropped, re-queued entries, heavy-list items; then
+  // workgroup-0-wave time (wall clock ticks) in successful claims, idle
+  // polling, item claims, hub chunks, publishing, relaxing
+  unsigned long long diag[16];
+  uint32_t pad7[kPad - 32];
   uint32_t head[kAddsSlots * kPad];  // pop counter of slot s at [s * kPad] (monotone)
   uint32_t tail[kAddsSlots * kPad];  // push counter of slot s
   int inflight[kAddsSlots * kPad];   // claims of slot s still being processed
   int avail[kAddsSlots * kPad];      // written, unclaimed entries of slot s (may dip below 0)
+  // heavy lists: vertices processed in bucket b whose heavy edges are still
+  // to be relaxed, in list slot b % kAddsSlots, claimable while base == b + 1
+  uint32_t hv_head[kAddsSlots * kPad];
+  uint32_t hv_tail[kAddsSlots * kPad];
+  int hv_avail[kAddsSlots * kPad];
 };
 
 // One 2048-edge piece of a high-degree vertex.

@@ -421,24 +421,6 @@ __global__ __launch_bounds__(kBlock) void k_ds_chunks(DsArrays a, BucketCtrl* __
   sink.flush();
 }
 
-// Light edges of every vertex: rows are sorted by weight, so they are the
-// prefix with w <= delta (the test of edge_selected); binary search per row.
-template <class W>
-__global__ void k_light_count(const int* __restrict__ rowptr, const uint2* __restrict__ edges,
-                              int64_t n, uint32_t delta_bits, int* __restrict__ split) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-    const int beg = rowptr[v];
-    int lo = beg, hi = rowptr[v + 1];  // first heavy edge in [lo, hi]
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (edge_selected<W>(kLightEdges, edges[mid].y, delta_bits)) lo = mid + 1;
-      else hi = mid;
-    }
-    split[v] = lo - beg;
-  }
-}
-
 template <class W>
 void run_ds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult& res,
                   hipStream_t s) {
@@ -481,15 +463,7 @@ void run_ds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
   a.inv_delta = 1.0f / delta;
   a.delta_bits = __builtin_bit_cast(uint32_t, delta);
   a.want_work = req.want_work ? 1 : 0;
-  if (!ws.split_valid || ws.split_delta_bits != a.delta_bits) {
-    ws.split.ensure((size_t)n);
-    k_light_count<W><<<std::min<unsigned>(div_up((uint64_t)n, kBlock), 8192u), kBlock, 0, s>>>(
-        g.rowptr.p, g.edges.p, n, a.delta_bits, ws.split.p);
-    MI_LAUNCH_CHECK();
-    ws.split_valid = true;
-    ws.split_delta_bits = a.delta_bits;
-  }
-  a.split = ws.split.p;
+  a.split = light_split(g, delta, s);
 
   const int64_t max_buckets = req.max_iters < 0 ? (int64_t)INT64_MAX : req.max_iters;
 
@@ -532,6 +506,23 @@ void run_ds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
   res.rounds = h.steps;
   res.edges = (int64_t)h.edges;
   res.initial_delta = res.final_delta = delta;
+}
+
+const int* light_split(mi355_graph& g, float delta, hipStream_t s) {
+  Workspace& ws = g.ws;
+  const uint32_t bits = __builtin_bit_cast(uint32_t, delta);
+  if (!ws.split_valid || ws.split_delta_bits != bits) {
+    ws.split.ensure((size_t)g.n);
+    const unsigned grid = std::min<unsigned>(div_up((uint64_t)g.n, kBlock), 8192u);
+    if (g.weight_type == MI355_W_F32)
+      k_light_count<WF32><<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, g.n, bits, ws.split.p);
+    else
+      k_light_count<WU32><<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, g.n, bits, ws.split.p);
+    MI_LAUNCH_CHECK();
+    ws.split_valid = true;
+    ws.split_delta_bits = bits;
+  }
+  return ws.split.p;
 }
 
 void run_delta_stepping(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,

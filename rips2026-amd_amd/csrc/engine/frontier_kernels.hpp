@@ -203,6 +203,24 @@ __global__ __launch_bounds__(kBlock) void k_relax_chunks(const uint2* __restrict
   sink.flush();
 }
 
+// Light edges of every vertex: rows are sorted by weight, so they are the
+// prefix with w <= delta (the test of edge_selected); binary search per row.
+template <class W>
+__global__ void k_light_count(const int* __restrict__ rowptr, const uint2* __restrict__ edges,
+                              int64_t n, uint32_t delta_bits, int* __restrict__ split) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const int beg = rowptr[v];
+    int lo = beg, hi = rowptr[v + 1];  // first heavy edge in [lo, hi]
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (edge_selected<W>(kLightEdges, edges[mid].y, delta_bits)) lo = mid + 1;
+      else hi = mid;
+    }
+    split[v] = lo - beg;
+  }
+}
+
 // Jacobi snapshot: keys of the next frontier as they stand after the round.
 __global__ void k_snapshot_keys(const int* __restrict__ q, const uint32_t* __restrict__ cnt,
                                 const unsigned long long* __restrict__ labels,

@@ -43,9 +43,9 @@ struct Workspace {
   uint32_t split_delta_bits = 0;
   bool split_valid = false;
   // ADDS state (allocated on first use)
-  DevBuf<int> adds_ring;              // kAddsSlots bucket rings
-  bool adds_ring_clean = false;       // ring holds only empty sentinels
-  DevBuf<uint32_t> adds_qstamp;
+  DevBuf<int> adds_ring;              // kAddsSlots bucket rings, then kAddsSlots heavy-list rings
+  bool adds_ring_clean = false;       // rings hold only empty sentinels
+  DevBuf<uint32_t> adds_qstamp;       // n queue stamps, then n heavy-list stamps
   DevBuf<HubChunk> adds_hub;
   DevBuf<unsigned char> adds_ctrl_mem;
   DevBuf<AddsPolicy> adds_policy;

@@ -54,6 +54,9 @@ void labels_to_output(mi355_graph& g, uint32_t source, int inf_mode, void* d_dis
                       unsigned long long* reach, hipStream_t stream);
 
 // Delta-stepping driver (delta.hip).
+// Light edges (w <= delta) of every vertex, a prefix of its weight-sorted row;
+// cached on the graph for the last delta asked for.
+const int* light_split(mi355_graph& g, float delta, hipStream_t stream);
 void run_delta_stepping(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
                         hipStream_t stream);
 
