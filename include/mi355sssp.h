@@ -47,6 +47,7 @@ int mi355_device_free(void* ptr);
 int mi355_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int mi355_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int mi355_memset_d8(void* dst, int value, size_t bytes, void* stream);
+int mi355_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int mi355_stream_create(void** stream);
 int mi355_stream_destroy(void* stream);
 int mi355_stream_sync(void* stream);
@@ -134,7 +135,8 @@ typedef struct {
   float initial_delta;
   float final_delta;
   float device_ms;          /* hipEvent time of the GPU work */
-  float reserved_f;
+  float kernel_ms;          /* hipEvent time of the persistent ADDS kernel alone (0 for the
+                               algorithms that run as a sequence of launches) */
   int32_t has_negative_cycle;
   int32_t reserved_i;
 } mi355_sssp_stats;

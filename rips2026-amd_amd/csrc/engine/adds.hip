@@ -14,40 +14,59 @@
 //    base + floor((d - T_base) / delta), clamped into the window (far items
 //    are clipped into the last slot and re-queued when they come up) and
 //    never below the bucket of the vertex that produced it.
-//  * Workers.  Every workgroup is a worker: wave 0 claims up to 256 items from
-//    the lowest non-empty slot of the active window [base, base + A) through
-//    the slot's counting semaphore `avail` (one fetch-sub, surplus given back,
-//    then a fetch-add on the slot head for the positions; no CAS retry loop
-//    for hundreds of workgroups to serialise on); the four waves relax them, 64 items per wave with
-//    their edges packed across the lanes as in the frontier kernels; vertices
-//    above kLargeDegree are cut into 2048-edge hub chunks any workgroup can
-//    claim.
+//  * Workers.  Every workgroup is a worker: wave 0 claims up to 256 items and
+//    the four waves relax them, 64 items per wave with their edges packed
+//    across the lanes as in the frontier kernels; vertices with more than
+//    kLargeDegree edges in the range being relaxed are cut into 2048-edge hub
+//    chunks any workgroup can claim.  Claims go through counting semaphores
+//    (`avail`: one fetch-sub, the surplus given back, then a fetch-add on the
+//    queue head for the positions), so hundreds of workgroups never retry a
+//    CAS on one queue head.  Order of preference: hub chunks, the heavy list
+//    of the bucket just finished, the active buckets [base, base + A).
+//  * Light and heavy edges.  Rows are sorted by weight, so the edges with
+//    w <= delta0 (the starting bucket width) are a row prefix.  Processing a
+//    vertex of bucket b relaxes that prefix and lists the vertex (once per
+//    bucket, stamp b + 1) for its heavy suffix, which is relaxed after bucket
+//    b is finished, while base == b + 1 (those claims hold slot b + 1 open).
+//    The suffix is relaxed from the label current at that time; a vertex
+//    improved later is processed and listed again, so every final label is
+//    relaxed over all its edges (the stamp is cleared before the label is
+//    read).  This is delta-stepping's light/heavy split made asynchronous:
+//    heavy edges are not re-relaxed every time a vertex is re-processed.
 //  * Pushes are staged per wave in LDS and published in batches: one atomic
 //    on `pending`, then one reservation per distinct target slot (an LDS
-//    histogram gives each entry its offset), then the entries.  A reserved
-//    entry is the sentinel -1 until written, so a claimer that gets there
-//    first waits for the write.
+//    histogram gives each entry its offset), then the entries, then `avail`.
+//    A reserved entry is the sentinel -1 until written, so a claimer that
+//    gets there first waits for the write.
 //  * Dedup.  qstamp[v] = smallest bucket v is queued in (atomicMin on push).
 //    A pop exchanges it with "none"; the returned stamp is the bucket the
 //    entry is processed in (if it falls in this slot), so every live entry is
-//    processed once and superseded entries are dropped.  The label read after
-//    the acquire-exchange sees every improvement that did not re-queue v, so
-//    each final distance is relaxed from.
+//    processed once and superseded entries are dropped.  The label is read
+//    after the exchange has returned, and every improvement completes its
+//    label atomicMin before it touches qstamp, so each final distance is
+//    relaxed from.
 //  * Phases.  The base advances (one 64-bit CAS of (base, T_base)) only when
-//    slot base is empty and no claim of it is still in flight, so no push can
-//    land behind the window.  The advancing wave evaluates the delta policy:
-//    fixed, the feedback rule of plain ADDS (grow delta when the window holds
-//    too little work for the workers, shrink it when it holds too much; after
+//    slot base and the heavy list of bucket base - 1 are empty and no claim
+//    holding slot base is in flight, so no push can land behind the window.
+//    The advancing wave evaluates the delta policy: fixed, the feedback rule
+//    of plain ADDS (grow delta when the window holds too little work for the
+//    workers, shrink it when it holds too much; after
 //    JAS/hip_adds_sssp.cpp:683-727), or the learned MLP, which reads 32
 //    features of the phase and returns log2 of the delta scale and the active
 //    bucket count.  Re-mapping changes priorities only; distances remain the
 //    exact fixpoint of the fp32 / uint32 left folds.
 //  * Termination.  `pending` counts items queued or being processed; a batch
-//    adds its count before its entries become visible and before its producer
+//    adds its count before its entries exist and before its producer
 //    finishes, a finished (or dropped) claim subtracts afterwards, so
 //    pending == 0 means there is no work anywhere.  Only running workgroups
 //    ever hold items, so idle workgroups (which back off exponentially) need
 //    not be co-resident for the run to make progress.
+//  * Memory ordering.  All shared words are device-scope atomics, and ring
+//    entries and hub records are written and read with device-coherent
+//    (relaxed atomic) stores and loads; where an order matters the wave waits
+//    for its outstanding memory operations (vm_wait) instead of using
+//    agent-scope release/acquire, which on this device writes back or
+//    invalidates the whole L2 and made the worklist 10x slower.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -841,20 +860,24 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
     ws.adds_ring_clean = true;
   }
 
-  hipEvent_t ev0, ev1;
+  hipEvent_t ev0, ev1, evk;
   MI_HIP(hipEventCreate(&ev0));
   MI_HIP(hipEventCreate(&ev1));
+  MI_HIP(hipEventCreate(&evk));
   MI_HIP(hipEventRecord(ev0, s));
   k_adds_init<W><<<std::min<unsigned>(div_up((uint64_t)std::max<int64_t>(n, hub_cap), kBlock), 4096u),
                    kBlock, 0, s>>>(a, n, src, req.init_dist, c_init);
   MI_LAUNCH_CHECK();
+  MI_HIP(hipEventRecord(evk, s));
   k_adds<W><<<workers, kBlock, 0, s>>>(a);
   MI_LAUNCH_CHECK();
   MI_HIP(hipEventRecord(ev1, s));
   MI_HIP(hipEventSynchronize(ev1));
   MI_HIP(hipEventElapsedTime(&res.device_ms, ev0, ev1));
+  MI_HIP(hipEventElapsedTime(&res.kernel_ms, evk, ev1));
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
+  (void)hipEventDestroy(evk);
   MI_HIP(hipMemcpyAsync(hbuf.data(), c, sizeof(AddsCtrl), hipMemcpyDeviceToHost, s));
   MI_HIP(hipStreamSynchronize(s));
   if (h.err) ws.adds_ring_clean = false;

@@ -106,6 +106,9 @@ int mi355_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
     MI_HIP(hipStreamSynchronize(as_stream(stream)));
   });
 }
+int mi355_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  return guard([&] { mi355::device_copy(dst, src, bytes, as_stream(stream)); });
+}
 int mi355_memset_d8(void* dst, int value, size_t bytes, void* stream) {
   return guard([&] { MI_HIP(hipMemsetAsync(dst, value, bytes, as_stream(stream))); });
 }
@@ -202,6 +205,7 @@ static void fill_stats(const mi355::RunResult& res, mi355_sssp_stats* st) {
   st->initial_delta = res.initial_delta;
   st->final_delta = res.final_delta;
   st->device_ms = res.device_ms;
+  st->kernel_ms = res.kernel_ms;
   st->has_negative_cycle = res.negative_cycle ? 1 : 0;
 }
 

@@ -36,6 +36,7 @@ struct RunResult {
   float initial_delta = 0.f;
   float final_delta = 0.f;
   float device_ms = 0.f;
+  float kernel_ms = 0.f;   // the persistent kernel alone (ADDS)
   bool negative_cycle = false;
   std::vector<uint32_t> frontier_log;  // BF: |F_r| for r = 0..rounds
 };
@@ -54,6 +55,9 @@ void labels_to_output(mi355_graph& g, uint32_t source, int inf_mode, void* d_dis
                       unsigned long long* reach, hipStream_t stream);
 
 // Delta-stepping driver (delta.hip).
+// Streaming device-to-device copy (copy.hip).
+void device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 // Light edges (w <= delta) of every vertex, a prefix of its weight-sorted row;
 // cached on the graph for the last delta asked for.
 const int* light_split(mi355_graph& g, float delta, hipStream_t stream);

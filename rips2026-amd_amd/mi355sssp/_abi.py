@@ -52,7 +52,7 @@ class SsspStats(C.Structure):
                 ("syncs", C.c_int32), ("edges_relaxed", C.c_int64),
                 ("vertices_processed", C.c_int64), ("reached", C.c_int64),
                 ("reached_edges", C.c_int64), ("initial_delta", C.c_float),
-                ("final_delta", C.c_float), ("device_ms", C.c_float), ("reserved_f", C.c_float),
+                ("final_delta", C.c_float), ("device_ms", C.c_float), ("kernel_ms", C.c_float),
                 ("has_negative_cycle", C.c_int32), ("reserved_i", C.c_int32)]
 
 
@@ -105,6 +105,7 @@ _ENGINE_SIGS = {
     "mi355_memcpy_h2d": (C.c_int, [P, P, C.c_size_t, P]),
     "mi355_memcpy_d2h": (C.c_int, [P, P, C.c_size_t, P]),
     "mi355_memset_d8": (C.c_int, [P, C.c_int, C.c_size_t, P]),
+    "mi355_memcpy_d2d": (C.c_int, [P, P, C.c_size_t, P]),
     "mi355_stream_create": (C.c_int, [C.POINTER(P)]),
     "mi355_stream_destroy": (C.c_int, [P]),
     "mi355_stream_sync": (C.c_int, [P]),

@@ -186,3 +186,23 @@ def test_hip_adds_warm_start_keeps_existing_distances():
     rc, dist, _ = M.hip_adds(g.rowptr, g.col, g.weights, 0, M.W_F32, o, init)
     assert rc == 0
     assert dist.tolist() == [0.0, 1.0, np.inf, np.inf]
+
+
+@pytest.mark.parametrize("nbytes,off", [(0, 0), (1, 0), (15, 0), (16, 0), (1 << 20, 0), ((1 << 20) + 7, 0),
+                                        (4099, 3)])
+def test_device_copy(nbytes, off):
+    """mi355_memcpy_d2d (the streaming copy kernel) copies every byte,
+    including unaligned ranges and tails, and nothing beyond."""
+    import ctypes as C
+    from mi355sssp._abi import check, engine
+
+    rng = np.random.default_rng(nbytes + off)
+    src = rng.integers(0, 256, nbytes + 64, dtype=np.uint8)
+    a = M.DeviceBuffer.from_array(src)
+    b = M.DeviceBuffer.from_array(np.full(nbytes + 64, 0xA5, np.uint8))
+    check(engine().mi355_memcpy_d2d(C.c_void_p(b.ptr.value + off), C.c_void_p(a.ptr.value + off),
+                                    nbytes, None))
+    check(engine().mi355_stream_sync(None))
+    got = b.to_array(np.uint8, nbytes + 64)
+    assert np.array_equal(got[off:off + nbytes], src[off:off + nbytes])
+    assert np.all(got[:off] == 0xA5) and np.all(got[off + nbytes:] == 0xA5)
