@@ -116,11 +116,12 @@ typedef struct {
   int32_t check_interval;   /* BF_JACOBI: test convergence when it % k == 0; 0 = never */
   float delta;              /* DS / ADDS bucket width; <= 0: heuristic 16*wbar/dbar */
   int32_t inf_mode;         /* MI355_INF_IEEE or MI355_INF_FLT_MAX (f32 outputs) */
-  int32_t dynamic_delta;    /* ADDS: adapt delta per phase */
-  int32_t use_mlp;          /* ADDS: ask the loaded MLP for delta per phase */
+  int32_t dynamic_delta;    /* ADDS: feedback delta per phase (0 = plain ADDS: fixed delta) */
+  int32_t use_mlp;          /* ADDS: the loaded delta model picks delta and active buckets */
   int32_t worker_blocks;    /* ADDS: persistent workgroups (0 = auto) */
   int32_t rounds_per_sync;  /* host sync cadence (0 = adaptive) */
-  int32_t reserved[7];
+  int32_t active_buckets;   /* ADDS: buckets served at once, 1..32 (0 = 1) */
+  int32_t reserved[6];
 } mi355_sssp_options;
 
 typedef struct {

@@ -166,9 +166,20 @@ struct AddsArgs {
   unsigned long long claim_limit;
 };
 
+// Write a ring entry.  The cell must hold the empty sentinel: its previous
+// occupant (one lap earlier) has been read and cleared by its consumer.  A
+// consumer that reads a cell before its own producer wrote it may take the
+// previous lap's value, and that lap's consumer then takes this one: each
+// value is still consumed once.  Overwriting an unread value would lose it,
+// so that case aborts the run (ring too small) instead.
+__device__ inline void ring_put(int* cell, int v, uint32_t* err, uint32_t* done) {
+  if (__hip_atomic_exchange(cell, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kEmptyEntry) {
+    atomicOr(err, kErrRing);
+    atomicExch(done, 1u);
+  }
+}
 __device__ inline void ring_store(const AddsArgs& a, uint32_t slot, uint32_t pos, int v) {
-  __hip_atomic_store(a.ring + (size_t)slot * (a.cap_mask + 1u) + (pos & a.cap_mask), v,
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  ring_put(a.ring + (size_t)slot * (a.cap_mask + 1u) + (pos & a.cap_mask), v, &a.c->err, &a.c->done);
 }
 
 // Per-wave push stage in LDS: vertex and target slot of each queued entry.
@@ -273,9 +284,7 @@ __device__ inline void heavy_publish(const AddsArgs& a, int* stage, uint32_t& n,
   }
   pos0 = __shfl(pos0, 0, 64);
   int* ring = a.hring + (size_t)hs * (a.cap_mask + 1u);
-  for (uint32_t i = lane; i < n; i += kWave)
-    __hip_atomic_store(ring + ((pos0 + i) & a.cap_mask), stage[i], __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t i = lane; i < n; i += kWave) ring_put(ring + ((pos0 + i) & a.cap_mask), stage[i], &c->err, &c->done);
   vm_wait();
   __builtin_amdgcn_wave_barrier();
   if (lane == 0) atomicAdd(&HV_AVAIL(c, hs), (int)n);
@@ -601,6 +610,8 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       continue;
     }
     const uint32_t base = s_base;
+    // absolute bucket of the slot this claim holds (the window starts at base)
+    const uint32_t held_bucket = base + (s_slot + kAddsSlots - base % kAddsSlots) % kAddsSlots;
     const float tbase = s_tbase, inv = s_inv;
     const uint32_t kind = s_kind;
 
@@ -682,14 +693,19 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         // Clearing the stamp first means a later light pass of u (after an
         // improvement) lists u again, or this pass reads the label it set.
         __hip_atomic_exchange(a.sstamp + u, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        vm_wait();  // the label is read after the stamp is cleared
         key = atomic_load_key(a.labels + u);
         bk = base;
         ++d_heavy;
       } else {
+        // Whoever takes the stamp processes the vertex.  (Putting back a
+        // stamp that belongs to an entry in another ring would race with that
+        // entry's own pop and could lose the vertex.)  The bucket is the one
+        // this claim holds; an entry whose stamp was already taken is stale.
         const uint32_t old = __hip_atomic_exchange(a.qstamp + u, kNoStamp, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT);
-        if (old != kNoStamp && old % kAddsSlots == slot) {
-          bk = old;
+        if (old != kNoStamp) {
+          bk = held_bucket;
           key = atomic_load_key(a.labels + u);
           want = map_bucket(W::bucket_value(key), base, tbase, inv, bk);
           if (want >= bk + s_active) {
@@ -699,7 +715,6 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
             ++d_req;
           }
         } else {
-          if (old != kNoStamp) atomicMin(a.qstamp + u, old);  // queued again elsewhere: restore
           ++d_drop;
         }
       }
@@ -829,7 +844,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   h.util_high = 0.85f;
   h.growth = 1.5f;
   h.shrink = 0.75f;
-  h.active = req.active_buckets > 0 ? (uint32_t)req.active_buckets : 1u;
+  h.active = (uint32_t)std::min(std::max(req.active_buckets, 1), kAddsSlots);
   h.workers = (uint32_t)workers;
   h.pending = 1;
   h.tail[0] = 1;

@@ -173,7 +173,7 @@ void mi355_sssp_options_default(mi355_sssp_options* opt) {
   opt->check_interval = 1;
   opt->delta = 0.0f;
   opt->inf_mode = MI355_INF_IEEE;
-  opt->dynamic_delta = 1;
+  opt->dynamic_delta = 0;
 }
 
 static mi355::RunRequest make_request(const mi355_sssp_options* opt, bool want_work) {
@@ -187,6 +187,7 @@ static mi355::RunRequest make_request(const mi355_sssp_options* opt, bool want_w
     r.dynamic_delta = opt->dynamic_delta;
     r.use_mlp = opt->use_mlp;
     r.worker_blocks = opt->worker_blocks;
+    r.active_buckets = opt->active_buckets;
   } else {
     r.algorithm = MI355_ALG_DELTA;
   }

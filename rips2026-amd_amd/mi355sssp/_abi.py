@@ -44,7 +44,7 @@ class SsspOptions(C.Structure):
                 ("check_interval", C.c_int32), ("delta", C.c_float), ("inf_mode", C.c_int32),
                 ("dynamic_delta", C.c_int32), ("use_mlp", C.c_int32),
                 ("worker_blocks", C.c_int32), ("rounds_per_sync", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("active_buckets", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class SsspStats(C.Structure):
