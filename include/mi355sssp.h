@@ -362,6 +362,54 @@ int mi355_hip_adds_sssp(int32_t weight_type, uint32_t num_vertices, uint32_t num
                         mi355_adds_stats* stats, const mi355_adds_options* options,
                         void* stream);
 
+/* ---- learned delta policy ---------------------------------------------
+ * No reference counterpart: the network replaces the fixed per-phase delta
+ * heuristics (JAS/hip_adds_sssp.cpp:683-727, ONE/sssp_hip.cpp:564-571,
+ * ADDSF/wl.h:349-410).  32 phase features -> 64 hidden (ReLU) -> 2 outputs:
+ * log2 of the bucket width relative to the heuristic delta0, and the number
+ * of buckets served at once.  The engine evaluates bf16-rounded weights with
+ * fp32 accumulation; training keeps fp32 master weights. */
+typedef struct {
+  float w1[64 * 32];       /* hidden x input, row-major */
+  float b1[64];
+  float w2[2 * 64];        /* output x hidden, row-major */
+  float b2[2];
+  float feat_mean[32];     /* inputs are normalised (x - mean) * inv_std; not trained */
+  float feat_inv_std[32];
+} mi355_mlp_params;
+
+typedef struct {
+  int32_t epochs;
+  int32_t batch;           /* samples per Adam step */
+  float lr, beta1, beta2, eps, weight_decay;
+  uint64_t seed;           /* sample order of each epoch */
+} mi355_mlp_train_options;
+
+void mi355_mlp_train_options_default(mi355_mlp_train_options* o);
+/* x: n x 32 features (host), out: n x 2 outputs (host). */
+int mi355_mlp_forward(const mi355_mlp_params* p, const float* x, int64_t n, float* out);
+/* Mean loss (1/n) sum w_s |o_s - y_s|^2 and its gradient (normalisation
+ * fields of grad are zero).  y: n x 2 targets, w: n sample weights or NULL. */
+int mi355_mlp_loss_grad(const mi355_mlp_params* p, const float* x, const float* y, const float* w,
+                        int64_t n, float* loss, mi355_mlp_params* grad);
+/* Adam on the device; p is updated in place; epoch_loss (host, may be NULL)
+ * receives the mean training loss of each epoch. */
+int mi355_mlp_train(mi355_mlp_params* p, const float* x, const float* y, const float* w, int64_t n,
+                    const mi355_mlp_train_options* o, float* epoch_loss);
+/* Load / drop the model used by mi355_sssp_options.use_mlp (process-wide). */
+int mi355_adds_mlp_set(const mi355_mlp_params* p);
+int mi355_adds_mlp_clear(void);
+
+/* One ADDS run that records, at every phase boundary, MI355_ADDS_TRACE_STRIDE
+ * floats: [0,32) the phase features (before the model's normalisation), 32
+ * items queued in the window, 33 the delta
+ * chosen for the next phase, 34 active buckets, 35 edges relaxed so far,
+ * 36 pushes so far.  trace: host array of cap_phases records; phases receives
+ * the number of phase boundaries (records beyond cap_phases are dropped). */
+#define MI355_ADDS_TRACE_STRIDE 40
+int mi355_adds_trace(mi355_graph* g, int32_t source, const mi355_sssp_options* opt, float* trace,
+                     int32_t cap_phases, int32_t* phases, mi355_sssp_stats* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -342,9 +342,9 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   float delta = ld_rlx(&c->delta);
   uint32_t active = ld_rlx(&c->active);
   const int mode = a.policy ? a.policy->mode : kPolicyFixed;
-  float feat = 0.0f;
+  float feat = 0.0f, raw = 0.0f;
   if (mode == kPolicyMlp || (a.policy && a.policy->trace))
-    mlp_features(a.policy, c, occ, window_items, delta, lane, &feat);
+    mlp_features(a.policy, c, occ, window_items, delta, lane, &feat, &raw);
   if (mode == kPolicyHeuristic) {
     const float capacity = (float)c->workers * (float)kClaim;
     if ((float)window_items < c->util_low * capacity) delta = fminf(delta * c->growth, c->max_delta);
@@ -359,7 +359,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   const uint32_t ph = ld_rlx(&c->phases);
   if (a.policy && a.policy->trace && ph < a.policy->trace_cap) {
     float* t = a.policy->trace + (size_t)ph * kTraceStride;
-    if (lane < (unsigned)kMlpIn) t[lane] = feat;
+    if (lane < (unsigned)kMlpIn) t[lane] = raw;
     if (lane == 0) {
       t[32] = (float)window_items;
       t[33] = delta;
@@ -837,9 +837,11 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   AddsCtrl& h = *reinterpret_cast<AddsCtrl*>(hbuf.data());
   h.base_word = 0;  // bucket 0 starts at distance 0.0f
   h.delta = delta;
-  h.base_delta = delta;
-  h.min_delta = delta / 64.0f;
-  h.max_delta = delta * 64.0f;
+  // the policies scale the heuristic width: the MLP's output and feature 15
+  // are log2(delta / delta0)
+  h.base_delta = hdelta;
+  h.min_delta = std::min(delta, hdelta) / 64.0f;
+  h.max_delta = std::max(delta, hdelta) * 64.0f;
   h.util_low = 0.35f;
   h.util_high = 0.85f;
   h.growth = 1.5f;

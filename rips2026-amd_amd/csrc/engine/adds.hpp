@@ -82,10 +82,12 @@ struct AddsPolicy {
 
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Feature i of the phase, held by lane i (< 32).  occ = items queued in slot
-// (base + lane) for lanes < kAddsSlots.
+// Feature i of the phase, held by lane i (< 32): *raw as computed, *feat
+// normalised for the network.  occ = items queued in slot (base + lane) for
+// lanes < kAddsSlots.
 __device__ inline void mlp_features(const AddsPolicy* p, const AddsCtrl* c, uint32_t occ,
-                                    uint32_t window_items, float delta, unsigned lane, float* feat) {
+                                    uint32_t window_items, float delta, unsigned lane, float* feat,
+                                    float* raw) {
   float x = 0.0f;
   if (lane < 14) x = p->graph_feat[lane];
   else if (lane == 14) x = log2f(1.0f + (float)window_items);
@@ -94,6 +96,7 @@ __device__ inline void mlp_features(const AddsPolicy* p, const AddsCtrl* c, uint
   // lanes 16..31 take the occupancy of window offsets 0..15
   const float shifted = __shfl(occ_log, (int)(lane - 16) & 63, 64);
   if (lane >= 16 && lane < 32) x = shifted;
+  *raw = lane < kMlpIn ? x : 0.0f;
   if (lane < kMlpIn) x = (x - p->feat_mean[lane]) * p->feat_inv_std[lane];
   *feat = lane < kMlpIn ? x : 0.0f;
 }

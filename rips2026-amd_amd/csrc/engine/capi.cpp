@@ -677,4 +677,81 @@ int mi355_hip_adds_sssp(int32_t weight_type, uint32_t num_vertices, uint32_t num
   return kDeviceError;
 }
 
+
+// ---- learned delta policy ---------------------------------------------------
+void mi355_mlp_train_options_default(mi355_mlp_train_options* o) {
+  if (!o) return;
+  *o = mi355_mlp_train_options{};
+  o->epochs = 60;
+  o->batch = 512;
+  o->lr = 2.0e-3f;
+  o->beta1 = 0.9f;
+  o->beta2 = 0.999f;
+  o->eps = 1.0e-8f;
+  o->weight_decay = 0.0f;
+  o->seed = 2026;
+}
+
+int mi355_mlp_forward(const mi355_mlp_params* p, const float* x, int64_t n, float* out) {
+  return guard([&] { mi355::mlp_forward(p, x, n, out, nullptr); });
+}
+
+int mi355_mlp_loss_grad(const mi355_mlp_params* p, const float* x, const float* y, const float* w,
+                        int64_t n, float* loss, mi355_mlp_params* grad) {
+  return guard([&] { mi355::mlp_loss_grad(p, x, y, w, n, loss, grad, nullptr); });
+}
+
+int mi355_mlp_train(mi355_mlp_params* p, const float* x, const float* y, const float* w, int64_t n,
+                    const mi355_mlp_train_options* o, float* epoch_loss) {
+  return guard([&] {
+    mi355_mlp_train_options d;
+    mi355_mlp_train_options_default(&d);
+    mi355::mlp_train(p, x, y, w, n, o ? *o : d, epoch_loss, nullptr);
+  });
+}
+
+int mi355_adds_mlp_set(const mi355_mlp_params* p) {
+  return guard([&] { mi355::set_mlp_params(p); });
+}
+
+int mi355_adds_mlp_clear(void) {
+  return guard([&] { mi355::clear_mlp(); });
+}
+
+int mi355_adds_trace(mi355_graph* g, int32_t source, const mi355_sssp_options* opt, float* trace,
+                     int32_t cap_phases, int32_t* phases, mi355_sssp_stats* stats, void* stream) {
+  return guard([&] {
+    if (!g) fail(MI355_EINVAL, "null graph");
+    if (source < 0 || source >= g->n) fail(MI355_ERANGE, "source vertex is outside the graph");
+    if (cap_phases < 0 || (cap_phases > 0 && !trace)) fail(MI355_EINVAL, "bad trace buffer");
+    MI_HIP(hipSetDevice(g->device));
+    const hipStream_t s = as_stream(stream);
+    mi355::RunRequest req = make_request(opt, true);
+    req.algorithm = MI355_ALG_ADDS;
+    mi355::DevBuf<float> d_trace((size_t)std::max(cap_phases, 1) * MI355_ADDS_TRACE_STRIDE);
+    MI_HIP(hipMemsetAsync(d_trace.p, 0, d_trace.n * sizeof(float), s));
+    req.trace = d_trace.p;
+    req.trace_cap = (uint32_t)std::max(cap_phases, 1);
+    mi355::RunResult res;
+    mi355::run_sssp(*g, (uint32_t)source, req, res, s);
+    const int64_t recorded = std::min<int64_t>(res.iterations_used, cap_phases);
+    if (recorded > 0)
+      MI_HIP(hipMemcpyAsync(trace, d_trace.p, (size_t)recorded * MI355_ADDS_TRACE_STRIDE * sizeof(float),
+                            hipMemcpyDeviceToHost, s));
+    mi355::DevBuf<unsigned long long> reach(2);
+    MI_HIP(hipMemsetAsync(reach.p, 0, 16, s));
+    mi355::labels_to_output(*g, (uint32_t)source, MI355_INF_IEEE, nullptr, nullptr, reach.p, s);
+    unsigned long long hr[2] = {0, 0};
+    MI_HIP(hipMemcpyAsync(hr, reach.p, 16, hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    if (phases) *phases = (int32_t)std::min<int64_t>(res.iterations_used, INT32_MAX);
+    if (stats) {
+      *stats = mi355_sssp_stats{};
+      fill_stats(res, stats);
+      stats->reached = (int64_t)hr[0];
+      stats->reached_edges = (int64_t)hr[1];
+    }
+  });
+}
+
 }  // extern "C"

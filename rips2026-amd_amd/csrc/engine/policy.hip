@@ -72,6 +72,11 @@ void set_mlp(const AddsPolicy& p) {
   g_mlp_loaded = true;
 }
 
+void clear_mlp() {
+  std::lock_guard<std::mutex> lk(g_mlp_mu);
+  g_mlp_loaded = false;
+}
+
 bool get_mlp(AddsPolicy* p) {
   std::lock_guard<std::mutex> lk(g_mlp_mu);
   if (g_mlp_loaded && p) *p = g_mlp;

@@ -68,6 +68,17 @@ void run_delta_stepping(mi355_graph& g, uint32_t source, const RunRequest& req, 
 // or learned MLP, plus the optional phase trace.
 const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hipStream_t stream);
 
+// Process-wide delta model (policy.hip) and its training kernels (mlp.hip).
+void set_mlp(const AddsPolicy& p);
+bool get_mlp(AddsPolicy* p);
+void clear_mlp();
+void set_mlp_params(const mi355_mlp_params* p);
+void mlp_forward(const mi355_mlp_params* p, const float* h_x, int64_t n, float* h_out, hipStream_t s);
+void mlp_loss_grad(const mi355_mlp_params* p, const float* h_x, const float* h_y, const float* h_w,
+                   int64_t n, float* loss, mi355_mlp_params* grad, hipStream_t s);
+void mlp_train(mi355_mlp_params* p, const float* h_x, const float* h_y, const float* h_w, int64_t n,
+               const mi355_mlp_train_options& o, float* h_epoch_loss, hipStream_t s);
+
 // Heuristic bucket width: 16 * mean positive weight / mean degree
 // (ONE/sssp_hip.cpp:59-81, JAS/hip_adds_sssp.cpp:1121).
 float heuristic_delta(const mi355_graph& g);

@@ -56,6 +56,20 @@ class SsspStats(C.Structure):
                 ("has_negative_cycle", C.c_int32), ("reserved_i", C.c_int32)]
 
 
+class MlpParams(C.Structure):
+    _fields_ = [("w1", C.c_float * (64 * 32)), ("b1", C.c_float * 64), ("w2", C.c_float * (2 * 64)),
+                ("b2", C.c_float * 2), ("feat_mean", C.c_float * 32), ("feat_inv_std", C.c_float * 32)]
+
+
+class MlpTrainOptions(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("weight_decay", C.c_float),
+                ("seed", C.c_uint64)]
+
+
+ADDS_TRACE_STRIDE = 40
+
+
 class BfProgress(C.Structure):
     _fields_ = [("iteration", C.c_int), ("max_iters", C.c_int),
                 ("convergence_checked", C.c_ubyte), ("changed", C.c_ubyte)]
@@ -122,6 +136,15 @@ _ENGINE_SIGS = {
     "mi355_delta_stepping_csr_f32": (C.c_int, [C.POINTER(CsrDesc), C.c_int, C.c_float, C.c_int, P,
                                                C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                PROGRESS_CB, P, P]),
+    "mi355_mlp_train_options_default": (None, [C.POINTER(MlpTrainOptions)]),
+    "mi355_mlp_forward": (C.c_int, [C.POINTER(MlpParams), P, I64, P]),
+    "mi355_mlp_loss_grad": (C.c_int, [C.POINTER(MlpParams), P, P, P, I64, C.POINTER(C.c_float),
+                                      C.POINTER(MlpParams)]),
+    "mi355_mlp_train": (C.c_int, [C.POINTER(MlpParams), P, P, P, I64, C.POINTER(MlpTrainOptions), P]),
+    "mi355_adds_mlp_set": (C.c_int, [C.POINTER(MlpParams)]),
+    "mi355_adds_mlp_clear": (C.c_int, []),
+    "mi355_adds_trace": (C.c_int, [P, I32, C.POINTER(SsspOptions), P, I32, C.POINTER(I32),
+                                   C.POINTER(SsspStats), P]),
 }
 
 _GRAPH_SIGS = {

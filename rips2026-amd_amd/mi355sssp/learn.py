@@ -1,0 +1,275 @@
+"""Learned delta policy for ADDS: the model, phase traces, trace datasets and
+training.  Every forward pass, gradient and optimiser step runs on the device
+through the C ABI (mi355_mlp_*); this module only moves arrays.
+
+The network maps the 32 features of a phase boundary to (log2 of the bucket
+width relative to the heuristic delta0, number of buckets served at once).
+It is trained by regression on traces that ADDS itself records: for each
+training graph ADDS is run under a grid of static (width scale, active
+buckets) settings, the fastest setting (kernel time) becomes the target of
+every phase record of that graph, and the work done and relaxations wasted
+of each setting are kept in the dataset's metadata.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from ._abi import (ADDS_TRACE_STRIDE, ALG_ADDS, MlpParams, MlpTrainOptions, SsspOptions, SsspStats,
+                   check, engine)
+
+N_IN, N_HID, N_OUT = 32, 64, 2
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_MODEL = os.path.join(_HERE, "data", "delta_mlp.json")
+
+# static (width scale, active buckets) settings searched per training graph
+SCALES = (0.25, 0.5, 1.0, 2.0, 4.0)
+ACTIVE = (1, 2, 4)
+SETTINGS = [(s, a) for a in ACTIVE for s in SCALES]
+
+
+def _f32(a, shape=None) -> np.ndarray:
+    out = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        out = out.reshape(shape)
+    return out
+
+
+class DeltaModel:
+    """fp32 master weights of the 32 -> 64 -> 2 network plus the input
+    normalisation (not trained)."""
+
+    def __init__(self, w1, b1, w2, b2, mean=None, inv_std=None):
+        self.w1 = _f32(w1, (N_HID, N_IN))
+        self.b1 = _f32(b1, (N_HID,))
+        self.w2 = _f32(w2, (N_OUT, N_HID))
+        self.b2 = _f32(b2, (N_OUT,))
+        self.mean = _f32(np.zeros(N_IN) if mean is None else mean, (N_IN,))
+        self.inv_std = _f32(np.ones(N_IN) if inv_std is None else inv_std, (N_IN,))
+
+    @classmethod
+    def init(cls, seed: int = 0) -> "DeltaModel":
+        rng = np.random.default_rng(seed)
+        w1 = rng.normal(0.0, math.sqrt(2.0 / N_IN), (N_HID, N_IN))
+        w2 = rng.normal(0.0, 0.1 * math.sqrt(1.0 / N_HID), (N_OUT, N_HID))
+        return cls(w1, np.zeros(N_HID), w2, np.array([0.0, 1.0]))
+
+    # -- conversions --------------------------------------------------------
+    def to_c(self) -> MlpParams:
+        p = MlpParams()
+        C.memmove(p.w1, self.w1.ctypes.data, self.w1.nbytes)
+        C.memmove(p.b1, self.b1.ctypes.data, self.b1.nbytes)
+        C.memmove(p.w2, self.w2.ctypes.data, self.w2.nbytes)
+        C.memmove(p.b2, self.b2.ctypes.data, self.b2.nbytes)
+        C.memmove(p.feat_mean, self.mean.ctypes.data, self.mean.nbytes)
+        C.memmove(p.feat_inv_std, self.inv_std.ctypes.data, self.inv_std.nbytes)
+        return p
+
+    @classmethod
+    def from_c(cls, p: MlpParams) -> "DeltaModel":
+        return cls(np.ctypeslib.as_array(p.w1).copy(), np.ctypeslib.as_array(p.b1).copy(),
+                   np.ctypeslib.as_array(p.w2).copy(), np.ctypeslib.as_array(p.b2).copy(),
+                   np.ctypeslib.as_array(p.feat_mean).copy(),
+                   np.ctypeslib.as_array(p.feat_inv_std).copy())
+
+    def flat(self) -> np.ndarray:
+        return np.concatenate([self.w1.ravel(), self.b1, self.w2.ravel(), self.b2])
+
+    def save(self, path: str, meta: dict | None = None) -> None:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        doc = {"w1": self.w1.tolist(), "b1": self.b1.tolist(), "w2": self.w2.tolist(),
+               "b2": self.b2.tolist(), "feat_mean": self.mean.tolist(),
+               "feat_inv_std": self.inv_std.tolist(), "meta": meta or {}}
+        with open(path, "w") as f:
+            json.dump(doc, f)
+
+    @classmethod
+    def load(cls, path: str) -> "DeltaModel":
+        with open(path) as f:
+            d = json.load(f)
+        m = cls(d["w1"], d["b1"], d["w2"], d["b2"], d["feat_mean"], d["feat_inv_std"])
+        m.meta = d.get("meta", {})
+        return m
+
+    # -- device compute -----------------------------------------------------
+    def forward(self, x) -> np.ndarray:
+        x = _f32(x, (-1, N_IN))
+        out = np.empty((x.shape[0], N_OUT), np.float32)
+        check(engine().mi355_mlp_forward(C.byref(self.to_c()), x.ctypes.data, x.shape[0],
+                                         out.ctypes.data))
+        return out
+
+    def loss_grad(self, x, y, w=None):
+        x = _f32(x, (-1, N_IN))
+        y = _f32(y, (-1, N_OUT))
+        wp = None if w is None else _f32(w, (-1,))
+        loss = C.c_float()
+        g = MlpParams()
+        check(engine().mi355_mlp_loss_grad(C.byref(self.to_c()), x.ctypes.data, y.ctypes.data,
+                                           None if wp is None else wp.ctypes.data, x.shape[0],
+                                           C.byref(loss), C.byref(g)))
+        return float(loss.value), DeltaModel.from_c(g)
+
+    def train(self, x, y, w=None, epochs: int = 60, batch: int = 512, lr: float = 2e-3,
+              weight_decay: float = 0.0, seed: int = 2026) -> np.ndarray:
+        x = _f32(x, (-1, N_IN))
+        y = _f32(y, (-1, N_OUT))
+        wp = None if w is None else _f32(w, (-1,))
+        o = MlpTrainOptions()
+        engine().mi355_mlp_train_options_default(C.byref(o))
+        o.epochs, o.batch, o.lr, o.weight_decay, o.seed = epochs, batch, lr, weight_decay, seed
+        p = self.to_c()
+        losses = np.zeros(epochs, np.float32)
+        check(engine().mi355_mlp_train(C.byref(p), x.ctypes.data, y.ctypes.data,
+                                       None if wp is None else wp.ctypes.data, x.shape[0],
+                                       C.byref(o), losses.ctypes.data))
+        t = DeltaModel.from_c(p)
+        self.w1, self.b1, self.w2, self.b2 = t.w1, t.b1, t.w2, t.b2
+        return losses
+
+    def install(self) -> None:
+        """Make this the model mi355_sssp uses when use_mlp is set."""
+        check(engine().mi355_adds_mlp_set(C.byref(self.to_c())))
+
+
+def clear_delta_model() -> None:
+    check(engine().mi355_adds_mlp_clear())
+
+
+def load_default_delta_model(install: bool = True) -> DeltaModel | None:
+    if not os.path.exists(DEFAULT_MODEL):
+        return None
+    m = DeltaModel.load(DEFAULT_MODEL)
+    if install:
+        m.install()
+    return m
+
+
+# -- traces -----------------------------------------------------------------
+def trace(dg, source: int, cap: int = 4096, **opts):
+    """One ADDS run with phase records: (records [phases x 40], stats)."""
+    o = dg.options(ALG_ADDS, **opts)
+    buf = np.zeros((cap, ADDS_TRACE_STRIDE), np.float32)
+    phases = C.c_int32()
+    st = SsspStats()
+    check(engine().mi355_adds_trace(dg.handle, int(source), C.byref(o), buf.ctypes.data, cap,
+                                    C.byref(phases), C.byref(st), None))
+    return buf[: min(phases.value, cap)].copy(), st
+
+
+def timed_adds(dg, source: int, scale: float, active: int, delta0: float, reps: int = 2,
+               use_mlp: bool = False):
+    """Best kernel time (ms) over reps of ADDS at width scale * delta0."""
+    o = dg.options(ALG_ADDS, delta=0.0 if use_mlp else scale * delta0, active_buckets=active,
+                   use_mlp=int(use_mlp))
+    best, st_best = float("inf"), None
+    for _ in range(reps):
+        st = dg.sssp_labels(int(source), o)
+        if st.kernel_ms < best:
+            best, st_best = st.kernel_ms, st
+    return best, st_best
+
+
+# -- training graphs --------------------------------------------------------
+def training_specs(count: int, seed: int):
+    """Seeded mix of R-MAT (scale 16-21, fp32 weights) and perturbed grids of
+    1e5-2e6 vertices (routing grids with lognormal integer weights, 4-neighbour
+    grids with uniform integer weights)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        kind = ("rmat", "route", "grid")[i % 3]
+        gseed = int(rng.integers(1, 2**62))
+        if kind == "rmat":
+            specs.append({"kind": kind, "scale": int(rng.integers(16, 22)), "seed": gseed})
+        else:
+            nv = math.exp(rng.uniform(math.log(1e5), math.log(2e6)))
+            side = int(round(math.sqrt(nv)))
+            specs.append({"kind": kind, "side": side, "seed": gseed})
+    return specs
+
+
+def make_graph(spec):
+    from . import graph as G
+
+    if spec["kind"] == "rmat":
+        return G.rmat(spec["scale"], 16, seed=spec["seed"])
+    if spec["kind"] == "route":
+        return G.routing_grid(spec["side"], seed=spec["seed"])
+    return G.grid4(spec["side"], spec["side"], 1, 100, seed=spec["seed"])
+
+
+def collect_graph(dg, source: int, reps: int = 2, max_records: int = 64):
+    """Time every static setting on one graph, pick the fastest, and return
+    its trace records (at the best and at the plain setting) with targets."""
+    _, st0 = trace(dg, source, cap=1)
+    delta0 = st0.initial_delta
+    m_r = int(st0.reached_edges)
+    costs = {}
+    for (s, a) in SETTINGS:
+        ms, st = timed_adds(dg, source, s, a, delta0, reps)
+        # (kernel ms, edges relaxed, relaxations beyond one per reached edge)
+        costs[(s, a)] = (ms, int(st.edges_relaxed), int(st.edges_relaxed) - m_r)
+    best = min(costs, key=lambda k: costs[k][0])
+    target = np.array([math.log2(best[0]), float(best[1])], np.float32)
+    xs = []
+    for (s, a) in [best] + ([(1.0, 1)] if best != (1.0, 1) else []):
+        rec, _ = trace(dg, source, cap=4096, delta=s * delta0, active_buckets=a)
+        if len(rec) > max_records:
+            rec = rec[np.linspace(0, len(rec) - 1, max_records).astype(int)]
+        xs.append(rec[:, :N_IN])
+    x = np.concatenate(xs) if xs else np.zeros((0, N_IN), np.float32)
+    y = np.repeat(target[None, :], len(x), axis=0)
+    meta = {"delta0": delta0, "best": list(best), "best_ms": costs[best][0],
+            "plain_ms": costs[(1.0, 1)][0],
+            "costs": {f"{s}x{a}": list(v) for (s, a), v in costs.items()}}
+    return x, y, meta
+
+
+def collect(specs, log=print, reps: int = 2, time_budget_s: float | None = None):
+    """Dataset over training specs: features, targets, per-sample weights
+    (each graph weighs the same in total) and per-graph metadata."""
+    from .device import DeviceGraph
+
+    X, Y, W, metas = [], [], [], []
+    t0 = time.time()
+    for i, spec in enumerate(specs):
+        if time_budget_s is not None and time.time() - t0 > time_budget_s:
+            log(f"collect: time budget reached after {i} graphs")
+            break
+        g = make_graph(spec)
+        dg = DeviceGraph(g)
+        src = int(g.pick_sources(1, seed=spec["seed"] & 0xFFFFFFFF)[0])
+        x, y, meta = collect_graph(dg, src, reps=reps)
+        meta.update(spec)
+        meta["n"], meta["m"], meta["source"] = g.n, g.m, src
+        X.append(x)
+        Y.append(y)
+        W.append(np.full(len(x), 1.0 / max(len(x), 1), np.float32))
+        metas.append(meta)
+        if i % 10 == 0:
+            log(f"collect {i + 1}/{len(specs)} {spec['kind']} n={g.n} best={meta['best']} "
+                f"gain={meta['plain_ms'] / meta['best_ms']:.2f} ({time.time() - t0:.0f}s)")
+        del dg, g
+    x = np.concatenate(X) if X else np.zeros((0, N_IN), np.float32)
+    y = np.concatenate(Y) if Y else np.zeros((0, N_OUT), np.float32)
+    w = np.concatenate(W) if W else np.zeros(0, np.float32)
+    if len(w):
+        w *= len(w) / w.sum()
+    return x, y, w, metas
+
+
+def fit(x, y, w=None, epochs: int = 60, batch: int = 512, lr: float = 2e-3, seed: int = 2026):
+    """Normalisation from the data, then Adam on the device."""
+    m = DeltaModel.init(seed)
+    mean = x.mean(axis=0)
+    std = x.std(axis=0)
+    m.mean = _f32(mean)
+    m.inv_std = _f32(np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-6), 0.0))
+    losses = m.train(x, y, w, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return m, losses

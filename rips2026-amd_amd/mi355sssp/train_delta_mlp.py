@@ -1,0 +1,95 @@
+"""Collect ADDS phase traces on seeded training graphs, train the delta
+network on the device, and write the model (and a summary) for the engine.
+
+    python -m mi355sssp.train_delta_mlp --graphs 2000 --out mi355sssp/data/delta_mlp.json
+
+Collection can be split over runs (--first/--graphs write one .npz shard
+each); --train-only trains on the shards given with --shards.  Training
+graphs come from learn.training_specs(seed); evaluation graphs use other seeds.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+if __package__ in (None, ""):
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    __package__ = "mi355sssp"
+
+from . import learn as L  # noqa: E402
+
+
+def log(msg: str) -> None:
+    print(f"[train_delta_mlp] {msg}", flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--graphs", type=int, default=2000, help="training graphs in this shard")
+    ap.add_argument("--first", type=int, default=0, help="index of the first spec of this shard")
+    ap.add_argument("--total", type=int, default=2000, help="size of the full spec list")
+    ap.add_argument("--seed", type=int, default=16001, help="training spec seed")
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--budget-s", type=float, default=None, help="stop collecting after this long")
+    ap.add_argument("--shard-out", default=None, help="write the collected shard here (.npz)")
+    ap.add_argument("--shards", nargs="*", default=[], help="train on these shards")
+    ap.add_argument("--train-only", action="store_true")
+    ap.add_argument("--epochs", type=int, default=80)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=None, help="model JSON to write")
+    args = ap.parse_args()
+
+    shards = list(args.shards)
+    if not args.train_only:
+        specs = L.training_specs(args.total, args.seed)[args.first:args.first + args.graphs]
+        t0 = time.time()
+        x, y, w, metas = L.collect(specs, log=log, reps=args.reps, time_budget_s=args.budget_s)
+        log(f"collected {len(metas)} graphs, {len(x)} samples in {time.time() - t0:.0f}s")
+        if args.shard_out:
+            np.savez_compressed(args.shard_out, x=x, y=y, w=w,
+                                meta=np.array(json.dumps(metas)))
+            shards.append(args.shard_out)
+        else:
+            shards = []
+            data = [(x, y, w, metas)]
+    if shards:
+        data = []
+        for p in shards:
+            z = np.load(p, allow_pickle=False)
+            data.append((z["x"], z["y"], z["w"], json.loads(str(z["meta"]))))
+    x = np.concatenate([d[0] for d in data])
+    y = np.concatenate([d[1] for d in data])
+    w = np.concatenate([d[2] for d in data])
+    metas = [m for d in data for m in d[3]]
+    if args.out is None or len(x) == 0:
+        return 0
+    w = w * (len(w) / w.sum())
+    t0 = time.time()
+    model, losses = L.fit(x, y, w, epochs=args.epochs, batch=args.batch, lr=args.lr)
+    log(f"trained on {len(x)} samples from {len(metas)} graphs in {time.time() - t0:.1f}s: "
+        f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    pred = model.forward(x)
+    best_gain = [m["plain_ms"] / m["best_ms"] for m in metas]
+    summary = {
+        "graphs": len(metas), "samples": int(len(x)), "epochs": args.epochs, "batch": args.batch,
+        "lr": args.lr, "train_seed": args.seed, "loss_first_epoch": float(losses[0]),
+        "loss_last_epoch": float(losses[-1]),
+        "mean_abs_err_log2_scale": float(np.abs(pred[:, 0] - y[:, 0]).mean()),
+        "mean_abs_err_active": float(np.abs(pred[:, 1] - y[:, 1]).mean()),
+        "oracle_static_gain_mean": float(np.mean(best_gain)),
+        "oracle_static_gain_median": float(np.median(best_gain)),
+        "settings": [list(s) for s in L.SETTINGS],
+    }
+    model.save(args.out, meta=summary)
+    log(f"wrote {args.out}: {json.dumps(summary)}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,132 @@
+"""The delta network's HIP kernels against the CPU oracle, training on the
+device, and the network inside the ADDS scheduler (bit-exact distances; the
+delta it picks is the network's output)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import mi355sssp as M
+from mi355sssp import learn as L
+from oracle import mlp_ref as R
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _model_and_data(seed, n=500):
+    rng = np.random.default_rng(seed)
+    m = L.DeltaModel.init(seed)
+    m.w2 = rng.normal(0, 0.3, (2, 64)).astype(np.float32)
+    m.b1 = rng.normal(0, 0.1, 64).astype(np.float32)
+    m.mean = rng.normal(0, 1, 32).astype(np.float32)
+    m.inv_std = rng.uniform(0.5, 2.0, 32).astype(np.float32)
+    x = (rng.normal(0, 1, (n, 32)) / m.inv_std + m.mean).astype(np.float32)
+    y = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    w = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    return m, x, y, w
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_forward_matches_oracle(seed):
+    m, x, _, _ = _model_and_data(seed)
+    got = m.forward(x)
+    want, _, _ = R.forward(m.flat(), m.mean, m.inv_std, x)
+    assert np.allclose(got, want, rtol=1e-4, atol=1e-4 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("seed,weighted", [(3, False), (4, True)])
+def test_gradient_matches_oracle(seed, weighted):
+    m, x, y, w = _model_and_data(seed, n=777)
+    loss, g = m.loss_grad(x, y, w if weighted else None)
+    L_ref, g_ref = R.loss_grad(m.flat(), m.mean, m.inv_std, x, y, w if weighted else None)
+    assert loss == pytest.approx(L_ref, rel=1e-4)
+    got = g.flat()
+    scale = np.abs(g_ref).max()
+    bad = np.flatnonzero(~np.isclose(got, g_ref, rtol=1e-3, atol=1e-4 * scale))
+    assert bad.size == 0, f"{bad.size} gradient entries differ, first {bad[:5].tolist()}"
+
+
+def test_training_fits_a_teacher_and_is_deterministic():
+    teacher, x, _, _ = _model_and_data(7, n=4096)
+    y = R.forward(teacher.flat(), teacher.mean, teacher.inv_std, x)[0].astype(np.float32)
+    a, la = L.fit(x, y, epochs=25, batch=256, lr=3e-3, seed=11)
+    b, lb = L.fit(x, y, epochs=25, batch=256, lr=3e-3, seed=11)
+    assert np.array_equal(a.flat(), b.flat()) and np.array_equal(la, lb)
+    assert la[-1] < 0.1 * la[0], la
+    final = R.loss(a.flat(), a.mean, a.inv_std, x, y)
+    assert final == pytest.approx(float(la[-1]), rel=0.5)
+
+
+def test_trace_records_features_and_policy():
+    g = M.grid4(200, 200, 1, 100, seed=3)
+    dg = M.DeviceGraph(g)
+    rec, st = L.trace(dg, 0)
+    assert len(rec) == st.iterations_used > 5
+    # graph features (lanes 0..13) are constant over the run, phase features vary
+    assert np.all(rec[:, :14] == rec[0, :14])
+    assert np.all(rec[:, 15] == 0.0)  # log2(delta / delta0) under the plain policy
+    assert np.all(rec[:, 33] == st.initial_delta) and np.all(rec[:, 34] == 1)
+    assert np.all(np.diff(rec[:, 35]) >= 0)  # edges relaxed so far
+
+
+def _check_adds(g, dg, src, **kw):
+    dist, pred, st = dg.sssp(src, M.ALG_ADDS, **kw)
+    want = orc.dijkstra(g.rowptr, g.col, g.weights, src)
+    assert dist.tobytes() == want.tobytes()
+    assert np.array_equal(pred, orc.canonical_pred(g.rowptr, g.col, g.weights, src, want))
+    return st
+
+
+def test_adds_with_network_is_exact_and_follows_the_network():
+    m, _, _, _ = _model_and_data(9)
+    m.mean[:] = 0.0
+    m.inv_std[:] = 0.25
+    m.install()
+    try:
+        for g in (M.grid4(128, 128, 1, 100, seed=5), M.rmat(13, 16, seed=8)):
+            dg = M.DeviceGraph(g)
+            for src in g.pick_sources(2, seed=4):
+                _check_adds(g, dg, int(src), use_mlp=1)
+            rec, st = L.trace(dg, int(g.pick_sources(1, seed=4)[0]), use_mlp=1)
+            out = m.forward(rec[:, :32])
+            d0 = np.float32(st.initial_delta)
+            scale = np.exp2(np.clip(out[:, 0], -4, 4)).astype(np.float32)
+            want_delta = np.clip(d0 * scale, d0 / 64, d0 * 64)
+            assert np.allclose(rec[:, 33], want_delta, rtol=2e-6)
+            want_active = np.clip(np.floor(out[:, 1] + 0.5), 1, 8)
+            assert np.array_equal(rec[:, 34], want_active)
+    finally:
+        L.clear_delta_model()
+
+
+def test_use_mlp_without_a_model_is_rejected():
+    L.clear_delta_model()
+    g = M.grid4(16, 16, 1, 9, seed=1)
+    dg = M.DeviceGraph(g)
+    with pytest.raises(M.Mi355Error):
+        dg.sssp(0, M.ALG_ADDS, use_mlp=1)
+
+
+@pytest.mark.skipif(not os.path.exists(L.DEFAULT_MODEL), reason="no trained delta model shipped")
+def test_trained_network_gain_on_held_out_graphs():
+    """What the trained network gains over plain ADDS (fixed heuristic delta)
+    on graphs whose seeds were not trained on; distances stay bit-exact."""
+    model = L.load_default_delta_model()
+    try:
+        gains = []
+        for name, g in (("routing grid 700^2", M.routing_grid(700, seed=424242)),
+                        ("R-MAT scale 18", M.rmat(18, 16, seed=424243))):
+            dg = M.DeviceGraph(g)
+            src = int(g.pick_sources(1, seed=77)[0])
+            _check_adds(g, dg, src, use_mlp=1)
+            plain = np.median([L.timed_adds(dg, src, 1.0, 1, 0.0, reps=1)[0] for _ in range(5)])
+            learned = np.median([L.timed_adds(dg, src, 1.0, 1, 0.0, reps=1, use_mlp=True)[0]
+                                 for _ in range(5)])
+            gains.append(plain / learned)
+            print(f"{name}: plain ADDS {plain:.2f} ms, learned delta {learned:.2f} ms, "
+                  f"gain {plain / learned:.2f}x", flush=True)
+        assert min(gains) > 0.9, gains  # never a real slowdown
+    finally:
+        L.clear_delta_model()
