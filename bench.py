@@ -74,6 +74,152 @@ def profile_traffic(path: str):
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_adds_rmat_s22_traffic.md")
 
 
+def routing_workloads(args, cpu_threads: int):
+    """The routing-grid configurations: plain vs learned ADDS single-source on
+    the 4900 x 4900 grid (16 sources), and 4096 box-limited multi-source
+    queries on it solved as one batch vs plain ADDS one query at a time."""
+    import mi355sssp as M
+    from mi355sssp import batch as B
+    from mi355sssp import learn as L
+    from oracle import orc
+
+    out = {}
+    t0 = time.perf_counter()
+    side = args.grid_side
+    g = M.routing_grid(side, seed=args.seed)
+    dg = M.DeviceGraph(g)
+    log(f"routing grid {side}^2: n={g.n} m={g.m} (avg degree {g.m / g.n:.2f}) in "
+        f"{time.perf_counter() - t0:.1f}s")
+    srcs = g.pick_sources(args.grid_sources, seed=args.seed)
+    # single source: plain (fixed heuristic delta) vs learned delta ADDS
+    plain_o = dg.options(M.ALG_ADDS)
+    model = L.load_default_delta_model()
+    res = {}
+    for name, o in (("plain_adds", plain_o),) + ((("learned_adds", dg.options(M.ALG_ADDS, use_mlp=1)),)
+                                                  if model else ()):
+        dg.sssp_labels(int(srcs[0]), o)
+        ms, mr = [], []
+        for s_ in srcs:
+            st = dg.sssp_labels(int(s_), o)
+            ms.append(st.kernel_ms)
+        res[name] = {"ms_per_sssp": float(np.mean(ms)), "phases_last": int(st.iterations_used)}
+    d, _, st = dg.sssp(int(srcs[0]), M.ALG_ADDS, use_mlp=int(model is not None))
+    if args.check > 0:
+        ok = d.tobytes() == orc.dijkstra(g.rowptr, g.col, g.weights, int(srcs[0])).tobytes()
+        log(f"routing grid source {srcs[0]}: dist bit-exact={ok}")
+        if not ok:
+            raise RuntimeError("routing grid parity failed")
+    for name in res:
+        res[name]["gteps"] = st.reached_edges / (res[name]["ms_per_sssp"] * 1e-3) / 1e9
+    if "learned_adds" in res:
+        res["learned_speedup"] = res["plain_adds"]["ms_per_sssp"] / res["learned_adds"]["ms_per_sssp"]
+    out["single_source"] = dict(res, sources=len(srcs), n=g.n, m=g.m, reached_edges=int(st.reached_edges))
+    log(f"routing grid single source: {json.dumps(out['single_source'])}")
+
+    # batched box-limited multi-source queries
+    qs = B.routing_queries(side, side, args.queries, seed=args.seed)
+    best, tune_ms = B.autotune(dg, qs, sample_stride=32)
+    opts = dict(rounds_per_graph=best.rounds_per_graph, blocks_per_cu=best.blocks_per_cu, order=best.order)
+    B.run(dg, qs, want_pred=False, download=False, **opts)  # warm
+    t0 = time.perf_counter()
+    _, _, off, bst = B.run(dg, qs, want_pred=True, download=False, **opts)
+    wall = time.perf_counter() - t0
+    qps = qs.count / wall
+    # one at a time: plain ADDS on each sampled query's induced subgraph
+    # (kernel time only; building the subgraph is not counted)
+    sample = np.linspace(0, qs.count - 1, args.one_at_a_time).astype(int)
+    one_ms = []
+    for q in sample:
+        sub, area = B.induced_subgraph(g, qs, int(q))
+        sdg = M.DeviceGraph(sub)
+        o = sdg.options(M.ALG_ADDS)
+        sdg.sssp_labels(area, o)
+        one_ms.append(min(sdg.sssp_labels(area, o).kernel_ms for _ in range(2)))
+        del sdg, sub
+    # the sampled queries cover the box-size distribution evenly by index;
+    # weight their mean by area to estimate the whole batch
+    areas = qs.areas()
+    per_slot = float(np.sum(one_ms) / np.sum(areas[sample]))
+    one_total_s = per_slot * float(areas.sum()) * 1e-3
+    # parity of a few queries against the oracle (full batch already run)
+    if args.check > 0:
+        dist, pred, off, _ = B.run(dg, qs.subset(sample[: args.check]))
+        for i, q in enumerate(sample[: args.check]):
+            sq = qs.subset([q])
+            want = orc.dijkstra_box(g.rowptr, g.col, g.weights, side, tuple(sq.boxes[0]), sq.sources)
+            ok = dist[off[i]:off[i + 1]].tobytes() == want.tobytes()
+            log(f"batched query {q}: dist bit-exact={ok}")
+            if not ok:
+                raise RuntimeError("batched parity failed")
+    out["batched"] = {
+        "queries": qs.count, "label_slots": int(bst.label_slots), "sources": int(bst.sources),
+        "wall_s": wall, "device_ms": float(bst.device_ms), "queries_per_s": qps,
+        "rounds": int(bst.rounds), "graph_replays": int(bst.graph_replays),
+        "tuned": opts, "tune_candidates_ms": [float(x) for x in tune_ms],
+        "one_at_a_time_plain_adds": {"sampled_queries": int(len(sample)),
+                                     "kernel_ms_sampled": [float(x) for x in one_ms],
+                                     "estimated_total_s": one_total_s,
+                                     "queries_per_s": qs.count / one_total_s},
+        "speedup_vs_one_at_a_time": one_total_s / wall,
+    }
+    log(f"batched: {qs.count} queries in {wall * 1e3:.1f} ms ({qps:.0f} queries/s); one at a time "
+        f"~{one_total_s:.1f} s -> x{one_total_s / wall:.1f}")
+    del dg, g
+    return out
+
+
+def batched_gemm(args):
+    """512 min-plus products with n log-uniform in [32, 512] and values in
+    (0, 1], grouped by tile class after the autotuner has picked a tile per
+    size class; the whole batch timed as one call."""
+    import ctypes as C
+    import math
+
+    import mi355sssp as M
+    from oracle import orc
+
+    rng = np.random.default_rng(args.seed)
+    ns = [int(round(math.exp(rng.uniform(math.log(32), math.log(512))))) for _ in range(512)]
+    mats = []
+    for n in ns:
+        a = ((rng.integers(1, 1 << 24, n * n) * 2.0 ** -24)).astype(np.float32)
+        b = ((rng.integers(1, 1 << 24, n * n) * 2.0 ** -24)).astype(np.float32)
+        mats.append((n, a, b, M.DeviceBuffer.from_array(a), M.DeviceBuffer.from_array(b),
+                     M.DeviceBuffer(4 * n * n)))
+    M.minplus_autotune(sorted(set([32, 48, 64, 96, 128, 192, 256, 384, 512])))
+    cnt = len(mats)
+    pa = (C.c_void_p * cnt)(*[m[3].ptr.value for m in mats])
+    pb = (C.c_void_p * cnt)(*[m[4].ptr.value for m in mats])
+    pc = (C.c_void_p * cnt)(*[m[5].ptr.value for m in mats])
+    dims = (C.c_int * cnt)(*ns)
+    eng = M.engine()
+
+    def once():
+        M._abi.check(eng.mi355_minplus_gemm_f32_batched(cnt, pa, pb, pc, dims, dims, dims, dims, dims,
+                                                        dims, None))
+
+    once()
+    eng.mi355_stream_sync(None)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        once()
+    eng.mi355_stream_sync(None)
+    dt = (time.perf_counter() - t0) / reps
+    ops = float(sum(2.0 * n ** 3 for n in ns))
+    ok = True
+    for i in (0, int(np.argmax(ns))):
+        n, a, b = mats[i][0], mats[i][1], mats[i][2]
+        got = mats[i][5].to_array(np.float32, n * n)
+        want = orc.minplus_gemm(a, b, n, n, n)
+        ok = ok and got.tobytes() == np.ascontiguousarray(want, np.float32).ravel().tobytes()
+    res = {"matrices": cnt, "n_range": [min(ns), max(ns)], "ops": ops, "ms": dt * 1e3,
+           "tops": ops / dt / 1e12, "valu_ceiling_tops": 52.4,
+           "frac_of_ceiling": ops / dt / 1e12 / 52.4, "bit_exact_spot_check": bool(ok)}
+    log(f"batched min-plus GEMM: {json.dumps(res)}")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +239,15 @@ def main() -> int:
     ap.add_argument("--cpu-sources", type=int, default=8, help="oracle Dijkstra runs timed")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size (counter calibration runs)")
+    ap.add_argument("--learned", type=int, default=1, help="also time learned-delta ADDS")
+    ap.add_argument("--routing", type=int, default=1,
+                    help="also run the routing-grid single-source and batched-query workloads")
+    ap.add_argument("--grid-side", type=int, default=4900)
+    ap.add_argument("--grid-sources", type=int, default=16)
+    ap.add_argument("--queries", type=int, default=4096)
+    ap.add_argument("--one-at-a-time", type=int, default=16,
+                    help="queries sampled for the one-at-a-time plain ADDS comparison")
+    ap.add_argument("--gemm", type=int, default=1, help="also time the batched min-plus GEMM workload")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -178,6 +333,24 @@ def main() -> int:
     log(f"{args.algorithm}: {ms_per:.3f} ms/SSSP wall, {dev_ms / args.steps:.3f} ms device, "
         f"{kernel_ms / args.steps:.3f} ms kernel, {gteps:.2f} GTEPS")
 
+    learned = None
+    if args.learned and args.algorithm == "adds":
+        from mi355sssp import learn as L
+
+        if L.load_default_delta_model() is not None:
+            lo = dg.options(M.ALG_ADDS, delta=args.delta, use_mlp=1)
+            lw, ld, lk = timed(lo, args.steps, 2)
+            d, _, _ = dg.sssp(int(sources[0]), M.ALG_ADDS, use_mlp=1)
+            from oracle import orc
+
+            ok = d.tobytes() == orc.dijkstra(g.rowptr, g.col, g.weights, int(sources[0])).tobytes()
+            learned = {"ms_per_sssp": lw * 1e3 / args.steps, "kernel_ms_per_sssp": lk / args.steps,
+                       "gteps": edges / lw / 1e9, "speedup_vs_plain": wall / lw, "bit_exact": ok}
+            log(f"learned-delta adds: {json.dumps(learned)}")
+            if not ok:
+                print("learned ADDS parity failed", file=sys.stderr)
+                return 1
+
     others = {}
     if args.compare_steps > 0:
         cidx = np.arange(args.compare_steps) % len(sources)
@@ -208,6 +381,8 @@ def main() -> int:
                          f"{os.cpu_count()} host CPUs)",
                "ms_per_sssp": cwall / len(cs) * 1e3}
 
+    routing = routing_workloads(args, os.cpu_count() or 1) if args.routing else None
+    gemm = batched_gemm(args) if args.gemm else None
     traffic = profile_traffic(TRAFFIC_FILE) if args.algorithm == "adds" and args.scale == 22 else None
     out = {
         "metric": "SSSP GTEPS + ms/query (learned-delta & batched vs plain ADDS), bit-exact dist",
@@ -232,7 +407,10 @@ def main() -> int:
                    "graph500_gteps": gteps / 2.0,
                    "device_ms_per_sssp": dev_ms / args.steps,
                    "kernel_ms_per_sssp": kernel_ms / args.steps,
-                   "others": others},
+                   "learned_delta_adds": learned,
+                   "others": others,
+                   "routing_grid": routing,
+                   "batched_minplus_gemm": gemm},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,

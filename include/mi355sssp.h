@@ -362,6 +362,64 @@ int mi355_hip_adds_sssp(int32_t weight_type, uint32_t num_vertices, uint32_t num
                         mi355_adds_stats* stats, const mi355_adds_options* options,
                         void* stream);
 
+/* ---- batched multi-source queries --------------------------------------
+ * No reference counterpart (the routing engine's nets; nearest reference code:
+ * the multi-column min-plus products of MP/src/minplus_sparse_hip.cpp:410-764
+ * and MP/src/minplus_hip.cpp:122-173).  Query q: sources
+ * sources[src_offsets[q] .. src_offsets[q+1]) (global vertex ids, all at
+ * distance 0) and the rectangle boxes[4q .. 4q+4) = row0, col0, row1, col1
+ * (half-open) of a grid graph whose vertex v is row * grid_cols + col; only
+ * vertices inside the rectangle are reached.  Outputs per query are the
+ * rectangle's vertices in row-major order, queries concatenated: query q at
+ * [out_offsets[q], out_offsets[q+1]) (mi355_batch_layout). */
+typedef struct {
+  int32_t grid_cols;
+  int32_t count;
+  const int32_t* src_offsets;  /* host, count + 1 */
+  const int32_t* sources;      /* host */
+  const int32_t* boxes;        /* host, 4 per query */
+} mi355_query_batch;
+
+typedef struct {
+  int32_t rounds_per_graph;    /* rounds per captured graph (multiple of 6; 0 = 24) */
+  int32_t blocks_per_cu;       /* relaxation workgroups per CU (0 = 4) */
+  int32_t use_graph;           /* 1: replay a captured HIP graph, 0: plain launches */
+  int32_t inf_mode;            /* MI355_INF_IEEE or MI355_INF_FLT_MAX (f32 outputs) */
+  int32_t count_edges;         /* count edge visits (stats) */
+  int32_t order;               /* label layout: 0 input order, 1 by rectangle position */
+  int32_t reserved[2];
+  int64_t queue_capacity;      /* frontier entries per queue (0 = auto) */
+} mi355_batch_options;
+
+typedef struct {
+  int64_t label_slots;         /* sum of rectangle areas */
+  int64_t sources;
+  int32_t rounds;              /* rounds with a non-empty frontier */
+  int32_t graph_replays;
+  int32_t rounds_per_graph;
+  int32_t reserved_i;
+  int64_t edges_relaxed;       /* when count_edges */
+  int64_t queue_capacity;
+  float device_ms;             /* initialisation + all rounds (hipEvent) */
+  float reserved_f;
+} mi355_batch_stats;
+
+void mi355_batch_options_default(mi355_batch_options* o);
+/* out_offsets: host array of count + 1 (int64): output offsets per query. */
+int mi355_batch_layout(const mi355_graph* g, const mi355_query_batch* qb, int64_t* out_offsets);
+/* d_dist: device array of label_slots 32-bit distances (fp32 or uint32 per the
+ * graph); d_pred: device int32 parents (global vertex ids, -1 for sources and
+ * unreached) or NULL.  Blocking. */
+int mi355_sssp_batched(mi355_graph* g, const mi355_query_batch* qb, const mi355_batch_options* o,
+                       void* d_dist, int32_t* d_pred, mi355_batch_stats* stats, void* stream);
+/* Autotuner: times every (blocks_per_cu, rounds_per_graph, order) candidate on
+ * every sample_stride-th query of the batch and writes the fastest into *best
+ * (other fields of *best are kept).  timings_ms (host, may be NULL) receives
+ * MI355_BATCH_TUNE_CANDIDATES device times in candidate order. */
+#define MI355_BATCH_TUNE_CANDIDATES 18
+int mi355_batch_autotune(mi355_graph* g, const mi355_query_batch* qb, int32_t sample_stride,
+                         mi355_batch_options* best, double* timings_ms, void* stream);
+
 /* ---- learned delta policy ---------------------------------------------
  * No reference counterpart: the network replaces the fixed per-phase delta
  * heuristics (JAS/hip_adds_sssp.cpp:683-727, ONE/sssp_hip.cpp:564-571,

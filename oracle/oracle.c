@@ -509,3 +509,140 @@ void orc_transpose(int64_t rows, int64_t cols, const int64_t* rowptr,
     }
   free(cur);
 }
+
+/* ---- box-restricted multi-source queries (routing nets) --------------------
+ * The batched multi-source workload has no reference counterpart; it runs the
+ * same Dijkstra (DS/testing/verify_bf_delta_vs_dijkstra.cpp:203-242) on the
+ * subgraph induced by a grid rectangle (vertex v = row * grid_cols + col,
+ * rows [r0, r1), cols [c0, c1)), every source starting at 0.  Outputs are
+ * indexed by the box's row-major local index; pred holds global vertex ids
+ * (-1 for sources and unreached vertices). */
+static int in_box(int64_t v, int32_t grid_cols, int32_t r0, int32_t c0, int32_t r1, int32_t c1,
+                  int64_t* local) {
+  const int64_t r = v / grid_cols, c = v % grid_cols;
+  if (r < r0 || r >= r1 || c < c0 || c >= c1) return 0;
+  *local = (r - r0) * (int64_t)(c1 - c0) + (c - c0);
+  return 1;
+}
+
+static int64_t box_global(int64_t local, int32_t grid_cols, int32_t r0, int32_t c0, int32_t c1) {
+  const int64_t bw = c1 - c0;
+  return (r0 + local / bw) * (int64_t)grid_cols + c0 + local % bw;
+}
+
+int orc_dijkstra_box_f32(int64_t n, const int64_t* rowptr, const int32_t* col, const float* w,
+                         int32_t grid_cols, int32_t r0, int32_t c0, int32_t r1, int32_t c1,
+                         const int32_t* sources, int32_t nsrc, float* dist) {
+  if (n <= 0 || grid_cols <= 0 || r0 < 0 || c0 < 0 || r1 <= r0 || c1 <= c0 || c1 > grid_cols ||
+      (int64_t)r1 * grid_cols > n || nsrc <= 0)
+    return -1;
+  const int64_t area = (int64_t)(r1 - r0) * (c1 - c0);
+  for (int64_t i = 0; i < area; ++i) dist[i] = INFINITY;
+  heap_f32 h = {0, 0, 0};
+  for (int32_t k = 0; k < nsrc; ++k) {
+    int64_t l;
+    if (!in_box(sources[k], grid_cols, r0, c0, r1, c1, &l)) {
+      free(h.a);
+      return -1;
+    }
+    dist[l] = 0.0f;
+    heap_f32_item s = {0.0f, (int32_t)l};
+    if (heap_f32_push(&h, s)) return -2;
+  }
+  while (h.size > 0) {
+    heap_f32_item t = heap_f32_pop(&h);
+    if (t.d != dist[t.v]) continue;
+    const int64_t u = box_global(t.v, grid_cols, r0, c0, c1);
+    for (int64_t e = rowptr[u]; e < rowptr[u + 1]; ++e) {
+      int64_t l;
+      if (!in_box(col[e], grid_cols, r0, c0, r1, c1, &l)) continue;
+      const float nd = t.d + w[e];
+      if (nd < dist[l]) {
+        dist[l] = nd;
+        heap_f32_item it = {nd, (int32_t)l};
+        if (heap_f32_push(&h, it)) {
+          free(h.a);
+          return -2;
+        }
+      }
+    }
+  }
+  free(h.a);
+  return 0;
+}
+
+int orc_dijkstra_box_u32(int64_t n, const int64_t* rowptr, const int32_t* col, const uint32_t* w,
+                         int32_t grid_cols, int32_t r0, int32_t c0, int32_t r1, int32_t c1,
+                         const int32_t* sources, int32_t nsrc, uint32_t* dist) {
+  if (n <= 0 || grid_cols <= 0 || r0 < 0 || c0 < 0 || r1 <= r0 || c1 <= c0 || c1 > grid_cols ||
+      (int64_t)r1 * grid_cols > n || nsrc <= 0)
+    return -1;
+  const int64_t area = (int64_t)(r1 - r0) * (c1 - c0);
+  for (int64_t i = 0; i < area; ++i) dist[i] = ORC_U32_INF;
+  heap_u32 h = {0, 0, 0};
+  for (int32_t k = 0; k < nsrc; ++k) {
+    int64_t l;
+    if (!in_box(sources[k], grid_cols, r0, c0, r1, c1, &l)) {
+      free(h.a);
+      return -1;
+    }
+    dist[l] = 0u;
+    heap_u32_item s = {0u, (int32_t)l};
+    if (heap_u32_push(&h, s)) return -2;
+  }
+  while (h.size > 0) {
+    heap_u32_item t = heap_u32_pop(&h);
+    if (t.d != dist[t.v]) continue;
+    const int64_t u = box_global(t.v, grid_cols, r0, c0, c1);
+    for (int64_t e = rowptr[u]; e < rowptr[u + 1]; ++e) {
+      int64_t l;
+      if (!in_box(col[e], grid_cols, r0, c0, r1, c1, &l)) continue;
+      const uint32_t nd = sat_add_u32(t.d, w[e]);
+      if (nd < dist[l]) {
+        dist[l] = nd;
+        heap_u32_item it = {nd, (int32_t)l};
+        if (heap_u32_push(&h, it)) {
+          free(h.a);
+          return -2;
+        }
+      }
+    }
+  }
+  free(h.a);
+  return 0;
+}
+
+/* Canonical predecessor inside the box: the smallest global id u (in the box)
+ * with an edge u->v and dist[u] (+) w == dist[v]; -1 for sources and for
+ * unreached vertices.  wbits: 32-bit weights, f32 when is_f32 != 0. */
+void orc_canonical_pred_box(int64_t n, const int64_t* rowptr, const int32_t* col,
+                            const uint32_t* wbits, int32_t is_f32, int32_t grid_cols, int32_t r0,
+                            int32_t c0, int32_t r1, int32_t c1, const int32_t* sources,
+                            int32_t nsrc, const uint32_t* distbits, int32_t* pred) {
+  (void)n;
+  const int64_t area = (int64_t)(r1 - r0) * (c1 - c0);
+  for (int64_t i = 0; i < area; ++i) pred[i] = -1;
+  for (int64_t lu = 0; lu < area; ++lu) {
+    const int64_t u = box_global(lu, grid_cols, r0, c0, c1);
+    for (int64_t e = rowptr[u]; e < rowptr[u + 1]; ++e) {
+      int64_t lv;
+      if (!in_box(col[e], grid_cols, r0, c0, r1, c1, &lv)) continue;
+      int hit;
+      if (is_f32) {
+        float du, dv, we;
+        memcpy(&du, &distbits[lu], 4);
+        memcpy(&dv, &distbits[lv], 4);
+        memcpy(&we, &wbits[e], 4);
+        hit = isfinite(du) && isfinite(dv) && (float)(du + we) == dv;
+      } else {
+        hit = distbits[lu] != ORC_U32_INF && distbits[lv] != ORC_U32_INF &&
+              sat_add_u32(distbits[lu], wbits[e]) == distbits[lv];
+      }
+      if (hit && (pred[lv] < 0 || u < pred[lv])) pred[lv] = (int32_t)u;
+    }
+  }
+  for (int32_t k = 0; k < nsrc; ++k) {
+    int64_t l;
+    if (in_box(sources[k], grid_cols, r0, c0, r1, c1, &l)) pred[l] = -1;
+  }
+}

@@ -91,6 +91,22 @@ void orc_transpose(int64_t rows, int64_t cols, const int64_t* rowptr,
                    const int32_t* col, const uint32_t* w, int64_t* t_rowptr,
                    int32_t* t_col, uint32_t* t_w);
 
+
+/* Multi-source Dijkstra on the subgraph induced by a grid rectangle (vertex
+ * v = row * grid_cols + col, rows [r0, r1), cols [c0, c1)); dist indexed by
+ * the box's row-major local index. */
+int orc_dijkstra_box_f32(int64_t n, const int64_t* rowptr, const int32_t* col, const float* w,
+                         int32_t grid_cols, int32_t r0, int32_t c0, int32_t r1, int32_t c1,
+                         const int32_t* sources, int32_t nsrc, float* dist);
+int orc_dijkstra_box_u32(int64_t n, const int64_t* rowptr, const int32_t* col, const uint32_t* w,
+                         int32_t grid_cols, int32_t r0, int32_t c0, int32_t r1, int32_t c1,
+                         const int32_t* sources, int32_t nsrc, uint32_t* dist);
+/* Smallest-global-id parent inside the box; -1 for sources / unreached. */
+void orc_canonical_pred_box(int64_t n, const int64_t* rowptr, const int32_t* col,
+                            const uint32_t* wbits, int32_t is_f32, int32_t grid_cols, int32_t r0,
+                            int32_t c0, int32_t r1, int32_t c1, const int32_t* sources,
+                            int32_t nsrc, const uint32_t* distbits, int32_t* pred);
+
 #ifdef __cplusplus
 }
 #endif

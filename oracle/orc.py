@@ -38,6 +38,10 @@ def lib() -> C.CDLL:
             ("orc_minplus_gemm_f32", None, [P, P, P] + [C.c_int] * 6),
             ("orc_minplus_spgemm_f32", C.c_int, [I64, I64, P, P, P, I64, P, P, P, P, P, P, P]),
             ("orc_transpose", None, [I64, I64, P, P, P, P, P, P]),
+            ("orc_dijkstra_box_f32", C.c_int, [I64, P, P, P] + [C.c_int32] * 5 + [P, C.c_int32, P]),
+            ("orc_dijkstra_box_u32", C.c_int, [I64, P, P, P] + [C.c_int32] * 5 + [P, C.c_int32, P]),
+            ("orc_canonical_pred_box", None, [I64, P, P, P, C.c_int32] + [C.c_int32] * 5 +
+             [P, C.c_int32, P, P]),
         ]:
             fn = getattr(_lib, name)
             fn.restype = res
@@ -176,3 +180,37 @@ def transpose(rowptr, col, wbits, cols: int):
     lib().orc_transpose(rows, cols, rp.ctypes.data, cl.ctypes.data, ww.ctypes.data,
                         trp.ctypes.data, tcl.ctypes.data, tw.ctypes.data)
     return trp, tcl, tw
+
+
+def dijkstra_box(rowptr, col, w, grid_cols: int, box, sources):
+    """Multi-source Dijkstra on the subgraph induced by box = (r0, c0, r1, c1)
+    of a grid with grid_cols columns; dist in the box's row-major order."""
+    rp, cl = _c(rowptr, np.int64), _c(col, np.int32)
+    n = len(rp) - 1
+    r0, c0, r1, c1 = (int(b) for b in box)
+    src = _c(sources, np.int32)
+    area = (r1 - r0) * (c1 - c0)
+    if np.asarray(w).dtype == np.uint32:
+        ww, dist, fn = _c(w, np.uint32), np.empty(area, np.uint32), lib().orc_dijkstra_box_u32
+    else:
+        ww, dist, fn = _c(w, np.float32), np.empty(area, np.float32), lib().orc_dijkstra_box_f32
+    rc = fn(n, rp.ctypes.data, cl.ctypes.data, ww.ctypes.data, grid_cols, r0, c0, r1, c1,
+            src.ctypes.data, len(src), dist.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_dijkstra_box failed: {rc}")
+    return dist
+
+
+def canonical_pred_box(rowptr, col, w, grid_cols: int, box, sources, dist):
+    rp, cl = _c(rowptr, np.int64), _c(col, np.int32)
+    n = len(rp) - 1
+    r0, c0, r1, c1 = (int(b) for b in box)
+    src = _c(sources, np.int32)
+    is_f32 = np.asarray(w).dtype != np.uint32
+    wb = _c(np.asarray(w, np.float32 if is_f32 else np.uint32), np.float32 if is_f32 else np.uint32)
+    db = _c(dist, np.float32 if is_f32 else np.uint32)
+    pred = np.empty(len(db), np.int32)
+    lib().orc_canonical_pred_box(n, rp.ctypes.data, cl.ctypes.data, wb.ctypes.data, int(is_f32),
+                                 grid_cols, r0, c0, r1, c1, src.ctypes.data, len(src),
+                                 db.ctypes.data, pred.ctypes.data)
+    return pred

@@ -1,0 +1,503 @@
+// Batched multi-source shortest paths: many routing queries (each a set of
+// sources and a grid rectangle that limits the expansion, the "net bounding
+// box") solved together as one min-plus SpMM frontier.
+//
+// Every query q owns a dense block of packed (dist key, parent) labels, one
+// per vertex of its rectangle in row-major order, at labels[base_q ..).  The
+// frontier is a list of (query, local index) pairs from all queries at once,
+// so one round relaxes every query's frontier in the same launch:
+// D_q <- min(D_q, A^T (x) D_q) restricted to the rectangle, for all q.  A
+// round is the frontier kernel of the single-source engine (64 entries per
+// wave, edges packed across lanes, LDS-staged appends) with the labels
+// addressed per query and out-of-rectangle targets skipped.  Each entry is
+// queued at most once per round (per-slot round stamp).
+//
+// Rounds run as a captured HIP graph of K kernel nodes: the frontier size is
+// read from device memory (an empty frontier makes a round a no-op), queue
+// counters rotate over three slots and each round clears the one two rounds
+// ahead (no memset nodes), and a one-thread node advances the stamp epoch.
+// The host replays the graph and looks at the frontier size once per replay.
+// Distances are the exact fixpoint of the fp32 / uint32 left folds, identical
+// to a multi-source Dijkstra on each query's induced subgraph.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "frontier_kernels.hpp"
+#include "sssp.hpp"
+
+namespace mi355 {
+
+namespace {
+
+struct QueryInfo {
+  int64_t base;      // first label slot of the query (schedule order)
+  int64_t out_base;  // first output element of the query (input order)
+  int32_t r0, c0;    // rectangle origin (grid row, column)
+  int32_t bh, bw;    // rectangle height and width
+};
+
+constexpr int kCntStride = 32;  // rotating counters on separate 128-byte lines
+
+struct BatchCtrl {
+  alignas(128) uint32_t cnt[3 * kCntStride];
+  alignas(128) uint32_t epoch;
+  alignas(128) uint32_t err;
+  alignas(128) uint32_t busy_rounds;
+  alignas(128) unsigned long long edges;
+};
+
+struct BatchArgs {
+  const int* rowptr;
+  const uint2* edges;
+  unsigned long long* labels;
+  uint32_t* stamp;
+  const QueryInfo* qinfo;
+  uint2* q[2];        // frontier queues of (query, local index)
+  uint32_t cap;
+  BatchCtrl* ctrl;
+  int32_t cols;       // grid width: vertex v = row * cols + col
+  int32_t rounds_per_graph;
+  int want_edges;
+};
+
+__device__ inline void batch_stage_flush(uint2* stage, uint32_t& staged, uint2* q, uint32_t* cnt,
+                                         uint32_t cap, uint32_t* err) {
+  if (staged == 0) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const unsigned lane = lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(cnt, staged);
+  base = __shfl(base, 0, 64);
+  for (uint32_t i = lane; i < staged; i += kWave) {
+    if (base + i < cap) q[base + i] = stage[i];
+    else atomicOr(err, 1u);
+  }
+  __builtin_amdgcn_wave_barrier();
+  staged = 0;
+}
+
+template <class W>
+__global__ __launch_bounds__(kBlock) void k_batch_round(BatchArgs a, int k) {
+  __shared__ int s_incl[kWavesPerBlock][kWave];
+  __shared__ int s_beg[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_key[kWavesPerBlock][kWave];
+  __shared__ int s_u[kWavesPerBlock][kWave];
+  __shared__ int s_q[kWavesPerBlock][kWave];
+  __shared__ uint2 s_stage[kWavesPerBlock][kStageCap];
+  BatchCtrl* c = a.ctrl;
+  // an overflowed queue (error flag set) holds only its first cap entries
+  const uint32_t n_in = min(c->cnt[(k % 3) * kCntStride], a.cap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->cnt[((k + 2) % 3) * kCntStride] = 0;  // output counter of round k + 2
+    if (n_in) atomicAdd(&c->busy_rounds, 1u);
+  }
+  if (n_in == 0) return;
+  const uint32_t stamp = c->epoch * (uint32_t)a.rounds_per_graph + (uint32_t)k + 1u;
+  const uint2* qin = a.q[k & 1];
+  uint2* qout = a.q[(k + 1) & 1];
+  uint32_t* cnt_out = &c->cnt[((k + 1) % 3) * kCntStride];
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  const unsigned wv = threadIdx.x / kWave;
+  uint2* stage = s_stage[wv];
+  uint32_t staged = 0;
+  unsigned long long edge_visits = 0;
+  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  for (uint32_t first = (blockIdx.x * kWavesPerBlock + wv) * kWave; first < n_in; first += nwaves * kWave) {
+    const uint32_t i = first + lane;
+    int u = 0, beg = 0, deg = 0, qid = 0;
+    uint32_t key = W::kInfKey;
+    if (i < n_in) {
+      const uint2 it = qin[i];
+      qid = (int)it.x;
+      const QueryInfo qi = a.qinfo[qid];
+      const int r = (int)(it.y / (uint32_t)qi.bw), col = (int)(it.y - (uint32_t)r * (uint32_t)qi.bw);
+      u = (qi.r0 + r) * a.cols + qi.c0 + col;
+      key = atomic_load_key(a.labels + qi.base + it.y);
+      beg = a.rowptr[u];
+      if (key != W::kInfKey) deg = a.rowptr[u + 1] - beg;
+    }
+    int incl = deg;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int t = __shfl_up(incl, off, kWave);
+      if ((int)lane >= off) incl += t;
+    }
+    const int total = __shfl(incl, kWave - 1, kWave);
+    s_incl[wv][lane] = incl;
+    s_beg[wv][lane] = beg - (incl - deg);
+    s_key[wv][lane] = key;
+    s_u[wv][lane] = u;
+    s_q[wv][lane] = qid;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    edge_visits += (unsigned long long)total;
+    for (int t = 0; t < total; t += kWave) {
+      const int j = t + (int)lane;
+      bool push = false;
+      uint2 item = make_uint2(0u, 0u);
+      if (j < total) {
+        int pos = 0;
+#pragma unroll
+        for (int s2 = 32; s2 > 0; s2 >>= 1)
+          if (s_incl[wv][pos + s2 - 1] <= j) pos += s2;
+        const uint2 ed = a.edges[s_beg[wv][pos] + j];
+        const int q = s_q[wv][pos];
+        const QueryInfo qi = a.qinfo[q];
+        const int v = (int)ed.x;
+        const int rv = v / a.cols, cv = v - rv * a.cols;
+        const uint32_t dr = (uint32_t)(rv - qi.r0), dc = (uint32_t)(cv - qi.c0);
+        if (dr < (uint32_t)qi.bh && dc < (uint32_t)qi.bw) {
+          const uint32_t lv = dr * (uint32_t)qi.bw + dc;
+          const uint32_t ck = W::relax(s_key[wv][pos], ed.y);
+          if (relax_edge<W>(a.labels + qi.base, lv, ck, (uint32_t)s_u[wv][pos])) {
+            uint32_t* sp = a.stamp + qi.base + lv;
+            if (*sp != stamp) push = atomicExch(sp, stamp) != stamp;
+            item = make_uint2((uint32_t)q, lv);
+          }
+        }
+      }
+      const unsigned long long mask = __ballot(push);
+      if (mask) {
+        const uint32_t kk = (uint32_t)__popcll(mask);
+        if (staged + kk > kStageCap) batch_stage_flush(stage, staged, qout, cnt_out, a.cap, &c->err);
+        if (push) stage[staged + prefix_count(mask)] = item;
+        staged += kk;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  batch_stage_flush(stage, staged, qout, cnt_out, a.cap, &c->err);
+  if (a.want_edges && lane == 0 && edge_visits) atomicAdd(&c->edges, edge_visits);
+}
+
+__global__ void k_batch_epoch(BatchCtrl* c) { c->epoch += 1u; }
+
+template <class W>
+__global__ void k_batch_init(unsigned long long* __restrict__ labels, uint32_t* __restrict__ stamp,
+                             int64_t total) {
+  const unsigned long long inf_label = ((unsigned long long)W::kInfKey << 32) | kNoPred;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    labels[i] = inf_label;
+    stamp[i] = 0u;
+  }
+}
+
+template <class W>
+__global__ void k_batch_sources(unsigned long long* __restrict__ labels, const QueryInfo* __restrict__ qinfo,
+                                const uint2* __restrict__ src, uint32_t nsrc, uint2* __restrict__ q0,
+                                BatchCtrl* __restrict__ c) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nsrc) {
+    const uint2 s = src[i];
+    labels[qinfo[s.x].base + s.y] = ((unsigned long long)W::kZeroKey << 32) | kNoPred;
+    q0[i] = s;
+  }
+  if (i == 0) {
+    for (int k = 0; k < 3; ++k) c->cnt[k * kCntStride] = 0;
+    c->cnt[0] = nsrc;
+    c->epoch = 0;
+    c->err = 0;
+    c->busy_rounds = 0;
+    c->edges = 0;
+  }
+}
+
+// Labels -> outputs, one query per blockIdx.y (its rectangle in chunks).
+template <class W>
+__global__ void k_batch_output(const unsigned long long* __restrict__ labels, const QueryInfo* __restrict__ qinfo,
+                               int inf_mode, uint32_t* __restrict__ dist, int32_t* __restrict__ pred) {
+  const QueryInfo qi = qinfo[blockIdx.y];
+  const int64_t area = (int64_t)qi.bh * qi.bw;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < area; i += stride) {
+    const unsigned long long l = labels[qi.base + i];
+    const uint32_t key = (uint32_t)(l >> 32);
+    const bool reached = key != W::kInfKey;
+    if (dist) {
+      uint32_t out = key;
+      if (W::kType == MI355_W_F32)
+        out = reached ? __float_as_uint(key_to_f32(key)) : (inf_mode == MI355_INF_FLT_MAX ? 0x7F7FFFFFu : 0x7F800000u);
+      dist[qi.out_base + i] = out;
+    }
+    if (pred) {
+      const uint32_t p = (uint32_t)l;
+      pred[qi.out_base + i] = (!reached || p == kNoPred) ? -1 : (int32_t)p;
+    }
+  }
+}
+
+__global__ void k_batch_source_preds(const QueryInfo* __restrict__ qinfo, const uint2* __restrict__ src,
+                                     uint32_t nsrc, int32_t* __restrict__ pred) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nsrc && pred) pred[qinfo[src[i].x].out_base + src[i].y] = -1;
+}
+
+}  // namespace
+
+void batch_layout(const mi355_query_batch& qb, int64_t n, std::vector<int64_t>& offsets) {
+  if (qb.count <= 0) fail(MI355_EINVAL, "batch needs at least one query");
+  if (qb.count > 65535) fail(MI355_EINVAL, "at most 65535 queries per batch");
+  if (qb.grid_cols <= 0 || n % qb.grid_cols != 0)
+    fail(MI355_EINVAL, "grid_cols must divide the vertex count (vertex = row * grid_cols + col)");
+  if (!qb.src_offsets || !qb.sources || !qb.boxes) fail(MI355_EINVAL, "null query array");
+  const int64_t rows = n / qb.grid_cols;
+  offsets.assign((size_t)qb.count + 1, 0);
+  for (int32_t q = 0; q < qb.count; ++q) {
+    const int32_t* b = qb.boxes + 4 * (size_t)q;
+    if (b[0] < 0 || b[1] < 0 || b[2] <= b[0] || b[3] <= b[1] || b[2] > rows || b[3] > qb.grid_cols)
+      fail(MI355_EINVAL, "query rectangle outside the grid or empty");
+    const int32_t s0 = qb.src_offsets[q], s1 = qb.src_offsets[q + 1];
+    if (s0 < 0 || s1 <= s0) fail(MI355_EINVAL, "every query needs at least one source");
+    for (int32_t i = s0; i < s1; ++i) {
+      const int64_t v = qb.sources[i];
+      if (v < 0 || v >= n) fail(MI355_ERANGE, "source vertex is outside the graph");
+      const int64_t r = v / qb.grid_cols, c = v % qb.grid_cols;
+      if (r < b[0] || r >= b[2] || c < b[1] || c >= b[3]) fail(MI355_EINVAL, "source outside its query rectangle");
+    }
+    const int64_t area = (int64_t)(b[2] - b[0]) * (int64_t)(b[3] - b[1]);
+    if (area > (int64_t)UINT32_MAX) fail(MI355_EOVERFLOW, "query rectangle too large");
+    offsets[(size_t)q + 1] = offsets[(size_t)q] + area;
+  }
+}
+
+template <class W>
+static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_options& o,
+                            uint32_t* d_dist, int32_t* d_pred, mi355_batch_stats& st, hipStream_t s) {
+  std::vector<int64_t> off;
+  batch_layout(qb, g.n, off);
+  const int64_t total = off.back();
+  const int32_t nq = qb.count;
+  // Label blocks are laid out in schedule order: input order, or by
+  // rectangle position (row, then column) so that queries whose rectangles
+  // overlap sit next to each other.
+  std::vector<int32_t> order((size_t)nq);
+  std::iota(order.begin(), order.end(), 0);
+  if (o.order == 1)
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+      const int32_t* bx = qb.boxes + 4 * (size_t)x;
+      const int32_t* by = qb.boxes + 4 * (size_t)y;
+      return bx[0] != by[0] ? bx[0] < by[0] : bx[1] < by[1];
+    });
+  std::vector<int64_t> lbase((size_t)nq);
+  int64_t acc = 0;
+  for (int32_t q : order) {
+    lbase[(size_t)q] = acc;
+    acc += off[(size_t)q + 1] - off[(size_t)q];
+  }
+  std::vector<QueryInfo> qi((size_t)nq);
+  std::vector<uint2> src;
+  for (int32_t q = 0; q < nq; ++q) {
+    const int32_t* b = qb.boxes + 4 * (size_t)q;
+    qi[(size_t)q] = QueryInfo{lbase[(size_t)q], off[(size_t)q], b[0], b[1], b[2] - b[0], b[3] - b[1]};
+    for (int32_t i = qb.src_offsets[q]; i < qb.src_offsets[q + 1]; ++i) {
+      const int64_t v = qb.sources[i];
+      const int64_t r = v / qb.grid_cols - b[0], c = v % qb.grid_cols - b[1];
+      src.push_back(make_uint2((uint32_t)q, (uint32_t)(r * (b[3] - b[1]) + c)));
+    }
+  }
+  const int64_t nsrc = (int64_t)src.size();
+  int64_t cap = o.queue_capacity > 0 ? o.queue_capacity : std::max<int64_t>((total + nsrc) / 4, int64_t(1) << 24);
+  cap = std::max<int64_t>(std::min<int64_t>(cap, total + nsrc), nsrc);
+  if (cap > (int64_t)UINT32_MAX) cap = UINT32_MAX;
+  // a multiple of 6: the queue parity (2) and the counter rotation (3)
+  // return to round 0's state after K rounds
+  const int K = std::max(6, (o.rounds_per_graph > 0 ? o.rounds_per_graph : 24) / 6 * 6);
+  const DeviceProps p = device_props();
+  const unsigned grid = (unsigned)std::max(1, (o.blocks_per_cu > 0 ? o.blocks_per_cu : 4) * p.cus);
+
+  // state cached on the graph handle and grown as needed
+  Workspace& ws = g.ws;
+  ws.batch_labels.ensure((size_t)total);
+  ws.batch_stamp.ensure((size_t)total);
+  ws.batch_q[0].ensure((size_t)cap);
+  ws.batch_q[1].ensure((size_t)cap);
+  ws.batch_qinfo.ensure((size_t)nq * sizeof(QueryInfo));
+  ws.batch_src.ensure((size_t)std::max<int64_t>(nsrc, 1));
+  ws.batch_ctrl.ensure(sizeof(BatchCtrl));
+  struct {
+    unsigned long long* labels;
+    uint32_t* stamp;
+    uint2* q[2];
+    QueryInfo* qinfo;
+    uint2* src;
+  } bs{ws.batch_labels.p, ws.batch_stamp.p, {ws.batch_q[0].p, ws.batch_q[1].p},
+       reinterpret_cast<QueryInfo*>(ws.batch_qinfo.p), ws.batch_src.p};
+  BatchCtrl* ctrl = reinterpret_cast<BatchCtrl*>(ws.batch_ctrl.p);
+  MI_HIP(hipMemcpyAsync(bs.qinfo, qi.data(), qi.size() * sizeof(QueryInfo), hipMemcpyHostToDevice, s));
+  MI_HIP(hipMemcpyAsync(bs.src, src.data(), src.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
+
+  BatchArgs a{};
+  a.rowptr = g.rowptr.p;
+  a.edges = g.edges.p;
+  a.labels = bs.labels;
+  a.stamp = bs.stamp;
+  a.qinfo = bs.qinfo;
+  a.q[0] = bs.q[0];
+  a.q[1] = bs.q[1];
+  a.cap = (uint32_t)cap;
+  a.ctrl = ctrl;
+  a.cols = qb.grid_cols;
+  a.rounds_per_graph = K;
+  a.want_edges = o.count_edges ? 1 : 0;
+
+  hipEvent_t ev0, ev1;
+  MI_HIP(hipEventCreate(&ev0));
+  MI_HIP(hipEventCreate(&ev1));
+  MI_HIP(hipEventRecord(ev0, s));
+  k_batch_init<W><<<(unsigned)std::min<int64_t>(div_up((uint64_t)total, kBlock), 8192), kBlock, 0, s>>>(
+      bs.labels, bs.stamp, total);
+  k_batch_sources<W><<<(unsigned)div_up((uint64_t)std::max<int64_t>(nsrc, 1), kBlock), kBlock, 0, s>>>(
+      bs.labels, bs.qinfo, bs.src, (uint32_t)nsrc, bs.q[0], ctrl);
+  MI_LAUNCH_CHECK();
+
+  // K rounds (+ the epoch node) as one graph, or as plain launches
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (o.use_graph) {
+    MI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < K; ++k) k_batch_round<W><<<grid, kBlock, 0, s>>>(a, k);
+    k_batch_epoch<<<1, 1, 0, s>>>(ctrl);
+    MI_HIP(hipStreamEndCapture(s, &graph));
+    MI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  uint32_t* h_cnt = nullptr;
+  MI_HIP(hipHostMalloc((void**)&h_cnt, 4 * sizeof(uint32_t), hipHostMallocDefault));
+  int64_t replays = 0;
+  const int64_t replay_limit = (total + nsrc) * 4 / K + 1024;  // far above any real run
+  int batch = 1;
+  bool done = false;
+  try {
+    while (!done) {
+      for (int r = 0; r < batch; ++r, ++replays) {
+        if (exec) {
+          MI_HIP(hipGraphLaunch(exec, s));
+        } else {
+          for (int k = 0; k < K; ++k) k_batch_round<W><<<grid, kBlock, 0, s>>>(a, k);
+          k_batch_epoch<<<1, 1, 0, s>>>(ctrl);
+          MI_LAUNCH_CHECK();
+        }
+      }
+      // the next replay's input counter (round K == round 0 of the next epoch)
+      MI_HIP(hipMemcpyAsync(h_cnt, &ctrl->cnt[(K % 3) * kCntStride], 4, hipMemcpyDeviceToHost, s));
+      MI_HIP(hipMemcpyAsync(h_cnt + 1, &ctrl->err, 4, hipMemcpyDeviceToHost, s));
+      MI_HIP(hipStreamSynchronize(s));
+      if (h_cnt[1]) fail(MI355_ECAPACITY, "batched frontier queue overflow (raise queue_capacity)");
+      done = h_cnt[0] == 0;
+      if (!done && replays > replay_limit) fail(MI355_EDEVICE, "batched frontier made no progress");
+      batch = std::min(batch * 2, 8);
+    }
+  } catch (...) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipHostFree(h_cnt);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    throw;
+  }
+  MI_HIP(hipEventRecord(ev1, s));
+  {
+    int64_t max_area = 0;
+    for (const QueryInfo& x : qi) max_area = std::max<int64_t>(max_area, (int64_t)x.bh * x.bw);
+    const dim3 og((unsigned)std::min<int64_t>(div_up((uint64_t)max_area, kBlock), 64), (unsigned)nq);
+    k_batch_output<W><<<og, kBlock, 0, s>>>(bs.labels, bs.qinfo, o.inf_mode, d_dist, d_pred);
+  }
+  if (d_pred && nsrc)
+    k_batch_source_preds<<<(unsigned)div_up((uint64_t)nsrc, kBlock), kBlock, 0, s>>>(bs.qinfo, bs.src,
+                                                                                    (uint32_t)nsrc, d_pred);
+  MI_LAUNCH_CHECK();
+  BatchCtrl hc;
+  MI_HIP(hipMemcpyAsync(&hc, ctrl, sizeof(BatchCtrl), hipMemcpyDeviceToHost, s));
+  MI_HIP(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  MI_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipHostFree(h_cnt);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  st = mi355_batch_stats{};
+  st.label_slots = total;
+  st.sources = nsrc;
+  st.rounds = (int32_t)hc.busy_rounds;
+  st.graph_replays = (int32_t)replays;
+  st.rounds_per_graph = K;
+  st.edges_relaxed = (int64_t)hc.edges;
+  st.device_ms = ms;
+  st.queue_capacity = cap;
+}
+
+void run_batch(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_options& o, void* d_dist,
+               int32_t* d_pred, mi355_batch_stats& st, hipStream_t caller) {
+  if (g.info.has_negative) fail(MI355_EINVAL, "batched queries require nonnegative edge weights");
+  // Graph capture needs a stream of its own (not the legacy default stream);
+  // it is ordered after the caller's work and the call is blocking.
+  hipStream_t s;
+  MI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t after;
+  MI_HIP(hipEventCreateWithFlags(&after, hipEventDisableTiming));
+  try {
+    MI_HIP(hipEventRecord(after, caller));
+    MI_HIP(hipStreamWaitEvent(s, after, 0));
+    if (g.weight_type == MI355_W_F32)
+      run_batch_typed<WF32>(g, qb, o, static_cast<uint32_t*>(d_dist), d_pred, st, s);
+    else
+      run_batch_typed<WU32>(g, qb, o, static_cast<uint32_t*>(d_dist), d_pred, st, s);
+    MI_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    (void)hipEventDestroy(after);
+    throw;
+  }
+  (void)hipStreamDestroy(s);
+  (void)hipEventDestroy(after);
+}
+
+void batch_autotune(mi355_graph& g, const mi355_query_batch& qb, int32_t stride, mi355_batch_options& best,
+                    double* timings, hipStream_t caller) {
+  std::vector<int64_t> off;
+  batch_layout(qb, g.n, off);
+  stride = std::max(1, stride);
+  // the sampled sub-batch
+  std::vector<int32_t> so{0}, srcs, boxes;
+  for (int32_t q = 0; q < qb.count; q += stride) {
+    for (int32_t i = qb.src_offsets[q]; i < qb.src_offsets[q + 1]; ++i) srcs.push_back(qb.sources[i]);
+    so.push_back((int32_t)srcs.size());
+    boxes.insert(boxes.end(), qb.boxes + 4 * (size_t)q, qb.boxes + 4 * (size_t)q + 4);
+  }
+  mi355_query_batch sub{qb.grid_cols, (int32_t)(so.size() - 1), so.data(), srcs.data(), boxes.data()};
+  std::vector<int64_t> soff;
+  batch_layout(sub, g.n, soff);
+  DevBuf<uint32_t> dist((size_t)soff.back());
+  const int bpc[3] = {2, 4, 8}, rpg[3] = {12, 24, 48};
+  double best_ms = 1e300;
+  int idx = 0;
+  mi355_batch_options chosen = best;
+  for (int order = 0; order < 2; ++order)
+    for (int r : rpg)
+      for (int b : bpc) {
+        mi355_batch_options o = best;
+        o.blocks_per_cu = b;
+        o.rounds_per_graph = r;
+        o.order = order;
+        o.use_graph = 1;
+        o.count_edges = 0;
+        mi355_batch_stats st{};
+        run_batch(g, sub, o, dist.p, nullptr, st, caller);
+        if (timings) timings[idx] = st.device_ms;
+        if (st.device_ms < best_ms) {
+          best_ms = st.device_ms;
+          chosen = o;
+        }
+        ++idx;
+      }
+  best = chosen;
+}
+
+}  // namespace mi355

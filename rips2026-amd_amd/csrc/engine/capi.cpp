@@ -754,4 +754,47 @@ int mi355_adds_trace(mi355_graph* g, int32_t source, const mi355_sssp_options* o
   });
 }
 
+
+// ---- batched multi-source queries -------------------------------------------
+void mi355_batch_options_default(mi355_batch_options* o) {
+  if (!o) return;
+  *o = mi355_batch_options{};
+  o->rounds_per_graph = 24;
+  o->blocks_per_cu = 4;
+  o->use_graph = 1;
+  o->inf_mode = MI355_INF_IEEE;
+}
+
+int mi355_batch_layout(const mi355_graph* g, const mi355_query_batch* qb, int64_t* out_offsets) {
+  return guard([&] {
+    if (!g || !qb || !out_offsets) fail(MI355_EINVAL, "null argument");
+    std::vector<int64_t> off;
+    mi355::batch_layout(*qb, g->n, off);
+    std::copy(off.begin(), off.end(), out_offsets);
+  });
+}
+
+int mi355_batch_autotune(mi355_graph* g, const mi355_query_batch* qb, int32_t sample_stride,
+                         mi355_batch_options* best, double* timings_ms, void* stream) {
+  return guard([&] {
+    if (!g || !qb || !best) fail(MI355_EINVAL, "null argument");
+    MI_HIP(hipSetDevice(g->device));
+    mi355::batch_autotune(*g, *qb, sample_stride, *best, timings_ms, as_stream(stream));
+  });
+}
+
+int mi355_sssp_batched(mi355_graph* g, const mi355_query_batch* qb, const mi355_batch_options* o,
+                       void* d_dist, int32_t* d_pred, mi355_batch_stats* stats, void* stream) {
+  return guard([&] {
+    if (!g || !qb) fail(MI355_EINVAL, "null argument");
+    if (!d_dist && !d_pred) fail(MI355_EINVAL, "no output requested");
+    MI_HIP(hipSetDevice(g->device));
+    mi355_batch_options d;
+    mi355_batch_options_default(&d);
+    mi355_batch_stats st{};
+    mi355::run_batch(*g, *qb, o ? *o : d, d_dist, d_pred, st, as_stream(stream));
+    if (stats) *stats = st;
+  });
+}
+
 }  // extern "C"

@@ -49,6 +49,13 @@ struct Workspace {
   DevBuf<HubChunk> adds_hub;
   DevBuf<unsigned char> adds_ctrl_mem;
   DevBuf<AddsPolicy> adds_policy;
+  // batched multi-source queries (batch.hip)
+  DevBuf<unsigned long long> batch_labels;
+  DevBuf<uint32_t> batch_stamp;
+  DevBuf<uint2> batch_q[2];
+  DevBuf<unsigned char> batch_qinfo;
+  DevBuf<uint2> batch_src;
+  DevBuf<unsigned char> batch_ctrl;
   int64_t slot_cap = 0;
   int nslots = 0;
 };

@@ -68,6 +68,13 @@ void run_delta_stepping(mi355_graph& g, uint32_t source, const RunRequest& req, 
 // or learned MLP, plus the optional phase trace.
 const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hipStream_t stream);
 
+// Batched multi-source queries (batch.hip).
+void batch_layout(const mi355_query_batch& qb, int64_t n, std::vector<int64_t>& offsets);
+void run_batch(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_options& o, void* d_dist,
+               int32_t* d_pred, mi355_batch_stats& st, hipStream_t stream);
+void batch_autotune(mi355_graph& g, const mi355_query_batch& qb, int32_t stride, mi355_batch_options& best,
+                    double* timings, hipStream_t stream);
+
 // Process-wide delta model (policy.hip) and its training kernels (mlp.hip).
 void set_mlp(const AddsPolicy& p);
 bool get_mlp(AddsPolicy* p);

@@ -70,6 +70,24 @@ class MlpTrainOptions(C.Structure):
 ADDS_TRACE_STRIDE = 40
 
 
+class QueryBatch(C.Structure):
+    _fields_ = [("grid_cols", C.c_int32), ("count", C.c_int32), ("src_offsets", C.c_void_p),
+                ("sources", C.c_void_p), ("boxes", C.c_void_p)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("rounds_per_graph", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("use_graph", C.c_int32), ("inf_mode", C.c_int32), ("count_edges", C.c_int32),
+                ("order", C.c_int32), ("reserved", C.c_int32 * 2), ("queue_capacity", C.c_int64)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("label_slots", C.c_int64), ("sources", C.c_int64), ("rounds", C.c_int32),
+                ("graph_replays", C.c_int32), ("rounds_per_graph", C.c_int32),
+                ("reserved_i", C.c_int32), ("edges_relaxed", C.c_int64),
+                ("queue_capacity", C.c_int64), ("device_ms", C.c_float), ("reserved_f", C.c_float)]
+
+
 class BfProgress(C.Structure):
     _fields_ = [("iteration", C.c_int), ("max_iters", C.c_int),
                 ("convergence_checked", C.c_ubyte), ("changed", C.c_ubyte)]
@@ -136,6 +154,12 @@ _ENGINE_SIGS = {
     "mi355_delta_stepping_csr_f32": (C.c_int, [C.POINTER(CsrDesc), C.c_int, C.c_float, C.c_int, P,
                                                C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                PROGRESS_CB, P, P]),
+    "mi355_batch_options_default": (None, [C.POINTER(BatchOptions)]),
+    "mi355_batch_layout": (C.c_int, [P, C.POINTER(QueryBatch), P]),
+    "mi355_sssp_batched": (C.c_int, [P, C.POINTER(QueryBatch), C.POINTER(BatchOptions), P, P,
+                                     C.POINTER(BatchStats), P]),
+    "mi355_batch_autotune": (C.c_int, [P, C.POINTER(QueryBatch), C.c_int32, C.POINTER(BatchOptions),
+                                       P, P]),
     "mi355_mlp_train_options_default": (None, [C.POINTER(MlpTrainOptions)]),
     "mi355_mlp_forward": (C.c_int, [C.POINTER(MlpParams), P, I64, P]),
     "mi355_mlp_loss_grad": (C.c_int, [C.POINTER(MlpParams), P, P, P, I64, C.POINTER(C.c_float),

@@ -1,0 +1,142 @@
+"""Batched multi-source queries on grid graphs (routing nets): the query
+workload generator and the ctypes front of mi355_sssp_batched."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._abi import BatchOptions, BatchStats, QueryBatch, W_F32, check, engine
+from .device import DeviceBuffer
+
+
+class Queries:
+    """count queries: sources (concatenated, global vertex ids) with
+    src_offsets, and rectangles boxes[q] = (row0, col0, row1, col1)."""
+
+    def __init__(self, grid_cols: int, src_offsets, sources, boxes):
+        self.grid_cols = int(grid_cols)
+        self.src_offsets = np.ascontiguousarray(src_offsets, np.int32)
+        self.sources = np.ascontiguousarray(sources, np.int32)
+        self.boxes = np.ascontiguousarray(boxes, np.int32).reshape(-1, 4)
+        self.count = len(self.boxes)
+
+    def c(self) -> QueryBatch:
+        return QueryBatch(self.grid_cols, self.count, self.src_offsets.ctypes.data,
+                          self.sources.ctypes.data, self.boxes.ctypes.data)
+
+    def query_sources(self, q: int) -> np.ndarray:
+        return self.sources[self.src_offsets[q]:self.src_offsets[q + 1]]
+
+    def areas(self) -> np.ndarray:
+        b = self.boxes.astype(np.int64)
+        return (b[:, 2] - b[:, 0]) * (b[:, 3] - b[:, 1])
+
+    def subset(self, idx) -> "Queries":
+        idx = np.asarray(idx)
+        srcs = [self.query_sources(int(q)) for q in idx]
+        off = np.concatenate([[0], np.cumsum([len(s) for s in srcs])])
+        return Queries(self.grid_cols, off, np.concatenate(srcs), self.boxes[idx])
+
+
+def routing_queries(rows: int, cols: int, count: int, seed: int, side_min: int = 50,
+                    side_max: int = 2000, max_sources: int = 8, square: bool = True) -> Queries:
+    """Net-like queries: a bounding box of side log-uniform in
+    [side_min, side_max] (square unless square=False, clipped to the grid),
+    placed uniformly; 1..max_sources sources uniform inside it."""
+    rng = np.random.default_rng(seed)
+    boxes = np.zeros((count, 4), np.int32)
+    srcs, off = [], [0]
+    for q in range(count):
+        side = math.exp(rng.uniform(math.log(side_min), math.log(side_max)))
+        other = side if square else math.exp(rng.uniform(math.log(side_min), math.log(side_max)))
+        h = min(rows, int(round(side)))
+        w = min(cols, int(round(other)))
+        r0 = int(rng.integers(0, rows - h + 1))
+        c0 = int(rng.integers(0, cols - w + 1))
+        boxes[q] = (r0, c0, r0 + h, c0 + w)
+        k = int(rng.integers(1, max_sources + 1))
+        rr = rng.integers(r0, r0 + h, k)
+        cc = rng.integers(c0, c0 + w, k)
+        srcs.append((rr * cols + cc).astype(np.int32))
+        off.append(off[-1] + k)
+    return Queries(cols, np.array(off), np.concatenate(srcs), boxes)
+
+
+def options(**kw) -> BatchOptions:
+    o = BatchOptions()
+    engine().mi355_batch_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def layout(dg, qs: Queries) -> np.ndarray:
+    off = np.zeros(qs.count + 1, np.int64)
+    check(engine().mi355_batch_layout(dg.handle, C.byref(qs.c()), off.ctypes.data))
+    return off
+
+
+def run(dg, qs: Queries, want_pred: bool = True, download: bool = True, **kw):
+    """Solve all queries; returns (dist, pred, offsets, stats) with dist/pred
+    concatenated per query (None when not downloaded)."""
+    off = layout(dg, qs)
+    total = int(off[-1])
+    d_dist = DeviceBuffer(4 * total)
+    d_pred = DeviceBuffer(4 * total) if want_pred else None
+    st = BatchStats()
+    check(engine().mi355_sssp_batched(dg.handle, C.byref(qs.c()), C.byref(options(**kw)), d_dist.ptr,
+                                      d_pred.ptr if d_pred else None, C.byref(st), None))
+    if not download:
+        return None, None, off, st
+    dist = d_dist.to_array(np.float32 if dg.weight_type == W_F32 else np.uint32, total)
+    pred = d_pred.to_array(np.int32, total) if d_pred else None
+    return dist, pred, off, st
+
+
+TUNE_CANDIDATES = 18
+
+
+def autotune(dg, qs: Queries, sample_stride: int = 8, **kw):
+    """(best options, device ms of every candidate) on every sample_stride-th query."""
+    best = options(**kw)
+    t = np.zeros(TUNE_CANDIDATES, np.float64)
+    check(engine().mi355_batch_autotune(dg.handle, C.byref(qs.c()), int(sample_stride), C.byref(best),
+                                        t.ctypes.data, None))
+    return best, t
+
+
+def induced_subgraph(g, qs: Queries, q: int):
+    """Host CSR of query q's rectangle (local ids, row-major) plus one extra
+    vertex (the last) joined to every source by a zero-weight edge: plain
+    single-source ADDS from it solves the multi-source query."""
+    from .graph import from_csr
+
+    r0, c0, r1, c1 = (int(x) for x in qs.boxes[q])
+    bw = c1 - c0
+    rows = np.arange(r0, r1)
+    verts = (rows[:, None] * qs.grid_cols + np.arange(c0, c1)[None, :]).ravel()
+    area = len(verts)
+    rp = np.asarray(g.rowptr, np.int64)
+    deg = rp[verts + 1] - rp[verts]
+    eidx = np.repeat(rp[verts], deg) + (np.arange(deg.sum()) - np.repeat(np.cumsum(deg) - deg, deg))
+    src_local = np.repeat(np.arange(area), deg)
+    dst = np.asarray(g.col)[eidx].astype(np.int64)
+    w = np.asarray(g.wbits)[eidx]
+    dr, dc = dst // qs.grid_cols - r0, dst % qs.grid_cols - c0
+    keep = (dr >= 0) & (dr < r1 - r0) & (dc >= 0) & (dc < bw)
+    src_local, w = src_local[keep], w[keep]
+    dst_local = (dr * bw + dc)[keep]
+    s = qs.query_sources(q).astype(np.int64)
+    s_local = np.unique((s // qs.grid_cols - r0) * bw + (s % qs.grid_cols - c0))
+    src_all = np.concatenate([src_local, np.full(len(s_local), area)])
+    dst_all = np.concatenate([dst_local, s_local])
+    w_all = np.concatenate([w, np.zeros(len(s_local), w.dtype)])
+    order = np.argsort(src_all, kind="stable")
+    rowptr = np.zeros(area + 2, np.int64)
+    np.add.at(rowptr, src_all + 1, 1)
+    rowptr = np.cumsum(rowptr)
+    wt = g.weight_type
+    wv = w_all[order].view(np.float32) if wt == W_F32 else w_all[order]
+    return from_csr(rowptr, dst_all[order], wv, wt), area

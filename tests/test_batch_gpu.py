@@ -1,0 +1,150 @@
+"""Batched multi-source queries (routing nets) against the CPU oracle's
+multi-source Dijkstra on each query's induced subgraph."""
+import numpy as np
+import pytest
+
+import mi355sssp as M
+from mi355sssp import batch as B
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(g, qs, dist, pred, off, sample=None):
+    w = g.weights
+    qids = range(qs.count) if sample is None else sample
+    for q in qids:
+        box = tuple(int(x) for x in qs.boxes[q])
+        srcs = qs.query_sources(q)
+        want = orc.dijkstra_box(g.rowptr, g.col, w, qs.grid_cols, box, srcs)
+        got = dist[off[q]:off[q + 1]]
+        bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+        assert bad.size == 0, f"query {q} box {box}: {bad.size} distances differ"
+        if pred is not None:
+            wp = orc.canonical_pred_box(g.rowptr, g.col, w, qs.grid_cols, box, srcs, want)
+            badp = np.flatnonzero(pred[off[q]:off[q + 1]] != wp)
+            assert badp.size == 0, f"query {q}: {badp.size} predecessors differ"
+
+
+def _f32_grid(rows, cols, seed):
+    rng = np.random.default_rng(seed)
+    src, dst = [], []
+    for r in range(rows):
+        for c in range(cols):
+            v = r * cols + c
+            for dr, dc in ((-1, 0), (0, -1), (0, 1), (1, 0)):
+                rr, cc = r + dr, c + dc
+                if 0 <= rr < rows and 0 <= cc < cols and rng.random() > 0.08:
+                    src.append(v)
+                    dst.append(rr * cols + cc)
+    w = ((rng.integers(1, 1 << 24, len(src)) * 2.0 ** -24) * 4).astype(np.float32)
+    return M.from_edges(rows * cols, np.array(src), np.array(dst), w, M.W_F32)
+
+
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_routing_grid_queries_bit_exact(use_graph):
+    side = 150
+    g = M.routing_grid(side, seed=21)
+    dg = M.DeviceGraph(g)
+    qs = B.routing_queries(side, side, 48, seed=5, side_min=5, side_max=140)
+    print(f"batch: {qs.count} queries, {int(qs.areas().sum())} slots, use_graph={use_graph}", flush=True)
+    dist, pred, off, st = B.run(dg, qs, use_graph=use_graph, count_edges=1)
+    assert st.label_slots == off[-1] == qs.areas().sum()
+    assert st.rounds > 0 and st.edges_relaxed > 0
+    _check(g, qs, dist, pred, off)
+    # the same batch again on the cached state, other schedule parameters
+    dist2, pred2, _, _ = B.run(dg, qs, use_graph=use_graph, rounds_per_graph=6, blocks_per_cu=1)
+    assert dist2.tobytes() == dist.tobytes() and np.array_equal(pred2, pred)
+
+
+def test_f32_grid_queries_bit_exact():
+    g = _f32_grid(60, 70, seed=3)
+    dg = M.DeviceGraph(g)
+    qs = B.routing_queries(60, 70, 30, seed=9, side_min=3, side_max=60)
+    dist, pred, off, _ = B.run(dg, qs)
+    _check(g, qs, dist, pred, off)
+
+
+def test_whole_grid_single_source_equals_sssp():
+    side = 90
+    g = M.routing_grid(side, seed=4)
+    dg = M.DeviceGraph(g)
+    src = 17 * side + 33
+    qs = B.Queries(side, [0, 1], [src], [[0, 0, side, side]])
+    dist, pred, off, _ = B.run(dg, qs)
+    d1, p1, _ = dg.sssp(src, M.ALG_DELTA)
+    assert dist.tobytes() == d1.tobytes() and np.array_equal(pred, p1)
+
+
+def test_degenerate_queries():
+    side = 40
+    g = M.routing_grid(side, seed=8)
+    dg = M.DeviceGraph(g)
+    v = 5 * side + 6
+    qs = B.Queries(side, [0, 1, 4, 6],
+                   [v, v, v, v, 0, side * side - 1],  # 1x1 box; duplicate sources; corners
+                   [[5, 6, 6, 7], [0, 0, side, side], [0, 0, side, side]])
+    dist, pred, off, _ = B.run(dg, qs)
+    assert dist[0] == 0 and pred[0] == -1
+    _check(g, qs, dist, pred, off)
+
+
+def test_invalid_batches_are_rejected():
+    side = 20
+    g = M.routing_grid(side, seed=2)
+    dg = M.DeviceGraph(g)
+    bad = [
+        B.Queries(side, [0, 1], [0], [[1, 1, 5, 5]]),          # source outside its box
+        B.Queries(side, [0, 1], [0], [[0, 0, side + 1, 5]]),   # box outside the grid
+        B.Queries(side, [0, 0], [], [[0, 0, 5, 5]]),           # no source
+        B.Queries(side + 1, [0, 1], [0], [[0, 0, 5, 5]]),      # grid width does not divide n
+    ]
+    for qs in bad:
+        with pytest.raises(M.Mi355Error):
+            B.run(dg, qs)
+
+
+def test_queue_overflow_is_reported():
+    side = 100
+    g = M.routing_grid(side, seed=6)
+    dg = M.DeviceGraph(g)
+    qs = B.routing_queries(side, side, 16, seed=1, side_min=60, side_max=100)
+    with pytest.raises(M.Mi355Error) as e:
+        B.run(dg, qs, queue_capacity=64)
+    assert "overflow" in str(e.value)
+    # and the handle still works afterwards
+    dist, pred, off, _ = B.run(dg, qs)
+    _check(g, qs, dist, pred, off, sample=[0, 5])
+
+
+def test_autotune_and_schedule_orders():
+    side = 120
+    g = M.routing_grid(side, seed=13)
+    dg = M.DeviceGraph(g)
+    qs = B.routing_queries(side, side, 40, seed=2, side_min=10, side_max=120)
+    best, t = B.autotune(dg, qs, sample_stride=4)
+    assert np.all(t > 0) and best.rounds_per_graph % 6 == 0 and best.blocks_per_cu in (2, 4, 8)
+    ref, pref, off, _ = B.run(dg, qs)
+    for order in (0, 1):
+        d, p, _, _ = B.run(dg, qs, order=order, rounds_per_graph=best.rounds_per_graph,
+                           blocks_per_cu=best.blocks_per_cu)
+        assert d.tobytes() == ref.tobytes() and np.array_equal(p, pref)
+    _check(g, qs, ref, pref, off, sample=range(0, 40, 7))
+
+
+def test_one_query_at_a_time_adds_matches_the_batch():
+    side = 110
+    g = M.routing_grid(side, seed=17)
+    dg = M.DeviceGraph(g)
+    qs = B.routing_queries(side, side, 6, seed=4, side_min=20, side_max=110)
+    dist, pred, off, _ = B.run(dg, qs)
+    for q in range(qs.count):
+        sub, area = B.induced_subgraph(g, qs, q)
+        sdg = M.DeviceGraph(sub)
+        d, p, _ = sdg.sssp(area, M.ALG_ADDS)
+        assert d[:area].tobytes() == dist[off[q]:off[q + 1]].tobytes()
+        r0, c0, r1, c1 = qs.boxes[q]
+        bw = c1 - c0
+        gp = np.where((p[:area] < 0) | (p[:area] == area), -1,
+                      (r0 + p[:area] // bw) * side + c0 + p[:area] % bw)
+        assert np.array_equal(gp, pred[off[q]:off[q + 1]])

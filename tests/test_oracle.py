@@ -286,3 +286,81 @@ def test_transpose_matches_scipy():
     assert np.array_equal(tw.view(np.float32), T.data)
     rp2, c2, w2 = orc.transpose(trp, tcol, tw, 50)
     assert np.array_equal(rp2, A.indptr) and np.array_equal(c2, A.indices)
+
+
+def _grid_graph(rows, cols, seed, wtype):
+    """4-neighbour grid as an outgoing CSR (v = r * cols + c)."""
+    rng = np.random.default_rng(seed)
+    src, dst = [], []
+    for r in range(rows):
+        for c in range(cols):
+            v = r * cols + c
+            for dr, dc in ((0, 1), (1, 0), (0, -1), (-1, 0)):
+                rr, cc = r + dr, c + dc
+                if 0 <= rr < rows and 0 <= cc < cols:
+                    src.append(v)
+                    dst.append(rr * cols + cc)
+    src, dst = np.array(src), np.array(dst)
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    rowptr = np.zeros(rows * cols + 1, np.int64)
+    np.add.at(rowptr, src + 1, 1)
+    rowptr = np.cumsum(rowptr)
+    if wtype == "u32":
+        w = rng.integers(1, 20, len(src)).astype(np.uint32)
+    else:
+        w = (rng.integers(1, 1 << 24, len(src)) * 2.0 ** -24).astype(np.float32)
+    return rowptr, dst.astype(np.int32), w
+
+
+@pytest.mark.parametrize("wtype", ["u32", "f32"])
+def test_box_dijkstra_closed_forms(wtype):
+    rows, cols = 17, 23
+    rp, col, w = _grid_graph(rows, cols, 5, wtype)
+    # the whole grid as the box and one source: plain Dijkstra
+    s = 5 * cols + 7
+    full = orc.dijkstra_box(rp, col, w, cols, (0, 0, rows, cols), [s])
+    assert full.tobytes() == orc.dijkstra(rp, col, w, s).tobytes()
+    # several sources: the element-wise minimum of the single-source runs
+    box = (3, 4, 14, 19)
+    srcs = [5 * cols + 7, 12 * cols + 15, 3 * cols + 4]
+    got = orc.dijkstra_box(rp, col, w, cols, box, srcs)
+    sub_rows, sub_cols = box[2] - box[0], box[3] - box[1]
+    # the induced subgraph built explicitly, relabelled to local ids
+    keep = {}
+    for r in range(box[0], box[2]):
+        for c in range(box[1], box[3]):
+            keep[r * cols + c] = (r - box[0]) * sub_cols + (c - box[1])
+    s_src, s_dst, s_w = [], [], []
+    for u, lu in keep.items():
+        for e in range(rp[u], rp[u + 1]):
+            if int(col[e]) in keep:
+                s_src.append(lu)
+                s_dst.append(keep[int(col[e])])
+                s_w.append(w[e])
+    order = np.lexsort((s_dst, s_src))
+    s_src, s_dst, s_w = np.array(s_src)[order], np.array(s_dst)[order], np.array(s_w)[order]
+    srp = np.zeros(sub_rows * sub_cols + 1, np.int64)
+    np.add.at(srp, s_src + 1, 1)
+    srp = np.cumsum(srp)
+    per = [orc.dijkstra(srp, s_dst.astype(np.int32), s_w.astype(w.dtype), keep[s]) for s in srcs]
+    assert got.tobytes() == np.minimum.reduce(per).tobytes()
+    # predecessors: sources have none; every other reached vertex has an in-box
+    # parent that gives its distance exactly, the smallest such global id
+    pred = orc.canonical_pred_box(rp, col, w, cols, box, srcs, got)
+    for s in srcs:
+        assert pred[keep[s]] == -1
+    inv = {lv: v for v, lv in keep.items()}
+    for lv in range(len(got)):
+        if lv in (keep[s] for s in srcs):
+            continue
+        v = inv[lv]
+        cands = []
+        for u, lu in keep.items():
+            for e in range(rp[u], rp[u + 1]):
+                if col[e] == v:
+                    du = got[lu]
+                    nd = (np.float32(du) + w[e]) if wtype == "f32" else min(int(du) + int(w[e]), 0xFFFFFFFF)
+                    if nd == got[lv]:
+                        cands.append(u)
+        assert pred[lv] == (min(cands) if cands else -1)
