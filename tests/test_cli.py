@@ -1,0 +1,73 @@
+"""The sssp_hip command-line driver: argument handling (CPU) and a run on the
+reference's tiny .gr fixture (GPU): output lines, INF, the fd 3 summary."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import mi355sssp as M
+from conftest import PKG
+
+CLI = os.path.join(PKG, "lib", "sssp_hip")
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")
+
+
+def _run(args, **kw):
+    return subprocess.run([CLI] + args, capture_output=True, text=True, timeout=120, **kw)
+
+
+@pytest.mark.parametrize("args,msg", [
+    ([], "usage"),
+    (["-s", "x", "g.gr"], "Invalid source vertex"),
+    (["-s", "-3", "g.gr"], "Invalid source vertex"),
+    (["-g", "1.5", "g.gr"], "Invalid GPU device"),
+    (["-z", "g.gr"], "usage"),
+    (["a.gr", "b.gr"], "Unexpected extra argument"),
+    (["-q"], "usage"),
+])
+def test_argument_errors(args, msg):
+    r = _run(args)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+def _tiny_gr(tmp_path):
+    ka = json.load(open(GOLDEN))["tiny_gr"]
+    ends = np.array(ka["row_ends"], np.int64)
+    rowptr = np.concatenate([[0], ends])
+    g = M.from_csr(rowptr, np.array(ka["destinations"], np.int32), np.array(ka["weights"], np.float32))
+    path = str(tmp_path / "tiny.gr")
+    g.write_galois_gr(path)
+    return path, ka
+
+
+@pytest.mark.gpu
+def test_tiny_graph_output_and_summary(tmp_path):
+    path, ka = _tiny_gr(tmp_path)
+    out = tmp_path / "dist.txt"
+    summary = tmp_path / "fd3.txt"
+    # a shell gives the driver its file descriptor 3
+    r = subprocess.run(["bash", "-c", 'exec "$0" "$@" 3>"$FD3_FILE"', CLI, "-q", "-s", str(ka["source"]),
+                        "-o", str(out), path], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, FD3_FILE=str(summary)))
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("processed vertices=") == 8  # ADDS_RUNS default
+    lines = out.read_text().splitlines()
+    want = [f"{i} INF" if d is None else f"{i} {d:.9g}" for i, d in enumerate(ka["expected"])]
+    assert lines == want
+    fields = summary.read_text().split()
+    assert fields[0] == path and float(fields[1]) > 0 and int(fields[2]) >= 4
+
+
+@pytest.mark.gpu
+def test_runs_env_and_stdout_output(tmp_path):
+    path, ka = _tiny_gr(tmp_path)
+    r = _run(["-s", "0", "-o", "-", path], env=dict(os.environ, ADDS_RUNS="2"))
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("processed vertices=") == 2
+    assert "0 0\n" in r.stdout and "4 INF" in r.stdout
+    assert "file descriptor 3 is not open" in r.stderr
+    r = _run(["-s", "7", path])
+    assert r.returncode == 1 and "outside" in r.stderr
