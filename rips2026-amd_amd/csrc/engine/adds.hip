@@ -198,8 +198,10 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   // count first: pending covers the entries before anyone can claim them
+  // pending covers the entries before they are offered (the wait before the
+  // avail update below); the producer's own claim keeps pending above zero
+  // meanwhile
   if (lane == 0) atomicAdd(&c->pending, (unsigned long long)st.n);
-  vm_wait();  // pending covers the entries before their reservations exist
   if (lane < (unsigned)kAddsSlots) st.hist[lane] = 0;
   __builtin_amdgcn_wave_barrier();
   for (uint32_t i = lane; i < st.n; i += kWave) {
@@ -274,8 +276,7 @@ __device__ inline void heavy_publish(const AddsArgs& a, int* stage, uint32_t& n,
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   uint32_t pos0 = 0;
   if (lane == 0) {
-    atomicAdd(&c->pending, (unsigned long long)n);
-    vm_wait();
+    atomicAdd(&c->pending, (unsigned long long)n);  // completed before the avail update below
     pos0 = atomicAdd(&HV_TAIL(c, hs), n);
     if ((int)(pos0 + n - ld_rlx(&HV_HEAD(c, hs))) > (int)(a.cap_mask + 1u)) {
       atomicOr(&c->err, kErrRing);
@@ -497,13 +498,27 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
     const long long t_top = wall_clock64();
     if (wv == 0) {
       uint32_t got = 0, slot = 0, qslot = 0, begin = 0, kind = kClaimItems, done = 0;
-      if (lane == 0) {
-        done = ld_rlx(&c->done);
+      // Probe the control words in two wave-wide rounds of loads (the base,
+      // then done / hub / heavy-list / active counts and the window's
+      // semaphores on separate lanes) instead of a chain of dependent loads.
+      const unsigned long long bw0 = __shfl(lane == 0 ? ld_rlx(&c->base_word) : 0ull, 0, 64);
+      const uint32_t base0 = base_of(bw0);
+      int probe = 0;
+      if (lane == 0) probe = (int)ld_rlx(&c->done);
+      else if (lane == 1) probe = ld_rlx(&c->hub_avail);
+      else if (lane == 2) probe = base0 > 0 ? ld_rlx(&HV_AVAIL(c, (base0 - 1) % kAddsSlots)) : 0;
+      else if (lane == 3) probe = (int)ld_rlx(&c->active);
+      else if (lane >= 4 && lane < 4u + kAddsSlots) probe = ld_rlx(&AVAIL(c, (base0 + lane - 4) % kAddsSlots));
+      done = (uint32_t)__shfl(probe, 0, 64);
+      const int hub_n = __shfl(probe, 1, 64), hv_n0 = __shfl(probe, 2, 64);
+      const uint32_t A = max(1u, min((uint32_t)__shfl(probe, 3, 64), (uint32_t)kAddsSlots));
+      const unsigned long long ready = __ballot(lane >= 4 && lane < 4 + A && probe > 0) >> 4;
+      if (lane == 0 && !done) {
         // hub chunks first (they finish claimed work), then the heavy list of
         // the bucket just finished, then the active buckets.  Every claim is a
         // counting-semaphore decrement: no retry loop for hundreds of
         // workgroups to serialise on.
-        if (!done && ld_rlx(&c->hub_avail) > 0) {
+        if (hub_n > 0) {
           if (atomicSub(&c->hub_avail, 1) >= 1) {
             got = 1;
             kind = kClaimHub;
@@ -513,36 +528,28 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
             ++d_cas;
           }
         }
-        if (!done && !got) {
-          const unsigned long long bw = ld_rlx(&c->base_word);
-          const uint32_t base = base_of(bw);
-          if (base > 0) {
-            const uint32_t hs = (base - 1) % kAddsSlots;
-            if (ld_rlx(&HV_AVAIL(c, hs)) > 0) {
-              // pinned on slot base: bucket base cannot finish before the heavy
-              // edges of bucket base - 1 are relaxed
-              const uint32_t ps = base % kAddsSlots;
-              got = sem_claim(&HV_AVAIL(c, hs), &HV_HEAD(c, hs), &INFL(c, ps), &begin);
-              if (got) {
-                kind = kClaimHeavy;
-                slot = ps;
-                qslot = hs;
-              } else {
-                ++d_cas;
-              }
-            }
+        if (!got && base0 > 0 && hv_n0 > 0) {
+          const uint32_t hs = (base0 - 1) % kAddsSlots;
+          // pinned on slot base: bucket base cannot finish before the heavy
+          // edges of bucket base - 1 are relaxed
+          const uint32_t ps = base0 % kAddsSlots;
+          got = sem_claim(&HV_AVAIL(c, hs), &HV_HEAD(c, hs), &INFL(c, ps), &begin);
+          if (got) {
+            kind = kClaimHeavy;
+            slot = ps;
+            qslot = hs;
+          } else {
+            ++d_cas;
           }
-          const uint32_t A = max(1u, min(ld_rlx(&c->active), (uint32_t)kAddsSlots));
-          for (uint32_t j = 0; j < A && !got; ++j) {
-            const uint32_t s = (base + j) % kAddsSlots;
-            if (ld_rlx(&AVAIL(c, s)) <= 0) continue;
-            got = sem_claim(&AVAIL(c, s), &HEAD(c, s), &INFL(c, s), &begin);
-            if (got) {
-              slot = s;
-              qslot = s;
-            } else {
-              ++d_cas;
-            }
+        }
+        for (unsigned long long m = ready; m && !got; m &= m - 1) {
+          const uint32_t s = (base0 + (uint32_t)__ffsll((long long)m) - 1) % kAddsSlots;
+          got = sem_claim(&AVAIL(c, s), &HEAD(c, s), &INFL(c, s), &begin);
+          if (got) {
+            slot = s;
+            qslot = s;
+          } else {
+            ++d_cas;
           }
         }
       }

@@ -64,10 +64,14 @@ def test_tiny_graph_output_and_summary(tmp_path):
 @pytest.mark.gpu
 def test_runs_env_and_stdout_output(tmp_path):
     path, ka = _tiny_gr(tmp_path)
-    r = _run(["-s", "0", "-o", "-", path], env=dict(os.environ, ADDS_RUNS="2"))
+    # file descriptor 3 always points at a file of the test's own
+    fd3 = tmp_path / "fd3.txt"
+    r = subprocess.run(["bash", "-c", 'exec "$0" "$@" 3>"$FD3_FILE"', CLI, "-s", "0", "-o", "-", path],
+                       capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, ADDS_RUNS="2", FD3_FILE=str(fd3)))
     assert r.returncode == 0, r.stderr
     assert r.stdout.count("processed vertices=") == 2
     assert "0 0\n" in r.stdout and "4 INF" in r.stdout
-    assert "file descriptor 3 is not open" in r.stderr
+    assert len(fd3.read_text().splitlines()) == 1
     r = _run(["-s", "7", path])
     assert r.returncode == 1 and "outside" in r.stderr
