@@ -423,15 +423,16 @@ int mi355_batch_autotune(mi355_graph* g, const mi355_query_batch* qb, int32_t sa
 /* ---- learned delta policy ---------------------------------------------
  * No reference counterpart: the network replaces the fixed per-phase delta
  * heuristics (JAS/hip_adds_sssp.cpp:683-727, ONE/sssp_hip.cpp:564-571,
- * ADDSF/wl.h:349-410).  32 phase features -> 64 hidden (ReLU) -> 2 outputs:
- * log2 of the bucket width relative to the heuristic delta0, and the number
- * of buckets served at once.  The engine evaluates bf16-rounded weights with
+ * ADDSF/wl.h:349-410).  32 phase features -> 64 hidden (ReLU) -> 3 outputs:
+ * log2 of the bucket width relative to the heuristic delta0, the number of
+ * buckets served at once, and log2 of the share of the persistent workgroups
+ * that claim work.  The engine evaluates bf16-rounded weights with
  * fp32 accumulation; training keeps fp32 master weights. */
 typedef struct {
   float w1[64 * 32];       /* hidden x input, row-major */
   float b1[64];
-  float w2[2 * 64];        /* output x hidden, row-major */
-  float b2[2];
+  float w2[3 * 64];        /* output x hidden, row-major */
+  float b2[3];
   float feat_mean[32];     /* inputs are normalised (x - mean) * inv_std; not trained */
   float feat_inv_std[32];
 } mi355_mlp_params;
@@ -444,16 +445,18 @@ typedef struct {
 } mi355_mlp_train_options;
 
 void mi355_mlp_train_options_default(mi355_mlp_train_options* o);
-/* x: n x 32 features (host), out: n x 2 outputs (host). */
+/* x: n x 32 features (host), out: n x 3 outputs (host). */
 int mi355_mlp_forward(const mi355_mlp_params* p, const float* x, int64_t n, float* out);
 /* Mean loss (1/n) sum w_s |o_s - y_s|^2 and its gradient (normalisation
- * fields of grad are zero).  y: n x 2 targets, w: n sample weights or NULL. */
+ * fields of grad are zero).  y: n x 3 targets, w: n sample weights or NULL. */
 int mi355_mlp_loss_grad(const mi355_mlp_params* p, const float* x, const float* y, const float* w,
                         int64_t n, float* loss, mi355_mlp_params* grad);
 /* Adam on the device; p is updated in place; epoch_loss (host, may be NULL)
  * receives the mean training loss of each epoch. */
 int mi355_mlp_train(mi355_mlp_params* p, const float* x, const float* y, const float* w, int64_t n,
                     const mi355_mlp_train_options* o, float* epoch_loss);
+/* Persistent workgroups an ADDS run launches when worker_blocks is 0. */
+int mi355_adds_default_workers(int32_t* workers);
 /* Load / drop the model used by mi355_sssp_options.use_mlp (process-wide). */
 int mi355_adds_mlp_set(const mi355_mlp_params* p);
 int mi355_adds_mlp_clear(void);
@@ -462,7 +465,8 @@ int mi355_adds_mlp_clear(void);
  * floats: [0,32) the phase features (before the model's normalisation), 32
  * items queued in the window, 33 the delta
  * chosen for the next phase, 34 active buckets, 35 edges relaxed so far,
- * 36 pushes so far.  trace: host array of cap_phases records; phases receives
+ * 36 pushes so far, 37 workgroups claiming work.  trace: host array of
+ * cap_phases records; phases receives
  * the number of phase boundaries (records beyond cap_phases are dropped). */
 #define MI355_ADDS_TRACE_STRIDE 40
 int mi355_adds_trace(mi355_graph* g, int32_t source, const mi355_sssp_options* opt, float* trace,

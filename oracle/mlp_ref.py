@@ -1,7 +1,7 @@
 """CPU restatement of the learned delta policy's network (test infrastructure
 only: loaded by tests, never by the product).
 
-The network the ADDS scheduler evaluates (32 -> 64 ReLU -> 2) has no
+The network the ADDS scheduler evaluates (32 -> 64 ReLU -> 3) has no
 counterpart in the reference; it replaces the fixed per-phase heuristics of
 JAS/hip_adds_sssp.cpp:683-727 and ONE/sssp_hip.cpp:564-571.  This module
 restates its forward pass and the gradient of the training loss in float64
@@ -22,17 +22,21 @@ def bf16_round(a) -> np.ndarray:
     return u.astype(np.uint32).view(np.float32)
 
 
+N_OUT = 3
+N_PARAMS = 64 * 32 + 64 + N_OUT * 64 + N_OUT  # 2307
+
+
 def unpack(flat):
     flat = np.asarray(flat, np.float64)
     w1 = flat[:2048].reshape(64, 32)
     b1 = flat[2048:2112]
-    w2 = flat[2112:2240].reshape(2, 64)
-    b2 = flat[2240:2242]
+    w2 = flat[2112:2112 + 64 * N_OUT].reshape(N_OUT, 64)
+    b2 = flat[2112 + 64 * N_OUT:N_PARAMS]
     return w1, b1, w2, b2
 
 
 def forward(flat, mean, inv_std, x, round_weights=True):
-    """Outputs (n x 2) and hidden activations (n x 64)."""
+    """Outputs (n x 3) and hidden activations (n x 64)."""
     w1, b1, w2, b2 = unpack(flat)
     if round_weights:
         w1 = bf16_round(w1).astype(np.float64)
@@ -50,7 +54,7 @@ def loss(flat, mean, inv_std, x, y, w=None, round_weights=True) -> float:
 
 
 def loss_grad(flat, mean, inv_std, x, y, w=None, round_weights=True):
-    """(loss, gradient w.r.t. the 2242 trained parameters).  With rounded
+    """(loss, gradient w.r.t. the 2307 trained parameters).  With rounded
     weights the gradient is taken through the rounding unchanged
     (straight-through), as the device does."""
     w1, b1, w2, b2 = unpack(flat)

@@ -342,6 +342,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   for (int off = 32; off > 0; off >>= 1) window_items += __shfl_xor(window_items, off, 64);
   float delta = ld_rlx(&c->delta);
   uint32_t active = ld_rlx(&c->active);
+  uint32_t workers_now = ld_rlx(&c->active_workers);
   const int mode = a.policy ? a.policy->mode : kPolicyFixed;
   float feat = 0.0f, raw = 0.0f;
   if (mode == kPolicyMlp || (a.policy && a.policy->trace))
@@ -351,8 +352,12 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
     if ((float)window_items < c->util_low * capacity) delta = fminf(delta * c->growth, c->max_delta);
     else if ((float)window_items > c->util_high * capacity) delta = fmaxf(delta * c->shrink, c->min_delta);
   } else if (mode == kPolicyMlp) {
-    float o0 = 0.0f, o1 = 0.0f;
-    mlp_forward_wave(a.policy, feat, lane, &o0, &o1);
+    float o[kMlpOut];
+    mlp_forward_wave(a.policy, feat, lane, o);
+    const float o0 = o[0], o1 = o[1];
+    const float share = exp2f(fminf(fmaxf(o[2], -5.0f), 0.0f));
+    workers_now = (uint32_t)fmaxf((float)c->workers * share, 16.0f);
+    workers_now = min(workers_now, c->workers);
     const float scale = exp2f(fminf(fmaxf(o0, -4.0f), 4.0f));
     delta = fminf(fmaxf(c->base_delta * scale, c->min_delta), c->max_delta);
     active = (uint32_t)fminf(fmaxf(o1 + 0.5f, 1.0f), 8.0f);
@@ -367,11 +372,13 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
       t[34] = (float)active;
       t[35] = (float)ld_rlx(&c->edges);
       t[36] = (float)ld_rlx(&c->pushes);
+      t[37] = (float)workers_now;
     }
   }
   if (lane == 0) {
     __hip_atomic_store(&c->delta, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&c->active, active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&c->active_workers, workers_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd(&c->phases, 1u);
   }
   return true;
@@ -513,7 +520,13 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       const int hub_n = __shfl(probe, 1, 64), hv_n0 = __shfl(probe, 2, 64);
       const uint32_t A = max(1u, min((uint32_t)__shfl(probe, 3, 64), (uint32_t)kAddsSlots));
       const unsigned long long ready = __ballot(lane >= 4 && lane < 4 + A && probe > 0) >> 4;
-      if (lane == 0 && !done) {
+      // Workgroups beyond the policy's worker share do not claim: fewer
+      // claimers contend less on the control words when a phase holds little
+      // work (high-diameter graphs).  They still end the run and may advance
+      // the phase.
+      const uint32_t share = __shfl(lane == 0 ? ld_rlx(&c->active_workers) : 0u, 0, 64);
+      const bool claimer = blockIdx.x < max(share, 1u);
+      if (lane == 0 && !done && claimer) {
         // hub chunks first (they finish claimed work), then the heavy list of
         // the bucket just finished, then the active buckets.  Every claim is a
         // counting-semaphore decrement: no retry loop for hundreds of
@@ -855,6 +868,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   h.shrink = 0.75f;
   h.active = (uint32_t)std::min(std::max(req.active_buckets, 1), kAddsSlots);
   h.workers = (uint32_t)workers;
+  h.active_workers = (uint32_t)workers;
   h.pending = 1;
   h.tail[0] = 1;
   h.avail[0] = 1;

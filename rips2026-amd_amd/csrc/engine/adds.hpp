@@ -9,7 +9,7 @@ namespace mi355 {
 constexpr int kAddsSlots = 32;     // circular bucket window
 constexpr int kMlpIn = 32;         // phase features
 constexpr int kMlpHidden = 64;
-constexpr int kMlpOut = 2;         // log2 delta scale, active buckets
+constexpr int kMlpOut = 3;         // log2 delta scale, active buckets, log2 worker share
 constexpr int kTraceStride = 40;   // floats per traced phase
 
 enum PolicyMode { kPolicyFixed = 0, kPolicyHeuristic = 1, kPolicyMlp = 2 };
@@ -28,7 +28,8 @@ struct AddsCtrl {
   uint32_t workers;
   uint32_t phases;
   uint32_t err;
-  uint32_t pad0[kPad - 12];
+  uint32_t active_workers;       // workgroups that claim work (the rest only idle)
+  uint32_t pad0[kPad - 13];
   unsigned long long base_word;  // (T_base float bits << 32) | base bucket
   uint32_t pad1[kPad - 2];
   unsigned long long pending;    // items queued or being processed (two's-complement adds)
@@ -101,22 +102,19 @@ __device__ inline void mlp_features(const AddsPolicy* p, const AddsCtrl* c, uint
   *feat = lane < kMlpIn ? x : 0.0f;
 }
 
-// 32 -> 64 (ReLU) -> 2 on one wave: lane j owns hidden unit j.
-__device__ inline void mlp_forward_wave(const AddsPolicy* p, float feat, unsigned lane, float* out0,
-                                        float* out1) {
+// 32 -> 64 (ReLU) -> 3 on one wave: lane j owns hidden unit j.
+__device__ inline void mlp_forward_wave(const AddsPolicy* p, float feat, unsigned lane, float* out) {
   float h = p->b1[lane];
 #pragma unroll
   for (int i = 0; i < kMlpIn; ++i) h += bf16_to_f32(p->w1[lane * kMlpIn + i]) * __shfl(feat, i, 64);
   h = fmaxf(h, 0.0f);
-  float o0 = bf16_to_f32(p->w2[lane]) * h;
-  float o1 = bf16_to_f32(p->w2[kMlpHidden + lane]) * h;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    o0 += __shfl_xor(o0, off, 64);
-    o1 += __shfl_xor(o1, off, 64);
+  for (int k = 0; k < kMlpOut; ++k) {
+    float o = bf16_to_f32(p->w2[k * kMlpHidden + lane]) * h;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off, 64);
+    out[k] = o + p->b2[k];
   }
-  *out0 = o0 + p->b2[0];
-  *out1 = o1 + p->b2[1];
 }
 
 }  // namespace mi355

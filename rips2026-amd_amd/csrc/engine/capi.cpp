@@ -710,6 +710,13 @@ int mi355_mlp_train(mi355_mlp_params* p, const float* x, const float* y, const f
   });
 }
 
+int mi355_adds_default_workers(int32_t* workers) {
+  return guard([&] {
+    if (!workers) fail(MI355_EINVAL, "null argument");
+    *workers = mi355::device_props().cus * 2;
+  });
+}
+
 int mi355_adds_mlp_set(const mi355_mlp_params* p) {
   return guard([&] { mi355::set_mlp_params(p); });
 }

@@ -1,7 +1,7 @@
 // The learned delta policy's network, trained and evaluated on the device.
 //
-// 32 phase features -> 64 hidden (ReLU) -> 2 outputs (log2 of the bucket-width
-// scale, active buckets).  Weights are kept as an fp32 master copy for
+// 32 phase features -> 64 hidden (ReLU) -> 3 outputs (log2 of the bucket-width
+// scale, active buckets, log2 of the claiming worker share).  Weights are kept as an fp32 master copy for
 // training; every evaluation (here and inside the ADDS kernel) uses the
 // weights rounded to bf16 (round to nearest even) with fp32 accumulation, in
 // the same operation order, so a trained model behaves on the worklist
@@ -12,7 +12,7 @@
 // the inputs are broadcast with shuffles, the two outputs reduced with an xor
 // butterfly.  Training: per-wave gradient partials, a fixed-order reduction
 // (deterministic), then Adam on the fp32 master weights.  Loss:
-// (1/N) sum_s w_s ((o0 - y0)^2 + (o1 - y1)^2).
+// (1/N) sum_s w_s sum_k (o_k - y_k)^2.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -26,7 +26,7 @@ namespace mi355 {
 
 namespace {
 
-constexpr int kP = kMlpHidden * kMlpIn + kMlpHidden + kMlpOut * kMlpHidden + kMlpOut;  // 2242
+constexpr int kP = kMlpHidden * kMlpIn + kMlpHidden + kMlpOut * kMlpHidden + kMlpOut;  // 2307
 constexpr int kOffB1 = kMlpHidden * kMlpIn;
 constexpr int kOffW2 = kOffB1 + kMlpHidden;
 constexpr int kOffB2 = kOffW2 + kMlpOut * kMlpHidden;
@@ -42,10 +42,12 @@ __device__ inline float bf16_round(float x) {
   return __uint_as_float(u & 0xFFFF0000u);
 }
 
-// Wave-resident copy of the model: lane j holds row j of w1 (bf16-rounded).
+// Wave-resident copy of the model: lane j holds row j of w1 and column j of
+// w2 (bf16-rounded).
 struct WaveModel {
   float w1[kMlpIn];
-  float b1, w20, w21, b20, b21;
+  float b1;
+  float w2[kMlpOut], b2[kMlpOut];
   float mean, inv;  // normalisation of feature `lane` (lanes < 32)
 };
 
@@ -53,29 +55,28 @@ __device__ inline void load_model(const float* __restrict__ p, unsigned lane, Wa
 #pragma unroll
   for (int i = 0; i < kMlpIn; ++i) m.w1[i] = bf16_round(p[lane * kMlpIn + i]);
   m.b1 = p[kOffB1 + lane];
-  m.w20 = bf16_round(p[kOffW2 + lane]);
-  m.w21 = bf16_round(p[kOffW2 + kMlpHidden + lane]);
-  m.b20 = p[kOffB2];
-  m.b21 = p[kOffB2 + 1];
+#pragma unroll
+  for (int k = 0; k < kMlpOut; ++k) {
+    m.w2[k] = bf16_round(p[kOffW2 + k * kMlpHidden + lane]);
+    m.b2[k] = p[kOffB2 + k];
+  }
   m.mean = lane < (unsigned)kMlpIn ? p[kOffMean + lane] : 0.0f;
   m.inv = lane < (unsigned)kMlpIn ? p[kOffInv + lane] : 0.0f;
 }
 
 // Same arithmetic as mlp_forward_wave() in the ADDS kernel.
-__device__ inline void forward(const WaveModel& m, float xn, float& h, float& o0, float& o1) {
+__device__ inline void forward(const WaveModel& m, float xn, float& h, float (&o)[kMlpOut]) {
   h = m.b1;
 #pragma unroll
   for (int i = 0; i < kMlpIn; ++i) h += m.w1[i] * __shfl(xn, i, 64);
   h = fmaxf(h, 0.0f);
-  o0 = m.w20 * h;
-  o1 = m.w21 * h;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    o0 += __shfl_xor(o0, off, 64);
-    o1 += __shfl_xor(o1, off, 64);
+  for (int k = 0; k < kMlpOut; ++k) {
+    float v = m.w2[k] * h;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    o[k] = v + m.b2[k];
   }
-  o0 += m.b20;
-  o1 += m.b21;
 }
 
 __global__ __launch_bounds__(kBlock) void k_mlp_forward(const float* __restrict__ p,
@@ -87,12 +88,10 @@ __global__ __launch_bounds__(kBlock) void k_mlp_forward(const float* __restrict_
   const int64_t waves = (int64_t)gridDim.x * kWavesPerBlock;
   for (int64_t s = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; s < n; s += waves) {
     const float xn = lane < (unsigned)kMlpIn ? (x[s * kMlpIn + lane] - m.mean) * m.inv : 0.0f;
-    float h, o0, o1;
-    forward(m, xn, h, o0, o1);
-    if (lane == 0) {
-      out[s * 2] = o0;
-      out[s * 2 + 1] = o1;
-    }
+    float h, o[kMlpOut];
+    forward(m, xn, h, o);
+    if (lane == 0)
+      for (int k = 0; k < kMlpOut; ++k) out[s * kMlpOut + k] = o[k];
   }
 }
 
@@ -111,24 +110,29 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* __restrict__ p
   float g1[kMlpIn];
 #pragma unroll
   for (int i = 0; i < kMlpIn; ++i) g1[i] = 0.0f;
-  float gb1 = 0.0f, gw20 = 0.0f, gw21 = 0.0f, gb20 = 0.0f, gb21 = 0.0f, loss = 0.0f;
+  float gb1 = 0.0f, loss = 0.0f, gw2[kMlpOut], gb2[kMlpOut];
+#pragma unroll
+  for (int k = 0; k < kMlpOut; ++k) gw2[k] = gb2[k] = 0.0f;
   const int64_t wave = (int64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   const int64_t waves = (int64_t)gridDim.x * kWavesPerBlock;
   const float inv_n = 1.0f / norm;
   for (int64_t s = wave; s < count; s += waves) {
     const int64_t idx = order ? (int64_t)order[first + s] : first + s;
     const float xn = lane < (unsigned)kMlpIn ? (x[idx * kMlpIn + lane] - m.mean) * m.inv : 0.0f;
-    float h, o0, o1;
-    forward(m, xn, h, o0, o1);
+    float h, o[kMlpOut];
+    forward(m, xn, h, o);
     const float wt = sw ? sw[idx] : 1.0f;
-    const float e0 = o0 - y[idx * 2], e1 = o1 - y[idx * 2 + 1];
-    const float d0 = 2.0f * wt * e0 * inv_n, d1 = 2.0f * wt * e1 * inv_n;
-    loss += wt * (e0 * e0 + e1 * e1) * inv_n;
-    gb20 += d0;
-    gb21 += d1;
-    gw20 += d0 * h;
-    gw21 += d1 * h;
-    const float dh = h > 0.0f ? d0 * m.w20 + d1 * m.w21 : 0.0f;
+    float dh = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kMlpOut; ++k) {
+      const float e = o[k] - y[idx * kMlpOut + k];
+      const float d = 2.0f * wt * e * inv_n;
+      loss += wt * e * e * inv_n;
+      gb2[k] += d;
+      gw2[k] += d * h;
+      dh += d * m.w2[k];
+    }
+    dh = h > 0.0f ? dh : 0.0f;
     gb1 += dh;
 #pragma unroll
     for (int i = 0; i < kMlpIn; ++i) g1[i] += dh * __shfl(xn, i, 64);
@@ -137,11 +141,10 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float* __restrict__ p
 #pragma unroll
   for (int i = 0; i < kMlpIn; ++i) rec[lane * kMlpIn + i] = g1[i];
   rec[kOffB1 + lane] = gb1;
-  rec[kOffW2 + lane] = gw20;
-  rec[kOffW2 + kMlpHidden + lane] = gw21;
+#pragma unroll
+  for (int k = 0; k < kMlpOut; ++k) rec[kOffW2 + k * kMlpHidden + lane] = gw2[k];
   if (lane == 0) {
-    rec[kOffB2] = gb20;
-    rec[kOffB2 + 1] = gb21;
+    for (int k = 0; k < kMlpOut; ++k) rec[kOffB2 + k] = gb2[k];
     rec[kP] = loss;
   }
 }
@@ -203,11 +206,11 @@ void mlp_forward(const mi355_mlp_params* prm, const float* h_x, int64_t n, float
   DevData d;
   upload(d.p, reinterpret_cast<const float*>(prm), kParamsWithNorm, s);
   upload(d.x, h_x, (size_t)n * kMlpIn, s);
-  DevBuf<float> out((size_t)n * 2);
+  DevBuf<float> out((size_t)n * kMlpOut);
   const unsigned grid = (unsigned)std::min<int64_t>(div_up((uint64_t)n, kWavesPerBlock), 4096);
   k_mlp_forward<<<grid, kBlock, 0, s>>>(d.p.p, d.x.p, n, out.p);
   MI_LAUNCH_CHECK();
-  MI_HIP(hipMemcpyAsync(h_out, out.p, (size_t)n * 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+  MI_HIP(hipMemcpyAsync(h_out, out.p, (size_t)n * kMlpOut * sizeof(float), hipMemcpyDeviceToHost, s));
   MI_HIP(hipStreamSynchronize(s));
 }
 
@@ -219,7 +222,7 @@ void mlp_loss_grad(const mi355_mlp_params* prm, const float* h_x, const float* h
   DevData d;
   upload(d.p, reinterpret_cast<const float*>(prm), kParamsWithNorm, s);
   upload(d.x, h_x, (size_t)n * kMlpIn, s);
-  upload(d.y, h_y, (size_t)n * 2, s);
+  upload(d.y, h_y, (size_t)n * kMlpOut, s);
   if (h_w) upload(d.sw, h_w, (size_t)n, s);
   d.partial.ensure((size_t)kGradWaves * kRec);
   d.grad.ensure(kRec);
@@ -249,7 +252,7 @@ void mlp_train(mi355_mlp_params* prm, const float* h_x, const float* h_y, const 
   DevData d;
   upload(d.p, reinterpret_cast<const float*>(prm), kParamsWithNorm, s);
   upload(d.x, h_x, (size_t)n * kMlpIn, s);
-  upload(d.y, h_y, (size_t)n * 2, s);
+  upload(d.y, h_y, (size_t)n * kMlpOut, s);
   if (h_w) upload(d.sw, h_w, (size_t)n, s);
   d.partial.ensure((size_t)kGradWaves * kRec);
   d.grad.ensure(kRec);

@@ -57,8 +57,8 @@ class SsspStats(C.Structure):
 
 
 class MlpParams(C.Structure):
-    _fields_ = [("w1", C.c_float * (64 * 32)), ("b1", C.c_float * 64), ("w2", C.c_float * (2 * 64)),
-                ("b2", C.c_float * 2), ("feat_mean", C.c_float * 32), ("feat_inv_std", C.c_float * 32)]
+    _fields_ = [("w1", C.c_float * (64 * 32)), ("b1", C.c_float * 64), ("w2", C.c_float * (3 * 64)),
+                ("b2", C.c_float * 3), ("feat_mean", C.c_float * 32), ("feat_inv_std", C.c_float * 32)]
 
 
 class MlpTrainOptions(C.Structure):
@@ -166,6 +166,7 @@ _ENGINE_SIGS = {
                                       C.POINTER(MlpParams)]),
     "mi355_mlp_train": (C.c_int, [C.POINTER(MlpParams), P, P, P, I64, C.POINTER(MlpTrainOptions), P]),
     "mi355_adds_mlp_set": (C.c_int, [C.POINTER(MlpParams)]),
+    "mi355_adds_default_workers": (C.c_int, [C.POINTER(C.c_int32)]),
     "mi355_adds_mlp_clear": (C.c_int, []),
     "mi355_adds_trace": (C.c_int, [P, I32, C.POINTER(SsspOptions), P, I32, C.POINTER(I32),
                                    C.POINTER(SsspStats), P]),

@@ -3,12 +3,14 @@ training.  Every forward pass, gradient and optimiser step runs on the device
 through the C ABI (mi355_mlp_*); this module only moves arrays.
 
 The network maps the 32 features of a phase boundary to (log2 of the bucket
-width relative to the heuristic delta0, number of buckets served at once).
-It is trained by regression on traces that ADDS itself records: for each
-training graph ADDS is run under a grid of static (width scale, active
-buckets) settings, the fastest setting (kernel time) becomes the target of
-every phase record of that graph, and the work done and relaxations wasted
-of each setting are kept in the dataset's metadata.
+width relative to the heuristic delta0, number of buckets served at once,
+log2 of the share of the persistent workgroups that claim work).  It is
+trained by regression on traces that ADDS itself records: for each training
+graph ADDS is timed under static settings (first the worker share, then the
+width scale x active buckets grid at the best share), the fastest setting
+(kernel time) becomes the target of every phase record of that graph, and the
+work done and relaxations wasted of each setting are kept in the dataset's
+metadata.
 """
 from __future__ import annotations
 
@@ -23,11 +25,14 @@ import numpy as np
 from ._abi import (ADDS_TRACE_STRIDE, ALG_ADDS, MlpParams, MlpTrainOptions, SsspOptions, SsspStats,
                    check, engine)
 
-N_IN, N_HID, N_OUT = 32, 64, 2
+N_IN, N_HID, N_OUT = 32, 64, 3
 _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(_HERE, "data", "delta_mlp.json")
 
-# static (width scale, active buckets) settings searched per training graph
+# static settings searched per training graph: the worker share (at width
+# scale 1/2, 2 active buckets), then (width scale, active buckets) at the best
+# share
+SHARES = (1.0, 0.5, 0.25, 0.125)
 SCALES = (0.25, 0.5, 1.0, 2.0, 4.0)
 ACTIVE = (1, 2, 4)
 SETTINGS = [(s, a) for a in ACTIVE for s in SCALES]
@@ -57,7 +62,7 @@ class DeltaModel:
         rng = np.random.default_rng(seed)
         w1 = rng.normal(0.0, math.sqrt(2.0 / N_IN), (N_HID, N_IN))
         w2 = rng.normal(0.0, 0.1 * math.sqrt(1.0 / N_HID), (N_OUT, N_HID))
-        return cls(w1, np.zeros(N_HID), w2, np.array([0.0, 1.0]))
+        return cls(w1, np.zeros(N_HID), w2, np.array([0.0, 1.0, 0.0]))
 
     # -- conversions --------------------------------------------------------
     def to_c(self) -> MlpParams:
@@ -162,11 +167,28 @@ def trace(dg, source: int, cap: int = 4096, **opts):
     return buf[: min(phases.value, cap)].copy(), st
 
 
+_default_workers = None
+
+
+def default_workers() -> int:
+    global _default_workers
+    if _default_workers is None:
+        w = C.c_int32()
+        check(engine().mi355_adds_default_workers(C.byref(w)))
+        _default_workers = int(w.value)
+    return _default_workers
+
+
+def workers_for(share: float) -> int:
+    """Launch size of a static worker share (0 = the default launch)."""
+    return 0 if share >= 1.0 else max(16, int(round(default_workers() * share)))
+
+
 def timed_adds(dg, source: int, scale: float, active: int, delta0: float, reps: int = 2,
-               use_mlp: bool = False):
+               use_mlp: bool = False, share: float = 1.0):
     """Best kernel time (ms) over reps of ADDS at width scale * delta0."""
     o = dg.options(ALG_ADDS, delta=0.0 if use_mlp else scale * delta0, active_buckets=active,
-                   use_mlp=int(use_mlp))
+                   use_mlp=int(use_mlp), worker_blocks=workers_for(share))
     best, st_best = float("inf"), None
     for _ in range(reps):
         st = dg.sssp_labels(int(source), o)
@@ -211,23 +233,34 @@ def collect_graph(dg, source: int, reps: int = 2, max_records: int = 64):
     delta0 = st0.initial_delta
     m_r = int(st0.reached_edges)
     costs = {}
-    for (s, a) in SETTINGS:
-        ms, st = timed_adds(dg, source, s, a, delta0, reps)
+
+    def run(s, a, w):
+        ms, st = timed_adds(dg, source, s, a, delta0, reps, share=w)
         # (kernel ms, edges relaxed, relaxations beyond one per reached edge)
-        costs[(s, a)] = (ms, int(st.edges_relaxed), int(st.edges_relaxed) - m_r)
+        costs[(s, a, w)] = (ms, int(st.edges_relaxed), int(st.edges_relaxed) - m_r)
+
+    for w in SHARES:
+        run(0.5, 2, w)
+    w_best = min((k for k in costs), key=lambda k: costs[k][0])[2]
+    for (s, a) in SETTINGS:
+        if (s, a, w_best) not in costs:
+            run(s, a, w_best)
+    if (1.0, 1, 1.0) not in costs:
+        run(1.0, 1, 1.0)
     best = min(costs, key=lambda k: costs[k][0])
-    target = np.array([math.log2(best[0]), float(best[1])], np.float32)
+    target = np.array([math.log2(best[0]), float(best[1]), math.log2(best[2])], np.float32)
     xs = []
-    for (s, a) in [best] + ([(1.0, 1)] if best != (1.0, 1) else []):
-        rec, _ = trace(dg, source, cap=4096, delta=s * delta0, active_buckets=a)
+    for (s, a, w) in [best] + ([(1.0, 1, 1.0)] if best != (1.0, 1, 1.0) else []):
+        rec, _ = trace(dg, source, cap=4096, delta=s * delta0, active_buckets=a,
+                       worker_blocks=workers_for(w))
         if len(rec) > max_records:
             rec = rec[np.linspace(0, len(rec) - 1, max_records).astype(int)]
         xs.append(rec[:, :N_IN])
     x = np.concatenate(xs) if xs else np.zeros((0, N_IN), np.float32)
     y = np.repeat(target[None, :], len(x), axis=0)
     meta = {"delta0": delta0, "best": list(best), "best_ms": costs[best][0],
-            "plain_ms": costs[(1.0, 1)][0],
-            "costs": {f"{s}x{a}": list(v) for (s, a), v in costs.items()}}
+            "plain_ms": costs[(1.0, 1, 1.0)][0],
+            "costs": {f"{s}x{a}x{w}": list(v) for (s, a, w), v in costs.items()}}
     return x, y, meta
 
 

@@ -82,9 +82,10 @@ def main() -> int:
         "loss_last_epoch": float(losses[-1]),
         "mean_abs_err_log2_scale": float(np.abs(pred[:, 0] - y[:, 0]).mean()),
         "mean_abs_err_active": float(np.abs(pred[:, 1] - y[:, 1]).mean()),
+        "mean_abs_err_log2_share": float(np.abs(pred[:, 2] - y[:, 2]).mean()),
         "oracle_static_gain_mean": float(np.mean(best_gain)),
         "oracle_static_gain_median": float(np.median(best_gain)),
-        "settings": [list(s) for s in L.SETTINGS],
+        "worker_shares": list(L.SHARES), "settings": [list(s) for s in L.SETTINGS],
     }
     model.save(args.out, meta=summary)
     log(f"wrote {args.out}: {json.dumps(summary)}")

@@ -18,12 +18,12 @@ pytestmark = pytest.mark.gpu
 def _model_and_data(seed, n=500):
     rng = np.random.default_rng(seed)
     m = L.DeltaModel.init(seed)
-    m.w2 = rng.normal(0, 0.3, (2, 64)).astype(np.float32)
+    m.w2 = rng.normal(0, 0.3, (L.N_OUT, 64)).astype(np.float32)
     m.b1 = rng.normal(0, 0.1, 64).astype(np.float32)
     m.mean = rng.normal(0, 1, 32).astype(np.float32)
     m.inv_std = rng.uniform(0.5, 2.0, 32).astype(np.float32)
     x = (rng.normal(0, 1, (n, 32)) / m.inv_std + m.mean).astype(np.float32)
-    y = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    y = rng.normal(0, 1, (n, L.N_OUT)).astype(np.float32)
     w = rng.uniform(0.5, 2.0, n).astype(np.float32)
     return m, x, y, w
 
@@ -68,6 +68,7 @@ def test_trace_records_features_and_policy():
     assert np.all(rec[:, :14] == rec[0, :14])
     assert np.all(rec[:, 15] == 0.0)  # log2(delta / delta0) under the plain policy
     assert np.all(rec[:, 33] == st.initial_delta) and np.all(rec[:, 34] == 1)
+    assert np.all(rec[:, 37] == L.default_workers())  # every workgroup claims
     assert np.all(np.diff(rec[:, 35]) >= 0)  # edges relaxed so far
 
 
@@ -97,6 +98,10 @@ def test_adds_with_network_is_exact_and_follows_the_network():
             assert np.allclose(rec[:, 33], want_delta, rtol=2e-6)
             want_active = np.clip(np.floor(out[:, 1] + 0.5), 1, 8)
             assert np.array_equal(rec[:, 34], want_active)
+            wk = np.float32(L.default_workers())
+            share = np.exp2(np.clip(out[:, 2], -5, 0)).astype(np.float32)
+            want_workers = np.minimum(np.floor(np.maximum(wk * share, np.float32(16))), wk)
+            assert np.all(np.abs(rec[:, 37] - want_workers) <= 1)
     finally:
         L.clear_delta_model()
 

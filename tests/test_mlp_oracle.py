@@ -48,11 +48,11 @@ def test_bf16_rounding_matches_exact_rule():
 
 def _problem(seed, n=37):
     rng = np.random.default_rng(seed)
-    flat = rng.normal(0, 0.3, 2242)
+    flat = rng.normal(0, 0.3, R.N_PARAMS)
     mean = rng.normal(0, 1, 32)
     inv = rng.uniform(0.5, 2.0, 32)
     x = rng.normal(0, 1, (n, 32)) / inv + mean
-    y = rng.normal(0, 1, (n, 2))
+    y = rng.normal(0, 1, (n, R.N_OUT))
     w = rng.uniform(0.5, 2.0, n)
     return flat, mean, inv, x, y, w
 
@@ -64,7 +64,8 @@ def test_gradient_matches_finite_differences(seed):
     assert L == pytest.approx(R.loss(flat, mean, inv, x, y, w, round_weights=False))
     rng = np.random.default_rng(seed + 10)
     idx = np.concatenate([rng.choice(2048, 40, replace=False), 2048 + rng.choice(64, 10, replace=False),
-                          2112 + rng.choice(128, 10, replace=False), [2240, 2241]])
+                          2112 + rng.choice(64 * R.N_OUT, 12, replace=False),
+                          2112 + 64 * R.N_OUT + np.arange(R.N_OUT)])
     eps = 1e-6
     for i in idx:
         p, m = flat.copy(), flat.copy()
@@ -78,7 +79,8 @@ def test_rounded_gradient_is_the_gradient_at_the_rounded_weights():
     flat, mean, inv, x, y, w = _problem(3)
     rounded = flat.copy()
     rounded[:2048] = R.bf16_round(flat[:2048].astype(np.float32)).astype(np.float64)
-    rounded[2112:2240] = R.bf16_round(flat[2112:2240].astype(np.float32)).astype(np.float64)
+    w2 = slice(2112, 2112 + 64 * R.N_OUT)
+    rounded[w2] = R.bf16_round(flat[w2].astype(np.float32)).astype(np.float64)
     La, ga = R.loss_grad(flat, mean, inv, x, y, w, round_weights=True)
     Lb, gb = R.loss_grad(rounded, mean, inv, x, y, w, round_weights=False)
     assert La == Lb
