@@ -362,7 +362,9 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
     delta = fminf(fmaxf(c->base_delta * scale, c->min_delta), c->max_delta);
     active = (uint32_t)fminf(fmaxf(o1 + 0.5f, 1.0f), 8.0f);
   }
-  const uint32_t ph = ld_rlx(&c->phases);
+  // the record index comes from the phase counter itself: two waves that
+  // advance one after the other never share a record
+  const uint32_t ph = __shfl(lane == 0 ? atomicAdd(&c->phases, 1u) : 0u, 0, 64);
   if (a.policy && a.policy->trace && ph < a.policy->trace_cap) {
     float* t = a.policy->trace + (size_t)ph * kTraceStride;
     if (lane < (unsigned)kMlpIn) t[lane] = raw;
@@ -379,7 +381,6 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
     __hip_atomic_store(&c->delta, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&c->active, active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&c->active_workers, workers_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    atomicAdd(&c->phases, 1u);
   }
   return true;
 }
