@@ -6,9 +6,12 @@ scale 22, edge factor 16, (a,b,c,d) = (.57,.19,.19,.05), self loops and
 duplicate pairs removed, symmetrised, fp32 weights in (0,1], 64 seeded
 sources with out-degree > 0.  The graph is generated on the host, uploaded
 once, and stays resident in HBM; a step is one full SSSP (label init
-included) from the next source with plain ADDS (persistent worklist kernel,
-fixed heuristic delta).  Delta-stepping and Bellman-Ford are timed beside it
-on the same sources.  Distances of the first --check sources are checked bit
+included) from the next source with ADDS (persistent worklist kernel) under the
+shipped learned delta policy; plain ADDS (fixed heuristic delta),
+delta-stepping and Bellman-Ford are timed beside it on the same sources.  The
+routing-grid legs (single source, and 4096 box-limited multi-source queries as
+one batch vs plain ADDS one query at a time) and the batched min-plus GEMM leg
+report into config.  Distances of the first --check sources are checked bit
 for bit against the CPU oracle outside the timed region.
 
 Prints one JSON line (the bench contract) with roofline and cpu_baseline.
@@ -230,8 +233,9 @@ def main() -> int:
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--algorithm", default="adds", choices=["bf", "bf_jacobi", "delta", "adds"])
     ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0: heuristic)")
-    ap.add_argument("--dynamic-delta", type=int, default=0,
-                    help="ADDS: 0 = plain fixed-heuristic delta, 1 = feedback policy")
+    ap.add_argument("--policy", default="auto", choices=["auto", "learned", "plain", "feedback"],
+                    help="ADDS phase policy (auto: the shipped learned-delta model when present, "
+                         "else plain fixed-heuristic delta)")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--check", type=int, default=4, help="sources checked against the oracle")
     ap.add_argument("--compare-steps", type=int, default=16,
@@ -239,7 +243,8 @@ def main() -> int:
     ap.add_argument("--cpu-sources", type=int, default=8, help="oracle Dijkstra runs timed")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size (counter calibration runs)")
-    ap.add_argument("--learned", type=int, default=1, help="also time learned-delta ADDS")
+    ap.add_argument("--learned", type=int, default=1,
+                    help="0: --policy auto falls back to plain ADDS even when a model is shipped")
     ap.add_argument("--routing", type=int, default=1,
                     help="also run the routing-grid single-source and batched-query workloads")
     ap.add_argument("--grid-side", type=int, default=4900)
@@ -276,8 +281,21 @@ def main() -> int:
         log(f"calibration copy of {nbytes} bytes done")
         del a, b
 
-    def opts_for(name):
-        kw = {"dynamic_delta": args.dynamic_delta} if name == "adds" else {}
+    from mi355sssp import learn as L
+
+    policy = args.policy
+    if policy in ("auto", "learned"):
+        model = L.load_default_delta_model() if args.learned or policy == "learned" else None
+        if model is None and policy == "learned":
+            print("no learned delta model shipped", file=sys.stderr)
+            return 1
+        policy = "learned" if model is not None else "plain"
+
+    def opts_for(name, pol=None):
+        pol = policy if pol is None else pol
+        kw = {}
+        if name == "adds":
+            kw = {"use_mlp": 1} if pol == "learned" else {"dynamic_delta": int(pol == "feedback")}
         return dg.options(algorithm_id(name), delta=args.delta, **kw), kw
 
     alg = algorithm_id(args.algorithm)
@@ -333,23 +351,15 @@ def main() -> int:
     log(f"{args.algorithm}: {ms_per:.3f} ms/SSSP wall, {dev_ms / args.steps:.3f} ms device, "
         f"{kernel_ms / args.steps:.3f} ms kernel, {gteps:.2f} GTEPS")
 
-    learned = None
-    if args.learned and args.algorithm == "adds":
-        from mi355sssp import learn as L
-
-        if L.load_default_delta_model() is not None:
-            lo = dg.options(M.ALG_ADDS, delta=args.delta, use_mlp=1)
-            lw, ld, lk = timed(lo, args.steps, 2)
-            d, _, _ = dg.sssp(int(sources[0]), M.ALG_ADDS, use_mlp=1)
-            from oracle import orc
-
-            ok = d.tobytes() == orc.dijkstra(g.rowptr, g.col, g.weights, int(sources[0])).tobytes()
-            learned = {"ms_per_sssp": lw * 1e3 / args.steps, "kernel_ms_per_sssp": lk / args.steps,
-                       "gteps": edges / lw / 1e9, "speedup_vs_plain": wall / lw, "bit_exact": ok}
-            log(f"learned-delta adds: {json.dumps(learned)}")
-            if not ok:
-                print("learned ADDS parity failed", file=sys.stderr)
-                return 1
+    # plain ADDS (fixed heuristic delta, every workgroup claiming) on the same
+    # sources: the comparator of the learned policy
+    plain = None
+    if args.algorithm == "adds" and policy != "plain":
+        po, _ = opts_for("adds", "plain")
+        pw, pd, pk = timed(po, args.steps, 2)
+        plain = {"ms_per_sssp": pw * 1e3 / args.steps, "kernel_ms_per_sssp": pk / args.steps,
+                 "gteps": edges / pw / 1e9, "speedup_of_timed_policy": pw / wall}
+        log(f"plain adds: {json.dumps(plain)}")
 
     others = {}
     if args.compare_steps > 0:
@@ -399,15 +409,16 @@ def main() -> int:
         "data": "synthetic (seeded R-MAT generator)",
         "config": {"workload": f"rmat_s{args.scale}_ef{args.edgefactor}_sym_f32_sssp",
                    "algorithm": args.algorithm,
-                   "policy": ("feedback" if args.dynamic_delta else "plain fixed-heuristic delta")
-                   if args.algorithm == "adds" else None,
+                   "policy": {"learned": "learned delta network (shipped model)",
+                              "plain": "plain fixed-heuristic delta",
+                              "feedback": "feedback delta"}[policy] if args.algorithm == "adds" else None,
                    "n": g.n, "m": g.m,
                    "sources": len(sources), "graph_sha256": g.sha256(),
                    "avg_reached_edges": float(mr.mean()),
                    "graph500_gteps": gteps / 2.0,
                    "device_ms_per_sssp": dev_ms / args.steps,
                    "kernel_ms_per_sssp": kernel_ms / args.steps,
-                   "learned_delta_adds": learned,
+                   "plain_adds": plain,
                    "others": others,
                    "routing_grid": routing,
                    "batched_minplus_gemm": gemm},
