@@ -83,6 +83,7 @@ namespace {
 
 constexpr uint32_t kNoStamp = 0xFFFFFFFFu;
 constexpr uint32_t kClaim = 256;    // items per workgroup claim
+static_assert(5 + kAddsSlots <= 64, "claim probe: lanes 0-3, the window's semaphores, the worker share");
 constexpr int kEmptyEntry = -1;     // ring sentinel: reserved, not yet written
 constexpr uint32_t kPushStage = 512;  // staged pushes per wave
 constexpr uint32_t kHeavyStage = 256; // staged heavy-list entries per wave
@@ -244,13 +245,18 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   st.n = 0;
 }
 
-// Stage the lanes' improved vertices (dedup by qstamp).
-__device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp, uint32_t v,
-                                     uint32_t bkt, unsigned long long& pushed) {
-  bool want = false;
-  // release: a popper that exchanges qstamp after this sees the new label
-  if (imp)
-    want = __hip_atomic_fetch_min(a.qstamp + v, bkt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > bkt;
+// Claim the queue stamp of an improved vertex: true when v is not yet queued
+// in bucket bkt or below.  The label update it follows has completed (its
+// returned value decided `imp`), so a popper that takes the stamp afterwards
+// reads the new label.
+__device__ inline bool claim_stamp(const AddsArgs& a, bool imp, uint32_t v, uint32_t bkt) {
+  if (!imp) return false;
+  return __hip_atomic_fetch_min(a.qstamp + v, bkt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > bkt;
+}
+
+// Stage the lanes that claimed a stamp.
+__device__ inline void stage_wanted(const AddsArgs& a, PushStage& st, bool want, uint32_t v,
+                                    uint32_t bkt, unsigned long long& pushed) {
   const unsigned long long mask = __ballot(want);
   if (mask == 0) return;
   const uint32_t k = (uint32_t)__popcll(mask);
@@ -262,6 +268,38 @@ __device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp,
   }
   st.n += k;
   pushed += k;
+}
+
+// Stage the lanes' improved vertices (dedup by qstamp).
+__device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp, uint32_t v,
+                                     uint32_t bkt, unsigned long long& pushed) {
+  stage_wanted(a, st, claim_stamp(a, imp, v, bkt), v, bkt, pushed);
+}
+
+// Relax two edges per lane (u0 -> v0 with candidate key ck0, u1 -> v1) and
+// stage the improved targets.  Both label probes, then both atomicMins, then
+// both stamp claims are issued before their results are used: the loop is
+// bound by the latency of these dependent accesses, so two independent
+// chains per lane double the accesses in flight.  Per edge this is the same
+// label-correcting step as relax_edge + push_improved.
+template <class W>
+__device__ inline void relax_pair(const AddsArgs& a, PushStage& st, bool ok0, uint32_t v0,
+                                  uint32_t ck0, uint32_t u0, uint32_t lo0, bool ok1, uint32_t v1,
+                                  uint32_t ck1, uint32_t u1, uint32_t lo1, uint32_t base,
+                                  float tbase, float inv, unsigned long long& pushed) {
+  bool imp0, imp1;
+  relax_edge_pair<W>(a.labels, v0, ok0 ? ck0 : W::kInfKey, u0, v1, ok1 ? ck1 : W::kInfKey, u1, imp0,
+                     imp1);
+  const uint32_t bk0 = imp0 ? map_bucket(W::bucket_value(ck0), base, tbase, inv, lo0) : 0u;
+  const uint32_t bk1 = imp1 ? map_bucket(W::bucket_value(ck1), base, tbase, inv, lo1) : 0u;
+  uint32_t q0 = 0u, q1 = 0u;  // 0 never exceeds a bucket: no claim
+  if (imp0) q0 = __hip_atomic_fetch_min(a.qstamp + v0, bk0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (imp1) q1 = __hip_atomic_fetch_min(a.qstamp + v1, bk1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // keep both claims in flight: the comparisons stay after the second issue
+  asm volatile("" : "+v"(q0), "+v"(q1));
+  const bool w0 = q0 > bk0, w1 = q1 > bk1;
+  stage_wanted(a, st, w0, v0, bk0, pushed);
+  stage_wanted(a, st, w1, v1, bk1, pushed);
 }
 
 // Publish a wave's staged heavy-list entries (all of one bucket, list slot
@@ -291,6 +329,39 @@ __device__ inline void heavy_publish(const AddsArgs& a, int* stage, uint32_t& n,
   if (lane == 0) atomicAdd(&HV_AVAIL(c, hs), (int)n);
   vm_wait();
   n = 0;
+}
+
+// The network's decision for the next phase: delta = delta0 * 2^clamp(o0, -4, 4)
+// (within the run's delta bounds), round(o1) active buckets in [1, 8], and
+// the share 2^clamp(o2, -5, 0) of the workgroups that claim (at least 16).
+__device__ inline void mlp_decide(const AddsPolicy* p, const AddsCtrl* c, float feat, unsigned lane,
+                                  float& delta, uint32_t& active, uint32_t& workers_now) {
+  float o[kMlpOut];
+  mlp_forward_wave(p, feat, lane, o);
+  const float share = exp2f(fminf(fmaxf(o[2], -5.0f), 0.0f));
+  workers_now = (uint32_t)fmaxf((float)c->workers * share, 16.0f);
+  workers_now = min(workers_now, c->workers);
+  const float scale = exp2f(fminf(fmaxf(o[0], -4.0f), 4.0f));
+  delta = fminf(fmaxf(c->base_delta * scale, c->min_delta), c->max_delta);
+  active = (uint32_t)fminf(fmaxf(o[1] + 0.5f, 1.0f), 8.0f);
+}
+
+// MLP mode, before the first claim: the network also sets the first phase
+// (the source alone in bucket 0, delta = delta0), so the run does not start
+// with the heuristic width on every workgroup.  One wave.
+__global__ void k_adds_first_phase(AddsArgs a) {
+  const unsigned lane = threadIdx.x;
+  AddsCtrl* c = a.c;
+  float feat = 0.0f, raw = 0.0f;
+  mlp_features(a.policy, c, lane == 0 ? 1u : 0u, 1u, c->delta, lane, &feat, &raw);
+  float delta;
+  uint32_t active, workers_now;
+  mlp_decide(a.policy, c, feat, lane, delta, active, workers_now);
+  if (lane == 0) {
+    c->delta = delta;
+    c->active = active;
+    c->active_workers = workers_now;
+  }
 }
 
 // Wave-level phase advance; true when this wave moved the base.
@@ -352,15 +423,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
     if ((float)window_items < c->util_low * capacity) delta = fminf(delta * c->growth, c->max_delta);
     else if ((float)window_items > c->util_high * capacity) delta = fmaxf(delta * c->shrink, c->min_delta);
   } else if (mode == kPolicyMlp) {
-    float o[kMlpOut];
-    mlp_forward_wave(a.policy, feat, lane, o);
-    const float o0 = o[0], o1 = o[1];
-    const float share = exp2f(fminf(fmaxf(o[2], -5.0f), 0.0f));
-    workers_now = (uint32_t)fmaxf((float)c->workers * share, 16.0f);
-    workers_now = min(workers_now, c->workers);
-    const float scale = exp2f(fminf(fmaxf(o0, -4.0f), 4.0f));
-    delta = fminf(fmaxf(c->base_delta * scale, c->min_delta), c->max_delta);
-    active = (uint32_t)fminf(fmaxf(o1 + 0.5f, 1.0f), 8.0f);
+    mlp_decide(a.policy, c, feat, lane, delta, active, workers_now);
   }
   // the record index comes from the phase counter itself: two waves that
   // advance one after the other never share a record
@@ -411,22 +474,24 @@ __device__ inline void relax_wave(const AddsArgs& a, PushStage& st, int (&s_incl
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (a.live_edges && lane == 0 && total > 0) atomicAdd(&a.c->edges, (unsigned long long)total);
   edges += (unsigned long long)total;
-  for (int t = 0; t < total; t += kWave) {
-    const int j = t + (int)lane;
-    bool imp = false;
-    uint32_t v = 0, bk = 0;
-    if (j < total) {
-      int pos = 0;
+  // edge j of the packed list belongs to the lane p with
+  // s_incl[p - 1] <= j < s_incl[p]; each lane takes edges j and j + 64
+  for (int t = 0; t < total; t += 2 * kWave) {
+    const int j0 = t + (int)lane, j1 = j0 + kWave;
+    const bool ok0 = j0 < total, ok1 = j1 < total;
+    int p0 = 0, p1 = 0;
 #pragma unroll
-      for (int s2 = 32; s2 > 0; s2 >>= 1)
-        if (s_incl[pos + s2 - 1] <= j) pos += s2;
-      const uint2 ed = a.edges[s_base[pos] + j];
-      v = ed.x;
-      const uint32_t ck = W::relax(s_key[pos], ed.y);
-      imp = relax_edge<W>(a.labels, v, ck, (uint32_t)s_u[pos]);
-      if (imp) bk = map_bucket(W::bucket_value(ck), base, tbase, inv, s_lo[pos]);
+    for (int s2 = 32; s2 > 0; s2 >>= 1) {
+      if (s_incl[p0 + s2 - 1] <= j0) p0 += s2;
+      if (s_incl[p1 + s2 - 1] <= j1) p1 += s2;
     }
-    push_improved(a, st, imp, v, bk, pushed);
+    // lanes past the end read edge 0 and discard it (no wait inside a branch)
+    const uint2 e0 = a.edges[ok0 ? s_base[p0] + j0 : 0];
+    const uint2 e1 = a.edges[ok1 ? s_base[p1] + j1 : 0];
+    const uint32_t ck0 = ok0 ? W::relax(s_key[p0], e0.y) : W::kInfKey;
+    const uint32_t ck1 = ok1 ? W::relax(s_key[p1], e1.y) : W::kInfKey;
+    relax_pair<W>(a, st, ok0, e0.x, ck0, (uint32_t)s_u[p0], s_lo[p0], ok1, e1.x, ck1,
+                  (uint32_t)s_u[p1], s_lo[p1], base, tbase, inv, pushed);
   }
   __builtin_amdgcn_wave_barrier();
 }
@@ -501,111 +566,128 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
   unsigned long long claims = 0;
   unsigned long long d_cas = 0, d_idle = 0, d_spin = 0, d_hub = 0, d_drop = 0, d_req = 0, d_heavy = 0;
   unsigned long long t_claim = 0, t_idle = 0, t_items = 0, t_hubs = 0, t_pub = 0, t_relax = 0;
+  // A workgroup beyond the policy's worker share is dormant: it sleeps and
+  // reads only the end flag and the share, so it adds no traffic on the
+  // control words that the claimers update (wave 0's copy is the one used).
+  bool dormant = false;
 
   for (;;) {
     const long long t_top = wall_clock64();
     if (wv == 0) {
       uint32_t got = 0, slot = 0, qslot = 0, begin = 0, kind = kClaimItems, done = 0;
-      // Probe the control words in two wave-wide rounds of loads (the base,
-      // then done / hub / heavy-list / active counts and the window's
-      // semaphores on separate lanes) instead of a chain of dependent loads.
-      const unsigned long long bw0 = __shfl(lane == 0 ? ld_rlx(&c->base_word) : 0ull, 0, 64);
-      const uint32_t base0 = base_of(bw0);
-      int probe = 0;
-      if (lane == 0) probe = (int)ld_rlx(&c->done);
-      else if (lane == 1) probe = ld_rlx(&c->hub_avail);
-      else if (lane == 2) probe = base0 > 0 ? ld_rlx(&HV_AVAIL(c, (base0 - 1) % kAddsSlots)) : 0;
-      else if (lane == 3) probe = (int)ld_rlx(&c->active);
-      else if (lane >= 4 && lane < 4u + kAddsSlots) probe = ld_rlx(&AVAIL(c, (base0 + lane - 4) % kAddsSlots));
-      done = (uint32_t)__shfl(probe, 0, 64);
-      const int hub_n = __shfl(probe, 1, 64), hv_n0 = __shfl(probe, 2, 64);
-      const uint32_t A = max(1u, min((uint32_t)__shfl(probe, 3, 64), (uint32_t)kAddsSlots));
-      const unsigned long long ready = __ballot(lane >= 4 && lane < 4 + A && probe > 0) >> 4;
-      // Workgroups beyond the policy's worker share do not claim: fewer
-      // claimers contend less on the control words when a phase holds little
-      // work (high-diameter graphs).  They still end the run and may advance
-      // the phase.
-      const uint32_t share = __shfl(lane == 0 ? ld_rlx(&c->active_workers) : 0u, 0, 64);
-      const bool claimer = blockIdx.x < max(share, 1u);
-      if (lane == 0 && !done && claimer) {
-        // hub chunks first (they finish claimed work), then the heavy list of
-        // the bucket just finished, then the active buckets.  Every claim is a
-        // counting-semaphore decrement: no retry loop for hundreds of
-        // workgroups to serialise on.
-        if (hub_n > 0) {
-          if (atomicSub(&c->hub_avail, 1) >= 1) {
-            got = 1;
-            kind = kClaimHub;
-            begin = atomicAdd(&c->hub_head, 1u);
-          } else {
-            atomicAdd(&c->hub_avail, 1);
-            ++d_cas;
-          }
-        }
-        if (!got && base0 > 0 && hv_n0 > 0) {
-          const uint32_t hs = (base0 - 1) % kAddsSlots;
-          // pinned on slot base: bucket base cannot finish before the heavy
-          // edges of bucket base - 1 are relaxed
-          const uint32_t ps = base0 % kAddsSlots;
-          got = sem_claim(&HV_AVAIL(c, hs), &HV_HEAD(c, hs), &INFL(c, ps), &begin);
-          if (got) {
-            kind = kClaimHeavy;
-            slot = ps;
-            qslot = hs;
-          } else {
-            ++d_cas;
-          }
-        }
-        for (unsigned long long m = ready; m && !got; m &= m - 1) {
-          const uint32_t s = (base0 + (uint32_t)__ffsll((long long)m) - 1) % kAddsSlots;
-          got = sem_claim(&AVAIL(c, s), &HEAD(c, s), &INFL(c, s), &begin);
-          if (got) {
-            slot = s;
-            qslot = s;
-          } else {
-            ++d_cas;
-          }
-        }
+      if (dormant) {
+        for (int k = 0; k < 4; ++k) __builtin_amdgcn_s_sleep(127);
+        int pr = 0;
+        if (lane == 0) pr = (int)ld_rlx(&c->done);
+        else if (lane == 1) pr = (int)ld_rlx(&c->active_workers);
+        done = (uint32_t)__shfl(pr, 0, 64);
+        dormant = blockIdx.x >= max((uint32_t)__shfl(pr, 1, 64), 1u);
       }
-      got = __shfl(got, 0, 64);
-      done = __shfl(done, 0, 64);
-      if (got) {
-        idle = 0;
-        backoff = 1;
-        if (++claims > a.claim_limit) {
-          if (lane == 0) {
-            atomicOr(&c->err, kErrStall);
-            atomicExch(&c->done, 1u);
+      if (!dormant && !done) {
+        // Probe the control words in two wave-wide rounds of loads (the base,
+        // then done / hub / heavy-list / active counts and the window's
+        // semaphores on separate lanes) instead of a chain of dependent loads.
+        const unsigned long long bw0 = __shfl(lane == 0 ? ld_rlx(&c->base_word) : 0ull, 0, 64);
+        const uint32_t base0 = base_of(bw0);
+        int probe = 0;
+        if (lane == 0) probe = (int)ld_rlx(&c->done);
+        else if (lane == 1) probe = ld_rlx(&c->hub_avail);
+        else if (lane == 2) probe = base0 > 0 ? ld_rlx(&HV_AVAIL(c, (base0 - 1) % kAddsSlots)) : 0;
+        else if (lane == 3) probe = (int)ld_rlx(&c->active);
+        else if (lane >= 4 && lane < 4u + kAddsSlots) probe = ld_rlx(&AVAIL(c, (base0 + lane - 4) % kAddsSlots));
+        else if (lane == 4u + kAddsSlots) probe = (int)ld_rlx(&c->active_workers);
+        done = (uint32_t)__shfl(probe, 0, 64);
+        const int hub_n = __shfl(probe, 1, 64), hv_n0 = __shfl(probe, 2, 64);
+        const uint32_t A = max(1u, min((uint32_t)__shfl(probe, 3, 64), (uint32_t)kAddsSlots));
+        const unsigned long long ready = __ballot(lane >= 4 && lane < 4 + A && probe > 0) >> 4;
+        // Workgroups beyond the policy's worker share do not claim: fewer
+        // claimers contend less on the control words when a phase holds little
+        // work (high-diameter graphs).  They turn dormant until the share
+        // includes them again or the run ends; the claimers advance the
+        // phases and detect the end.
+        const uint32_t share = (uint32_t)__shfl(probe, 4 + kAddsSlots, 64);
+        const bool claimer = blockIdx.x < max(share, 1u);
+        dormant = !claimer;
+        if (lane == 0 && !done && claimer) {
+          // hub chunks first (they finish claimed work), then the heavy list of
+          // the bucket just finished, then the active buckets.  Every claim is a
+          // counting-semaphore decrement: no retry loop for hundreds of
+          // workgroups to serialise on.
+          if (hub_n > 0) {
+            if (atomicSub(&c->hub_avail, 1) >= 1) {
+              got = 1;
+              kind = kClaimHub;
+              begin = atomicAdd(&c->hub_head, 1u);
+            } else {
+              atomicAdd(&c->hub_avail, 1);
+              ++d_cas;
+            }
           }
-          done = 1;
-        }
-      }
-      if (!got && !done) {
-        const unsigned long long pend = __shfl(lane == 0 ? ld_rlx(&c->pending) : 0ull, 0, 64);
-        if (pend == 0) {
-          if (lane == 0) atomicExch(&c->done, 1u);
-          done = 1;
-        } else {
-          const unsigned long long bw = __shfl(lane == 0 ? ld_rlx(&c->base_word) : 0ull, 0, 64);
-          if (try_advance(a, bw)) {
-            idle = 0;
-            backoff = 1;
-          } else {
-            // exponential back-off keeps idle workgroups from flooding the
-            // control words that the busy ones update
-            ++d_idle;
-            for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(8);
-            if (backoff < 32) backoff <<= 1;
-            if (++idle > kIdleLimit) {  // no progress for seconds: abort, do not hang
-              if (lane == 0) {
-                atomicOr(&c->err, kErrStall);
-                atomicExch(&c->done, 1u);
-              }
-              done = 1;
+          if (!got && base0 > 0 && hv_n0 > 0) {
+            const uint32_t hs = (base0 - 1) % kAddsSlots;
+            // pinned on slot base: bucket base cannot finish before the heavy
+            // edges of bucket base - 1 are relaxed
+            const uint32_t ps = base0 % kAddsSlots;
+            got = sem_claim(&HV_AVAIL(c, hs), &HV_HEAD(c, hs), &INFL(c, ps), &begin);
+            if (got) {
+              kind = kClaimHeavy;
+              slot = ps;
+              qslot = hs;
+            } else {
+              ++d_cas;
+            }
+          }
+          for (unsigned long long m = ready; m && !got; m &= m - 1) {
+            const uint32_t s = (base0 + (uint32_t)__ffsll((long long)m) - 1) % kAddsSlots;
+            got = sem_claim(&AVAIL(c, s), &HEAD(c, s), &INFL(c, s), &begin);
+            if (got) {
+              slot = s;
+              qslot = s;
+            } else {
+              ++d_cas;
             }
           }
         }
-      }
+        got = __shfl(got, 0, 64);
+        done = __shfl(done, 0, 64);
+        if (got) {
+          idle = 0;
+          backoff = 1;
+          if (++claims > a.claim_limit) {
+            if (lane == 0) {
+              atomicOr(&c->err, kErrStall);
+              atomicExch(&c->done, 1u);
+            }
+            done = 1;
+          }
+        }
+        if (!got && !done && claimer) {
+          const unsigned long long pend = __shfl(lane == 0 ? ld_rlx(&c->pending) : 0ull, 0, 64);
+          if (pend == 0) {
+            if (lane == 0) atomicExch(&c->done, 1u);
+            done = 1;
+          } else {
+            const unsigned long long bw = __shfl(lane == 0 ? ld_rlx(&c->base_word) : 0ull, 0, 64);
+            if (try_advance(a, bw)) {
+              idle = 0;
+              backoff = 1;
+            } else {
+              // exponential back-off keeps idle workgroups from flooding the
+              // control words that the busy ones update
+              ++d_idle;
+              for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(8);
+              if (backoff < 32) backoff <<= 1;
+              if (++idle > kIdleLimit) {  // no progress for seconds: abort, do not hang
+                if (lane == 0) {
+                  atomicOr(&c->err, kErrStall);
+                  atomicExch(&c->done, 1u);
+                }
+                done = 1;
+              }
+            }
+          }
+        }
+      }  // !dormant && !done
       if (lane == 0) {
         s_done = done;
         s_count = got;
@@ -613,11 +695,13 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         s_qslot = qslot;
         s_begin = begin;
         s_kind = kind;
-        const unsigned long long cur = ld_rlx(&c->base_word);
-        s_base = base_of(cur);
-        s_tbase = tbase_of(cur);
-        s_inv = 1.0f / ld_rlx(&c->delta);
-        s_active = max(1u, ld_rlx(&c->active));
+        if (got) {  // the window a claim is mapped against
+          const unsigned long long cur = ld_rlx(&c->base_word);
+          s_base = base_of(cur);
+          s_tbase = tbase_of(cur);
+          s_inv = 1.0f / ld_rlx(&c->delta);
+          s_active = max(1u, ld_rlx(&c->active));
+        }
       }
     }
     __syncthreads();
@@ -654,18 +738,16 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       const uint32_t hu = ld_rlx(&hc->u), hkey = ld_rlx(&hc->key), hfirst = ld_rlx(&hc->first);
       const uint32_t hcnt = min(ld_rlx(&hc->cnt), (uint32_t)kChunk);
       const uint32_t hslot = ld_rlx(&hc->slot) % kAddsSlots, hbucket = ld_rlx(&hc->bucket);
-      for (uint32_t t0 = wv * kWave; t0 < hcnt; t0 += kBlock) {
-        const uint32_t e = hfirst + t0 + lane;
-        bool imp = false;
-        uint32_t v = 0, bk = 0;
-        if (t0 + lane < hcnt) {
-          const uint2 ed = a.edges[e];
-          v = ed.x;
-          const uint32_t ck = W::relax(hkey, ed.y);
-          imp = relax_edge<W>(a.labels, v, ck, hu);
-          if (imp) bk = map_bucket(W::bucket_value(ck), base, tbase, inv, hbucket);
-        }
-        push_improved(a, st, imp, v, bk, pushed_local);
+      // wave wv takes edges [wv * 128, wv * 128 + 128) of every 512, two per lane
+      for (uint32_t t0 = wv * 2 * kWave; t0 < hcnt; t0 += 2 * kBlock) {
+        const uint32_t i0 = t0 + lane, i1 = i0 + kWave;
+        const bool ok0 = i0 < hcnt, ok1 = i1 < hcnt;
+        const uint2 e0 = a.edges[hfirst + (ok0 ? i0 : 0u)];
+        const uint2 e1 = a.edges[hfirst + (ok1 ? i1 : 0u)];
+        const uint32_t ck0 = ok0 ? W::relax(hkey, e0.y) : W::kInfKey;
+        const uint32_t ck1 = ok1 ? W::relax(hkey, e1.y) : W::kInfKey;
+        relax_pair<W>(a, st, ok0, e0.x, ck0, hu, hbucket, ok1, e1.x, ck1, hu, hbucket, base, tbase,
+                      inv, pushed_local);
       }
       stage_publish(a, st);
       if (a.live_edges && wv == 0 && lane == 0) atomicAdd(&c->edges, (unsigned long long)hcnt);
@@ -709,6 +791,14 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         __hip_atomic_store(ep, kEmptyEntry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
+    // row bounds and light-prefix length of u depend only on u: issued now,
+    // they arrive while the stamp exchange and the label read complete
+    int r_beg = 0, r_end = 0, r_split = 0;
+    if (i < count && entry_ok) {
+      r_beg = a.rowptr[u];
+      r_end = a.rowptr[u + 1];
+      r_split = a.split[u];
+    }
     if (i < count && entry_ok) {
       if (heavy) {
         // Clearing the stamp first means a later light pass of u (after an
@@ -741,9 +831,9 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       }
       if (key != W::kInfKey) {
         ++items_local;
-        beg = a.rowptr[u];
-        int end = a.rowptr[u + 1];
-        const int split = beg + a.split[u];
+        beg = r_beg;
+        int end = r_end;
+        const int split = beg + r_split;
         if (heavy) {
           beg = split;
         } else {
@@ -907,6 +997,10 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   k_adds_init<W><<<std::min<unsigned>(div_up((uint64_t)std::max<int64_t>(n, hub_cap), kBlock), 4096u),
                    kBlock, 0, s>>>(a, n, src, req.init_dist, c_init);
   MI_LAUNCH_CHECK();
+  if (req.use_mlp) {
+    k_adds_first_phase<<<1, kWave, 0, s>>>(a);
+    MI_LAUNCH_CHECK();
+  }
   MI_HIP(hipEventRecord(evk, s));
   k_adds<W><<<workers, kBlock, 0, s>>>(a);
   MI_LAUNCH_CHECK();

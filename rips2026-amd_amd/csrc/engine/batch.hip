@@ -136,37 +136,64 @@ __global__ __launch_bounds__(kBlock) void k_batch_round(BatchArgs a, int k) {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     edge_visits += (unsigned long long)total;
-    for (int t = 0; t < total; t += kWave) {
-      const int j = t + (int)lane;
-      bool push = false;
-      uint2 item = make_uint2(0u, 0u);
-      if (j < total) {
-        int pos = 0;
+    // Two edges per lane per step: each step is four rounds of independent
+    // accesses (edges + query records, labels + round stamps, atomicMins,
+    // stamp exchanges), so both edges' chains are in flight together.
+    // Lanes past the end of the packed list read edge 0 / query 0 / slot 0
+    // and discard them: guarded loads would each wait inside their branch.
+    for (int t = 0; t < total; t += 2 * kWave) {
+      const int j0 = t + (int)lane, j1 = j0 + kWave;
+      const bool ok0 = j0 < total, ok1 = j1 < total;
+      int p0 = 0, p1 = 0;
 #pragma unroll
-        for (int s2 = 32; s2 > 0; s2 >>= 1)
-          if (s_incl[wv][pos + s2 - 1] <= j) pos += s2;
-        const uint2 ed = a.edges[s_beg[wv][pos] + j];
-        const int q = s_q[wv][pos];
-        const QueryInfo qi = a.qinfo[q];
-        const int v = (int)ed.x;
-        const int rv = v / a.cols, cv = v - rv * a.cols;
-        const uint32_t dr = (uint32_t)(rv - qi.r0), dc = (uint32_t)(cv - qi.c0);
-        if (dr < (uint32_t)qi.bh && dc < (uint32_t)qi.bw) {
-          const uint32_t lv = dr * (uint32_t)qi.bw + dc;
-          const uint32_t ck = W::relax(s_key[wv][pos], ed.y);
-          if (relax_edge<W>(a.labels + qi.base, lv, ck, (uint32_t)s_u[wv][pos])) {
-            uint32_t* sp = a.stamp + qi.base + lv;
-            if (*sp != stamp) push = atomicExch(sp, stamp) != stamp;
-            item = make_uint2((uint32_t)q, lv);
-          }
-        }
+      for (int s2 = 32; s2 > 0; s2 >>= 1) {
+        if (s_incl[wv][p0 + s2 - 1] <= j0) p0 += s2;
+        if (s_incl[wv][p1 + s2 - 1] <= j1) p1 += s2;
       }
-      const unsigned long long mask = __ballot(push);
-      if (mask) {
-        const uint32_t kk = (uint32_t)__popcll(mask);
-        if (staged + kk > kStageCap) batch_stage_flush(stage, staged, qout, cnt_out, a.cap, &c->err);
-        if (push) stage[staged + prefix_count(mask)] = item;
-        staged += kk;
+      const int q0 = ok0 ? s_q[wv][p0] : 0, q1 = ok1 ? s_q[wv][p1] : 0;
+      const uint2 e0 = a.edges[ok0 ? s_beg[wv][p0] + j0 : 0];
+      const uint2 e1 = a.edges[ok1 ? s_beg[wv][p1] + j1 : 0];
+      const QueryInfo qi0 = a.qinfo[q0], qi1 = a.qinfo[q1];
+      // target inside the query's rectangle -> its local slot
+      const int rv0 = (int)e0.x / a.cols, cv0 = (int)e0.x - rv0 * a.cols;
+      const int rv1 = (int)e1.x / a.cols, cv1 = (int)e1.x - rv1 * a.cols;
+      const uint32_t dr0 = (uint32_t)(rv0 - qi0.r0), dc0 = (uint32_t)(cv0 - qi0.c0);
+      const uint32_t dr1 = (uint32_t)(rv1 - qi1.r0), dc1 = (uint32_t)(cv1 - qi1.c0);
+      const bool in0 = ok0 && dr0 < (uint32_t)qi0.bh && dc0 < (uint32_t)qi0.bw;
+      const bool in1 = ok1 && dr1 < (uint32_t)qi1.bh && dc1 < (uint32_t)qi1.bw;
+      const uint32_t lv0 = in0 ? dr0 * (uint32_t)qi0.bw + dc0 : 0u;
+      const uint32_t lv1 = in1 ? dr1 * (uint32_t)qi1.bw + dc1 : 0u;
+      const int64_t slot0 = in0 ? qi0.base + lv0 : 0, slot1 = in1 ? qi1.base + lv1 : 0;
+      const uint32_t ck0 = in0 ? W::relax(s_key[wv][p0], e0.y) : W::kInfKey;
+      const uint32_t ck1 = in1 ? W::relax(s_key[wv][p1], e1.y) : W::kInfKey;
+      constexpr unsigned long long kNone = ~0ull;
+      const unsigned long long cand0 =
+          ck0 != W::kInfKey ? ((unsigned long long)ck0 << 32) | (uint32_t)s_u[wv][p0] : kNone;
+      const unsigned long long cand1 =
+          ck1 != W::kInfKey ? ((unsigned long long)ck1 << 32) | (uint32_t)s_u[wv][p1] : kNone;
+      // label probes and round stamps (the stamp read is only a filter: the
+      // exchange below decides)
+      unsigned long long cur0 = a.labels[slot0], cur1 = a.labels[slot1];
+      uint32_t so0 = a.stamp[slot0], so1 = a.stamp[slot1];
+      asm volatile("" : "+v"(cur0), "+v"(cur1), "+v"(so0), "+v"(so1));
+      const bool try0 = cand0 < cur0, try1 = cand1 < cur1;
+      unsigned long long old0 = kNone, old1 = kNone;
+      if (try0) old0 = atomicMin(a.labels + slot0, cand0);
+      if (try1) old1 = atomicMin(a.labels + slot1, cand1);
+      const bool imp0 = try0 && ck0 < (uint32_t)(old0 >> 32);
+      const bool imp1 = try1 && ck1 < (uint32_t)(old1 >> 32);
+      uint32_t x0 = stamp, x1 = stamp;  // stamp already set: no push
+      if (imp0 && so0 != stamp) x0 = atomicExch(a.stamp + slot0, stamp);
+      if (imp1 && so1 != stamp) x1 = atomicExch(a.stamp + slot1, stamp);
+      asm volatile("" : "+v"(x0), "+v"(x1));
+      const bool push0 = x0 != stamp, push1 = x1 != stamp;
+      const unsigned long long m0 = __ballot(push0), m1 = __ballot(push1);
+      const uint32_t k0 = (uint32_t)__popcll(m0), k1 = (uint32_t)__popcll(m1);
+      if (k0 + k1) {
+        if (staged + k0 + k1 > kStageCap) batch_stage_flush(stage, staged, qout, cnt_out, a.cap, &c->err);
+        if (push0) stage[staged + prefix_count(m0)] = make_uint2((uint32_t)q0, lv0);
+        if (push1) stage[staged + k0 + prefix_count(m1)] = make_uint2((uint32_t)q1, lv1);
+        staged += k0 + k1;
       }
     }
     __builtin_amdgcn_wave_barrier();

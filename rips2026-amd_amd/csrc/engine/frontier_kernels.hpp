@@ -120,23 +120,27 @@ __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     edge_visits += (unsigned long long)total;
 
-    for (int t = 0; t < total; t += kWave) {
-      const int j = t + (int)lane;
-      bool imp = false;
-      uint32_t v = 0, ck = 0;
-      if (j < total) {
-        int pos = 0;
+    // two edges per lane per step (j and j + 64), relaxed as a pair
+    for (int t = 0; t < total; t += 2 * kWave) {
+      const int j0 = t + (int)lane, j1 = j0 + kWave;
+      const bool ok0 = j0 < total, ok1 = j1 < total;
+      int p0 = 0, p1 = 0;
 #pragma unroll
-        for (int s = 32; s > 0; s >>= 1)
-          if (s_incl[wv][pos + s - 1] <= j) pos += s;
-        const uint2 ed = edges[s_base[wv][pos] + j];
-        v = ed.x;
-        if (edge_selected<W>(cls, ed.y, delta_bits)) {
-          ck = W::relax(s_key[wv][pos], ed.y);
-          imp = relax_edge<W>(labels, v, ck, (uint32_t)s_u[wv][pos]);
-        }
+      for (int s = 32; s > 0; s >>= 1) {
+        if (s_incl[wv][p0 + s - 1] <= j0) p0 += s;
+        if (s_incl[wv][p1 + s - 1] <= j1) p1 += s;
       }
-      sink.improved(imp, v, ck);
+      const uint2 e0 = edges[ok0 ? s_base[wv][p0] + j0 : 0];
+      const uint2 e1 = edges[ok1 ? s_base[wv][p1] + j1 : 0];
+      const bool sel0 = ok0 && edge_selected<W>(cls, e0.y, delta_bits);
+      const bool sel1 = ok1 && edge_selected<W>(cls, e1.y, delta_bits);
+      const uint32_t ck0 = sel0 ? W::relax(s_key[wv][p0], e0.y) : W::kInfKey;
+      const uint32_t ck1 = sel1 ? W::relax(s_key[wv][p1], e1.y) : W::kInfKey;
+      bool imp0, imp1;
+      relax_edge_pair<W>(labels, e0.x, ck0, (uint32_t)s_u[wv][p0], e1.x, ck1, (uint32_t)s_u[wv][p1],
+                         imp0, imp1);
+      sink.improved(imp0, e0.x, ck0);
+      sink.improved(imp1, e1.x, ck1);
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -156,19 +160,19 @@ __device__ inline void relax_chunks_body(const uint2* __restrict__ edges,
   for (uint32_t c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave; c < n_ch; c += nwaves) {
     const uint4 ch = cq.q[c];
     edge_visits += ch.w - ch.z;
-    for (uint32_t t = ch.z; t < ch.w; t += kWave) {
-      const uint32_t e = t + lane;
-      bool imp = false;
-      uint32_t v = 0, ck = 0;
-      if (e < ch.w) {
-        const uint2 ed = edges[e];
-        v = ed.x;
-        if (edge_selected<W>(cls, ed.y, delta_bits)) {
-          ck = W::relax(ch.y, ed.y);
-          imp = relax_edge<W>(labels, v, ck, ch.x);
-        }
-      }
-      sink.improved(imp, v, ck);
+    for (uint32_t t = ch.z; t < ch.w; t += 2 * kWave) {
+      const uint32_t i0 = t + lane, i1 = i0 + kWave;
+      const bool ok0 = i0 < ch.w, ok1 = i1 < ch.w;
+      const uint2 e0 = edges[ok0 ? i0 : ch.z];
+      const uint2 e1 = edges[ok1 ? i1 : ch.z];
+      const bool sel0 = ok0 && edge_selected<W>(cls, e0.y, delta_bits);
+      const bool sel1 = ok1 && edge_selected<W>(cls, e1.y, delta_bits);
+      const uint32_t ck0 = sel0 ? W::relax(ch.y, e0.y) : W::kInfKey;
+      const uint32_t ck1 = sel1 ? W::relax(ch.y, e1.y) : W::kInfKey;
+      bool imp0, imp1;
+      relax_edge_pair<W>(labels, e0.x, ck0, ch.x, e1.x, ck1, ch.x, imp0, imp1);
+      sink.improved(imp0, e0.x, ck0);
+      sink.improved(imp1, e1.x, ck1);
     }
   }
   if (work && lane == 0 && edge_visits) atomicAdd(work, edge_visits);

@@ -173,4 +173,29 @@ __device__ inline bool relax_edge(unsigned long long* __restrict__ labels, uint3
   return cand_key < (uint32_t)(old >> 32);
 }
 
+// relax_edge for two edges at once (candidate keys ck0 / ck1, kInfKey = no
+// edge): both label probes, then both atomicMins, are issued before either
+// result is used, so the two dependent chains overlap.  A lane without an
+// edge probes label 0 and discards it (a guarded load would wait inside its
+// branch).
+template <class W>
+__device__ inline void relax_edge_pair(unsigned long long* __restrict__ labels, uint32_t v0,
+                                       uint32_t ck0, uint32_t parent0, uint32_t v1, uint32_t ck1,
+                                       uint32_t parent1, bool& imp0, bool& imp1) {
+  constexpr unsigned long long kNone = ~0ull;
+  const bool has0 = ck0 != W::kInfKey, has1 = ck1 != W::kInfKey;
+  const unsigned long long cand0 = has0 ? ((unsigned long long)ck0 << 32) | parent0 : kNone;
+  const unsigned long long cand1 = has1 ? ((unsigned long long)ck1 << 32) | parent1 : kNone;
+  unsigned long long cur0 = labels[has0 ? v0 : 0u];
+  unsigned long long cur1 = labels[has1 ? v1 : 0u];
+  // both probes complete here, so no wait separates the two atomics below
+  asm volatile("" : "+v"(cur0), "+v"(cur1));
+  const bool try0 = cand0 < cur0, try1 = cand1 < cur1;
+  unsigned long long old0 = kNone, old1 = kNone;
+  if (try0) old0 = atomicMin(labels + v0, cand0);
+  if (try1) old1 = atomicMin(labels + v1, cand1);
+  imp0 = try0 && ck0 < (uint32_t)(old0 >> 32);
+  imp1 = try1 && ck1 < (uint32_t)(old1 >> 32);
+}
+
 }  // namespace mi355
