@@ -241,6 +241,8 @@ def main() -> int:
     ap.add_argument("--compare-steps", type=int, default=16,
                     help="SSSPs timed for each other algorithm (reported, not the value)")
     ap.add_argument("--cpu-sources", type=int, default=8, help="oracle Dijkstra runs timed")
+    ap.add_argument("--compare-plain", type=int, default=1,
+                    help="0: skip the plain ADDS comparator (profiling runs of one policy)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size (counter calibration runs)")
     ap.add_argument("--learned", type=int, default=1,
@@ -354,7 +356,7 @@ def main() -> int:
     # plain ADDS (fixed heuristic delta, every workgroup claiming) on the same
     # sources: the comparator of the learned policy
     plain = None
-    if args.algorithm == "adds" and policy != "plain":
+    if args.algorithm == "adds" and policy != "plain" and args.compare_plain:
         po, _ = opts_for("adds", "plain")
         pw, pd, pk = timed(po, args.steps, 2)
         plain = {"ms_per_sssp": pw * 1e3 / args.steps, "kernel_ms_per_sssp": pk / args.steps,
