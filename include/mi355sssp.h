@@ -465,7 +465,8 @@ int mi355_adds_mlp_clear(void);
  * floats: [0,32) the phase features (before the model's normalisation), 32
  * items queued in the window, 33 the delta
  * chosen for the next phase, 34 active buckets, 35 edges relaxed so far,
- * 36 pushes so far, 37 workgroups claiming work.  trace: host array of
+ * 36 pushes so far, 37 workgroups claiming work, 38 microseconds from the
+ * run's start to this boundary.  trace: host array of
  * cap_phases records; phases receives
  * the number of phase boundaries (records beyond cap_phases are dropped). */
 #define MI355_ADDS_TRACE_STRIDE 40

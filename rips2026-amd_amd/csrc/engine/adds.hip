@@ -400,6 +400,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   }
   ok = __shfl(ok, 0, 64);
   if (!ok) return false;
+  const long long t_adv = wall_clock64();
   // phase boundary: occupancy of the new window, then the policy
   const uint32_t nb = base + 1;
   uint32_t occ = 0;
@@ -438,50 +439,63 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
       t[35] = (float)ld_rlx(&c->edges);
       t[36] = (float)ld_rlx(&c->pushes);
       t[37] = (float)workers_now;
+      t[38] = (float)(t_adv - (long long)ld_rlx(&c->diag[15])) * 0.01f;  // 100 MHz clock -> us
     }
   }
   if (lane == 0) {
     __hip_atomic_store(&c->delta, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&c->active, active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&c->active_workers, workers_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicAdd(&c->diag[14], (unsigned long long)(wall_clock64() - t_adv));
   }
   return true;
 }
 
-// Relax the packed edges of up to 64 vertices held one per lane (deg = 0 for
-// lanes without work) and stage the improved targets.  lo = producer bucket.
+// Relax the packed edges of up to kBlock vertices held one per thread (deg = 0
+// for threads without work) and stage the improved targets; lo = producer
+// bucket.  The edge list is packed across the whole workgroup and its waves
+// take 128-edge steps of it in turn (two edges per lane), so one high-degree
+// vertex does not keep the other waves waiting at the next barrier.  The
+// arrays are kBlock entries in LDS; s_wsum holds kWavesPerBlock totals.
 template <class W>
-__device__ inline void relax_wave(const AddsArgs& a, PushStage& st, int (&s_incl)[kWave],
-                                  int (&s_base)[kWave], uint32_t (&s_key)[kWave], int (&s_u)[kWave],
-                                  uint32_t (&s_lo)[kWave], int u, int beg, int deg, uint32_t key,
-                                  uint32_t lo, uint32_t base, float tbase, float inv,
-                                  unsigned long long& edges, unsigned long long& pushed) {
+__device__ inline void relax_block(const AddsArgs& a, PushStage& st, int* s_incl, int* s_base,
+                                   uint32_t* s_key, int* s_u, uint32_t* s_lo, int* s_wsum, int u,
+                                   int beg, int deg, uint32_t key, uint32_t lo, uint32_t base,
+                                   float tbase, float inv, unsigned long long& edges,
+                                   unsigned long long& pushed) {
   const unsigned lane = lane_id();
+  const unsigned wv = threadIdx.x / kWave;
   int incl = deg;
 #pragma unroll
   for (int off = 1; off < kWave; off <<= 1) {
     const int t = __shfl_up(incl, off, kWave);
     if ((int)lane >= off) incl += t;
   }
-  const int total = __shfl(incl, kWave - 1, kWave);
-  s_incl[lane] = incl;
-  s_base[lane] = beg - (incl - deg);
-  s_key[lane] = key;
-  s_u[lane] = u;
-  s_lo[lane] = lo;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (a.live_edges && lane == 0 && total > 0) atomicAdd(&a.c->edges, (unsigned long long)total);
-  edges += (unsigned long long)total;
-  // edge j of the packed list belongs to the lane p with
-  // s_incl[p - 1] <= j < s_incl[p]; each lane takes edges j and j + 64
-  for (int t = 0; t < total; t += 2 * kWave) {
+  if (lane == kWave - 1) s_wsum[wv] = incl;
+  __syncthreads();
+  int total = 0;
+#pragma unroll
+  for (int w = 0; w < kWavesPerBlock; ++w) {
+    const int ws = s_wsum[w];
+    if (w < (int)wv) incl += ws;
+    total += ws;
+  }
+  s_incl[threadIdx.x] = incl;
+  s_base[threadIdx.x] = beg - (incl - deg);
+  s_key[threadIdx.x] = key;
+  s_u[threadIdx.x] = u;
+  s_lo[threadIdx.x] = lo;
+  __syncthreads();
+  if (a.live_edges && threadIdx.x == 0 && total > 0) atomicAdd(&a.c->edges, (unsigned long long)total);
+  if (wv == 0) edges += (unsigned long long)total;
+  // edge j belongs to the vertex p with s_incl[p - 1] <= j < s_incl[p]
+  static_assert(kBlock == 256, "binary search over 256 packed vertices");
+  for (int t = (int)wv * 2 * kWave; t < total; t += 2 * kBlock) {
     const int j0 = t + (int)lane, j1 = j0 + kWave;
     const bool ok0 = j0 < total, ok1 = j1 < total;
     int p0 = 0, p1 = 0;
 #pragma unroll
-    for (int s2 = 32; s2 > 0; s2 >>= 1) {
+    for (int s2 = kBlock / 2; s2 > 0; s2 >>= 1) {
       if (s_incl[p0 + s2 - 1] <= j0) p0 += s2;
       if (s_incl[p1 + s2 - 1] <= j1) p1 += s2;
     }
@@ -552,6 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
   __shared__ uint32_t s_key[kWavesPerBlock][kWave];
   __shared__ int s_u[kWavesPerBlock][kWave];
   __shared__ uint32_t s_lo[kWavesPerBlock][kWave];
+  __shared__ int s_wsum[kWavesPerBlock];
   __shared__ int s_pv[kWavesPerBlock][kPushStage];
   __shared__ uint32_t s_ps[kWavesPerBlock][kPushStage];
   __shared__ uint32_t s_ph[kWavesPerBlock][kAddsSlots];
@@ -559,6 +574,11 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
   AddsCtrl* c = a.c;
   const unsigned lane = threadIdx.x & 63;
   const unsigned wv = threadIdx.x >> 6;
+  // run start for the trace's timestamps (block 0 starts first; bucket 0 is
+  // not finished before it does)
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    __hip_atomic_store(&c->diag[15], (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   PushStage st{s_pv[wv], s_ps[wv], s_ph[wv], 0};
   uint32_t hv_n = 0;
   unsigned long long edges_local = 0, pushed_local = 0, items_local = 0;
@@ -863,8 +883,8 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
     // re-queued items go through the stage like any push
     push_improved(a, st, requeue, (uint32_t)u, want, pushed_local);
     const long long t_r0 = wall_clock64();
-    relax_wave<W>(a, st, s_incl[wv], s_beg[wv], s_key[wv], s_u[wv], s_lo[wv], u, beg, deg, key, bk,
-                  base, tbase, inv, edges_local, pushed_local);
+    relax_block<W>(a, st, &s_incl[0][0], &s_beg[0][0], &s_key[0][0], &s_u[0][0], &s_lo[0][0], s_wsum,
+                   u, beg, deg, key, bk, base, tbase, inv, edges_local, pushed_local);
     const long long t_r1 = wall_clock64();
     stage_publish(a, st);
     heavy_publish(a, s_hv[wv], hv_n, slot);
@@ -972,8 +992,9 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.ring = ws.adds_ring.p;
   a.hring = ws.adds_ring.p + (size_t)kAddsSlots * cap;
   a.cap_mask = cap - 1;
-  // light/heavy split at the starting bucket width
-  a.split = light_split(g, delta, s);
+  // light/heavy split at the starting bucket width (MLP mode: at the
+  // network's first width, set below; one cached split per graph either way)
+  a.split = req.use_mlp ? nullptr : light_split(g, delta, s);
   a.hub = ws.adds_hub.p;
   a.hub_mask = hub_cap - 1;
   a.c = c;
@@ -1000,6 +1021,15 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   if (req.use_mlp) {
     k_adds_first_phase<<<1, kWave, 0, s>>>(a);
     MI_LAUNCH_CHECK();
+    // The light prefix of each (weight-sorted) row follows the network's
+    // first width, not the heuristic one: a split at a wider delta makes the
+    // light pass relax edges of up to that width eagerly from vertices that
+    // are not settled yet (on R-MAT scale 22, 1.4x the relaxations).  The
+    // split is cached per width, so runs on one graph compute it once.
+    float first_delta = delta;
+    MI_HIP(hipMemcpyAsync(&first_delta, &c->delta, sizeof(float), hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    a.split = light_split(g, first_delta > 0.0f && std::isfinite(first_delta) ? first_delta : delta, s);
   }
   MI_HIP(hipEventRecord(evk, s));
   k_adds<W><<<workers, kBlock, 0, s>>>(a);
@@ -1018,11 +1048,12 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
     std::fprintf(stderr,
                  "adds: %.3f ms phases %u items %llu edges %llu pushes %llu | claims %llu cas_fail %llu "
                  "idle %llu ring_wait %llu hub %llu dropped %llu requeued %llu heavy %llu\n"
-                 "  per-workgroup ms: claim %.2f idle %.2f items %.2f hubs %.2f (publish %.2f relax %.2f)\n",
+                 "  per-workgroup ms: claim %.2f idle %.2f items %.2f hubs %.2f (publish %.2f relax %.2f)"
+                 " | phase advances %.3f ms in total\n",
                  res.device_ms, h.phases, h.items, h.edges, h.pushes, h.diag[0], h.diag[1], h.diag[2],
                  h.diag[3], h.diag[4], h.diag[5], h.diag[6], h.diag[7], h.diag[8] * 1e-5 / workers,
                  h.diag[9] * 1e-5 / workers, h.diag[10] * 1e-5 / workers, h.diag[11] * 1e-5 / workers,
-                 h.diag[12] * 1e-5 / workers, h.diag[13] * 1e-5 / workers);
+                 h.diag[12] * 1e-5 / workers, h.diag[13] * 1e-5 / workers, h.diag[14] * 1e-5);
   if (h.err & kErrRing) fail(MI355_ECAPACITY, "ADDS bucket queue overflow");
   if (h.err & kErrHub) fail(MI355_ECAPACITY, "ADDS hub chunk queue overflow");
   if (h.err & kErrStall) fail(MI355_EDEVICE, "ADDS workers stalled (aborted by the spin bound)");
