@@ -105,7 +105,10 @@ def routing_workloads(args, cpu_threads: int):
         for s_ in srcs:
             st = dg.sssp_labels(int(s_), o)
             ms.append(st.kernel_ms)
-        res[name] = {"ms_per_sssp": float(np.mean(ms)), "phases_last": int(st.iterations_used)}
+        res[name] = {"ms_per_sssp": float(np.mean(ms)), "phases_last": int(st.iterations_used),
+                     # 2: the network ran; 0: fixed delta (also when the grid lies
+                     # outside the model's training range)
+                     "policy_used": int(st.policy)}
     d, _, st = dg.sssp(int(srcs[0]), M.ALG_ADDS, use_mlp=int(model is not None))
     if args.check > 0:
         ok = d.tobytes() == orc.dijkstra(g.rowptr, g.col, g.weights, int(srcs[0])).tobytes()
@@ -310,6 +313,7 @@ def main() -> int:
         _, _, st = dg.sssp(int(s), alg, delta=args.delta, want_pred=False, **kw)
         mr[i], nr[i] = st.reached_edges, st.reached
         if i == 0:
+            policy_used = int(st.policy)
             log(f"source {s}: iterations {st.iterations_used} rounds {st.rounds} "
                 f"delta {st.final_delta:.4g} reached {st.reached} m_r {st.reached_edges}")
 
@@ -420,6 +424,7 @@ def main() -> int:
                    "graph500_gteps": gteps / 2.0,
                    "device_ms_per_sssp": dev_ms / args.steps,
                    "kernel_ms_per_sssp": kernel_ms / args.steps,
+                   "policy_used": policy_used if args.algorithm == "adds" else None,
                    "plain_adds": plain,
                    "others": others,
                    "routing_grid": routing,

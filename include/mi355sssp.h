@@ -139,7 +139,9 @@ typedef struct {
   float kernel_ms;          /* hipEvent time of the persistent ADDS kernel alone (0 for the
                                algorithms that run as a sequence of launches) */
   int32_t has_negative_cycle;
-  int32_t reserved_i;
+  int32_t policy;           /* ADDS phase policy used: 0 fixed delta, 1 feedback, 2 learned
+                               network (0 when use_mlp is set but the graph lies outside the
+                               model's training range) */
 } mi355_sssp_stats;
 
 void mi355_sssp_options_default(mi355_sssp_options* opt);
@@ -435,6 +437,13 @@ typedef struct {
   float b2[3];
   float feat_mean[32];     /* inputs are normalised (x - mean) * inv_std; not trained */
   float feat_inv_std[32];
+  /* Range of the 14 graph features (inputs 0..13) over the training graphs.
+   * A graph with a feature outside [lo, hi] (1 % of the range as tolerance)
+   * runs with the fixed heuristic delta instead of the network
+   * (mi355_sssp_stats.policy = 0); a feature with hi <= lo is not bounded
+   * (zero-initialised params bound nothing).  Not trained. */
+  float graph_feat_lo[14];
+  float graph_feat_hi[14];
 } mi355_mlp_params;
 
 typedef struct {

@@ -243,9 +243,15 @@ void run_sssp(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult&
     case MI355_ALG_DELTA:
       run_delta_stepping(g, source, req, res, stream);
       break;
-    case MI355_ALG_ADDS:
-      run_adds(g, source, req, res, stream, prepare_adds_policy(g, req, stream));
+    case MI355_ALG_ADDS: {
+      int mode = kPolicyFixed;
+      const AddsPolicy* pol = prepare_adds_policy(g, req, stream, &mode);
+      RunRequest r = req;
+      r.use_mlp = mode == kPolicyMlp;
+      run_adds(g, source, r, res, stream, pol);
+      res.policy = mode;
       break;
+    }
     default:
       fail(MI355_EINVAL, "unknown SSSP algorithm");
   }

@@ -207,6 +207,7 @@ static void fill_stats(const mi355::RunResult& res, mi355_sssp_stats* st) {
   st->final_delta = res.final_delta;
   st->device_ms = res.device_ms;
   st->kernel_ms = res.kernel_ms;
+  st->policy = res.policy;
   st->has_negative_cycle = res.negative_cycle ? 1 : 0;
 }
 

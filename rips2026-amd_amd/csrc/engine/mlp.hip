@@ -34,7 +34,9 @@ constexpr int kOffMean = kP;
 constexpr int kOffInv = kP + kMlpIn;
 constexpr int kParamsWithNorm = kP + 2 * kMlpIn;
 constexpr int kRec = kP + 1;  // gradient partial record: P gradients + loss
-static_assert(sizeof(mi355_mlp_params) == kParamsWithNorm * sizeof(float), "parameter layout");
+constexpr int kGraphFeat = 14;  // graph features: inputs 0..13, range-checked per graph
+static_assert(sizeof(mi355_mlp_params) == (kParamsWithNorm + 2 * kGraphFeat) * sizeof(float),
+              "parameter layout");
 
 __device__ inline float bf16_round(float x) {
   uint32_t u = __float_as_uint(x);
@@ -237,7 +239,8 @@ void mlp_loss_grad(const mi355_mlp_params* prm, const float* h_x, const float* h
   if (grad) {
     float* out = reinterpret_cast<float*>(grad);
     std::memcpy(out, g.data(), kP * sizeof(float));
-    std::fill(out + kP, out + kParamsWithNorm, 0.0f);  // normalisation is not trained
+    // normalisation and the feature range are not trained
+    std::fill(out + kP, out + kParamsWithNorm + 2 * kGraphFeat, 0.0f);
   }
 }
 
@@ -319,7 +322,7 @@ void set_mlp_params(const mi355_mlp_params* prm) {
     h.feat_mean[i] = prm->feat_mean[i];
     h.feat_inv_std[i] = prm->feat_inv_std[i];
   }
-  set_mlp(h);
+  set_mlp(h, prm->graph_feat_lo, prm->graph_feat_hi);
 }
 
 }  // namespace mi355

@@ -14,6 +14,18 @@ namespace {
 std::mutex g_mlp_mu;
 bool g_mlp_loaded = false;
 AddsPolicy g_mlp{};  // weights + feature normalisation (mode/trace unused)
+float g_feat_lo[14], g_feat_hi[14];  // training range of the graph features
+
+// True when every bounded graph feature lies in the model's training range
+// (widened by 1 % of the range on each side).
+bool in_training_range(const float* f, const float* lo, const float* hi) {
+  for (int i = 0; i < 14; ++i) {
+    if (!(hi[i] > lo[i])) continue;  // not bounded
+    const float tol = 0.01f * (hi[i] - lo[i]);
+    if (!(f[i] >= lo[i] - tol && f[i] <= hi[i] + tol)) return false;
+  }
+  return true;
+}
 
 constexpr int kSamples = 4096;
 
@@ -66,9 +78,11 @@ void graph_features(mi355_graph& g, hipStream_t s, float* out14) {
   std::copy(g.graph_feat, g.graph_feat + 14, out14);
 }
 
-void set_mlp(const AddsPolicy& p) {
+void set_mlp(const AddsPolicy& p, const float* feat_lo, const float* feat_hi) {
   std::lock_guard<std::mutex> lk(g_mlp_mu);
   g_mlp = p;
+  std::copy(feat_lo, feat_lo + 14, g_feat_lo);
+  std::copy(feat_hi, feat_hi + 14, g_feat_hi);
   g_mlp_loaded = true;
 }
 
@@ -77,28 +91,38 @@ void clear_mlp() {
   g_mlp_loaded = false;
 }
 
-bool get_mlp(AddsPolicy* p) {
+bool get_mlp(AddsPolicy* p, float* feat_lo, float* feat_hi) {
   std::lock_guard<std::mutex> lk(g_mlp_mu);
   if (g_mlp_loaded && p) *p = g_mlp;
+  if (g_mlp_loaded && feat_lo) std::copy(g_feat_lo, g_feat_lo + 14, feat_lo);
+  if (g_mlp_loaded && feat_hi) std::copy(g_feat_hi, g_feat_hi + 14, feat_hi);
   return g_mlp_loaded;
 }
 
-const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hipStream_t s) {
+const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hipStream_t s,
+                                      int* mode_out) {
   AddsPolicy h{};
+  float lo[14], hi[14];
+  int mode = req.dynamic_delta ? kPolicyHeuristic : kPolicyFixed;
   if (req.use_mlp) {
-    if (!get_mlp(&h)) fail(MI355_EINVAL, "no delta model loaded (mi355_adds_mlp_set)");
-    h.mode = kPolicyMlp;
-  } else {
-    h.mode = req.dynamic_delta ? kPolicyHeuristic : kPolicyFixed;
+    if (!get_mlp(&h, lo, hi)) fail(MI355_EINVAL, "no delta model loaded (mi355_adds_mlp_set)");
+    mode = kPolicyMlp;
+  }
+  if (mode == kPolicyMlp || req.trace) graph_features(g, s, h.graph_feat);
+  // a graph unlike every training graph runs with the fixed heuristic delta:
+  // the network's output there is an extrapolation
+  if (mode == kPolicyMlp && !in_training_range(h.graph_feat, lo, hi)) mode = kPolicyFixed;
+  if (mode != kPolicyMlp) {
     for (int i = 0; i < kMlpIn; ++i) {
       h.feat_mean[i] = 0.0f;
       h.feat_inv_std[i] = 1.0f;
     }
   }
+  h.mode = mode;
   h.trace = req.trace;
   h.trace_cap = req.trace ? req.trace_cap : 0u;
-  if (h.mode == kPolicyFixed && !h.trace) return nullptr;
-  graph_features(g, s, h.graph_feat);
+  *mode_out = mode;
+  if (mode == kPolicyFixed && !h.trace) return nullptr;
   g.ws.adds_policy.ensure(1);
   MI_HIP(hipMemcpyAsync(g.ws.adds_policy.p, &h, sizeof(AddsPolicy), hipMemcpyHostToDevice, s));
   return g.ws.adds_policy.p;

@@ -37,6 +37,7 @@ struct RunResult {
   float final_delta = 0.f;
   float device_ms = 0.f;
   float kernel_ms = 0.f;   // the persistent kernel alone (ADDS)
+  int policy = 0;          // ADDS: PolicyMode used
   bool negative_cycle = false;
   std::vector<uint32_t> frontier_log;  // BF: |F_r| for r = 0..rounds
 };
@@ -65,8 +66,11 @@ void run_delta_stepping(mi355_graph& g, uint32_t source, const RunRequest& req, 
                         hipStream_t stream);
 
 // Device policy object of an ADDS run (adds.hip): fixed, feedback heuristic
-// or learned MLP, plus the optional phase trace.
-const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hipStream_t stream);
+// or learned MLP, plus the optional phase trace (nullptr: fixed, no trace).
+// *mode_out receives the policy used (PolicyMode): the MLP only when asked for
+// and the graph lies in the model's training range.
+const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hipStream_t stream,
+                                      int* mode_out);
 
 // Batched multi-source queries (batch.hip).
 void batch_layout(const mi355_query_batch& qb, int64_t n, std::vector<int64_t>& offsets);
@@ -76,8 +80,8 @@ void batch_autotune(mi355_graph& g, const mi355_query_batch& qb, int32_t stride,
                     double* timings, hipStream_t stream);
 
 // Process-wide delta model (policy.hip) and its training kernels (mlp.hip).
-void set_mlp(const AddsPolicy& p);
-bool get_mlp(AddsPolicy* p);
+void set_mlp(const AddsPolicy& p, const float* feat_lo, const float* feat_hi);
+bool get_mlp(AddsPolicy* p, float* feat_lo = nullptr, float* feat_hi = nullptr);
 void clear_mlp();
 void set_mlp_params(const mi355_mlp_params* p);
 void mlp_forward(const mi355_mlp_params* p, const float* h_x, int64_t n, float* h_out, hipStream_t s);

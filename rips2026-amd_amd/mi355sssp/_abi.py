@@ -53,12 +53,13 @@ class SsspStats(C.Structure):
                 ("vertices_processed", C.c_int64), ("reached", C.c_int64),
                 ("reached_edges", C.c_int64), ("initial_delta", C.c_float),
                 ("final_delta", C.c_float), ("device_ms", C.c_float), ("kernel_ms", C.c_float),
-                ("has_negative_cycle", C.c_int32), ("reserved_i", C.c_int32)]
+                ("has_negative_cycle", C.c_int32), ("policy", C.c_int32)]
 
 
 class MlpParams(C.Structure):
     _fields_ = [("w1", C.c_float * (64 * 32)), ("b1", C.c_float * 64), ("w2", C.c_float * (3 * 64)),
-                ("b2", C.c_float * 3), ("feat_mean", C.c_float * 32), ("feat_inv_std", C.c_float * 32)]
+                ("b2", C.c_float * 3), ("feat_mean", C.c_float * 32), ("feat_inv_std", C.c_float * 32),
+                ("graph_feat_lo", C.c_float * 14), ("graph_feat_hi", C.c_float * 14)]
 
 
 class MlpTrainOptions(C.Structure):

@@ -26,6 +26,7 @@ from ._abi import (ADDS_TRACE_STRIDE, ALG_ADDS, MlpParams, MlpTrainOptions, Sssp
                    check, engine)
 
 N_IN, N_HID, N_OUT = 32, 64, 3
+N_GRAPH = 14  # graph features: inputs 0..13
 _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(_HERE, "data", "delta_mlp.json")
 
@@ -49,13 +50,17 @@ class DeltaModel:
     """fp32 master weights of the 32 -> 64 -> 3 network plus the input
     normalisation (not trained)."""
 
-    def __init__(self, w1, b1, w2, b2, mean=None, inv_std=None):
+    def __init__(self, w1, b1, w2, b2, mean=None, inv_std=None, feat_lo=None, feat_hi=None):
         self.w1 = _f32(w1, (N_HID, N_IN))
         self.b1 = _f32(b1, (N_HID,))
         self.w2 = _f32(w2, (N_OUT, N_HID))
         self.b2 = _f32(b2, (N_OUT,))
         self.mean = _f32(np.zeros(N_IN) if mean is None else mean, (N_IN,))
         self.inv_std = _f32(np.ones(N_IN) if inv_std is None else inv_std, (N_IN,))
+        # training range of the graph features (inputs 0..13); lo == hi
+        # (the default) leaves a feature unbounded
+        self.feat_lo = _f32(np.zeros(N_GRAPH) if feat_lo is None else feat_lo, (N_GRAPH,))
+        self.feat_hi = _f32(np.zeros(N_GRAPH) if feat_hi is None else feat_hi, (N_GRAPH,))
 
     @classmethod
     def init(cls, seed: int = 0) -> "DeltaModel":
@@ -73,6 +78,8 @@ class DeltaModel:
         C.memmove(p.b2, self.b2.ctypes.data, self.b2.nbytes)
         C.memmove(p.feat_mean, self.mean.ctypes.data, self.mean.nbytes)
         C.memmove(p.feat_inv_std, self.inv_std.ctypes.data, self.inv_std.nbytes)
+        C.memmove(p.graph_feat_lo, self.feat_lo.ctypes.data, self.feat_lo.nbytes)
+        C.memmove(p.graph_feat_hi, self.feat_hi.ctypes.data, self.feat_hi.nbytes)
         return p
 
     @classmethod
@@ -80,7 +87,9 @@ class DeltaModel:
         return cls(np.ctypeslib.as_array(p.w1).copy(), np.ctypeslib.as_array(p.b1).copy(),
                    np.ctypeslib.as_array(p.w2).copy(), np.ctypeslib.as_array(p.b2).copy(),
                    np.ctypeslib.as_array(p.feat_mean).copy(),
-                   np.ctypeslib.as_array(p.feat_inv_std).copy())
+                   np.ctypeslib.as_array(p.feat_inv_std).copy(),
+                   np.ctypeslib.as_array(p.graph_feat_lo).copy(),
+                   np.ctypeslib.as_array(p.graph_feat_hi).copy())
 
     def flat(self) -> np.ndarray:
         return np.concatenate([self.w1.ravel(), self.b1, self.w2.ravel(), self.b2])
@@ -89,7 +98,8 @@ class DeltaModel:
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
         doc = {"w1": self.w1.tolist(), "b1": self.b1.tolist(), "w2": self.w2.tolist(),
                "b2": self.b2.tolist(), "feat_mean": self.mean.tolist(),
-               "feat_inv_std": self.inv_std.tolist(), "meta": meta or {}}
+               "feat_inv_std": self.inv_std.tolist(), "graph_feat_lo": self.feat_lo.tolist(),
+               "graph_feat_hi": self.feat_hi.tolist(), "meta": meta or {}}
         with open(path, "w") as f:
             json.dump(doc, f)
 
@@ -97,7 +107,8 @@ class DeltaModel:
     def load(cls, path: str) -> "DeltaModel":
         with open(path) as f:
             d = json.load(f)
-        m = cls(d["w1"], d["b1"], d["w2"], d["b2"], d["feat_mean"], d["feat_inv_std"])
+        m = cls(d["w1"], d["b1"], d["w2"], d["b2"], d["feat_mean"], d["feat_inv_std"],
+                d.get("graph_feat_lo"), d.get("graph_feat_hi"))
         m.meta = d.get("meta", {})
         return m
 
@@ -219,6 +230,19 @@ def training_specs(count: int, seed: int):
     return specs
 
 
+def large_grid_specs(count: int, seed: int):
+    """Seeded routing and 4-neighbour grids of 4e6-3e7 vertices (the size of
+    a full-chip routing graph): few of them, since each takes seconds per
+    setting."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        gseed = int(rng.integers(1, 2**62))
+        nv = math.exp(rng.uniform(math.log(4e6), math.log(3e7)))
+        specs.append({"kind": ("route", "grid")[i % 2], "side": int(round(math.sqrt(nv))), "seed": gseed})
+    return specs
+
+
 def make_graph(spec):
     from . import graph as G
 
@@ -303,12 +327,21 @@ def collect(specs, log=print, reps: int = 2, time_budget_s: float | None = None)
     return x, y, w, metas
 
 
+def graph_feature_range(x):
+    """Per-feature min / max of the graph features (inputs 0..13) over a
+    dataset: the range the engine trusts the network in."""
+    x = _f32(x, (-1, N_IN))
+    return _f32(x[:, :N_GRAPH].min(axis=0)), _f32(x[:, :N_GRAPH].max(axis=0))
+
+
 def fit(x, y, w=None, epochs: int = 60, batch: int = 512, lr: float = 2e-3, seed: int = 2026):
-    """Normalisation from the data, then Adam on the device."""
+    """Normalisation and the graph-feature range from the data, then Adam on
+    the device."""
     m = DeltaModel.init(seed)
     mean = x.mean(axis=0)
     std = x.std(axis=0)
     m.mean = _f32(mean)
     m.inv_std = _f32(np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-6), 0.0))
+    m.feat_lo, m.feat_hi = graph_feature_range(x)
     losses = m.train(x, y, w, epochs=epochs, batch=batch, lr=lr, seed=seed)
     return m, losses

@@ -106,6 +106,43 @@ def test_adds_with_network_is_exact_and_follows_the_network():
         L.clear_delta_model()
 
 
+def test_graph_outside_the_training_range_runs_the_fixed_delta():
+    """The model's graph-feature range decides whether the network is applied:
+    inside it the run reports policy 2 (learned), outside it the engine keeps
+    the fixed heuristic delta (policy 0, same phases as a plain run)."""
+    m, _, _, _ = _model_and_data(10)
+    m.mean[:] = 0.0
+    m.inv_std[:] = 0.25
+    g = M.rmat(12, 16, seed=21)
+    dg = M.DeviceGraph(g)
+    src = int(g.pick_sources(1, seed=5)[0])
+    rec, _ = L.trace(dg, src)
+    f = rec[0, :L.N_GRAPH]
+    try:
+        m.feat_lo, m.feat_hi = f - 1.0, f + 1.0  # contains the graph
+        m.install()
+        st = _check_adds(g, dg, src, use_mlp=1)
+        assert st.policy == 2
+        m.feat_lo, m.feat_hi = f - 1.0, f + 1.0
+        m.feat_hi[13] = f[13] - 0.5  # graph larger than every training graph
+        m.feat_lo[13] = f[13] - 1.0
+        m.install()
+        st = _check_adds(g, dg, src, use_mlp=1)
+        plain = _check_adds(g, dg, src)
+        assert st.policy == 0 and plain.policy == 0
+        assert st.initial_delta == plain.initial_delta
+        rec2, st2 = L.trace(dg, src, use_mlp=1)  # every phase at the heuristic delta
+        assert st2.policy == 0
+        assert np.all(rec2[:, 33] == plain.initial_delta) and np.all(rec2[:, 34] == 1)
+        assert np.all(rec2[:, 37] == L.default_workers())
+        m.feat_lo[:] = 0.0  # lo == hi: unbounded
+        m.feat_hi[:] = 0.0
+        m.install()
+        assert _check_adds(g, dg, src, use_mlp=1).policy == 2
+    finally:
+        L.clear_delta_model()
+
+
 def test_use_mlp_without_a_model_is_rejected():
     L.clear_delta_model()
     g = M.grid4(16, 16, 1, 9, seed=1)
