@@ -125,7 +125,8 @@ def routing_workloads(args, cpu_threads: int):
     # batched box-limited multi-source queries
     qs = B.routing_queries(side, side, args.queries, seed=args.seed)
     best, tune_ms = B.autotune(dg, qs, sample_stride=32)
-    opts = dict(rounds_per_graph=best.rounds_per_graph, blocks_per_cu=best.blocks_per_cu, order=best.order)
+    opts = dict(rounds_per_graph=best.rounds_per_graph, blocks_per_cu=best.blocks_per_cu, order=best.order,
+                delta_scale=best.delta_scale)
     B.run(dg, qs, want_pred=False, download=False, **opts)  # warm
     t0 = time.perf_counter()
     _, _, off, bst = B.run(dg, qs, want_pred=True, download=False, **opts)
@@ -160,7 +161,8 @@ def routing_workloads(args, cpu_threads: int):
     out["batched"] = {
         "queries": qs.count, "label_slots": int(bst.label_slots), "sources": int(bst.sources),
         "wall_s": wall, "device_ms": float(bst.device_ms), "queries_per_s": qps,
-        "rounds": int(bst.rounds), "graph_replays": int(bst.graph_replays),
+        "rounds": int(bst.rounds), "far_passes": int(bst.far_passes),
+        "graph_replays": int(bst.graph_replays),
         "tuned": opts, "tune_candidates_ms": [float(x) for x in tune_ms],
         "one_at_a_time_plain_adds": {"sampled_queries": int(len(sample)),
                                      "kernel_ms_sampled": [float(x) for x in one_ms],

@@ -383,13 +383,15 @@ typedef struct {
 } mi355_query_batch;
 
 typedef struct {
-  int32_t rounds_per_graph;    /* rounds per captured graph (multiple of 6; 0 = 24) */
+  int32_t rounds_per_graph;    /* rounds per captured graph (even; 0 = 24) */
   int32_t blocks_per_cu;       /* relaxation workgroups per CU (0 = 4) */
   int32_t use_graph;           /* 1: replay a captured HIP graph, 0: plain launches */
   int32_t inf_mode;            /* MI355_INF_IEEE or MI355_INF_FLT_MAX (f32 outputs) */
   int32_t count_edges;         /* count edge visits (stats) */
   int32_t order;               /* label layout: 0 input order, 1 by rectangle position */
-  int32_t reserved[2];
+  float delta_scale;           /* near-far bucket width = delta_scale x the graph's heuristic
+                                  delta0 (0 = 1); negative: Bellman-Ford rounds (no far pile) */
+  int32_t reserved;
   int64_t queue_capacity;      /* frontier entries per queue (0 = auto) */
 } mi355_batch_options;
 
@@ -399,7 +401,7 @@ typedef struct {
   int32_t rounds;              /* rounds with a non-empty frontier */
   int32_t graph_replays;
   int32_t rounds_per_graph;
-  int32_t reserved_i;
+  int32_t far_passes;          /* rounds that moved the far pile (threshold advances) */
   int64_t edges_relaxed;       /* when count_edges */
   int64_t queue_capacity;
   float device_ms;             /* initialisation + all rounds (hipEvent) */
@@ -414,9 +416,10 @@ int mi355_batch_layout(const mi355_graph* g, const mi355_query_batch* qb, int64_
  * unreached) or NULL.  Blocking. */
 int mi355_sssp_batched(mi355_graph* g, const mi355_query_batch* qb, const mi355_batch_options* o,
                        void* d_dist, int32_t* d_pred, mi355_batch_stats* stats, void* stream);
-/* Autotuner: times every (blocks_per_cu, rounds_per_graph, order) candidate on
- * every sample_stride-th query of the batch and writes the fastest into *best
- * (other fields of *best are kept).  timings_ms (host, may be NULL) receives
+/* Autotuner: times every (order, delta_scale, blocks_per_cu) candidate, order
+ * in {0, 1} x delta_scale in {0.5, 1, 2} x blocks_per_cu in {2, 4, 8}, on every
+ * sample_stride-th query of the batch and writes the fastest into *best (other
+ * fields of *best are kept).  timings_ms (host, may be NULL) receives
  * MI355_BATCH_TUNE_CANDIDATES device times in candidate order. */
 #define MI355_BATCH_TUNE_CANDIDATES 18
 int mi355_batch_autotune(mi355_graph* g, const mi355_query_batch* qb, int32_t sample_stride,

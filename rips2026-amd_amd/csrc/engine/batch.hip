@@ -7,19 +7,32 @@
 // frontier is a list of (query, local index) pairs from all queries at once,
 // so one round relaxes every query's frontier in the same launch:
 // D_q <- min(D_q, A^T (x) D_q) restricted to the rectangle, for all q.  A
-// round is the frontier kernel of the single-source engine (64 entries per
-// wave, edges packed across lanes, LDS-staged appends) with the labels
-// addressed per query and out-of-rectangle targets skipped.  Each entry is
-// queued at most once per round (per-slot round stamp).
+// round packs the edges of 64 entries per wave across lanes (two edges per
+// lane in flight), skips out-of-rectangle targets and stages appends in LDS.
 //
-// Rounds run as a captured HIP graph of K kernel nodes: the frontier size is
-// read from device memory (an empty frontier makes a round a no-op), queue
-// counters rotate over three slots and each round clears the one two rounds
-// ahead (no memset nodes), and a one-thread node advances the stamp epoch.
-// The host replays the graph and looks at the frontier size once per replay.
+// Near-far frontier (bucketed label correcting): an improved slot whose new
+// key is below the threshold T joins the next round's frontier ("near", at
+// most once per round: per-slot round stamp); otherwise it joins the far pile
+// (at most once per far epoch: per-slot far stamp).  When a round's input is
+// empty the round becomes a far pass instead: T advances by delta (or past
+// the smallest far key, skipping empty buckets), far entries below the new T
+// become the next frontier, entries above it move to the other far pile, and
+// entries below the old T are stale (their slot improved below T since and
+// was relaxed as near).  Bellman-Ford rounds are the case T = infinity; on
+// weighted routing grids they relax each slot 40-90 times, near-far a few.
+//
+// Every round is two kernel nodes: a one-thread decision kernel that reads
+// the counters the previous round produced and writes this round's
+// descriptor (relax / far pass / idle, threshold, pile sizes) and clears this
+// round's output counters, then the work kernel, which reads only the
+// descriptor and appends.  So no kernel reads a word that a workgroup of the
+// same kernel may be changing.  K rounds (K even: the queue parity returns)
+// plus a one-thread epoch node form a captured HIP graph (no memset nodes);
+// the host replays it and reads the number of entries left once per replay.
 // Distances are the exact fixpoint of the fp32 / uint32 left folds, identical
 // to a multi-source Dijkstra on each query's induced subgraph.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -38,28 +51,43 @@ struct QueryInfo {
   int32_t bh, bw;    // rectangle height and width
 };
 
-constexpr int kCntStride = 32;  // rotating counters on separate 128-byte lines
+constexpr int kCntStride = 32;  // counters on separate 128-byte lines
+constexpr uint32_t kRoundIdle = 0, kRoundRelax = 1, kRoundFar = 2;
+constexpr uint32_t kNearStage = 256;  // staged near appends per wave
+constexpr uint32_t kFarStage = 128;   // staged far appends per wave
 
 struct BatchCtrl {
-  alignas(128) uint32_t cnt[3 * kCntStride];
-  alignas(128) uint32_t epoch;
+  alignas(128) uint32_t cnt[2 * kCntStride];   // near queue sizes (round parity)
+  alignas(128) uint32_t fcnt[2 * kCntStride];  // far pile sizes (far epoch parity)
+  alignas(128) uint32_t fmin[2 * kCntStride];  // smallest key in each far pile
+  alignas(128) uint32_t epoch;                 // graph replays (round stamps)
   alignas(128) uint32_t err;
-  alignas(128) uint32_t busy_rounds;
   alignas(128) unsigned long long edges;
+  // written only by the decision kernel (and initialisation)
+  alignas(128) uint32_t fepoch;  // far epoch: appends go to pile fepoch & 1, stamp fepoch + 1
+  uint32_t tkey;                 // near threshold (key domain)
+  uint32_t busy_rounds, far_passes;
+  // this round's descriptor
+  uint32_t mode, n_in, far_n, t_old, t_new;
+  // entries left after the last round of a replay (read by the host)
+  uint32_t left;
 };
 
 struct BatchArgs {
   const int* rowptr;
   const uint2* edges;
   unsigned long long* labels;
-  uint32_t* stamp;
+  uint32_t* stamp;    // round stamp of the slot's last near append
+  uint32_t* fstamp;   // far stamp of the slot's last far append
   const QueryInfo* qinfo;
-  uint2* q[2];        // frontier queues of (query, local index)
-  uint32_t cap;
+  uint2* q[2];        // near queues of (query, local index)
+  uint2* far[2];      // far piles
+  uint32_t cap;       // entries per near queue / far pile
   BatchCtrl* ctrl;
   int32_t cols;       // grid width: vertex v = row * cols + col
   int32_t rounds_per_graph;
   int want_edges;
+  float delta;        // near-far bucket width (value domain); <= 0: Bellman-Ford rounds
 };
 
 __device__ inline void batch_stage_flush(uint2* stage, uint32_t& staged, uint2* q, uint32_t* cnt,
@@ -80,145 +108,251 @@ __device__ inline void batch_stage_flush(uint2* stage, uint32_t& staged, uint2* 
   staged = 0;
 }
 
+// Next near threshold (key domain) after the far pile whose smallest key is
+// kmin: one bucket further, or the end of kmin's bucket when the buckets in
+// between are empty, and always past kmin so that a far pass moves at least
+// one entry.  Correctness does not depend on T; only the work per slot does.
 template <class W>
-__global__ __launch_bounds__(kBlock) void k_batch_round(BatchArgs a, int k) {
+__device__ inline uint32_t next_threshold(uint32_t t_old, uint32_t kmin, float delta) {
+  uint32_t t;
+  if (W::kType == MI355_W_F32) {
+    const float told = key_to_f32(t_old), km = key_to_f32(kmin);
+    t = f32_to_key(fmaxf(told + delta, (floorf(km / delta) + 1.0f) * delta));
+  } else {
+    const double d = fmax(1.0, (double)delta);
+    const double tn = fmax((double)t_old + d, (floor((double)kmin / d) + 1.0) * d);
+    t = tn >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)tn;
+  }
+  return max(t, kmin + 1u);  // far keys are finite: kmin + 1 does not wrap
+}
+
+// The round's decision (one thread): relax the near frontier, or (frontier
+// empty) run a far pass with the next threshold, or idle (nothing left).
+// Clears the counters this round appends to.
+template <class W>
+__global__ void k_batch_decide(BatchArgs a, int k) {
+  BatchCtrl* c = a.ctrl;
+  const uint32_t n_in = min(c->cnt[(k & 1) * kCntStride], a.cap);
+  c->cnt[((k + 1) & 1) * kCntStride] = 0;  // this round's near output
+  uint32_t mode = kRoundIdle;
+  if (n_in > 0) {
+    mode = kRoundRelax;
+  } else {
+    const uint32_t e = c->fepoch;
+    const uint32_t nf = min(c->fcnt[(e & 1) * kCntStride], a.cap);
+    if (nf > 0) {
+      // the pass reads pile e & 1; entries above the new threshold go to the
+      // other pile, which the previous pass emptied
+      mode = kRoundFar;
+      const uint32_t t_old = c->tkey;
+      const uint32_t t_new = next_threshold<W>(t_old, c->fmin[(e & 1) * kCntStride], a.delta);
+      c->fcnt[((e + 1) & 1) * kCntStride] = 0;
+      c->fmin[((e + 1) & 1) * kCntStride] = W::kInfKey;
+      c->far_n = nf;
+      c->t_old = t_old;
+      c->t_new = t_new;
+      c->tkey = t_new;
+      c->fepoch = e + 1;
+      c->far_passes += 1;
+    }
+  }
+  if (mode != kRoundIdle) c->busy_rounds += 1;
+  c->mode = mode;
+  c->n_in = n_in;
+}
+
+// Stage one lane's entry (wave-uniform call) into a per-wave LDS stage.
+__device__ inline void stage_entry(bool push, uint2 item, uint2* stage, uint32_t& staged,
+                                   uint32_t stage_cap, uint2* q, uint32_t* cnt, uint32_t cap,
+                                   uint32_t* err) {
+  const unsigned long long m = __ballot(push);
+  if (m == 0) return;
+  const uint32_t kk = (uint32_t)__popcll(m);
+  if (staged + kk > stage_cap) batch_stage_flush(stage, staged, q, cnt, cap, err);
+  if (push) stage[staged + prefix_count(m)] = item;
+  staged += kk;
+}
+
+template <class W>
+__global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
   __shared__ int s_incl[kWavesPerBlock][kWave];
   __shared__ int s_beg[kWavesPerBlock][kWave];
   __shared__ uint32_t s_key[kWavesPerBlock][kWave];
   __shared__ int s_u[kWavesPerBlock][kWave];
   __shared__ int s_q[kWavesPerBlock][kWave];
-  __shared__ uint2 s_stage[kWavesPerBlock][kStageCap];
+  __shared__ uint2 s_near[kWavesPerBlock][kNearStage];
+  __shared__ uint2 s_far[kWavesPerBlock][kFarStage];
   BatchCtrl* c = a.ctrl;
-  // an overflowed queue (error flag set) holds only its first cap entries
-  const uint32_t n_in = min(c->cnt[(k % 3) * kCntStride], a.cap);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    c->cnt[((k + 2) % 3) * kCntStride] = 0;  // output counter of round k + 2
-    if (n_in) atomicAdd(&c->busy_rounds, 1u);
-  }
-  if (n_in == 0) return;
+  const uint32_t mode = c->mode;
+  if (mode == kRoundIdle) return;
   const uint32_t stamp = c->epoch * (uint32_t)a.rounds_per_graph + (uint32_t)k + 1u;
-  const uint2* qin = a.q[k & 1];
+  const uint32_t fe = c->fepoch;  // this epoch's pile: fe & 1, far stamp fe + 1
+  const uint32_t fst = fe + 1u;
+  uint2* fout = a.far[fe & 1];
+  uint32_t* fcnt_out = &c->fcnt[(fe & 1) * kCntStride];
   uint2* qout = a.q[(k + 1) & 1];
-  uint32_t* cnt_out = &c->cnt[((k + 1) % 3) * kCntStride];
+  uint32_t* cnt_out = &c->cnt[((k + 1) & 1) * kCntStride];
   const unsigned lane = threadIdx.x & (kWave - 1);
   const unsigned wv = threadIdx.x / kWave;
-  uint2* stage = s_stage[wv];
-  uint32_t staged = 0;
+  uint2* nst = s_near[wv];
+  uint2* fst_stage = s_far[wv];
+  uint32_t n_staged = 0, f_staged = 0;
+  uint32_t fmin_lane = W::kInfKey;
   unsigned long long edge_visits = 0;
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-  for (uint32_t first = (blockIdx.x * kWavesPerBlock + wv) * kWave; first < n_in; first += nwaves * kWave) {
-    const uint32_t i = first + lane;
-    int u = 0, beg = 0, deg = 0, qid = 0;
-    uint32_t key = W::kInfKey;
-    if (i < n_in) {
-      const uint2 it = qin[i];
-      qid = (int)it.x;
-      const QueryInfo qi = a.qinfo[qid];
-      const int r = (int)(it.y / (uint32_t)qi.bw), col = (int)(it.y - (uint32_t)r * (uint32_t)qi.bw);
-      u = (qi.r0 + r) * a.cols + qi.c0 + col;
-      key = atomic_load_key(a.labels + qi.base + it.y);
-      beg = a.rowptr[u];
-      if (key != W::kInfKey) deg = a.rowptr[u + 1] - beg;
+
+  if (mode == kRoundFar) {
+    // far pass over the pile of the epoch just closed
+    const uint2* fin = a.far[(fe - 1u) & 1];
+    const uint32_t nf = c->far_n, t_old = c->t_old, t_new = c->t_new;
+    for (uint32_t first = (blockIdx.x * kWavesPerBlock + wv) * kWave; first < nf; first += nwaves * kWave) {
+      const uint32_t i = first + lane;
+      const bool ok = i < nf;
+      const uint2 it = fin[ok ? i : 0u];
+      const int64_t slot = a.qinfo[it.x].base + it.y;
+      const uint32_t key = (uint32_t)(a.labels[slot] >> 32);
+      // below t_old: stale (the slot improved below the threshold since and
+      // was relaxed as near); [t_old, t_new): next frontier; above: stays far
+      const bool to_near = ok && key >= t_old && key < t_new;
+      const bool keep = ok && key >= t_new;
+      uint32_t x = stamp, y = fst;
+      if (to_near && a.stamp[slot] != stamp) x = atomicExch(a.stamp + slot, stamp);
+      if (keep && a.fstamp[slot] != fst) y = atomicExch(a.fstamp + slot, fst);
+      if (keep) fmin_lane = min(fmin_lane, key);
+      stage_entry(x != stamp, it, nst, n_staged, kNearStage, qout, cnt_out, a.cap, &c->err);
+      stage_entry(y != fst, it, fst_stage, f_staged, kFarStage, fout, fcnt_out, a.cap, &c->err);
     }
-    int incl = deg;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const int t = __shfl_up(incl, off, kWave);
-      if ((int)lane >= off) incl += t;
-    }
-    const int total = __shfl(incl, kWave - 1, kWave);
-    s_incl[wv][lane] = incl;
-    s_beg[wv][lane] = beg - (incl - deg);
-    s_key[wv][lane] = key;
-    s_u[wv][lane] = u;
-    s_q[wv][lane] = qid;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    edge_visits += (unsigned long long)total;
-    // Two edges per lane per step: each step is four rounds of independent
-    // accesses (edges + query records, labels + round stamps, atomicMins,
-    // stamp exchanges), so both edges' chains are in flight together.
-    // Lanes past the end of the packed list read edge 0 / query 0 / slot 0
-    // and discard them: guarded loads would each wait inside their branch.
-    for (int t = 0; t < total; t += 2 * kWave) {
-      const int j0 = t + (int)lane, j1 = j0 + kWave;
-      const bool ok0 = j0 < total, ok1 = j1 < total;
-      int p0 = 0, p1 = 0;
-#pragma unroll
-      for (int s2 = 32; s2 > 0; s2 >>= 1) {
-        if (s_incl[wv][p0 + s2 - 1] <= j0) p0 += s2;
-        if (s_incl[wv][p1 + s2 - 1] <= j1) p1 += s2;
+  } else {
+    const uint32_t n_in = c->n_in, tkey = c->tkey;
+    const uint2* qin = a.q[k & 1];
+    for (uint32_t first = (blockIdx.x * kWavesPerBlock + wv) * kWave; first < n_in; first += nwaves * kWave) {
+      const uint32_t i = first + lane;
+      int u = 0, beg = 0, deg = 0, qid = 0;
+      uint32_t key = W::kInfKey;
+      if (i < n_in) {
+        const uint2 it = qin[i];
+        qid = (int)it.x;
+        const QueryInfo qi = a.qinfo[qid];
+        const int r = (int)(it.y / (uint32_t)qi.bw), col = (int)(it.y - (uint32_t)r * (uint32_t)qi.bw);
+        u = (qi.r0 + r) * a.cols + qi.c0 + col;
+        key = atomic_load_key(a.labels + qi.base + it.y);
+        beg = a.rowptr[u];
+        if (key != W::kInfKey) deg = a.rowptr[u + 1] - beg;
       }
-      const int q0 = ok0 ? s_q[wv][p0] : 0, q1 = ok1 ? s_q[wv][p1] : 0;
-      const uint2 e0 = a.edges[ok0 ? s_beg[wv][p0] + j0 : 0];
-      const uint2 e1 = a.edges[ok1 ? s_beg[wv][p1] + j1 : 0];
-      const QueryInfo qi0 = a.qinfo[q0], qi1 = a.qinfo[q1];
-      // target inside the query's rectangle -> its local slot
-      const int rv0 = (int)e0.x / a.cols, cv0 = (int)e0.x - rv0 * a.cols;
-      const int rv1 = (int)e1.x / a.cols, cv1 = (int)e1.x - rv1 * a.cols;
-      const uint32_t dr0 = (uint32_t)(rv0 - qi0.r0), dc0 = (uint32_t)(cv0 - qi0.c0);
-      const uint32_t dr1 = (uint32_t)(rv1 - qi1.r0), dc1 = (uint32_t)(cv1 - qi1.c0);
-      const bool in0 = ok0 && dr0 < (uint32_t)qi0.bh && dc0 < (uint32_t)qi0.bw;
-      const bool in1 = ok1 && dr1 < (uint32_t)qi1.bh && dc1 < (uint32_t)qi1.bw;
-      const uint32_t lv0 = in0 ? dr0 * (uint32_t)qi0.bw + dc0 : 0u;
-      const uint32_t lv1 = in1 ? dr1 * (uint32_t)qi1.bw + dc1 : 0u;
-      const int64_t slot0 = in0 ? qi0.base + lv0 : 0, slot1 = in1 ? qi1.base + lv1 : 0;
-      const uint32_t ck0 = in0 ? W::relax(s_key[wv][p0], e0.y) : W::kInfKey;
-      const uint32_t ck1 = in1 ? W::relax(s_key[wv][p1], e1.y) : W::kInfKey;
-      constexpr unsigned long long kNone = ~0ull;
-      const unsigned long long cand0 =
-          ck0 != W::kInfKey ? ((unsigned long long)ck0 << 32) | (uint32_t)s_u[wv][p0] : kNone;
-      const unsigned long long cand1 =
-          ck1 != W::kInfKey ? ((unsigned long long)ck1 << 32) | (uint32_t)s_u[wv][p1] : kNone;
-      // label probes and round stamps (the stamp read is only a filter: the
-      // exchange below decides)
-      unsigned long long cur0 = a.labels[slot0], cur1 = a.labels[slot1];
-      uint32_t so0 = a.stamp[slot0], so1 = a.stamp[slot1];
-      asm volatile("" : "+v"(cur0), "+v"(cur1), "+v"(so0), "+v"(so1));
-      const bool try0 = cand0 < cur0, try1 = cand1 < cur1;
-      unsigned long long old0 = kNone, old1 = kNone;
-      if (try0) old0 = atomicMin(a.labels + slot0, cand0);
-      if (try1) old1 = atomicMin(a.labels + slot1, cand1);
-      const bool imp0 = try0 && ck0 < (uint32_t)(old0 >> 32);
-      const bool imp1 = try1 && ck1 < (uint32_t)(old1 >> 32);
-      uint32_t x0 = stamp, x1 = stamp;  // stamp already set: no push
-      if (imp0 && so0 != stamp) x0 = atomicExch(a.stamp + slot0, stamp);
-      if (imp1 && so1 != stamp) x1 = atomicExch(a.stamp + slot1, stamp);
-      asm volatile("" : "+v"(x0), "+v"(x1));
-      const bool push0 = x0 != stamp, push1 = x1 != stamp;
-      const unsigned long long m0 = __ballot(push0), m1 = __ballot(push1);
-      const uint32_t k0 = (uint32_t)__popcll(m0), k1 = (uint32_t)__popcll(m1);
-      if (k0 + k1) {
-        if (staged + k0 + k1 > kStageCap) batch_stage_flush(stage, staged, qout, cnt_out, a.cap, &c->err);
-        if (push0) stage[staged + prefix_count(m0)] = make_uint2((uint32_t)q0, lv0);
-        if (push1) stage[staged + k0 + prefix_count(m1)] = make_uint2((uint32_t)q1, lv1);
-        staged += k0 + k1;
+      int incl = deg;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const int t = __shfl_up(incl, off, kWave);
+        if ((int)lane >= off) incl += t;
       }
+      const int total = __shfl(incl, kWave - 1, kWave);
+      s_incl[wv][lane] = incl;
+      s_beg[wv][lane] = beg - (incl - deg);
+      s_key[wv][lane] = key;
+      s_u[wv][lane] = u;
+      s_q[wv][lane] = qid;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      edge_visits += (unsigned long long)total;
+      // Two edges per lane per step: four rounds of independent accesses
+      // (edges + query records; labels + both stamps; atomicMins; stamp
+      // exchanges).  Lanes past the end read edge 0 / query 0 / slot 0 and
+      // discard them (guarded loads would each wait inside their branch).
+      for (int t = 0; t < total; t += 2 * kWave) {
+        const int j0 = t + (int)lane, j1 = j0 + kWave;
+        const bool ok0 = j0 < total, ok1 = j1 < total;
+        int p0 = 0, p1 = 0;
+#pragma unroll
+        for (int s2 = 32; s2 > 0; s2 >>= 1) {
+          if (s_incl[wv][p0 + s2 - 1] <= j0) p0 += s2;
+          if (s_incl[wv][p1 + s2 - 1] <= j1) p1 += s2;
+        }
+        const int q0 = ok0 ? s_q[wv][p0] : 0, q1 = ok1 ? s_q[wv][p1] : 0;
+        const uint2 e0 = a.edges[ok0 ? s_beg[wv][p0] + j0 : 0];
+        const uint2 e1 = a.edges[ok1 ? s_beg[wv][p1] + j1 : 0];
+        const QueryInfo qi0 = a.qinfo[q0], qi1 = a.qinfo[q1];
+        // target inside the query's rectangle -> its local slot
+        const int rv0 = (int)e0.x / a.cols, cv0 = (int)e0.x - rv0 * a.cols;
+        const int rv1 = (int)e1.x / a.cols, cv1 = (int)e1.x - rv1 * a.cols;
+        const uint32_t dr0 = (uint32_t)(rv0 - qi0.r0), dc0 = (uint32_t)(cv0 - qi0.c0);
+        const uint32_t dr1 = (uint32_t)(rv1 - qi1.r0), dc1 = (uint32_t)(cv1 - qi1.c0);
+        const bool in0 = ok0 && dr0 < (uint32_t)qi0.bh && dc0 < (uint32_t)qi0.bw;
+        const bool in1 = ok1 && dr1 < (uint32_t)qi1.bh && dc1 < (uint32_t)qi1.bw;
+        const uint32_t lv0 = in0 ? dr0 * (uint32_t)qi0.bw + dc0 : 0u;
+        const uint32_t lv1 = in1 ? dr1 * (uint32_t)qi1.bw + dc1 : 0u;
+        const int64_t slot0 = in0 ? qi0.base + lv0 : 0, slot1 = in1 ? qi1.base + lv1 : 0;
+        const uint32_t ck0 = in0 ? W::relax(s_key[wv][p0], e0.y) : W::kInfKey;
+        const uint32_t ck1 = in1 ? W::relax(s_key[wv][p1], e1.y) : W::kInfKey;
+        constexpr unsigned long long kNone = ~0ull;
+        const unsigned long long cand0 =
+            ck0 != W::kInfKey ? ((unsigned long long)ck0 << 32) | (uint32_t)s_u[wv][p0] : kNone;
+        const unsigned long long cand1 =
+            ck1 != W::kInfKey ? ((unsigned long long)ck1 << 32) | (uint32_t)s_u[wv][p1] : kNone;
+        // label probes and both stamps (the stamp reads are only filters: the
+        // exchanges below decide)
+        unsigned long long cur0 = a.labels[slot0], cur1 = a.labels[slot1];
+        uint32_t so0 = a.stamp[slot0], so1 = a.stamp[slot1];
+        uint32_t fo0 = a.fstamp[slot0], fo1 = a.fstamp[slot1];
+        asm volatile("" : "+v"(cur0), "+v"(cur1), "+v"(so0), "+v"(so1), "+v"(fo0), "+v"(fo1));
+        const bool try0 = cand0 < cur0, try1 = cand1 < cur1;
+        unsigned long long old0 = kNone, old1 = kNone;
+        if (try0) old0 = atomicMin(a.labels + slot0, cand0);
+        if (try1) old1 = atomicMin(a.labels + slot1, cand1);
+        const bool imp0 = try0 && ck0 < (uint32_t)(old0 >> 32);
+        const bool imp1 = try1 && ck1 < (uint32_t)(old1 >> 32);
+        // below the threshold: next round (once per round); above: far pile
+        // (once per far epoch)
+        const bool near0 = imp0 && ck0 < tkey, near1 = imp1 && ck1 < tkey;
+        const bool far0 = imp0 && !near0, far1 = imp1 && !near1;
+        if (far0) fmin_lane = min(fmin_lane, ck0);
+        if (far1) fmin_lane = min(fmin_lane, ck1);
+        uint32_t x0 = stamp, x1 = stamp, y0 = fst, y1 = fst;  // already stamped: no push
+        if (near0 && so0 != stamp) x0 = atomicExch(a.stamp + slot0, stamp);
+        if (near1 && so1 != stamp) x1 = atomicExch(a.stamp + slot1, stamp);
+        if (far0 && fo0 != fst) y0 = atomicExch(a.fstamp + slot0, fst);
+        if (far1 && fo1 != fst) y1 = atomicExch(a.fstamp + slot1, fst);
+        asm volatile("" : "+v"(x0), "+v"(x1), "+v"(y0), "+v"(y1));
+        const uint2 it0 = make_uint2((uint32_t)q0, lv0), it1 = make_uint2((uint32_t)q1, lv1);
+        stage_entry(x0 != stamp, it0, nst, n_staged, kNearStage, qout, cnt_out, a.cap, &c->err);
+        stage_entry(x1 != stamp, it1, nst, n_staged, kNearStage, qout, cnt_out, a.cap, &c->err);
+        stage_entry(y0 != fst, it0, fst_stage, f_staged, kFarStage, fout, fcnt_out, a.cap, &c->err);
+        stage_entry(y1 != fst, it1, fst_stage, f_staged, kFarStage, fout, fcnt_out, a.cap, &c->err);
+      }
+      __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
   }
-  batch_stage_flush(stage, staged, qout, cnt_out, a.cap, &c->err);
+  batch_stage_flush(nst, n_staged, qout, cnt_out, a.cap, &c->err);
+  batch_stage_flush(fst_stage, f_staged, fout, fcnt_out, a.cap, &c->err);
+  // smallest far key of the pile (one atomic per wave)
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) fmin_lane = min(fmin_lane, (uint32_t)__shfl_xor((int)fmin_lane, off, 64));
+  if (lane == 0 && fmin_lane != W::kInfKey) atomicMin(&c->fmin[(fe & 1) * kCntStride], fmin_lane);
   if (a.want_edges && lane == 0 && edge_visits) atomicAdd(&c->edges, edge_visits);
 }
 
-__global__ void k_batch_epoch(BatchCtrl* c) { c->epoch += 1u; }
+// End of a replay: the graph epoch (round stamps) and the entries left.
+__global__ void k_batch_epoch(BatchCtrl* c, int K) {
+  c->epoch += 1u;
+  c->left = c->cnt[(K & 1) * kCntStride] + c->fcnt[(c->fepoch & 1) * kCntStride];
+}
 
 template <class W>
 __global__ void k_batch_init(unsigned long long* __restrict__ labels, uint32_t* __restrict__ stamp,
-                             int64_t total) {
+                             uint32_t* __restrict__ fstamp, int64_t total) {
   const unsigned long long inf_label = ((unsigned long long)W::kInfKey << 32) | kNoPred;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
     labels[i] = inf_label;
     stamp[i] = 0u;
+    fstamp[i] = 0u;
   }
 }
 
 template <class W>
 __global__ void k_batch_sources(unsigned long long* __restrict__ labels, const QueryInfo* __restrict__ qinfo,
                                 const uint2* __restrict__ src, uint32_t nsrc, uint2* __restrict__ q0,
-                                BatchCtrl* __restrict__ c) {
+                                BatchCtrl* __restrict__ c, uint32_t tkey0) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nsrc) {
     const uint2 s = src[i];
@@ -226,12 +360,21 @@ __global__ void k_batch_sources(unsigned long long* __restrict__ labels, const Q
     q0[i] = s;
   }
   if (i == 0) {
-    for (int k = 0; k < 3; ++k) c->cnt[k * kCntStride] = 0;
+    for (int k = 0; k < 2; ++k) {
+      c->cnt[k * kCntStride] = 0;
+      c->fcnt[k * kCntStride] = 0;
+      c->fmin[k * kCntStride] = W::kInfKey;
+    }
     c->cnt[0] = nsrc;
     c->epoch = 0;
     c->err = 0;
-    c->busy_rounds = 0;
     c->edges = 0;
+    c->fepoch = 0;
+    c->tkey = tkey0;
+    c->busy_rounds = 0;
+    c->far_passes = 0;
+    c->mode = kRoundIdle;
+    c->left = nsrc;
   }
 }
 
@@ -332,9 +475,17 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   int64_t cap = o.queue_capacity > 0 ? o.queue_capacity : std::max<int64_t>((total + nsrc) / 4, int64_t(1) << 24);
   cap = std::max<int64_t>(std::min<int64_t>(cap, total + nsrc), nsrc);
   if (cap > (int64_t)UINT32_MAX) cap = UINT32_MAX;
-  // a multiple of 6: the queue parity (2) and the counter rotation (3)
-  // return to round 0's state after K rounds
-  const int K = std::max(6, (o.rounds_per_graph > 0 ? o.rounds_per_graph : 24) / 6 * 6);
+  // even: the queue and counter parity return to round 0's after K rounds
+  const int K = std::max(2, (o.rounds_per_graph > 0 ? o.rounds_per_graph : 24) / 2 * 2);
+  // near-far bucket width; Bellman-Ford rounds (threshold infinity) when
+  // asked for or when the graph gives no positive width
+  if (!std::isfinite(o.delta_scale)) fail(MI355_EINVAL, "delta_scale must be finite");
+  const float delta = o.delta_scale < 0.0f ? 0.0f : heuristic_delta(g) * (o.delta_scale > 0.0f ? o.delta_scale : 1.0f);
+  const bool bucketed = delta > 0.0f && std::isfinite(delta);
+  uint32_t tkey0 = W::kInfKey;
+  if (bucketed)
+    tkey0 = W::kType == MI355_W_F32 ? f32_to_key(delta)
+                                    : (uint32_t)std::min(4294967294.0, std::max(1.0, std::ceil((double)delta)));
   const DeviceProps p = device_props();
   const unsigned grid = (unsigned)std::max(1, (o.blocks_per_cu > 0 ? o.blocks_per_cu : 4) * p.cus);
 
@@ -344,6 +495,11 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   ws.batch_stamp.ensure((size_t)total);
   ws.batch_q[0].ensure((size_t)cap);
   ws.batch_q[1].ensure((size_t)cap);
+  if (bucketed) {
+    ws.batch_fstamp.ensure((size_t)total);
+    ws.batch_far[0].ensure((size_t)cap);
+    ws.batch_far[1].ensure((size_t)cap);
+  }
   ws.batch_qinfo.ensure((size_t)nq * sizeof(QueryInfo));
   ws.batch_src.ensure((size_t)std::max<int64_t>(nsrc, 1));
   ws.batch_ctrl.ensure(sizeof(BatchCtrl));
@@ -364,10 +520,16 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   a.edges = g.edges.p;
   a.labels = bs.labels;
   a.stamp = bs.stamp;
+  // Bellman-Ford rounds never touch the far stamps or piles: they alias the
+  // round stamps and the queues (never read there)
+  a.fstamp = bucketed ? ws.batch_fstamp.p : bs.stamp;
   a.qinfo = bs.qinfo;
   a.q[0] = bs.q[0];
   a.q[1] = bs.q[1];
+  a.far[0] = bucketed ? ws.batch_far[0].p : bs.q[0];
+  a.far[1] = bucketed ? ws.batch_far[1].p : bs.q[1];
   a.cap = (uint32_t)cap;
+  a.delta = bucketed ? delta : 0.0f;
   a.ctrl = ctrl;
   a.cols = qb.grid_cols;
   a.rounds_per_graph = K;
@@ -378,9 +540,9 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   MI_HIP(hipEventCreate(&ev1));
   MI_HIP(hipEventRecord(ev0, s));
   k_batch_init<W><<<(unsigned)std::min<int64_t>(div_up((uint64_t)total, kBlock), 8192), kBlock, 0, s>>>(
-      bs.labels, bs.stamp, total);
+      bs.labels, bs.stamp, a.fstamp, total);
   k_batch_sources<W><<<(unsigned)div_up((uint64_t)std::max<int64_t>(nsrc, 1), kBlock), kBlock, 0, s>>>(
-      bs.labels, bs.qinfo, bs.src, (uint32_t)nsrc, bs.q[0], ctrl);
+      bs.labels, bs.qinfo, bs.src, (uint32_t)nsrc, bs.q[0], ctrl, tkey0);
   MI_LAUNCH_CHECK();
 
   // K rounds (+ the epoch node) as one graph, or as plain launches
@@ -388,15 +550,18 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   hipGraph_t graph = nullptr;
   if (o.use_graph) {
     MI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < K; ++k) k_batch_round<W><<<grid, kBlock, 0, s>>>(a, k);
-    k_batch_epoch<<<1, 1, 0, s>>>(ctrl);
+    for (int k = 0; k < K; ++k) {
+      k_batch_decide<W><<<1, 1, 0, s>>>(a, k);
+      k_batch_work<W><<<grid, kBlock, 0, s>>>(a, k);
+    }
+    k_batch_epoch<<<1, 1, 0, s>>>(ctrl, K);
     MI_HIP(hipStreamEndCapture(s, &graph));
     MI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
   uint32_t* h_cnt = nullptr;
   MI_HIP(hipHostMalloc((void**)&h_cnt, 4 * sizeof(uint32_t), hipHostMallocDefault));
   int64_t replays = 0;
-  const int64_t replay_limit = (total + nsrc) * 4 / K + 1024;  // far above any real run
+  const int64_t replay_limit = (total + nsrc) * 8 / K + 4096;  // far above any real run
   int batch = 1;
   bool done = false;
   try {
@@ -405,16 +570,19 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
         if (exec) {
           MI_HIP(hipGraphLaunch(exec, s));
         } else {
-          for (int k = 0; k < K; ++k) k_batch_round<W><<<grid, kBlock, 0, s>>>(a, k);
-          k_batch_epoch<<<1, 1, 0, s>>>(ctrl);
+          for (int k = 0; k < K; ++k) {
+            k_batch_decide<W><<<1, 1, 0, s>>>(a, k);
+            k_batch_work<W><<<grid, kBlock, 0, s>>>(a, k);
+          }
+          k_batch_epoch<<<1, 1, 0, s>>>(ctrl, K);
           MI_LAUNCH_CHECK();
         }
       }
-      // the next replay's input counter (round K == round 0 of the next epoch)
-      MI_HIP(hipMemcpyAsync(h_cnt, &ctrl->cnt[(K % 3) * kCntStride], 4, hipMemcpyDeviceToHost, s));
+      // entries left (next frontier + far pile) after the replay's last round
+      MI_HIP(hipMemcpyAsync(h_cnt, &ctrl->left, 4, hipMemcpyDeviceToHost, s));
       MI_HIP(hipMemcpyAsync(h_cnt + 1, &ctrl->err, 4, hipMemcpyDeviceToHost, s));
       MI_HIP(hipStreamSynchronize(s));
-      if (h_cnt[1]) fail(MI355_ECAPACITY, "batched frontier queue overflow (raise queue_capacity)");
+      if (h_cnt[1]) fail(MI355_ECAPACITY, "batched frontier queue or far pile overflow (raise queue_capacity)");
       done = h_cnt[0] == 0;
       if (!done && replays > replay_limit) fail(MI355_EDEVICE, "batched frontier made no progress");
       batch = std::min(batch * 2, 8);
@@ -452,6 +620,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   st.label_slots = total;
   st.sources = nsrc;
   st.rounds = (int32_t)hc.busy_rounds;
+  st.far_passes = (int32_t)hc.far_passes;
   st.graph_replays = (int32_t)replays;
   st.rounds_per_graph = K;
   st.edges_relaxed = (int64_t)hc.edges;
@@ -502,16 +671,17 @@ void batch_autotune(mi355_graph& g, const mi355_query_batch& qb, int32_t stride,
   std::vector<int64_t> soff;
   batch_layout(sub, g.n, soff);
   DevBuf<uint32_t> dist((size_t)soff.back());
-  const int bpc[3] = {2, 4, 8}, rpg[3] = {12, 24, 48};
+  const int bpc[3] = {2, 4, 8};
+  const float scales[3] = {0.5f, 1.0f, 2.0f};
   double best_ms = 1e300;
   int idx = 0;
   mi355_batch_options chosen = best;
   for (int order = 0; order < 2; ++order)
-    for (int r : rpg)
+    for (float sc : scales)
       for (int b : bpc) {
         mi355_batch_options o = best;
         o.blocks_per_cu = b;
-        o.rounds_per_graph = r;
+        o.delta_scale = sc;
         o.order = order;
         o.use_graph = 1;
         o.count_edges = 0;

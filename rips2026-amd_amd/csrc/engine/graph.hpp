@@ -52,7 +52,9 @@ struct Workspace {
   // batched multi-source queries (batch.hip)
   DevBuf<unsigned long long> batch_labels;
   DevBuf<uint32_t> batch_stamp;
+  DevBuf<uint32_t> batch_fstamp;  // near-far: far stamp per label slot
   DevBuf<uint2> batch_q[2];
+  DevBuf<uint2> batch_far[2];     // near-far: the two far piles
   DevBuf<unsigned char> batch_qinfo;
   DevBuf<uint2> batch_src;
   DevBuf<unsigned char> batch_ctrl;

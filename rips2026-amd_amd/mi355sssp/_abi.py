@@ -79,13 +79,14 @@ class QueryBatch(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [("rounds_per_graph", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("use_graph", C.c_int32), ("inf_mode", C.c_int32), ("count_edges", C.c_int32),
-                ("order", C.c_int32), ("reserved", C.c_int32 * 2), ("queue_capacity", C.c_int64)]
+                ("order", C.c_int32), ("delta_scale", C.c_float), ("reserved", C.c_int32),
+                ("queue_capacity", C.c_int64)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("label_slots", C.c_int64), ("sources", C.c_int64), ("rounds", C.c_int32),
                 ("graph_replays", C.c_int32), ("rounds_per_graph", C.c_int32),
-                ("reserved_i", C.c_int32), ("edges_relaxed", C.c_int64),
+                ("far_passes", C.c_int32), ("edges_relaxed", C.c_int64),
                 ("queue_capacity", C.c_int64), ("device_ms", C.c_float), ("reserved_f", C.c_float)]
 
 

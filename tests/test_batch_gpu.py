@@ -57,6 +57,27 @@ def test_routing_grid_queries_bit_exact(use_graph):
     assert dist2.tobytes() == dist.tobytes() and np.array_equal(pred2, pred)
 
 
+@pytest.mark.parametrize("delta_scale", [0.1, 1.0, 8.0, -1.0])
+def test_near_far_widths_and_bellman_ford_rounds_agree(delta_scale):
+    """Any bucket width (and Bellman-Ford rounds, delta_scale < 0) gives the
+    oracle's distances and parents; bucketed runs use the far pile, Bellman-
+    Ford rounds never do, and narrow buckets relax fewer edges."""
+    side = 130
+    g = M.routing_grid(side, seed=31)
+    dg = M.DeviceGraph(g)
+    qs = B.routing_queries(side, side, 40, seed=12, side_min=8, side_max=130)
+    print(f"batch: {qs.count} queries, delta_scale={delta_scale}", flush=True)
+    dist, pred, off, st = B.run(dg, qs, delta_scale=delta_scale, count_edges=1)
+    _check(g, qs, dist, pred, off)
+    if delta_scale < 0:
+        assert st.far_passes == 0
+    else:
+        assert st.far_passes > 0
+    if delta_scale == 1.0:
+        _, _, _, bf = B.run(dg, qs, delta_scale=-1.0, count_edges=1, download=False)
+        assert st.edges_relaxed < bf.edges_relaxed, (st.edges_relaxed, bf.edges_relaxed)
+
+
 def test_f32_grid_queries_bit_exact():
     g = _f32_grid(60, 70, seed=3)
     dg = M.DeviceGraph(g)
@@ -123,11 +144,12 @@ def test_autotune_and_schedule_orders():
     dg = M.DeviceGraph(g)
     qs = B.routing_queries(side, side, 40, seed=2, side_min=10, side_max=120)
     best, t = B.autotune(dg, qs, sample_stride=4)
-    assert np.all(t > 0) and best.rounds_per_graph % 6 == 0 and best.blocks_per_cu in (2, 4, 8)
+    assert np.all(t > 0) and best.rounds_per_graph % 2 == 0 and best.blocks_per_cu in (2, 4, 8)
+    assert best.delta_scale in (0.5, 1.0, 2.0)
     ref, pref, off, _ = B.run(dg, qs)
     for order in (0, 1):
         d, p, _, _ = B.run(dg, qs, order=order, rounds_per_graph=best.rounds_per_graph,
-                           blocks_per_cu=best.blocks_per_cu)
+                           blocks_per_cu=best.blocks_per_cu, delta_scale=best.delta_scale)
         assert d.tobytes() == ref.tobytes() and np.array_equal(p, pref)
     _check(g, qs, ref, pref, off, sample=range(0, 40, 7))
 
