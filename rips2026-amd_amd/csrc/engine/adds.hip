@@ -276,30 +276,50 @@ __device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp,
   stage_wanted(a, st, claim_stamp(a, imp, v, bkt), v, bkt, pushed);
 }
 
-// Relax two edges per lane (u0 -> v0 with candidate key ck0, u1 -> v1) and
-// stage the improved targets.  Both label probes, then both atomicMins, then
-// both stamp claims are issued before their results are used: the loop is
-// bound by the latency of these dependent accesses, so two independent
-// chains per lane double the accesses in flight.  Per edge this is the same
-// label-correcting step as relax_edge + push_improved.
+// Relax four edges per lane (edge p: u[p] -> v[p], candidate key ck[p],
+// kInfKey = none, lo[p] = producer bucket) and stage the improved targets.
+// The four label probes, then the four atomicMins, then the four stamp
+// claims are issued before any of their results is used: the loop is bound by
+// the latency of these dependent accesses, so four independent chains per
+// lane keep four times the accesses in flight.  A lane without an edge probes
+// label 0 and discards it (a guarded load would wait inside its branch).  Per
+// edge this is the same label-correcting step as relax_edge + push_improved.
 template <class W>
-__device__ inline void relax_pair(const AddsArgs& a, PushStage& st, bool ok0, uint32_t v0,
-                                  uint32_t ck0, uint32_t u0, uint32_t lo0, bool ok1, uint32_t v1,
-                                  uint32_t ck1, uint32_t u1, uint32_t lo1, uint32_t base,
-                                  float tbase, float inv, unsigned long long& pushed) {
-  bool imp0, imp1;
-  relax_edge_pair<W>(a.labels, v0, ok0 ? ck0 : W::kInfKey, u0, v1, ok1 ? ck1 : W::kInfKey, u1, imp0,
-                     imp1);
-  const uint32_t bk0 = imp0 ? map_bucket(W::bucket_value(ck0), base, tbase, inv, lo0) : 0u;
-  const uint32_t bk1 = imp1 ? map_bucket(W::bucket_value(ck1), base, tbase, inv, lo1) : 0u;
-  uint32_t q0 = 0u, q1 = 0u;  // 0 never exceeds a bucket: no claim
-  if (imp0) q0 = __hip_atomic_fetch_min(a.qstamp + v0, bk0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (imp1) q1 = __hip_atomic_fetch_min(a.qstamp + v1, bk1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  // keep both claims in flight: the comparisons stay after the second issue
-  asm volatile("" : "+v"(q0), "+v"(q1));
-  const bool w0 = q0 > bk0, w1 = q1 > bk1;
-  stage_wanted(a, st, w0, v0, bk0, pushed);
-  stage_wanted(a, st, w1, v1, bk1, pushed);
+__device__ inline void relax_quad(const AddsArgs& a, PushStage& st, const uint32_t (&v)[4],
+                                  const uint32_t (&ck)[4], const uint32_t (&u)[4],
+                                  const uint32_t (&lo)[4], uint32_t base, float tbase, float inv,
+                                  unsigned long long& pushed) {
+  constexpr unsigned long long kNone = ~0ull;
+  unsigned long long cand[4], cur[4], old[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const bool has = ck[p] != W::kInfKey;
+    cand[p] = has ? ((unsigned long long)ck[p] << 32) | u[p] : kNone;
+    cur[p] = a.labels[has ? v[p] : 0u];
+  }
+  // all four probes have arrived before the first atomic is issued
+  asm volatile("" : "+v"(cur[0]), "+v"(cur[1]), "+v"(cur[2]), "+v"(cur[3]));
+  bool imp[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    old[p] = kNone;
+    if (cand[p] < cur[p]) old[p] = atomicMin(a.labels + v[p], cand[p]);
+  }
+  uint32_t bk[4], q[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    imp[p] = cand[p] < cur[p] && ck[p] < (uint32_t)(old[p] >> 32);
+    bk[p] = imp[p] ? map_bucket(W::bucket_value(ck[p]), base, tbase, inv, lo[p]) : 0u;
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    q[p] = 0u;  // 0 never exceeds a bucket: no claim
+    if (imp[p]) q[p] = __hip_atomic_fetch_min(a.qstamp + v[p], bk[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // keep the four claims in flight: the comparisons stay after the last issue
+  asm volatile("" : "+v"(q[0]), "+v"(q[1]), "+v"(q[2]), "+v"(q[3]));
+#pragma unroll
+  for (int p = 0; p < 4; ++p) stage_wanted(a, st, q[p] > bk[p], v[p], bk[p], pushed);
 }
 
 // Publish a wave's staged heavy-list entries (all of one bucket, list slot
@@ -488,24 +508,37 @@ __device__ inline void relax_block(const AddsArgs& a, PushStage& st, int* s_incl
   __syncthreads();
   if (a.live_edges && threadIdx.x == 0 && total > 0) atomicAdd(&a.c->edges, (unsigned long long)total);
   if (wv == 0) edges += (unsigned long long)total;
-  // edge j belongs to the vertex p with s_incl[p - 1] <= j < s_incl[p]
+  // edge j belongs to the vertex p with s_incl[p - 1] <= j < s_incl[p];
+  // a wave takes 256-edge steps, four edges per lane (j, j + 64, j + 128,
+  // j + 192)
   static_assert(kBlock == 256, "binary search over 256 packed vertices");
-  for (int t = (int)wv * 2 * kWave; t < total; t += 2 * kBlock) {
-    const int j0 = t + (int)lane, j1 = j0 + kWave;
-    const bool ok0 = j0 < total, ok1 = j1 < total;
-    int p0 = 0, p1 = 0;
+  for (int t = (int)wv * 4 * kWave; t < total; t += 4 * kBlock) {
+    int pos[4];
+    bool ok[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      pos[p] = 0;
+      ok[p] = t + p * kWave + (int)lane < total;
+    }
 #pragma unroll
     for (int s2 = kBlock / 2; s2 > 0; s2 >>= 1) {
-      if (s_incl[p0 + s2 - 1] <= j0) p0 += s2;
-      if (s_incl[p1 + s2 - 1] <= j1) p1 += s2;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        if (s_incl[pos[p] + s2 - 1] <= t + p * kWave + (int)lane) pos[p] += s2;
     }
+    uint2 e[4];
     // lanes past the end read edge 0 and discard it (no wait inside a branch)
-    const uint2 e0 = a.edges[ok0 ? s_base[p0] + j0 : 0];
-    const uint2 e1 = a.edges[ok1 ? s_base[p1] + j1 : 0];
-    const uint32_t ck0 = ok0 ? W::relax(s_key[p0], e0.y) : W::kInfKey;
-    const uint32_t ck1 = ok1 ? W::relax(s_key[p1], e1.y) : W::kInfKey;
-    relax_pair<W>(a, st, ok0, e0.x, ck0, (uint32_t)s_u[p0], s_lo[p0], ok1, e1.x, ck1,
-                  (uint32_t)s_u[p1], s_lo[p1], base, tbase, inv, pushed);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) e[p] = a.edges[ok[p] ? s_base[pos[p]] + t + p * kWave + (int)lane : 0];
+    uint32_t v[4], ck[4], u[4], lo[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      v[p] = e[p].x;
+      ck[p] = ok[p] ? W::relax(s_key[pos[p]], e[p].y) : W::kInfKey;
+      u[p] = (uint32_t)s_u[pos[p]];
+      lo[p] = s_lo[pos[p]];
+    }
+    relax_quad<W>(a, st, v, ck, u, lo, base, tbase, inv, pushed);
   }
   __builtin_amdgcn_wave_barrier();
 }
@@ -758,16 +791,19 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       const uint32_t hu = ld_rlx(&hc->u), hkey = ld_rlx(&hc->key), hfirst = ld_rlx(&hc->first);
       const uint32_t hcnt = min(ld_rlx(&hc->cnt), (uint32_t)kChunk);
       const uint32_t hslot = ld_rlx(&hc->slot) % kAddsSlots, hbucket = ld_rlx(&hc->bucket);
-      // wave wv takes edges [wv * 128, wv * 128 + 128) of every 512, two per lane
-      for (uint32_t t0 = wv * 2 * kWave; t0 < hcnt; t0 += 2 * kBlock) {
-        const uint32_t i0 = t0 + lane, i1 = i0 + kWave;
-        const bool ok0 = i0 < hcnt, ok1 = i1 < hcnt;
-        const uint2 e0 = a.edges[hfirst + (ok0 ? i0 : 0u)];
-        const uint2 e1 = a.edges[hfirst + (ok1 ? i1 : 0u)];
-        const uint32_t ck0 = ok0 ? W::relax(hkey, e0.y) : W::kInfKey;
-        const uint32_t ck1 = ok1 ? W::relax(hkey, e1.y) : W::kInfKey;
-        relax_pair<W>(a, st, ok0, e0.x, ck0, hu, hbucket, ok1, e1.x, ck1, hu, hbucket, base, tbase,
-                      inv, pushed_local);
+      // wave wv takes edges [wv * 256, wv * 256 + 256) of every 1024, four per lane
+      for (uint32_t t0 = wv * 4 * kWave; t0 < hcnt; t0 += 4 * kBlock) {
+        uint32_t v[4], ck[4], uu[4], lo[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const uint32_t i = t0 + p * kWave + lane;
+          const uint2 e = a.edges[hfirst + (i < hcnt ? i : 0u)];
+          v[p] = e.x;
+          ck[p] = i < hcnt ? W::relax(hkey, e.y) : W::kInfKey;
+          uu[p] = hu;
+          lo[p] = hbucket;
+        }
+        relax_quad<W>(a, st, v, ck, uu, lo, base, tbase, inv, pushed_local);
       }
       stage_publish(a, st);
       if (a.live_edges && wv == 0 && lane == 0) atomicAdd(&c->edges, (unsigned long long)hcnt);
