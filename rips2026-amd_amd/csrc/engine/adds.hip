@@ -15,10 +15,11 @@
 //    are clipped into the last slot and re-queued when they come up) and
 //    never below the bucket of the vertex that produced it.
 //  * Workers.  Every workgroup is a worker: wave 0 claims up to 256 items and
-//    the four waves relax them, 64 items per wave with their edges packed
-//    across the lanes as in the frontier kernels; vertices with more than
-//    kLargeDegree edges in the range being relaxed are cut into 2048-edge hub
-//    chunks any workgroup can claim.  Claims go through counting semaphores
+//    the four waves relax them, their edges packed across the workgroup's
+//    lanes (four edges per lane in flight); vertices with more than
+//    kLargeDegree edges in the range being relaxed are cut into 8192-edge hub
+//    chunks any workgroup can claim (R-MAT scale 22: 2048-edge chunks cost
+//    12 % more time in claims and publishes).  Claims go through counting semaphores
 //    (`avail`: one fetch-sub, the surplus given back, then a fetch-add on the
 //    queue head for the positions), so hundreds of workgroups never retry a
 //    CAS on one queue head.  Order of preference: hub chunks, the heavy list
@@ -83,6 +84,7 @@ namespace {
 
 constexpr uint32_t kNoStamp = 0xFFFFFFFFu;
 constexpr uint32_t kClaim = 256;    // items per workgroup claim
+constexpr int kHubChunk = 8192;     // edges per hub chunk (rows above kLargeDegree)
 static_assert(5 + kAddsSlots <= 64, "claim probe: lanes 0-3, the window's semaphores, the worker share");
 constexpr int kEmptyEntry = -1;     // ring sentinel: reserved, not yet written
 constexpr uint32_t kPushStage = 512;  // staged pushes per wave
@@ -550,7 +552,7 @@ enum ClaimKind : uint32_t { kClaimItems = 0, kClaimHub = 1, kClaimHeavy = 2 };
 __device__ inline void make_hub_chunks(const AddsArgs& a, uint32_t u, uint32_t key, int beg, int end,
                                        uint32_t slot, uint32_t lo) {
   AddsCtrl* c = a.c;
-  const uint32_t nch = (uint32_t)((end - beg + kChunk - 1) / kChunk);
+  const uint32_t nch = (uint32_t)((end - beg + kHubChunk - 1) / kHubChunk);
   atomicAdd(&c->pending, (unsigned long long)nch);
   atomicAdd(&INFL(c, slot), (int)nch);
   const uint32_t pos = atomicAdd(&c->hub_reserve, nch);
@@ -560,11 +562,11 @@ __device__ inline void make_hub_chunks(const AddsArgs& a, uint32_t u, uint32_t k
   }
   for (uint32_t q = 0; q < nch; ++q) {
     HubChunk* hc = a.hub + ((pos + q) & a.hub_mask);
-    const int e0 = beg + (int)q * kChunk;
+    const int e0 = beg + (int)q * kHubChunk;
     st_rlx(&hc->u, u);
     st_rlx(&hc->key, key);
     st_rlx(&hc->first, (uint32_t)e0);
-    st_rlx(&hc->cnt, (uint32_t)(min(end, e0 + kChunk) - e0));
+    st_rlx(&hc->cnt, (uint32_t)(min(end, e0 + kHubChunk) - e0));
     st_rlx(&hc->slot, slot);
     st_rlx(&hc->bucket, lo);
     vm_wait();  // fields before the sequence number that publishes them
@@ -789,7 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       if (!s_hub_ok) break;
       // coherent reads: the record slot may hold an older chunk in a cache
       const uint32_t hu = ld_rlx(&hc->u), hkey = ld_rlx(&hc->key), hfirst = ld_rlx(&hc->first);
-      const uint32_t hcnt = min(ld_rlx(&hc->cnt), (uint32_t)kChunk);
+      const uint32_t hcnt = min(ld_rlx(&hc->cnt), (uint32_t)kHubChunk);
       const uint32_t hslot = ld_rlx(&hc->slot) % kAddsSlots, hbucket = ld_rlx(&hc->bucket);
       // wave wv takes edges [wv * 256, wv * 256 + 256) of every 1024, four per lane
       for (uint32_t t0 = wv * 4 * kWave; t0 < hcnt; t0 += 4 * kBlock) {
