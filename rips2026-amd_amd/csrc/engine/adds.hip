@@ -17,7 +17,7 @@
 //  * Workers.  Every workgroup is a worker: wave 0 claims up to 256 items and
 //    the four waves relax them, their edges packed across the workgroup's
 //    lanes (four edges per lane in flight); vertices with more than
-//    kLargeDegree edges in the range being relaxed are cut into 8192-edge hub
+//    kHubDegree edges in the range being relaxed are cut into 8192-edge hub
 //    chunks any workgroup can claim (R-MAT scale 22: 2048-edge chunks cost
 //    12 % more time in claims and publishes).  Claims go through counting semaphores
 //    (`avail`: one fetch-sub, the surplus given back, then a fetch-add on the
@@ -84,7 +84,8 @@ namespace {
 
 constexpr uint32_t kNoStamp = 0xFFFFFFFFu;
 constexpr uint32_t kClaim = 256;    // items per workgroup claim
-constexpr int kHubChunk = 8192;     // edges per hub chunk (rows above kLargeDegree)
+constexpr int kHubChunk = 8192;     // edges per hub chunk
+constexpr int kHubDegree = 4096;  // rows with more edges become hub chunks
 static_assert(5 + kAddsSlots <= 64, "claim probe: lanes 0-3, the window's semaphores, the worker share");
 constexpr int kEmptyEntry = -1;     // ring sentinel: reserved, not yet written
 constexpr uint32_t kPushStage = 512;  // staged pushes per wave
@@ -902,7 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           end = split;
         }
         deg = end - beg;
-        if (deg > kLargeDegree) {
+        if (deg > kHubDegree) {
           make_hub_chunks(a, (uint32_t)u, key, beg, end, slot, bk);
           deg = 0;
         }

@@ -327,6 +327,54 @@ def collect(specs, log=print, reps: int = 2, time_budget_s: float | None = None)
     return x, y, w, metas
 
 
+def time_weights(w, metas, power: float = 1.0):
+    """Per-sample weights that give each graph a total weight proportional to
+    its plain-ADDS kernel time to the given power (power 1: the time a policy
+    can save on it) instead of an equal share.  w gives every graph the same total weight c, spread
+    evenly over its records, in the order of metas (as collect() writes it),
+    so a graph's record count is c / w."""
+    w = np.asarray(w, np.float32)
+    out = np.empty_like(w)
+    c = float(w.astype(np.float64).sum()) / max(len(metas), 1)
+    i = 0
+    for m in metas:
+        k = int(round(c / float(w[i])))
+        out[i:i + k] = w[i:i + k] * float(m["plain_ms"]) ** power
+        i += k
+    if i != len(w):
+        raise ValueError("sample weights do not match the graphs' record counts")
+    return out
+
+
+def graph_class(meta) -> str:
+    """Size class of a training graph: R-MAT by scale, grids by kind and
+    decade of the vertex count."""
+    if meta["kind"] == "rmat":
+        return f"rmat{meta['scale']}"
+    decade = int(math.floor(math.log10(max(int(meta["n"]), 1))))
+    return f"{meta['kind']}{min(max(decade, 5), 7)}"  # 1e5-1e6, 1e6-1e7, above
+
+
+def class_weights(w, metas, class_of=graph_class):
+    """Per-sample weights that give every size class the same total weight
+    (graphs of a class share it equally), from the equal-per-graph weights w
+    in the order of metas."""
+    w = np.asarray(w, np.float32)
+    counts = {}
+    for m in metas:
+        counts[class_of(m)] = counts.get(class_of(m), 0) + 1
+    out = np.empty_like(w)
+    c = float(w.astype(np.float64).sum()) / max(len(metas), 1)
+    i = 0
+    for m in metas:
+        k = int(round(c / float(w[i])))
+        out[i:i + k] = w[i:i + k] / counts[class_of(m)]
+        i += k
+    if i != len(w):
+        raise ValueError("sample weights do not match the graphs' record counts")
+    return out
+
+
 def graph_feature_range(x):
     """Per-feature min / max of the graph features (inputs 0..13) over a
     dataset: the range the engine trusts the network in."""

@@ -45,6 +45,10 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=None, help="model JSON to write")
+    ap.add_argument("--graph-weight", choices=["class", "time", "sqrt-time", "equal"], default="class",
+                    help="weight of a graph's records: every size class (R-MAT scale, grid kind x "
+                         "decade of n) the same, its plain-ADDS time (the time at stake), its square "
+                         "root, or every graph the same")
     args = ap.parse_args()
 
     shards = list(args.shards)
@@ -70,10 +74,16 @@ def main() -> int:
             data.append((z["x"], z["y"], z["w"], json.loads(str(z["meta"]))))
     x = np.concatenate([d[0] for d in data])
     y = np.concatenate([d[1] for d in data])
-    w = np.concatenate([d[2] for d in data])
     metas = [m for d in data for m in d[3]]
     if args.out is None or len(x) == 0:
         return 0
+    # every shard (or collection) gives its graphs equal weights; bring them to
+    # one scale (1 per graph), then re-weight
+    w = np.concatenate([d[2] / (d[2].sum() / max(len(d[3]), 1)) for d in data])
+    if args.graph_weight == "class":
+        w = L.class_weights(w, metas)
+    elif args.graph_weight != "equal":
+        w = L.time_weights(w, metas, 1.0 if args.graph_weight == "time" else 0.5)
     w = w * (len(w) / w.sum())
     t0 = time.time()
     model, losses = L.fit(x, y, w, epochs=args.epochs, batch=args.batch, lr=args.lr)
@@ -83,7 +93,8 @@ def main() -> int:
     best_gain = [m["plain_ms"] / m["best_ms"] for m in metas]
     summary = {
         "graphs": len(metas), "samples": int(len(x)), "epochs": args.epochs, "batch": args.batch,
-        "lr": args.lr, "train_seed": args.seed, "loss_first_epoch": float(losses[0]),
+        "lr": args.lr, "train_seed": args.seed, "graph_weight": args.graph_weight,
+        "loss_first_epoch": float(losses[0]),
         "loss_last_epoch": float(losses[-1]),
         "mean_abs_err_log2_scale": float(np.abs(pred[:, 0] - y[:, 0]).mean()),
         "mean_abs_err_active": float(np.abs(pred[:, 1] - y[:, 1]).mean()),
