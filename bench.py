@@ -124,7 +124,7 @@ def routing_workloads(args, cpu_threads: int):
 
     # batched box-limited multi-source queries
     qs = B.routing_queries(side, side, args.queries, seed=args.seed)
-    best, tune_ms = B.autotune(dg, qs, sample_stride=32)
+    best, tune_ms = B.autotune(dg, qs, sample_stride=8)
     opts = dict(rounds_per_graph=best.rounds_per_graph, blocks_per_cu=best.blocks_per_cu, order=best.order,
                 delta_scale=best.delta_scale)
     B.run(dg, qs, want_pred=False, download=False, **opts)  # warm

@@ -255,69 +255,89 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       edge_visits += (unsigned long long)total;
-      // Two edges per lane per step: four rounds of independent accesses
-      // (edges + query records; labels + both stamps; atomicMins; stamp
-      // exchanges).  Lanes past the end read edge 0 / query 0 / slot 0 and
-      // discard them (guarded loads would each wait inside their branch).
-      for (int t = 0; t < total; t += 2 * kWave) {
-        const int j0 = t + (int)lane, j1 = j0 + kWave;
-        const bool ok0 = j0 < total, ok1 = j1 < total;
-        int p0 = 0, p1 = 0;
+      // Four edges per lane per step (j, j + 64, j + 128, j + 192): four
+      // rounds of independent accesses (edges + query records; labels + both
+      // stamps; atomicMins; stamp exchanges), each issued for all four edges
+      // before any result is used.  Lanes past the end read edge 0 / query 0
+      // / slot 0 and discard them (guarded loads would each wait inside
+      // their branch).
+      for (int t = 0; t < total; t += 4 * kWave) {
+        int pos[4], qq[4];
+        bool ok[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          pos[p] = 0;
+          ok[p] = t + p * kWave + (int)lane < total;
+        }
 #pragma unroll
         for (int s2 = 32; s2 > 0; s2 >>= 1) {
-          if (s_incl[wv][p0 + s2 - 1] <= j0) p0 += s2;
-          if (s_incl[wv][p1 + s2 - 1] <= j1) p1 += s2;
+#pragma unroll
+          for (int p = 0; p < 4; ++p)
+            if (s_incl[wv][pos[p] + s2 - 1] <= t + p * kWave + (int)lane) pos[p] += s2;
         }
-        const int q0 = ok0 ? s_q[wv][p0] : 0, q1 = ok1 ? s_q[wv][p1] : 0;
-        const uint2 e0 = a.edges[ok0 ? s_beg[wv][p0] + j0 : 0];
-        const uint2 e1 = a.edges[ok1 ? s_beg[wv][p1] + j1 : 0];
-        const QueryInfo qi0 = a.qinfo[q0], qi1 = a.qinfo[q1];
-        // target inside the query's rectangle -> its local slot
-        const int rv0 = (int)e0.x / a.cols, cv0 = (int)e0.x - rv0 * a.cols;
-        const int rv1 = (int)e1.x / a.cols, cv1 = (int)e1.x - rv1 * a.cols;
-        const uint32_t dr0 = (uint32_t)(rv0 - qi0.r0), dc0 = (uint32_t)(cv0 - qi0.c0);
-        const uint32_t dr1 = (uint32_t)(rv1 - qi1.r0), dc1 = (uint32_t)(cv1 - qi1.c0);
-        const bool in0 = ok0 && dr0 < (uint32_t)qi0.bh && dc0 < (uint32_t)qi0.bw;
-        const bool in1 = ok1 && dr1 < (uint32_t)qi1.bh && dc1 < (uint32_t)qi1.bw;
-        const uint32_t lv0 = in0 ? dr0 * (uint32_t)qi0.bw + dc0 : 0u;
-        const uint32_t lv1 = in1 ? dr1 * (uint32_t)qi1.bw + dc1 : 0u;
-        const int64_t slot0 = in0 ? qi0.base + lv0 : 0, slot1 = in1 ? qi1.base + lv1 : 0;
-        const uint32_t ck0 = in0 ? W::relax(s_key[wv][p0], e0.y) : W::kInfKey;
-        const uint32_t ck1 = in1 ? W::relax(s_key[wv][p1], e1.y) : W::kInfKey;
+        uint2 e[4];
+        QueryInfo qi[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          qq[p] = ok[p] ? s_q[wv][pos[p]] : 0;
+          e[p] = a.edges[ok[p] ? s_beg[wv][pos[p]] + t + p * kWave + (int)lane : 0];
+          qi[p] = a.qinfo[qq[p]];
+        }
         constexpr unsigned long long kNone = ~0ull;
-        const unsigned long long cand0 =
-            ck0 != W::kInfKey ? ((unsigned long long)ck0 << 32) | (uint32_t)s_u[wv][p0] : kNone;
-        const unsigned long long cand1 =
-            ck1 != W::kInfKey ? ((unsigned long long)ck1 << 32) | (uint32_t)s_u[wv][p1] : kNone;
-        // label probes and both stamps (the stamp reads are only filters: the
-        // exchanges below decide)
-        unsigned long long cur0 = a.labels[slot0], cur1 = a.labels[slot1];
-        uint32_t so0 = a.stamp[slot0], so1 = a.stamp[slot1];
-        uint32_t fo0 = a.fstamp[slot0], fo1 = a.fstamp[slot1];
-        asm volatile("" : "+v"(cur0), "+v"(cur1), "+v"(so0), "+v"(so1), "+v"(fo0), "+v"(fo1));
-        const bool try0 = cand0 < cur0, try1 = cand1 < cur1;
-        unsigned long long old0 = kNone, old1 = kNone;
-        if (try0) old0 = atomicMin(a.labels + slot0, cand0);
-        if (try1) old1 = atomicMin(a.labels + slot1, cand1);
-        const bool imp0 = try0 && ck0 < (uint32_t)(old0 >> 32);
-        const bool imp1 = try1 && ck1 < (uint32_t)(old1 >> 32);
-        // below the threshold: next round (once per round); above: far pile
-        // (once per far epoch)
-        const bool near0 = imp0 && ck0 < tkey, near1 = imp1 && ck1 < tkey;
-        const bool far0 = imp0 && !near0, far1 = imp1 && !near1;
-        if (far0) fmin_lane = min(fmin_lane, ck0);
-        if (far1) fmin_lane = min(fmin_lane, ck1);
-        uint32_t x0 = stamp, x1 = stamp, y0 = fst, y1 = fst;  // already stamped: no push
-        if (near0 && so0 != stamp) x0 = atomicExch(a.stamp + slot0, stamp);
-        if (near1 && so1 != stamp) x1 = atomicExch(a.stamp + slot1, stamp);
-        if (far0 && fo0 != fst) y0 = atomicExch(a.fstamp + slot0, fst);
-        if (far1 && fo1 != fst) y1 = atomicExch(a.fstamp + slot1, fst);
-        asm volatile("" : "+v"(x0), "+v"(x1), "+v"(y0), "+v"(y1));
-        const uint2 it0 = make_uint2((uint32_t)q0, lv0), it1 = make_uint2((uint32_t)q1, lv1);
-        stage_entry(x0 != stamp, it0, nst, n_staged, kNearStage, qout, cnt_out, a.cap, &c->err);
-        stage_entry(x1 != stamp, it1, nst, n_staged, kNearStage, qout, cnt_out, a.cap, &c->err);
-        stage_entry(y0 != fst, it0, fst_stage, f_staged, kFarStage, fout, fcnt_out, a.cap, &c->err);
-        stage_entry(y1 != fst, it1, fst_stage, f_staged, kFarStage, fout, fcnt_out, a.cap, &c->err);
+        int64_t slot[4];
+        uint32_t lv[4], ck[4];
+        unsigned long long cand[4], cur[4], old[4];
+        uint32_t so[4], fo[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          // target inside the query's rectangle -> its local slot
+          const int rv = (int)e[p].x / a.cols, cv = (int)e[p].x - rv * a.cols;
+          const uint32_t dr = (uint32_t)(rv - qi[p].r0), dc = (uint32_t)(cv - qi[p].c0);
+          const bool in = ok[p] && dr < (uint32_t)qi[p].bh && dc < (uint32_t)qi[p].bw;
+          lv[p] = in ? dr * (uint32_t)qi[p].bw + dc : 0u;
+          slot[p] = in ? qi[p].base + lv[p] : 0;
+          ck[p] = in ? W::relax(s_key[wv][pos[p]], e[p].y) : W::kInfKey;
+          cand[p] = ck[p] != W::kInfKey ? ((unsigned long long)ck[p] << 32) | (uint32_t)s_u[wv][pos[p]] : kNone;
+          // label probe and both stamps (the stamp reads are only filters:
+          // the exchanges below decide)
+          cur[p] = a.labels[slot[p]];
+          so[p] = a.stamp[slot[p]];
+          fo[p] = a.fstamp[slot[p]];
+        }
+        asm volatile("" : "+v"(cur[0]), "+v"(cur[1]), "+v"(cur[2]), "+v"(cur[3]));
+        asm volatile("" : "+v"(so[0]), "+v"(so[1]), "+v"(so[2]), "+v"(so[3]), "+v"(fo[0]), "+v"(fo[1]),
+                     "+v"(fo[2]), "+v"(fo[3]));
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          old[p] = kNone;
+          if (cand[p] < cur[p]) old[p] = atomicMin(a.labels + slot[p], cand[p]);
+        }
+        uint32_t x[4], y[4];
+        bool nr[4], fr[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const bool imp = cand[p] < cur[p] && ck[p] < (uint32_t)(old[p] >> 32);
+          // below the threshold: next round (once per round); above: far
+          // pile (once per far epoch)
+          nr[p] = imp && ck[p] < tkey;
+          fr[p] = imp && !nr[p];
+          if (fr[p]) fmin_lane = min(fmin_lane, ck[p]);
+          x[p] = stamp;  // already stamped: no push
+          y[p] = fst;
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          if (nr[p] && so[p] != stamp) x[p] = atomicExch(a.stamp + slot[p], stamp);
+          if (fr[p] && fo[p] != fst) y[p] = atomicExch(a.fstamp + slot[p], fst);
+        }
+        asm volatile("" : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(y[0]), "+v"(y[1]),
+                     "+v"(y[2]), "+v"(y[3]));
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const uint2 it = make_uint2((uint32_t)qq[p], lv[p]);
+          stage_entry(x[p] != stamp, it, nst, n_staged, kNearStage, qout, cnt_out, a.cap, &c->err);
+          stage_entry(y[p] != fst, it, fst_stage, f_staged, kFarStage, fout, fcnt_out, a.cap, &c->err);
+        }
       }
       __builtin_amdgcn_wave_barrier();
     }
