@@ -158,6 +158,29 @@ def routing_workloads(args, cpu_threads: int):
             log(f"batched query {q}: dist bit-exact={ok}")
             if not ok:
                 raise RuntimeError("batched parity failed")
+    # CPU baseline for the batch: the oracle's multi-source Dijkstra on each
+    # sampled query's induced subgraph, one query per thread on the host's
+    # cores (at most 16, this machine's share), scaled to the batch by area
+    cpu = None
+    if args.cpu_queries > 0:
+        from concurrent.futures import ThreadPoolExecutor
+
+        csample = np.linspace(0, qs.count - 1, args.cpu_queries).astype(int)
+
+        def one(q):
+            sq = qs.subset([int(q)])
+            return orc.dijkstra_box(g.rowptr, g.col, g.weights, side, tuple(sq.boxes[0]), sq.sources)
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(cpu_threads) as ex:
+            list(ex.map(one, csample))
+        cwall = time.perf_counter() - t0
+        est = cwall * float(areas.sum()) / float(areas[csample].sum())
+        cpu = {"threads": cpu_threads, "sampled_queries": int(len(csample)), "sample_wall_s": cwall,
+               "estimated_batch_s": est, "queries_per_s": qs.count / est,
+               "kind": "port (oracle multi-source Dijkstra per query, one query per thread)"}
+        log(f"batched CPU baseline: {len(csample)} queries on {cpu_threads} threads in {cwall:.1f} s "
+            f"-> ~{est:.1f} s for the batch ({qs.count / est:.0f} queries/s)")
     out["batched"] = {
         "queries": qs.count, "label_slots": int(bst.label_slots), "sources": int(bst.sources),
         "wall_s": wall, "device_ms": float(bst.device_ms), "queries_per_s": qps,
@@ -169,6 +192,7 @@ def routing_workloads(args, cpu_threads: int):
                                      "estimated_total_s": one_total_s,
                                      "queries_per_s": qs.count / one_total_s},
         "speedup_vs_one_at_a_time": one_total_s / wall,
+        "cpu_baseline": cpu,
     }
     log(f"batched: {qs.count} queries in {wall * 1e3:.1f} ms ({qps:.0f} queries/s); one at a time "
         f"~{one_total_s:.1f} s -> x{one_total_s / wall:.1f}")
@@ -257,6 +281,8 @@ def main() -> int:
     ap.add_argument("--grid-side", type=int, default=4900)
     ap.add_argument("--grid-sources", type=int, default=16)
     ap.add_argument("--queries", type=int, default=4096)
+    ap.add_argument("--cpu-queries", type=int, default=64,
+                    help="queries of the batch timed with the CPU oracle (0: skip)")
     ap.add_argument("--one-at-a-time", type=int, default=16,
                     help="queries sampled for the one-at-a-time plain ADDS comparison")
     ap.add_argument("--gemm", type=int, default=1, help="also time the batched min-plus GEMM workload")
@@ -399,7 +425,7 @@ def main() -> int:
                          f"{os.cpu_count()} host CPUs)",
                "ms_per_sssp": cwall / len(cs) * 1e3}
 
-    routing = routing_workloads(args, os.cpu_count() or 1) if args.routing else None
+    routing = routing_workloads(args, min(16, os.cpu_count() or 1)) if args.routing else None
     gemm = batched_gemm(args) if args.gemm else None
     traffic = profile_traffic(TRAFFIC_FILE) if args.algorithm == "adds" and args.scale == 22 else None
     out = {
