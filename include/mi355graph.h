@@ -127,6 +127,58 @@ float mi355g_rng_unit_f32(uint64_t seed, uint64_t stream, uint64_t index);
 void mi355g_fill_unit_f32(uint64_t seed, uint64_t stream, uint64_t first,
                           int64_t count, float* buf);
 
+/* ---- RIPSIFM1 routing metadata sidecar ---------------------------------
+ * The FPGA-interchange converter writes a RIPSCSR1 graph plus this sidecar
+ * (Routing/interchange_to_csr.cpp:1390-1587, write_metadata; magic
+ * "RIPSIFM1", format version 2, incoming orientation): a string table, the
+ * device node id of every CSR node, one (tile string, pip index) record per
+ * CSR edge in colind order, the pip table (wire0, wire1, forward), sink
+ * site-pin nodes, route requests (net, logical net, source and sink site-pin
+ * nodes), logical cell / net / port-instance summaries, blocked and
+ * sink-stop node lists and the raw netlist payloads.  All records are
+ * little-endian u64; "no index" is UINT64_MAX. */
+typedef struct mi355g_ifm mi355g_ifm;
+
+typedef struct {
+  uint64_t version, orientation;
+  uint64_t string_count, node_count, edge_attr_count, pip_data_count;
+  uint64_t site_pin_attr_count, route_request_count;
+  uint64_t blocked_node_count, sink_stop_node_count;
+  uint64_t logical_cell_count, logical_net_count, logical_port_instance_count;
+  uint64_t physical_netlist_byte_count, logical_netlist_byte_count;
+  uint64_t device_path_string, physical_path_string, logical_path_string;
+  uint64_t logical_design_name_string;
+  uint64_t route_source_count, route_sink_count;  /* totals over all requests */
+} mi355g_ifm_header_t;
+
+/* Reads and checks the whole sidecar (counts against the file size, node and
+ * pip references in range, string references in range or UINT64_MAX). */
+int mi355g_read_ripsifm1(const char* path, mi355g_ifm** out);
+void mi355g_ifm_free(mi355g_ifm* m);
+int mi355g_ifm_header(const mi355g_ifm* m, mi355g_ifm_header_t* out);
+/* String `index`: bytes (not NUL-terminated) valid until mi355g_ifm_free. */
+int mi355g_ifm_string(const mi355g_ifm* m, uint64_t index, const char** data, uint64_t* len);
+/* Fixed-width tables copied into caller arrays sized from the header (NULL
+ * skips a table): node_device_ids[node_count]; edge_attrs[2 * edge_attr_count]
+ * (tile string, pip index); pip_data[3 * pip_data_count] (wire0, wire1,
+ * forward); site_pins[3 * site_pin_attr_count] (node, site, pin);
+ * blocked[blocked_node_count]; sink_stop[sink_stop_node_count]. */
+int mi355g_ifm_tables(const mi355g_ifm* m, uint64_t* node_device_ids, uint64_t* edge_attrs,
+                      uint64_t* pip_data, uint64_t* site_pins, uint64_t* blocked,
+                      uint64_t* sink_stop);
+/* Route requests: nets[2 * count] (net string, logical net index);
+ * src_offsets[count + 1] and sources[3 * route_source_count] (node, site,
+ * pin); sink_offsets[count + 1] and sinks[3 * route_sink_count]. */
+int mi355g_ifm_requests(const mi355g_ifm* m, uint64_t* nets, uint64_t* src_offsets,
+                        uint64_t* sources, uint64_t* sink_offsets, uint64_t* sinks);
+/* Logical summaries: cells[3 * cell_count] (declaration name, net begin, net
+ * count); nets[4 * net_count] (name, cell, port-instance begin, count);
+ * ports[7 * port_instance_count] (port name, instance name, port index,
+ * instance index, bus index, has bus index, is external port). */
+int mi355g_ifm_logical(const mi355g_ifm* m, uint64_t* cells, uint64_t* nets, uint64_t* ports);
+/* The raw decompressed .phys / .netlist payloads (valid until mi355g_ifm_free). */
+int mi355g_ifm_payloads(const mi355g_ifm* m, const uint8_t** physical, const uint8_t** logical);
+
 #ifdef __cplusplus
 }
 #endif

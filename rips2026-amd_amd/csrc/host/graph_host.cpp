@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -25,6 +26,19 @@ struct mi355g_graph {
   std::vector<int32_t> col;
   std::vector<uint32_t> w;
   int64_t nnz() const { return (int64_t)col.size(); }
+};
+
+// RIPSIFM1 sidecar held in memory: the header counts, the string table (one
+// byte buffer plus offsets) and every table as flat u64 records.
+struct mi355g_ifm {
+  mi355g_ifm_header_t h{};
+  std::vector<char> str_bytes;
+  std::vector<uint64_t> str_off;  // string_count + 1
+  std::vector<uint64_t> node_ids, edge_attrs, pips, site_pins;
+  std::vector<uint64_t> nets, src_off, sources, sink_off, sinks;
+  std::vector<uint64_t> cells, lnets, ports;
+  std::vector<uint64_t> blocked, sink_stop;
+  std::vector<uint8_t> physical, logical;
 };
 
 namespace {
@@ -924,6 +938,216 @@ int mi355g_sha256(const mi355g_graph* g, char* hex65) {
     hex65[64] = 0;
     return 0;
   });
+}
+
+}  // extern "C"
+
+// ---- RIPSIFM1 ------------------------------------------------------------
+// Layout as written by Routing/interchange_to_csr.cpp:1390-1587 (all u64
+// little-endian): magic "RIPSIFM1"; version, orientation; the 13 counts and
+// 4 path / design-name string references of mi355g_ifm_header_t; the strings
+// (u64 length + bytes); node device ids; (tile, pip) per edge; (wire0, wire1,
+// forward) per pip; (node, site, pin) per sink site pin; per route request
+// net string, logical net index, source count + (node, site, pin) records,
+// sink count + records; logical cells (3 u64), nets (4), port instances (7);
+// blocked nodes; sink-stop nodes; the physical and logical netlist bytes.
+namespace {
+
+constexpr uint64_t kIfmNoIndex = ~0ull;
+
+struct IfmReader {
+  FILE* f;
+  uint64_t left;  // bytes not yet read
+  bool u64(uint64_t& v) {
+    if (left < 8 || std::fread(&v, 8, 1, f) != 1) return false;
+    left -= 8;
+    return true;
+  }
+  // n records of k u64 each; refuses counts the file cannot hold before
+  // allocating
+  bool table(std::vector<uint64_t>& out, uint64_t n, uint64_t k) {
+    if (k && n > left / (8 * k)) return false;
+    out.resize((size_t)(n * k));
+    if (!read_arr(f, out.data(), out.size())) return false;
+    left -= n * k * 8;
+    return true;
+  }
+  bool bytes(std::vector<uint8_t>& out, uint64_t n) {
+    if (n > left) return false;
+    out.resize((size_t)n);
+    if (!read_arr(f, out.data(), out.size())) return false;
+    left -= n;
+    return true;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int mi355g_read_ripsifm1(const char* path, mi355g_ifm** out) {
+  return guarded([&] {
+    if (!path || !out) return fail(MI355G_EINVAL, "null argument");
+    File f(path, "rb");
+    if (!f.f) return fail(MI355G_EIO, std::string("cannot open ") + path);
+    if (std::fseek(f.f, 0, SEEK_END) != 0) return fail(MI355G_EIO, "cannot size the file");
+    const long size = std::ftell(f.f);
+    std::rewind(f.f);
+    if (size < 8) return fail(MI355G_EFORMAT, "truncated RIPSIFM1 file");
+    IfmReader r{f.f, (uint64_t)size};
+    char magic[8];
+    if (!read_arr(f.f, magic, 8) || std::memcmp(magic, "RIPSIFM1", 8) != 0)
+      return fail(MI355G_EFORMAT, "bad RIPSIFM1 magic");
+    r.left -= 8;
+    auto m = std::make_unique<mi355g_ifm>();
+    mi355g_ifm_header_t& h = m->h;
+    uint64_t* fields[] = {&h.version, &h.orientation, &h.string_count, &h.node_count,
+                          &h.edge_attr_count, &h.pip_data_count, &h.site_pin_attr_count,
+                          &h.route_request_count, &h.blocked_node_count, &h.sink_stop_node_count,
+                          &h.logical_cell_count, &h.logical_net_count,
+                          &h.logical_port_instance_count, &h.physical_netlist_byte_count,
+                          &h.logical_netlist_byte_count, &h.device_path_string,
+                          &h.physical_path_string, &h.logical_path_string,
+                          &h.logical_design_name_string};
+    for (uint64_t* v : fields)
+      if (!r.u64(*v)) return fail(MI355G_EFORMAT, "truncated RIPSIFM1 header");
+    if (h.version != 2) return fail(MI355G_EFORMAT, "unsupported RIPSIFM1 version (expected 2)");
+    if (h.orientation != 1) return fail(MI355G_EFORMAT, "RIPSIFM1 orientation must be 1 (incoming)");
+    // strings
+    if (h.string_count > r.left / 8) return fail(MI355G_EFORMAT, "truncated RIPSIFM1 strings");
+    m->str_off.assign((size_t)h.string_count + 1, 0);
+    for (uint64_t i = 0; i < h.string_count; ++i) {
+      uint64_t len = 0;
+      if (!r.u64(len) || len > r.left) return fail(MI355G_EFORMAT, "truncated RIPSIFM1 strings");
+      const size_t at = m->str_bytes.size();
+      m->str_bytes.resize(at + (size_t)len);
+      if (!read_arr(f.f, m->str_bytes.data() + at, (size_t)len))
+        return fail(MI355G_EFORMAT, "truncated RIPSIFM1 strings");
+      r.left -= len;
+      m->str_off[(size_t)i + 1] = m->str_bytes.size();
+    }
+    if (!r.table(m->node_ids, h.node_count, 1) || !r.table(m->edge_attrs, h.edge_attr_count, 2) ||
+        !r.table(m->pips, h.pip_data_count, 3) || !r.table(m->site_pins, h.site_pin_attr_count, 3))
+      return fail(MI355G_EFORMAT, "truncated RIPSIFM1 tables");
+    // route requests: variable-length source / sink lists
+    m->nets.reserve((size_t)std::min<uint64_t>(h.route_request_count, r.left / 32) * 2);
+    m->src_off.assign(1, 0);
+    m->sink_off.assign(1, 0);
+    for (uint64_t q = 0; q < h.route_request_count; ++q) {
+      uint64_t net = 0, lnet = 0, k = 0;
+      if (!r.u64(net) || !r.u64(lnet) || !r.u64(k) || k > r.left / 24)
+        return fail(MI355G_EFORMAT, "truncated RIPSIFM1 route requests");
+      m->nets.push_back(net);
+      m->nets.push_back(lnet);
+      std::vector<uint64_t> rec;
+      if (!r.table(rec, k, 3)) return fail(MI355G_EFORMAT, "truncated RIPSIFM1 route requests");
+      m->sources.insert(m->sources.end(), rec.begin(), rec.end());
+      m->src_off.push_back(m->sources.size() / 3);
+      if (!r.u64(k) || !r.table(rec, k, 3)) return fail(MI355G_EFORMAT, "truncated RIPSIFM1 route requests");
+      m->sinks.insert(m->sinks.end(), rec.begin(), rec.end());
+      m->sink_off.push_back(m->sinks.size() / 3);
+    }
+    h.route_source_count = m->sources.size() / 3;
+    h.route_sink_count = m->sinks.size() / 3;
+    if (!r.table(m->cells, h.logical_cell_count, 3) || !r.table(m->lnets, h.logical_net_count, 4) ||
+        !r.table(m->ports, h.logical_port_instance_count, 7) ||
+        !r.table(m->blocked, h.blocked_node_count, 1) || !r.table(m->sink_stop, h.sink_stop_node_count, 1) ||
+        !r.bytes(m->physical, h.physical_netlist_byte_count) ||
+        !r.bytes(m->logical, h.logical_netlist_byte_count))
+      return fail(MI355G_EFORMAT, "truncated RIPSIFM1 trailing sections");
+    if (r.left != 0) return fail(MI355G_EFORMAT, "trailing bytes after the RIPSIFM1 sections");
+    // references
+    auto str_ok = [&](uint64_t s, bool none_ok) {
+      return s < h.string_count || (none_ok && s == kIfmNoIndex);
+    };
+    for (uint64_t i = 0; i < h.edge_attr_count; ++i)
+      if (!str_ok(m->edge_attrs[2 * i], false) || m->edge_attrs[2 * i + 1] >= h.pip_data_count)
+        return fail(MI355G_EFORMAT, "RIPSIFM1 edge attribute refers outside the tables");
+    for (uint64_t i = 0; i < h.pip_data_count; ++i)
+      if (!str_ok(m->pips[3 * i], false) || !str_ok(m->pips[3 * i + 1], false) || m->pips[3 * i + 2] > 1)
+        return fail(MI355G_EFORMAT, "RIPSIFM1 pip record refers outside the tables");
+    auto pins_ok = [&](const std::vector<uint64_t>& v) {
+      for (size_t i = 0; i < v.size(); i += 3)
+        if (v[i] >= h.node_count || !str_ok(v[i + 1], false) || !str_ok(v[i + 2], false)) return false;
+      return true;
+    };
+    if (!pins_ok(m->site_pins) || !pins_ok(m->sources) || !pins_ok(m->sinks))
+      return fail(MI355G_EFORMAT, "RIPSIFM1 site pin refers outside the tables");
+    for (size_t q = 0; q < m->nets.size(); q += 2)
+      if (!str_ok(m->nets[q], false) ||
+          (m->nets[q + 1] != kIfmNoIndex && m->nets[q + 1] >= h.logical_net_count))
+        return fail(MI355G_EFORMAT, "RIPSIFM1 route request refers outside the tables");
+    for (uint64_t v : m->blocked)
+      if (v >= h.node_count) return fail(MI355G_EFORMAT, "RIPSIFM1 blocked node out of range");
+    for (uint64_t v : m->sink_stop)
+      if (v >= h.node_count) return fail(MI355G_EFORMAT, "RIPSIFM1 sink-stop node out of range");
+    for (uint64_t s2 : {h.device_path_string, h.physical_path_string, h.logical_path_string,
+                        h.logical_design_name_string})
+      if (!str_ok(s2, true)) return fail(MI355G_EFORMAT, "RIPSIFM1 header string out of range");
+    *out = m.release();
+    return 0;
+  });
+}
+
+void mi355g_ifm_free(mi355g_ifm* m) { delete m; }
+
+int mi355g_ifm_header(const mi355g_ifm* m, mi355g_ifm_header_t* out) {
+  if (!m || !out) return fail(MI355G_EINVAL, "null argument");
+  *out = m->h;
+  return 0;
+}
+
+int mi355g_ifm_string(const mi355g_ifm* m, uint64_t index, const char** data, uint64_t* len) {
+  if (!m || !data || !len) return fail(MI355G_EINVAL, "null argument");
+  if (index >= m->h.string_count) return fail(MI355G_EINVAL, "string index out of range");
+  *data = m->str_bytes.data() + m->str_off[(size_t)index];
+  *len = m->str_off[(size_t)index + 1] - m->str_off[(size_t)index];
+  return 0;
+}
+
+namespace {
+void copy_out(const std::vector<uint64_t>& v, uint64_t* dst) {
+  if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * 8);
+}
+}  // namespace
+
+int mi355g_ifm_tables(const mi355g_ifm* m, uint64_t* node_device_ids, uint64_t* edge_attrs,
+                      uint64_t* pip_data, uint64_t* site_pins, uint64_t* blocked,
+                      uint64_t* sink_stop) {
+  if (!m) return fail(MI355G_EINVAL, "null argument");
+  copy_out(m->node_ids, node_device_ids);
+  copy_out(m->edge_attrs, edge_attrs);
+  copy_out(m->pips, pip_data);
+  copy_out(m->site_pins, site_pins);
+  copy_out(m->blocked, blocked);
+  copy_out(m->sink_stop, sink_stop);
+  return 0;
+}
+
+int mi355g_ifm_requests(const mi355g_ifm* m, uint64_t* nets, uint64_t* src_offsets,
+                        uint64_t* sources, uint64_t* sink_offsets, uint64_t* sinks) {
+  if (!m) return fail(MI355G_EINVAL, "null argument");
+  copy_out(m->nets, nets);
+  copy_out(m->src_off, src_offsets);
+  copy_out(m->sources, sources);
+  copy_out(m->sink_off, sink_offsets);
+  copy_out(m->sinks, sinks);
+  return 0;
+}
+
+int mi355g_ifm_logical(const mi355g_ifm* m, uint64_t* cells, uint64_t* nets, uint64_t* ports) {
+  if (!m) return fail(MI355G_EINVAL, "null argument");
+  copy_out(m->cells, cells);
+  copy_out(m->lnets, nets);
+  copy_out(m->ports, ports);
+  return 0;
+}
+
+int mi355g_ifm_payloads(const mi355g_ifm* m, const uint8_t** physical, const uint8_t** logical) {
+  if (!m) return fail(MI355G_EINVAL, "null argument");
+  if (physical) *physical = m->physical.data();
+  if (logical) *logical = m->logical.data();
+  return 0;
 }
 
 }  // extern "C"

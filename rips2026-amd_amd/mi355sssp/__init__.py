@@ -37,4 +37,4 @@ __all__ = [
 from .kernels import (minplus_gemm, minplus_gemm_batched, minplus_autotune, minplus_tuning,  # noqa: E402
                       spgemm, bf_dense, gauss_seidel, one_sssp, hip_adds)
 from ._abi import OneOptions, OneResult, AddsOptions, AddsStats  # noqa: E402
-from . import batch, learn  # noqa: E402,F401
+from . import batch, learn, routing  # noqa: E402,F401

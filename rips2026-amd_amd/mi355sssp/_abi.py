@@ -201,7 +201,27 @@ _GRAPH_SIGS = {
     "mi355g_rng_u64": (U64, [U64, U64, U64]),
     "mi355g_rng_unit_f32": (C.c_float, [U64, U64, U64]),
     "mi355g_fill_unit_f32": (None, [U64, U64, U64, I64, P]),
+    "mi355g_read_ripsifm1": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "mi355g_ifm_free": (None, [P]),
+    "mi355g_ifm_header": (C.c_int, [P, P]),
+    "mi355g_ifm_string": (C.c_int, [P, U64, C.POINTER(C.c_void_p), C.POINTER(U64)]),
+    "mi355g_ifm_tables": (C.c_int, [P, P, P, P, P, P, P]),
+    "mi355g_ifm_requests": (C.c_int, [P, P, P, P, P, P]),
+    "mi355g_ifm_logical": (C.c_int, [P, P, P, P]),
+    "mi355g_ifm_payloads": (C.c_int, [P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
 }
+
+IFM_HEADER_FIELDS = ("version", "orientation", "string_count", "node_count", "edge_attr_count",
+                     "pip_data_count", "site_pin_attr_count", "route_request_count",
+                     "blocked_node_count", "sink_stop_node_count", "logical_cell_count",
+                     "logical_net_count", "logical_port_instance_count",
+                     "physical_netlist_byte_count", "logical_netlist_byte_count",
+                     "device_path_string", "physical_path_string", "logical_path_string",
+                     "logical_design_name_string", "route_source_count", "route_sink_count")
+
+
+class IfmHeader(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in IFM_HEADER_FIELDS]
 
 
 def _bind(lib: C.CDLL, sigs: dict) -> C.CDLL:
