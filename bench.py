@@ -290,7 +290,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print("bench.py measures one GPU: the SSSP engine runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(sharding across GPUs is designed, not run)", file=sys.stderr)
         return 2
 
     import mi355sssp as M

@@ -286,7 +286,7 @@ __device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp,
 // the latency of these dependent accesses, so four independent chains per
 // lane keep four times the accesses in flight.  A lane without an edge probes
 // label 0 and discards it (a guarded load would wait inside its branch).  Per
-// edge this is the same label-correcting step as relax_edge + push_improved.
+// edge this is the step of relax_edge_pair followed by push_improved.
 template <class W>
 __device__ inline void relax_quad(const AddsArgs& a, PushStage& st, const uint32_t (&v)[4],
                                   const uint32_t (&ck)[4], const uint32_t (&u)[4],

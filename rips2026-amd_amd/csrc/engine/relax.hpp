@@ -161,21 +161,14 @@ __device__ inline bool edge_selected(int cls, uint32_t wbits, uint32_t delta_bit
   return cls == kLightEdges ? light : !light;
 }
 
-// Relax one edge; returns true when the distance part strictly decreased.
-template <class W>
-__device__ inline bool relax_edge(unsigned long long* __restrict__ labels, uint32_t v,
-                                  uint32_t cand_key, uint32_t parent) {
-  if (cand_key == W::kInfKey) return false;
-  const unsigned long long cand = ((unsigned long long)cand_key << 32) | parent;
-  const unsigned long long cur = labels[v];
-  if (cand >= cur) return false;
-  const unsigned long long old = atomicMin(labels + v, cand);
-  return cand_key < (uint32_t)(old >> 32);
-}
-
-// relax_edge for two edges at once (candidate keys ck0 / ck1, kInfKey = no
-// edge): both label probes, then both atomicMins, are issued before either
-// result is used, so the two dependent chains overlap.  A lane without an
+// The packed-label relaxation step for two edges at once.  A label is
+// (distance key << 32 | parent): a candidate (ck << 32 | parent) replaces it
+// when smaller, by one 64-bit atomicMin, so the smaller distance wins and
+// among equal distances the smaller parent (the canonical predecessor); a
+// plain load first filters candidates that cannot win.  imp = the distance
+// part strictly decreased.  Candidate keys ck0 / ck1 (kInfKey = no edge):
+// both label probes, then both atomicMins, are issued before either result
+// is used, so the two dependent chains overlap.  A lane without an
 // edge probes label 0 and discards it (a guarded load would wait inside its
 // branch).
 template <class W>
