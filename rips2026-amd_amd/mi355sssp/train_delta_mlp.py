@@ -34,6 +34,8 @@ def main() -> int:
     ap.add_argument("--first", type=int, default=0, help="index of the first spec of this shard")
     ap.add_argument("--total", type=int, default=2000, help="size of the full spec list")
     ap.add_argument("--seed", type=int, default=16001, help="training spec seed")
+    ap.add_argument("--kind", choices=["rmat", "route", "grid"], default=None,
+                    help="collect only the training specs of this graph kind")
     ap.add_argument("--large-grids", type=int, default=0,
                     help="collect this many 4e6-3e7 vertex grids instead (learn.large_grid_specs)")
     ap.add_argument("--reps", type=int, default=2)
@@ -57,6 +59,8 @@ def main() -> int:
             specs = L.large_grid_specs(args.large_grids, args.seed)
         else:
             specs = L.training_specs(args.total, args.seed)[args.first:args.first + args.graphs]
+        if args.kind:
+            specs = [s for s in specs if s["kind"] == args.kind]
         t0 = time.time()
         x, y, w, metas = L.collect(specs, log=log, reps=args.reps, time_budget_s=args.budget_s)
         log(f"collected {len(metas)} graphs, {len(x)} samples in {time.time() - t0:.0f}s")
