@@ -14,7 +14,9 @@
 //    base + floor((d - T_base) / delta), clamped into the window (far items
 //    are clipped into the last slot and re-queued when they come up) and
 //    never below the bucket of the vertex that produced it.
-//  * Workers.  Every workgroup is a worker: wave 0 claims up to 256 items and
+//  * Workers.  Every workgroup is a worker: wave 0 claims up to 256 items
+//    (512 from a bucket holding 64K or more: fewer claims when work is
+//    plentiful, more claimers when it is scarce) and, 256 at a time,
 //    the four waves relax them, their edges packed across the workgroup's
 //    lanes (four edges per lane in flight); vertices with more than
 //    kHubDegree edges in the range being relaxed are cut into 8192-edge hub
@@ -83,7 +85,8 @@ namespace mi355 {
 namespace {
 
 constexpr uint32_t kNoStamp = 0xFFFFFFFFu;
-constexpr uint32_t kClaim = 256;    // items per workgroup claim
+constexpr uint32_t kClaim = 512;    // most items per workgroup claim (two passes of kBlock)
+constexpr uint32_t kBigClaimAvail = 65536;  // slot size above which claims take kClaim
 constexpr int kHubChunk = 8192;     // edges per hub chunk
 constexpr int kHubDegree = 4096;  // rows with more edges become hub chunks
 static_assert(5 + kAddsSlots <= 64, "claim probe: lanes 0-3, the window's semaphores, the worker share");
@@ -443,7 +446,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   if (mode == kPolicyMlp || (a.policy && a.policy->trace))
     mlp_features(a.policy, c, occ, window_items, delta, lane, &feat, &raw);
   if (mode == kPolicyHeuristic) {
-    const float capacity = (float)c->workers * (float)kClaim;
+    const float capacity = (float)c->workers * (float)kBlock;
     if ((float)window_items < c->util_low * capacity) delta = fminf(delta * c->growth, c->max_delta);
     else if ((float)window_items > c->util_high * capacity) delta = fmaxf(delta * c->shrink, c->min_delta);
   } else if (mode == kPolicyMlp) {
@@ -580,11 +583,12 @@ __device__ inline void make_hub_chunks(const AddsArgs& a, uint32_t u, uint32_t k
 
 // Claim from a (ring, head, avail) set through its counting semaphore while
 // holding inflight on slot `pin`; returns the count and sets *begin.
-__device__ inline uint32_t sem_claim(int* avail, uint32_t* head, int* infl, uint32_t* begin) {
+__device__ inline uint32_t sem_claim(int* avail, uint32_t* head, int* infl, uint32_t* begin,
+                                     int want = (int)kBlock) {
   atomic_add_done(infl, 1);  // before the claim is visible
-  const int old = atomic_add_done(avail, -(int)kClaim);
-  const int k = min(old, (int)kClaim);
-  if (k < (int)kClaim) atomic_add_done(avail, (int)kClaim - max(k, 0));  // give back the surplus
+  const int old = atomic_add_done(avail, -want);
+  const int k = min(old, want);
+  if (k < want) atomic_add_done(avail, want - max(k, 0));  // give back the surplus
   if (k <= 0) {
     atomic_add_done(infl, -1);
     return 0;
@@ -656,6 +660,10 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         const int hub_n = __shfl(probe, 1, 64), hv_n0 = __shfl(probe, 2, 64);
         const uint32_t A = max(1u, min((uint32_t)__shfl(probe, 3, 64), (uint32_t)kAddsSlots));
         const unsigned long long ready = __ballot(lane >= 4 && lane < 4 + A && probe > 0) >> 4;
+        // slots holding many items are claimed kClaim at a time (fewer
+        // claims); otherwise kBlock, so that scarce work spreads over more
+        // workgroups
+        const unsigned long long big = __ballot(lane >= 4 && lane < 4 + A && probe >= (int)kBigClaimAvail) >> 4;
         // Workgroups beyond the policy's worker share do not claim: fewer
         // claimers contend less on the control words when a phase holds little
         // work (high-diameter graphs).  They turn dormant until the share
@@ -695,7 +703,8 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           }
           for (unsigned long long m = ready; m && !got; m &= m - 1) {
             const uint32_t s = (base0 + (uint32_t)__ffsll((long long)m) - 1) % kAddsSlots;
-            got = sem_claim(&AVAIL(c, s), &HEAD(c, s), &INFL(c, s), &begin);
+            const int want = (big >> ((uint32_t)__ffsll((long long)m) - 1)) & 1ull ? (int)kClaim : (int)kBlock;
+            got = sem_claim(&AVAIL(c, s), &HEAD(c, s), &INFL(c, s), &begin, want);
             if (got) {
               slot = s;
               qslot = s;
@@ -821,114 +830,116 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       continue;
     }
 
-    // items of a bucket ring or a heavy list: wave w takes claim entries
-    // [w*64, w*64 + 64)
+    // items of a bucket ring or a heavy list, in passes of kBlock: in pass p
+    // wave w takes claim entries [p*256 + w*64, p*256 + w*64 + 64)
     const bool heavy = kind == kClaimHeavy;
     const uint32_t slot = s_slot, begin = s_begin;
     int* ring = (heavy ? a.hring : a.ring) + (size_t)s_qslot * (a.cap_mask + 1u);
-    const uint32_t i = wv * kWave + lane;
-    int u = 0, beg = 0, deg = 0;
-    uint32_t key = W::kInfKey, bk = 0;
-    bool entry_ok = true;
-    bool requeue = false, to_heavy = false;
-    uint32_t want = 0;
-    if (i < count) {
-      // The producer reserved this position before writing it: wait for the
-      // write (bounded), then hand the entry back to the ring as empty.
-      int* ep = ring + ((begin + i) & a.cap_mask);
-      uint32_t spins = 0;
-      while ((u = __hip_atomic_load(ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == kEmptyEntry &&
-             ++spins < kSpinLimit)
-        __builtin_amdgcn_s_sleep(1);
-      d_spin += spins;
-      entry_ok = u != kEmptyEntry && u >= 0 && (uint32_t)u < a.n;
-      if (!entry_ok) {
-        atomicOr(&c->err, kErrRing);
-        atomicExch(&c->done, 1u);
-        u = 0;
-      } else {
-        __hip_atomic_store(ep, kEmptyEntry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    // row bounds and light-prefix length of u depend only on u: issued now,
-    // they arrive while the stamp exchange and the label read complete
-    int r_beg = 0, r_end = 0, r_split = 0;
-    if (i < count && entry_ok) {
-      r_beg = a.rowptr[u];
-      r_end = a.rowptr[u + 1];
-      r_split = a.split[u];
-    }
-    if (i < count && entry_ok) {
-      if (heavy) {
-        // Clearing the stamp first means a later light pass of u (after an
-        // improvement) lists u again, or this pass reads the label it set.
-        __hip_atomic_exchange(a.sstamp + u, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        vm_wait();  // the label is read after the stamp is cleared
-        key = atomic_load_key(a.labels + u);
-        bk = base;
-        ++d_heavy;
-      } else {
-        // Whoever takes the stamp processes the vertex.  (Putting back a
-        // stamp that belongs to an entry in another ring would race with that
-        // entry's own pop and could lose the vertex.)  The bucket is the one
-        // this claim holds; an entry whose stamp was already taken is stale.
-        const uint32_t old = __hip_atomic_exchange(a.qstamp + u, kNoStamp, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-        if (old != kNoStamp) {
-          bk = held_bucket;
-          key = atomic_load_key(a.labels + u);
-          want = map_bucket(W::bucket_value(key), base, tbase, inv, bk);
-          if (want >= bk + s_active) {
-            // clipped or re-mapped far item: queue it where it belongs now
-            key = W::kInfKey;
-            requeue = true;
-            ++d_req;
-          }
+    for (uint32_t pass = 0; pass * kBlock < count; ++pass) {
+      const uint32_t i = pass * kBlock + wv * kWave + lane;
+      int u = 0, beg = 0, deg = 0;
+      uint32_t key = W::kInfKey, bk = 0;
+      bool entry_ok = true;
+      bool requeue = false, to_heavy = false;
+      uint32_t want = 0;
+      if (i < count) {
+        // The producer reserved this position before writing it: wait for the
+        // write (bounded), then hand the entry back to the ring as empty.
+        int* ep = ring + ((begin + i) & a.cap_mask);
+        uint32_t spins = 0;
+        while ((u = __hip_atomic_load(ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == kEmptyEntry &&
+               ++spins < kSpinLimit)
+          __builtin_amdgcn_s_sleep(1);
+        d_spin += spins;
+        entry_ok = u != kEmptyEntry && u >= 0 && (uint32_t)u < a.n;
+        if (!entry_ok) {
+          atomicOr(&c->err, kErrRing);
+          atomicExch(&c->done, 1u);
+          u = 0;
         } else {
-          ++d_drop;
+          __hip_atomic_store(ep, kEmptyEntry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
-      if (key != W::kInfKey) {
-        ++items_local;
-        beg = r_beg;
-        int end = r_end;
-        const int split = beg + r_split;
+      // row bounds and light-prefix length of u depend only on u: issued now,
+      // they arrive while the stamp exchange and the label read complete
+      int r_beg = 0, r_end = 0, r_split = 0;
+      if (i < count && entry_ok) {
+        r_beg = a.rowptr[u];
+        r_end = a.rowptr[u + 1];
+        r_split = a.split[u];
+      }
+      if (i < count && entry_ok) {
         if (heavy) {
-          beg = split;
+          // Clearing the stamp first means a later light pass of u (after an
+          // improvement) lists u again, or this pass reads the label it set.
+          __hip_atomic_exchange(a.sstamp + u, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          vm_wait();  // the label is read after the stamp is cleared
+          key = atomic_load_key(a.labels + u);
+          bk = base;
+          ++d_heavy;
         } else {
-          // light prefix now; the heavy suffix once bucket bk is finished
-          if (split < end)
-            to_heavy = __hip_atomic_exchange(a.sstamp + u, bk + 1u, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT) != bk + 1u;
-          end = split;
+          // Whoever takes the stamp processes the vertex.  (Putting back a
+          // stamp that belongs to an entry in another ring would race with that
+          // entry's own pop and could lose the vertex.)  The bucket is the one
+          // this claim holds; an entry whose stamp was already taken is stale.
+          const uint32_t old = __hip_atomic_exchange(a.qstamp + u, kNoStamp, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+          if (old != kNoStamp) {
+            bk = held_bucket;
+            key = atomic_load_key(a.labels + u);
+            want = map_bucket(W::bucket_value(key), base, tbase, inv, bk);
+            if (want >= bk + s_active) {
+              // clipped or re-mapped far item: queue it where it belongs now
+              key = W::kInfKey;
+              requeue = true;
+              ++d_req;
+            }
+          } else {
+            ++d_drop;
+          }
         }
-        deg = end - beg;
-        if (deg > kHubDegree) {
-          make_hub_chunks(a, (uint32_t)u, key, beg, end, slot, bk);
-          deg = 0;
+        if (key != W::kInfKey) {
+          ++items_local;
+          beg = r_beg;
+          int end = r_end;
+          const int split = beg + r_split;
+          if (heavy) {
+            beg = split;
+          } else {
+            // light prefix now; the heavy suffix once bucket bk is finished
+            if (split < end)
+              to_heavy = __hip_atomic_exchange(a.sstamp + u, bk + 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT) != bk + 1u;
+            end = split;
+          }
+          deg = end - beg;
+          if (deg > kHubDegree) {
+            make_hub_chunks(a, (uint32_t)u, key, beg, end, slot, bk);
+            deg = 0;
+          }
         }
       }
-    }
-    // heavy-list entries of this claim's bucket (list slot = bucket slot)
-    {
-      const unsigned long long hm = __ballot(to_heavy);
-      if (hm) {
-        const uint32_t k = (uint32_t)__popcll(hm);
-        if (hv_n + k > kHeavyStage) heavy_publish(a, s_hv[wv], hv_n, slot);
-        if (to_heavy) s_hv[wv][hv_n + prefix_count(hm)] = u;
-        hv_n += k;
+      // heavy-list entries of this claim's bucket (list slot = bucket slot)
+      {
+        const unsigned long long hm = __ballot(to_heavy);
+        if (hm) {
+          const uint32_t k = (uint32_t)__popcll(hm);
+          if (hv_n + k > kHeavyStage) heavy_publish(a, s_hv[wv], hv_n, slot);
+          if (to_heavy) s_hv[wv][hv_n + prefix_count(hm)] = u;
+          hv_n += k;
+        }
       }
+      // re-queued items go through the stage like any push
+      push_improved(a, st, requeue, (uint32_t)u, want, pushed_local);
+      const long long t_r0 = wall_clock64();
+      relax_block<W>(a, st, &s_incl[0][0], &s_beg[0][0], &s_key[0][0], &s_u[0][0], &s_lo[0][0], s_wsum,
+                     u, beg, deg, key, bk, base, tbase, inv, edges_local, pushed_local);
+      const long long t_r1 = wall_clock64();
+      stage_publish(a, st);
+      heavy_publish(a, s_hv[wv], hv_n, slot);
+      t_relax += t_r1 - t_r0;
+      t_pub += wall_clock64() - t_r1;
     }
-    // re-queued items go through the stage like any push
-    push_improved(a, st, requeue, (uint32_t)u, want, pushed_local);
-    const long long t_r0 = wall_clock64();
-    relax_block<W>(a, st, &s_incl[0][0], &s_beg[0][0], &s_key[0][0], &s_u[0][0], &s_lo[0][0], s_wsum,
-                   u, beg, deg, key, bk, base, tbase, inv, edges_local, pushed_local);
-    const long long t_r1 = wall_clock64();
-    stage_publish(a, st);
-    heavy_publish(a, s_hv[wv], hv_n, slot);
-    t_relax += t_r1 - t_r0;
-    t_pub += wall_clock64() - t_r1;
     __syncthreads();
     if (threadIdx.x == 0) {
       atomic_add_done(&INFL(c, slot), -1);
