@@ -401,6 +401,7 @@ namespace {
 // the thread's lifetime so asynchronous launches never see a freed slot).
 thread_local mi355::DevBuf<mi355::MinplusItem> t_item;
 thread_local mi355::DevBuf<mi355::MinplusItem> t_table;
+thread_local mi355::DevBuf<int2> t_work;
 
 // Upload a dense matrix if it lives on the host.
 const float* stage_dense(const float* adj, int32_t memory, int n, mi355::DevBuf<float>& tmp,
@@ -445,7 +446,7 @@ int mi355_minplus_gemm_f32_batched(int count, const float* const* dA, const floa
     }
     // Synchronise before reusing the descriptor table of a previous call.
     MI_HIP(hipStreamSynchronize(as_stream(stream)));
-    mi355::minplus_gemm_batched(items, as_stream(stream), t_table);
+    mi355::minplus_gemm_batched(items, as_stream(stream), t_table, t_work);
   });
 }
 

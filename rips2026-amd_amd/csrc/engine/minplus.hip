@@ -13,8 +13,14 @@
 //     each k-slice costs one barrier instead of two;
 //   * two consecutive k feed one v_min3_f32 (acc = min(acc, a0+b0, a1+b1)):
 //     3 VALU instructions per 2 (add, min) pairs.  Packed fp32 adds are not
-//     used (the build disables them), so the instruction ceiling is 2/3 of
-//     the packed-add VALU peak.
+//     used (the build disables them);
+//   * the k loop within a slice is unrolled UK pairs at a time.  Fully
+//     unrolled, the compiler hoists every LDS fragment of the slice into
+//     registers (193-256 VGPRs, one or two waves per SIMD); UK = 2 or 4 keeps
+//     64x64 tiles at 80-120 VGPRs and four to six waves per SIMD, which is
+//     what a launch with many workgroups needs.  A launch with fewer
+//     workgroups than CUs runs faster fully unrolled (more independent work
+//     per wave).
 // Out-of-range elements are padded with +INF, as the reference does (:49-73),
 // which leaves every real output unchanged.  NaN candidates never win (the
 // reference's select `cand < acc ? cand : acc`, :96-97, behaves the same);
@@ -22,6 +28,7 @@
 // tie with +0.0, the only divergence from its k-ordered select.
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -47,14 +54,26 @@ struct TileCfg {
   static constexpr int kBPad = 4;
 };
 
-// One workgroup per (tile_m, tile_n, batch item).  `items` holds per-item
-// descriptors; for the single-GEMM entry point it has one element.
-template <int BM, int BN, int BK, int TM, int TN>
-__global__ __launch_bounds__(256) void k_minplus_tile(const MinplusItem* __restrict__ items) {
+// One workgroup per C tile.  Without a work list the grid is (tiles along N,
+// tiles along M) over items[0].  A batch passes `work`: entry blockIdx.x is
+// {item, tile index within the item (row-major over its tile grid)}, item -1
+// marks padding.
+template <int BM, int BN, int BK, int TM, int TN, int UK>
+__global__ __launch_bounds__(256) void k_minplus_tile(const MinplusItem* __restrict__ items,
+                                                      const int2* __restrict__ work) {
   using Cfg = TileCfg<BM, BN, BK, TM, TN>;
-  const MinplusItem it = items[blockIdx.z];
-  const int row0 = blockIdx.y * BM;
-  const int col0 = blockIdx.x * BN;
+  int item = 0, tile_m = blockIdx.y, tile_n = blockIdx.x;
+  if (work) {
+    const int2 w = work[blockIdx.x];
+    if (w.x < 0) return;
+    item = w.x;
+    const int tiles_n = (items[item].N + BN - 1) / BN;
+    tile_m = w.y / tiles_n;
+    tile_n = w.y - tile_m * tiles_n;
+  }
+  const MinplusItem it = items[item];
+  const int row0 = tile_m * BM;
+  const int col0 = tile_n * BN;
   if (row0 >= it.M || col0 >= it.N) return;
 
   __shared__ float As[2][BK][BM + Cfg::kAPad];
@@ -112,7 +131,7 @@ __global__ __launch_bounds__(256) void k_minplus_tile(const MinplusItem* __restr
   for (int s = 0; s < nslices; ++s) {
     const int cur = s & 1;
     if (s + 1 < nslices) load_slice((s + 1) * BK);
-#pragma unroll
+#pragma unroll (UK)
     for (int k = 0; k < BK; k += 2) {
       float a0[TM], a1[TM], b0[TN], b1[TN];
 #pragma unroll
@@ -151,7 +170,7 @@ __global__ __launch_bounds__(256) void k_minplus_tile(const MinplusItem* __restr
   }
 }
 
-using TileKernel = void (*)(const MinplusItem*);
+using TileKernel = void (*)(const MinplusItem*, const int2*);
 
 struct TileEntry {
   int bm, bn, bk, tm, tn;
@@ -159,12 +178,17 @@ struct TileEntry {
   const char* name;
 };
 
+// Index 2 is the default for launches of many workgroups, index 5 for
+// launches of few.
+constexpr int kTileMany = 2;
+constexpr int kTileFew = 5;
 const TileEntry kTiles[] = {
-    {128, 128, 16, 8, 8, k_minplus_tile<128, 128, 16, 8, 8>, "128x128x16/8x8"},
-    {128, 64, 16, 8, 4, k_minplus_tile<128, 64, 16, 8, 4>, "128x64x16/8x4"},
-    {64, 64, 16, 4, 4, k_minplus_tile<64, 64, 16, 4, 4>, "64x64x16/4x4"},
-    {64, 64, 32, 4, 4, k_minplus_tile<64, 64, 32, 4, 4>, "64x64x32/4x4"},
-    {64, 128, 16, 4, 8, k_minplus_tile<64, 128, 16, 4, 8>, "64x128x16/4x8"},
+    {128, 128, 16, 8, 8, k_minplus_tile<128, 128, 16, 8, 8, 2>, "128x128x16/8x8/u2"},
+    {128, 64, 16, 8, 4, k_minplus_tile<128, 64, 16, 8, 4, 2>, "128x64x16/8x4/u2"},
+    {64, 64, 16, 4, 4, k_minplus_tile<64, 64, 16, 4, 4, 4>, "64x64x16/4x4/u4"},
+    {64, 64, 32, 4, 4, k_minplus_tile<64, 64, 32, 4, 4, 4>, "64x64x32/4x4/u4"},
+    {64, 128, 16, 4, 8, k_minplus_tile<64, 128, 16, 4, 8, 2>, "64x128x16/4x8/u2"},
+    {64, 64, 16, 4, 4, k_minplus_tile<64, 64, 16, 4, 4, 8>, "64x64x16/4x4/full"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -174,28 +198,51 @@ std::mutex g_tune_mu;
 int g_tuned[32];
 bool g_tuned_init = false;
 
-int static_choice(int M, int N) {
-  const int s = std::max(M, N);
-  if (s >= 384) return 0;
-  return 2;
+// Static rule by the launch's 64x64 tile count: below 512 (two workgroups
+// per CU) the fully unrolled 64x64 tile, else the UK = 4 one.  Measured on
+// MI355X, square products: n = 512 single 6.2 vs 5.4 T ops/s, n = 1024 single
+// 24.5 vs 19.6, n = 2048 single 45.4 vs 52.9, 512 products of n in [32, 512]
+// 28.9 vs 40.5.
+constexpr uint64_t kFewTiles = 512;
+
+uint64_t tiles64(int M, int N) { return (uint64_t)div_up((uint64_t)M, 64) * div_up((uint64_t)N, 64); }
+
+int static_choice(uint64_t launch_tiles64) { return launch_tiles64 < kFewTiles ? kTileFew : kTileMany; }
+
+int size_class(int M, int N) {
+  int cls = 0;
+  while ((1 << cls) < std::max(M, N) && cls < 31) ++cls;
+  return cls;
 }
 
-int choose_tile(int M, int N) {
+// launch_tiles64: 64x64 tiles of the whole launch the product is part of.
+int choose_tile(int M, int N, uint64_t launch_tiles64) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   if (!g_tuned_init) {
     for (int& x : g_tuned) x = -1;
     g_tuned_init = true;
   }
-  int cls = 0;
-  while ((1 << cls) < std::max(M, N) && cls < 31) ++cls;
-  return g_tuned[cls] >= 0 ? g_tuned[cls] : static_choice(M, N);
+  const int cls = size_class(M, N);
+  return g_tuned[cls] >= 0 ? g_tuned[cls] : static_choice(launch_tiles64);
 }
 
-void launch_items(int tile, const MinplusItem* d_items, int count, int maxM, int maxN,
-                  hipStream_t s) {
+void launch_single(int tile, const MinplusItem* d_item, int M, int N, hipStream_t s) {
   const TileEntry& t = kTiles[tile];
-  dim3 grid(div_up((uint64_t)maxN, t.bn), div_up((uint64_t)maxM, t.bm), (unsigned)count);
-  hipLaunchKernelGGL(t.fn, grid, dim3(256), 0, s, d_items);
+  dim3 grid(div_up((uint64_t)N, t.bn), div_up((uint64_t)M, t.bm), 1);
+  hipLaunchKernelGGL(t.fn, grid, dim3(256), 0, s, d_item, static_cast<const int2*>(nullptr));
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin by workgroup id.  Entry p of
+// the item-ordered tile list goes to workgroup id (p % chunk) * 8 + p / chunk,
+// so XCD x runs the contiguous run [x * chunk, (x + 1) * chunk): the tiles of
+// one product share an XCD and its L2, which holds that product's A rows and
+// B columns.
+constexpr int kXcds = 8;
+
+void xcd_order(const std::vector<int2>& list, std::vector<int2>& out) {
+  const size_t chunk = (list.size() + kXcds - 1) / kXcds;
+  out.assign(chunk * kXcds, int2{-1, 0});
+  for (size_t p = 0; p < list.size(); ++p) out[(p % chunk) * kXcds + p / chunk] = list[p];
 }
 
 }  // namespace
@@ -207,39 +254,53 @@ void minplus_gemm(const float* A, const float* B, float* C, int M, int N, int K,
                   int ldb, int ldc, hipStream_t s, MinplusItem* scratch_item, int force_tile) {
   const MinplusItem it{A, B, C, M, N, K, lda, ldb, ldc, 0};
   MI_HIP(hipMemcpyAsync(scratch_item, &it, sizeof(it), hipMemcpyHostToDevice, s));
-  const int tile = force_tile >= 0 ? force_tile : choose_tile(M, N);
-  launch_items(tile, scratch_item, 1, M, N, s);
+  const int tile = force_tile >= 0 ? force_tile : choose_tile(M, N, tiles64(M, N));
+  launch_single(tile, scratch_item, M, N, s);
   MI_LAUNCH_CHECK();
 }
 
 void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s,
-                          DevBuf<MinplusItem>& table) {
+                          DevBuf<MinplusItem>& table, DevBuf<int2>& work) {
   if (items.empty()) return;
-  // Group items by the tile their size class selects; one launch per group.
-  std::vector<std::vector<MinplusItem>> groups(kNumTiles);
-  for (const auto& it : items) groups[choose_tile(it.M, it.N)].push_back(it);
+  // One launch per tile shape (each product takes its size class's tuned
+  // tile, else the static rule over the whole batch's tile count) over a flat
+  // list of (product, tile) work entries: every workgroup owns a real tile,
+  // whatever the spread of sizes.  512 products of n in [32, 512] on the
+  // 64x64 full-unroll tile: one grid sized for the largest product, 0.93 ms;
+  // one grid per size class, 0.75 ms; the work list, 0.54 ms.
+  table.ensure(items.size());
+  MI_HIP(hipMemcpyAsync(table.p, items.data(), items.size() * sizeof(MinplusItem), hipMemcpyHostToDevice, s));
+  uint64_t all_tiles64 = 0;
+  for (const MinplusItem& it : items) all_tiles64 += tiles64(it.M, it.N);
+  std::vector<std::vector<int2>> lists(kNumTiles);
+  for (size_t i = 0; i < items.size(); ++i) {
+    const MinplusItem& it = items[i];
+    const int t = choose_tile(it.M, it.N, all_tiles64);
+    const uint64_t tiles = (uint64_t)div_up((uint64_t)it.M, kTiles[t].bm) * div_up((uint64_t)it.N, kTiles[t].bn);
+    if (tiles > (uint64_t)INT32_MAX) fail(MI355_EINVAL, "min-plus product too large for one batch item");
+    for (int64_t k = 0; k < (int64_t)tiles; ++k) lists[t].push_back(int2{(int)i, (int)k});
+  }
+  std::vector<std::vector<int2>> ordered(kNumTiles);
   size_t total = 0;
-  for (auto& gi : groups) total += gi.size();
-  table.ensure(total);
-  std::vector<MinplusItem> flat;
+  for (int t = 0; t < kNumTiles; ++t) {
+    if (lists[t].empty()) continue;
+    xcd_order(lists[t], ordered[t]);
+    // a launch holds at most 2^32 - 1 threads
+    if (ordered[t].size() > (size_t)UINT32_MAX / 256) fail(MI355_EINVAL, "min-plus batch too large");
+    total += ordered[t].size();
+  }
+  std::vector<int2> flat;
   flat.reserve(total);
-  for (auto& gi : groups) flat.insert(flat.end(), gi.begin(), gi.end());
-  MI_HIP(hipMemcpyAsync(table.p, flat.data(), total * sizeof(MinplusItem), hipMemcpyHostToDevice, s));
+  for (auto& o : ordered) flat.insert(flat.end(), o.begin(), o.end());
+  work.ensure(flat.size());
+  MI_HIP(hipMemcpyAsync(work.p, flat.data(), flat.size() * sizeof(int2), hipMemcpyHostToDevice, s));
   size_t off = 0;
   for (int t = 0; t < kNumTiles; ++t) {
-    if (groups[t].empty()) continue;
-    int maxM = 0, maxN = 0;
-    for (auto& it : groups[t]) {
-      maxM = std::max(maxM, it.M);
-      maxN = std::max(maxN, it.N);
-    }
-    // grid.z is limited to 65535 items per launch
-    for (size_t b = 0; b < groups[t].size(); b += 65535) {
-      const int cnt = (int)std::min<size_t>(65535, groups[t].size() - b);
-      launch_items(t, table.p + off + b, cnt, maxM, maxN, s);
-      MI_LAUNCH_CHECK();
-    }
-    off += groups[t].size();
+    if (ordered[t].empty()) continue;
+    hipLaunchKernelGGL(kTiles[t].fn, dim3((unsigned)ordered[t].size()), dim3(256), 0, s,
+                       static_cast<const MinplusItem*>(table.p), static_cast<const int2*>(work.p + off));
+    MI_LAUNCH_CHECK();
+    off += ordered[t].size();
   }
 }
 
@@ -265,7 +326,8 @@ void minplus_autotune(const int* sizes, int count, hipStream_t s, double* report
       for (int r = 0; r < reps; ++r) {
         const TileEntry& te = kTiles[t];
         dim3 grid(div_up((uint64_t)n, te.bn), div_up((uint64_t)n, te.bm), 1);
-        hipLaunchKernelGGL(te.fn, grid, dim3(256), 0, s, item.p);
+        hipLaunchKernelGGL(te.fn, grid, dim3(256), 0, s, static_cast<const MinplusItem*>(item.p),
+                           static_cast<const int2*>(nullptr));
       }
       MI_HIP(hipEventRecord(e1, s));
       MI_HIP(hipEventSynchronize(e1));

@@ -21,8 +21,10 @@ const char* minplus_tile_name(int i);
 // Asynchronous on stream s; scratch_item is a device slot for the descriptor.
 void minplus_gemm(const float* A, const float* B, float* C, int M, int N, int K, int lda, int ldb,
                   int ldc, hipStream_t s, MinplusItem* scratch_item, int force_tile = -1);
+// table / work: grow-only device buffers for the descriptors and the
+// (product, tile) work list; the caller keeps them until the launches end.
 void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s,
-                          DevBuf<MinplusItem>& table);
+                          DevBuf<MinplusItem>& table, DevBuf<int2>& work);
 void minplus_autotune(const int* sizes, int count, hipStream_t s, double* report);
 void minplus_tuning_get(int* table32);
 void minplus_tuning_set(const int* table32);

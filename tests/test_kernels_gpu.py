@@ -49,6 +49,32 @@ def test_minplus_gemm_batched_and_autotune():
     assert all(x.tobytes() == y.tobytes() for x, y in zip(outs, outs2))
 
 
+def test_minplus_gemm_batched_ragged_classes():
+    """160 products from two size classes in shuffled order, ragged M, N and
+    K (K not a multiple of the tile's k-step), once on the default tile and
+    once with each class forced onto another tile shape."""
+    rng = np.random.default_rng(11)
+    shapes = [(int(rng.integers(257, 301)), int(rng.integers(1, 41)), int(rng.integers(257, 301))) for _ in range(120)]
+    shapes += [(int(rng.integers(65, 129)), int(rng.integers(1, 41)), int(rng.integers(65, 129))) for _ in range(40)]
+    order = rng.permutation(len(shapes))
+    shapes = [shapes[i] for i in order]
+    As = [rng.random((m, k), dtype=np.float32) for m, k, _ in shapes]
+    Bs = [rng.random((k, n), dtype=np.float32) for _, k, n in shapes]
+    want = [orc.minplus_gemm(a.ravel(), b.ravel(), m, n, k).reshape(m, n).tobytes()
+            for (m, k, n), a, b in zip(shapes, As, Bs)]
+    saved = M.minplus_tuning()
+    try:
+        for table in (None, {7: 0, 9: 4}, {7: 3, 9: 1}, {7: 2, 9: 5}):
+            t = [-1] * 32
+            for cls, tile in (table or {}).items():
+                t[cls] = tile
+            M.minplus_tuning(t)
+            outs = M.minplus_gemm_batched(As, Bs)
+            assert [c.tobytes() for c in outs] == want, table
+    finally:
+        M.minplus_tuning(saved)
+
+
 def _rand_csr(rng, rows, cols, p):
     dense = np.where(rng.random((rows, cols)) < p, rng.random((rows, cols)).astype(np.float32), np.inf)
     rp, col, val = [0], [], []
