@@ -175,9 +175,10 @@ int mi355_minplus_gemm_f32(const float* dA, const float* dB, float* dC, int M, i
 
 /* Batched variant (no reference counterpart; the batched min-plus workload of
  * the routing engine): count independent products.  The pointer and size
- * arrays are HOST arrays of device pointers / sizes.  Each item takes the tile
- * its size class (ceil(log2(max(M, N)))) selects; each tile shape is one
- * launch over a flat list of (item, tile) entries.  Asynchronous. */
+ * arrays are HOST arrays of device pointers / sizes.  Each item takes its size
+ * class's (ceil(log2(max(M, N)))) autotuned tile, else the tile the static
+ * rule picks for the whole batch's size; each tile shape is one launch over a
+ * flat list of (item, tile) entries.  Asynchronous. */
 int mi355_minplus_gemm_f32_batched(int count, const float* const* dA, const float* const* dB,
                                    float* const* dC, const int* M, const int* N, const int* K,
                                    const int* lda, const int* ldb, const int* ldc, void* stream);
