@@ -124,9 +124,15 @@ def routing_workloads(args, cpu_threads: int):
 
     # batched box-limited multi-source queries
     qs = B.routing_queries(side, side, args.queries, seed=args.seed)
-    best, tune_ms = B.autotune(dg, qs, sample_stride=8)
-    opts = dict(rounds_per_graph=best.rounds_per_graph, blocks_per_cu=best.blocks_per_cu, order=best.order,
-                delta_scale=best.delta_scale)
+    if args.batch_graph:
+        best, tune_ms = B.autotune(dg, qs, sample_stride=8)
+        opts = dict(rounds_per_graph=best.rounds_per_graph, blocks_per_cu=best.blocks_per_cu,
+                    order=best.order, delta_scale=best.delta_scale)
+    else:
+        # plain launches, no tuning (the tuner replays captured graphs): the
+        # options the tuner picks for this batch
+        tune_ms = []
+        opts = dict(use_graph=0, blocks_per_cu=2, order=1, delta_scale=1.0)
     B.run(dg, qs, want_pred=False, download=False, **opts)  # warm
     t0 = time.perf_counter()
     _, _, off, bst = B.run(dg, qs, want_pred=True, download=False, **opts)
@@ -136,6 +142,7 @@ def routing_workloads(args, cpu_threads: int):
     # (kernel time only; building the subgraph is not counted)
     sample = np.linspace(0, qs.count - 1, args.one_at_a_time).astype(int)
     one_ms = []
+    areas = qs.areas()
     for q in sample:
         sub, area = B.induced_subgraph(g, qs, int(q))
         sdg = M.DeviceGraph(sub)
@@ -145,13 +152,15 @@ def routing_workloads(args, cpu_threads: int):
         del sdg, sub
     # the sampled queries cover the box-size distribution evenly by index;
     # weight their mean by area to estimate the whole batch
-    areas = qs.areas()
-    per_slot = float(np.sum(one_ms) / np.sum(areas[sample]))
-    one_total_s = per_slot * float(areas.sum()) * 1e-3
+    one_total_s = None
+    if len(sample):
+        per_slot = float(np.sum(one_ms) / np.sum(areas[sample]))
+        one_total_s = per_slot * float(areas.sum()) * 1e-3
     # parity of a few queries against the oracle (full batch already run)
     if args.check > 0:
-        dist, pred, off, _ = B.run(dg, qs.subset(sample[: args.check]))
-        for i, q in enumerate(sample[: args.check]):
+        chk = np.linspace(0, qs.count - 1, args.check + 2).astype(int)[1:-1]
+        dist, pred, off, _ = B.run(dg, qs.subset(chk))
+        for i, q in enumerate(chk):
             sq = qs.subset([q])
             want = orc.dijkstra_box(g.rowptr, g.col, g.weights, side, tuple(sq.boxes[0]), sq.sources)
             ok = dist[off[i]:off[i + 1]].tobytes() == want.tobytes()
@@ -190,12 +199,12 @@ def routing_workloads(args, cpu_threads: int):
         "one_at_a_time_plain_adds": {"sampled_queries": int(len(sample)),
                                      "kernel_ms_sampled": [float(x) for x in one_ms],
                                      "estimated_total_s": one_total_s,
-                                     "queries_per_s": qs.count / one_total_s},
-        "speedup_vs_one_at_a_time": one_total_s / wall,
+                                     "queries_per_s": qs.count / one_total_s if one_total_s else None},
+        "speedup_vs_one_at_a_time": one_total_s / wall if one_total_s else None,
         "cpu_baseline": cpu,
     }
-    log(f"batched: {qs.count} queries in {wall * 1e3:.1f} ms ({qps:.0f} queries/s); one at a time "
-        f"~{one_total_s:.1f} s -> x{one_total_s / wall:.1f}")
+    log(f"batched: {qs.count} queries in {wall * 1e3:.1f} ms ({qps:.0f} queries/s)" +
+        (f"; one at a time ~{one_total_s:.1f} s -> x{one_total_s / wall:.1f}" if one_total_s else ""))
     del dg, g
     return out
 
@@ -289,6 +298,9 @@ def main() -> int:
     ap.add_argument("--one-at-a-time", type=int, default=16,
                     help="queries sampled for the one-at-a-time plain ADDS comparison")
     ap.add_argument("--gemm", type=int, default=1, help="also time the batched min-plus GEMM workload")
+    ap.add_argument("--batch-graph", type=int, default=1,
+                    help="0: batched queries as plain launches with fixed options, no tuning "
+                         "(kernel-trace profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
