@@ -232,17 +232,40 @@ void launch_single(int tile, const MinplusItem* d_item, int M, int N, hipStream_
   hipLaunchKernelGGL(t.fn, grid, dim3(256), 0, s, d_item, static_cast<const int2*>(nullptr));
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin by workgroup id.  Entry p of
-// the item-ordered tile list goes to workgroup id (p % chunk) * 8 + p / chunk,
-// so XCD x runs the contiguous run [x * chunk, (x + 1) * chunk): the tiles of
-// one product share an XCD and its L2, which holds that product's A rows and
-// B columns.
+// Workgroups are dealt to the 8 XCDs round-robin by workgroup id, so the
+// entry at workgroup id j * 8 + x runs on XCD x.  The tiles of one tile shape
+// are cut into pieces (runs of tiles of one product, at most 1/64 of the
+// launch's tiles each), the pieces go longest-first (a tile's work is its
+// product's K) to the XCD with the least work so far, and each XCD runs its
+// pieces in that order: the XCDs finish together, long tiles start first, and
+// a product's tiles share an XCD and its L2, which holds the product's A rows
+// and B columns.  Short XCD lists are padded with item -1.
 constexpr int kXcds = 8;
 
-void xcd_order(const std::vector<int2>& list, std::vector<int2>& out) {
-  const size_t chunk = (list.size() + kXcds - 1) / kXcds;
-  out.assign(chunk * kXcds, int2{-1, 0});
-  for (size_t p = 0; p < list.size(); ++p) out[(p % chunk) * kXcds + p / chunk] = list[p];
+struct TileRun {
+  int item, first, count, k;
+};
+
+void xcd_order(std::vector<TileRun>& runs, uint64_t total_tiles, std::vector<int2>& out) {
+  const int piece = (int)std::max<uint64_t>(1, (total_tiles + 63) / 64);
+  std::vector<TileRun> pieces;
+  for (const TileRun& r : runs)
+    for (int f = 0; f < r.count; f += piece) pieces.push_back(TileRun{r.item, r.first + f, std::min(piece, r.count - f), r.k});
+  std::stable_sort(pieces.begin(), pieces.end(), [](const TileRun& x, const TileRun& y) { return x.k > y.k; });
+  std::vector<std::vector<int2>> bins(kXcds);
+  uint64_t work[kXcds] = {};
+  for (const TileRun& r : pieces) {
+    int b = 0;
+    for (int x = 1; x < kXcds; ++x)
+      if (work[x] < work[b]) b = x;
+    work[b] += (uint64_t)r.count * (uint64_t)std::max(r.k, 1);
+    for (int t = 0; t < r.count; ++t) bins[b].push_back(int2{r.item, r.first + t});
+  }
+  size_t len = 0;
+  for (auto& bn : bins) len = std::max(len, bn.size());
+  out.assign(len * kXcds, int2{-1, 0});
+  for (int x = 0; x < kXcds; ++x)
+    for (size_t j = 0; j < bins[x].size(); ++j) out[j * kXcds + x] = bins[x][j];
 }
 
 }  // namespace
@@ -272,19 +295,21 @@ void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s,
   MI_HIP(hipMemcpyAsync(table.p, items.data(), items.size() * sizeof(MinplusItem), hipMemcpyHostToDevice, s));
   uint64_t all_tiles64 = 0;
   for (const MinplusItem& it : items) all_tiles64 += tiles64(it.M, it.N);
-  std::vector<std::vector<int2>> lists(kNumTiles);
+  std::vector<std::vector<TileRun>> lists(kNumTiles);
+  std::vector<uint64_t> list_tiles(kNumTiles, 0);
   for (size_t i = 0; i < items.size(); ++i) {
     const MinplusItem& it = items[i];
     const int t = choose_tile(it.M, it.N, all_tiles64);
     const uint64_t tiles = (uint64_t)div_up((uint64_t)it.M, kTiles[t].bm) * div_up((uint64_t)it.N, kTiles[t].bn);
     if (tiles > (uint64_t)INT32_MAX) fail(MI355_EINVAL, "min-plus product too large for one batch item");
-    for (int64_t k = 0; k < (int64_t)tiles; ++k) lists[t].push_back(int2{(int)i, (int)k});
+    lists[t].push_back(TileRun{(int)i, 0, (int)tiles, it.K});
+    list_tiles[t] += tiles;
   }
   std::vector<std::vector<int2>> ordered(kNumTiles);
   size_t total = 0;
   for (int t = 0; t < kNumTiles; ++t) {
     if (lists[t].empty()) continue;
-    xcd_order(lists[t], ordered[t]);
+    xcd_order(lists[t], list_tiles[t], ordered[t]);
     // a launch holds at most 2^32 - 1 threads
     if (ordered[t].size() > (size_t)UINT32_MAX / 256) fail(MI355_EINVAL, "min-plus batch too large");
     total += ordered[t].size();
