@@ -3,7 +3,12 @@
 // box") solved together as one min-plus SpMM frontier.
 //
 // Every query q owns a dense block of packed (dist key, parent) labels, one
-// per vertex of its rectangle in row-major order, at labels[base_q ..).  The
+// per vertex of its rectangle, at labels[base_q ..), laid out in 4 x 4 tiles
+// (tile-major, row-major inside a tile; the block is padded to multiples of
+// 4): one 128-byte line of labels is a 4 x 4 patch, so a frontier that
+// sweeps across the rectangle touches about a quarter of the lines that a
+// row-major block's vertical fronts would, and reuses each line for about
+// four rounds.  The
 // frontier is a list of (query, local index) pairs from all queries at once,
 // so one round relaxes every query's frontier in the same launch:
 // D_q <- min(D_q, A^T (x) D_q) restricted to the rectangle, for all q.  A
@@ -45,13 +50,28 @@ namespace mi355 {
 namespace {
 
 struct QueryInfo {
-  int64_t base;      // first label slot of the query (schedule order)
+  int64_t base;      // first label slot of the query (schedule order, 4 x 4 tiles)
   int64_t out_base;  // first output element of the query (input order)
   int32_t r0, c0;    // rectangle origin (grid row, column)
   int32_t bh, bw;    // rectangle height and width
 };
 
 constexpr int kCntStride = 32;  // counters on separate 128-byte lines
+
+// Local slot of rectangle cell (dr, dc) in a block of width bw: 4 x 4 tiles,
+// tile-major, row-major inside the tile.
+__host__ __device__ inline uint32_t tiled_slot(uint32_t dr, uint32_t dc, uint32_t bw) {
+  const uint32_t tw = (bw + 3u) >> 2;
+  return ((((dr >> 2) * tw) + (dc >> 2)) << 4) | ((dr & 3u) << 2) | (dc & 3u);
+}
+__host__ __device__ inline void tiled_cell(uint32_t l, uint32_t bw, uint32_t& dr, uint32_t& dc) {
+  const uint32_t tw = (bw + 3u) >> 2;
+  const uint32_t t = l >> 4, tr = t / tw;
+  dr = (tr << 2) | ((l >> 2) & 3u);
+  dc = ((t - tr * tw) << 2) | (l & 3u);
+}
+// Slots of a bh x bw block (padded to whole tiles).
+__host__ __device__ inline int64_t tiled_area(int64_t bh, int64_t bw) { return ((bh + 3) & ~3LL) * ((bw + 3) & ~3LL); }
 constexpr uint32_t kRoundIdle = 0, kRoundRelax = 1, kRoundFar = 2;
 constexpr uint32_t kNearStage = 256;  // staged near appends per wave
 constexpr uint32_t kFarStage = 128;   // staged far appends per wave
@@ -233,8 +253,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
         const uint2 it = qin[i];
         qid = (int)it.x;
         const QueryInfo qi = a.qinfo[qid];
-        const int r = (int)(it.y / (uint32_t)qi.bw), col = (int)(it.y - (uint32_t)r * (uint32_t)qi.bw);
-        u = (qi.r0 + r) * a.cols + qi.c0 + col;
+        uint32_t r, col;
+        tiled_cell(it.y, (uint32_t)qi.bw, r, col);
+        u = (qi.r0 + (int)r) * a.cols + qi.c0 + (int)col;
         key = atomic_load_key(a.labels + qi.base + it.y);
         beg = a.rowptr[u];
         if (key != W::kInfKey) deg = a.rowptr[u + 1] - beg;
@@ -294,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
           const int rv = (int)e[p].x / a.cols, cv = (int)e[p].x - rv * a.cols;
           const uint32_t dr = (uint32_t)(rv - qi[p].r0), dc = (uint32_t)(cv - qi[p].c0);
           const bool in = ok[p] && dr < (uint32_t)qi[p].bh && dc < (uint32_t)qi[p].bw;
-          lv[p] = in ? dr * (uint32_t)qi[p].bw + dc : 0u;
+          lv[p] = in ? tiled_slot(dr, dc, (uint32_t)qi[p].bw) : 0u;
           slot[p] = in ? qi[p].base + lv[p] : 0;
           ck[p] = in ? W::relax(s_key[wv][pos[p]], e[p].y) : W::kInfKey;
           cand[p] = ck[p] != W::kInfKey ? ((unsigned long long)ck[p] << 32) | (uint32_t)s_u[wv][pos[p]] : kNone;
@@ -406,7 +427,8 @@ __global__ void k_batch_output(const unsigned long long* __restrict__ labels, co
   const int64_t area = (int64_t)qi.bh * qi.bw;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < area; i += stride) {
-    const unsigned long long l = labels[qi.base + i];
+    const uint32_t dr = (uint32_t)(i / qi.bw), dc = (uint32_t)(i - (int64_t)dr * qi.bw);
+    const unsigned long long l = labels[qi.base + tiled_slot(dr, dc, (uint32_t)qi.bw)];
     const uint32_t key = (uint32_t)(l >> 32);
     const bool reached = key != W::kInfKey;
     if (dist) {
@@ -425,7 +447,12 @@ __global__ void k_batch_output(const unsigned long long* __restrict__ labels, co
 __global__ void k_batch_source_preds(const QueryInfo* __restrict__ qinfo, const uint2* __restrict__ src,
                                      uint32_t nsrc, int32_t* __restrict__ pred) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nsrc && pred) pred[qinfo[src[i].x].out_base + src[i].y] = -1;
+  if (i < nsrc && pred) {
+    const QueryInfo qi = qinfo[src[i].x];
+    uint32_t dr, dc;
+    tiled_cell(src[i].y, (uint32_t)qi.bw, dr, dc);
+    pred[qi.out_base + (int64_t)dr * qi.bw + dc] = -1;
+  }
 }
 
 }  // namespace
@@ -451,7 +478,7 @@ void batch_layout(const mi355_query_batch& qb, int64_t n, std::vector<int64_t>& 
       if (r < b[0] || r >= b[2] || c < b[1] || c >= b[3]) fail(MI355_EINVAL, "source outside its query rectangle");
     }
     const int64_t area = (int64_t)(b[2] - b[0]) * (int64_t)(b[3] - b[1]);
-    if (area > (int64_t)UINT32_MAX) fail(MI355_EOVERFLOW, "query rectangle too large");
+    if (tiled_area(b[2] - b[0], b[3] - b[1]) > (int64_t)UINT32_MAX) fail(MI355_EOVERFLOW, "query rectangle too large");
     offsets[(size_t)q + 1] = offsets[(size_t)q] + area;
   }
 }
@@ -477,9 +504,11 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   std::vector<int64_t> lbase((size_t)nq);
   int64_t acc = 0;
   for (int32_t q : order) {
+    const int32_t* b = qb.boxes + 4 * (size_t)q;
     lbase[(size_t)q] = acc;
-    acc += off[(size_t)q + 1] - off[(size_t)q];
+    acc += tiled_area(b[2] - b[0], b[3] - b[1]);
   }
+  const int64_t slots = acc;  // label slots incl. tile padding
   std::vector<QueryInfo> qi((size_t)nq);
   std::vector<uint2> src;
   for (int32_t q = 0; q < nq; ++q) {
@@ -488,7 +517,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
     for (int32_t i = qb.src_offsets[q]; i < qb.src_offsets[q + 1]; ++i) {
       const int64_t v = qb.sources[i];
       const int64_t r = v / qb.grid_cols - b[0], c = v % qb.grid_cols - b[1];
-      src.push_back(make_uint2((uint32_t)q, (uint32_t)(r * (b[3] - b[1]) + c)));
+      src.push_back(make_uint2((uint32_t)q, tiled_slot((uint32_t)r, (uint32_t)c, (uint32_t)(b[3] - b[1]))));
     }
   }
   const int64_t nsrc = (int64_t)src.size();
@@ -511,12 +540,12 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
 
   // state cached on the graph handle and grown as needed
   Workspace& ws = g.ws;
-  ws.batch_labels.ensure((size_t)total);
-  ws.batch_stamp.ensure((size_t)total);
+  ws.batch_labels.ensure((size_t)slots);
+  ws.batch_stamp.ensure((size_t)slots);
   ws.batch_q[0].ensure((size_t)cap);
   ws.batch_q[1].ensure((size_t)cap);
   if (bucketed) {
-    ws.batch_fstamp.ensure((size_t)total);
+    ws.batch_fstamp.ensure((size_t)slots);
     ws.batch_far[0].ensure((size_t)cap);
     ws.batch_far[1].ensure((size_t)cap);
   }
@@ -559,8 +588,8 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   MI_HIP(hipEventCreate(&ev0));
   MI_HIP(hipEventCreate(&ev1));
   MI_HIP(hipEventRecord(ev0, s));
-  k_batch_init<W><<<(unsigned)std::min<int64_t>(div_up((uint64_t)total, kBlock), 8192), kBlock, 0, s>>>(
-      bs.labels, bs.stamp, a.fstamp, total);
+  k_batch_init<W><<<(unsigned)std::min<int64_t>(div_up((uint64_t)slots, kBlock), 8192), kBlock, 0, s>>>(
+      bs.labels, bs.stamp, a.fstamp, slots);
   k_batch_sources<W><<<(unsigned)div_up((uint64_t)std::max<int64_t>(nsrc, 1), kBlock), kBlock, 0, s>>>(
       bs.labels, bs.qinfo, bs.src, (uint32_t)nsrc, bs.q[0], ctrl, tkey0);
   MI_LAUNCH_CHECK();
