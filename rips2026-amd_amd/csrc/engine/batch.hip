@@ -57,6 +57,15 @@ struct QueryInfo {
 };
 
 constexpr int kCntStride = 32;  // counters on separate 128-byte lines
+// Queries are split into kParts parts of about equal area; part p's frontier
+// queues and far piles are segments of the shared arrays and only workgroups
+// with blockIdx.x % kParts == p process them.  Workgroups are dealt to the 8
+// XCDs round-robin by workgroup id, so a query's frontier stays on one XCD
+// and its label lines in that XCD's L2 from round to round.
+constexpr int kParts = 8;
+__host__ __device__ constexpr int cidx(uint32_t parity, uint32_t part) {
+  return (int)((parity * kParts + part) * kCntStride);
+}
 
 // Local slot of rectangle cell (dr, dc) in a block of width bw: 4 x 4 tiles,
 // tile-major, row-major inside the tile.
@@ -77,9 +86,9 @@ constexpr uint32_t kNearStage = 256;  // staged near appends per wave
 constexpr uint32_t kFarStage = 128;   // staged far appends per wave
 
 struct BatchCtrl {
-  alignas(128) uint32_t cnt[2 * kCntStride];   // near queue sizes (round parity)
-  alignas(128) uint32_t fcnt[2 * kCntStride];  // far pile sizes (far epoch parity)
-  alignas(128) uint32_t fmin[2 * kCntStride];  // smallest key in each far pile
+  alignas(128) uint32_t cnt[2 * kParts * kCntStride];   // near queue sizes (round parity, part)
+  alignas(128) uint32_t fcnt[2 * kParts * kCntStride];  // far pile sizes (far epoch parity, part)
+  alignas(128) uint32_t fmin[2 * kParts * kCntStride];  // smallest key in each far pile
   alignas(128) uint32_t epoch;                 // graph replays (round stamps)
   alignas(128) uint32_t err;
   alignas(128) unsigned long long edges;
@@ -88,7 +97,8 @@ struct BatchCtrl {
   uint32_t tkey;                 // near threshold (key domain)
   uint32_t busy_rounds, far_passes;
   // this round's descriptor
-  uint32_t mode, n_in, far_n, t_old, t_new;
+  uint32_t mode, t_old, t_new;
+  uint32_t n_in[kParts], far_n[kParts];
   // entries left after the last round of a replay (read by the host)
   uint32_t left;
 };
@@ -102,7 +112,7 @@ struct BatchArgs {
   const QueryInfo* qinfo;
   uint2* q[2];        // near queues of (query, local index)
   uint2* far[2];      // far piles
-  uint32_t cap;       // entries per near queue / far pile
+  uint32_t qoff[kParts + 1];  // part p's segment of every queue / pile: [qoff[p], qoff[p + 1])
   BatchCtrl* ctrl;
   int32_t cols;       // grid width: vertex v = row * cols + col
   int32_t rounds_per_graph;
@@ -152,23 +162,34 @@ __device__ inline uint32_t next_threshold(uint32_t t_old, uint32_t kmin, float d
 template <class W>
 __global__ void k_batch_decide(BatchArgs a, int k) {
   BatchCtrl* c = a.ctrl;
-  const uint32_t n_in = min(c->cnt[(k & 1) * kCntStride], a.cap);
-  c->cnt[((k + 1) & 1) * kCntStride] = 0;  // this round's near output
+  uint32_t n_in = 0;
+  for (uint32_t p = 0; p < kParts; ++p) {
+    const uint32_t np = min(c->cnt[cidx(k & 1, p)], a.qoff[p + 1] - a.qoff[p]);
+    c->n_in[p] = np;
+    n_in += np;
+    c->cnt[cidx((k + 1) & 1, p)] = 0;  // this round's near output
+  }
   uint32_t mode = kRoundIdle;
   if (n_in > 0) {
     mode = kRoundRelax;
   } else {
     const uint32_t e = c->fepoch;
-    const uint32_t nf = min(c->fcnt[(e & 1) * kCntStride], a.cap);
+    uint32_t nf = 0, kmin = W::kInfKey;
+    for (uint32_t p = 0; p < kParts; ++p) {
+      nf += min(c->fcnt[cidx(e & 1, p)], a.qoff[p + 1] - a.qoff[p]);
+      kmin = min(kmin, c->fmin[cidx(e & 1, p)]);
+    }
     if (nf > 0) {
       // the pass reads pile e & 1; entries above the new threshold go to the
       // other pile, which the previous pass emptied
       mode = kRoundFar;
       const uint32_t t_old = c->tkey;
-      const uint32_t t_new = next_threshold<W>(t_old, c->fmin[(e & 1) * kCntStride], a.delta);
-      c->fcnt[((e + 1) & 1) * kCntStride] = 0;
-      c->fmin[((e + 1) & 1) * kCntStride] = W::kInfKey;
-      c->far_n = nf;
+      const uint32_t t_new = next_threshold<W>(t_old, kmin, a.delta);
+      for (uint32_t p = 0; p < kParts; ++p) {
+        c->far_n[p] = min(c->fcnt[cidx(e & 1, p)], a.qoff[p + 1] - a.qoff[p]);
+        c->fcnt[cidx((e + 1) & 1, p)] = 0;
+        c->fmin[cidx((e + 1) & 1, p)] = W::kInfKey;
+      }
       c->t_old = t_old;
       c->t_new = t_new;
       c->tkey = t_new;
@@ -178,7 +199,6 @@ __global__ void k_batch_decide(BatchArgs a, int k) {
   }
   if (mode != kRoundIdle) c->busy_rounds += 1;
   c->mode = mode;
-  c->n_in = n_in;
 }
 
 // Stage one lane's entry (wave-uniform call) into a per-wave LDS stage.
@@ -208,10 +228,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
   const uint32_t stamp = c->epoch * (uint32_t)a.rounds_per_graph + (uint32_t)k + 1u;
   const uint32_t fe = c->fepoch;  // this epoch's pile: fe & 1, far stamp fe + 1
   const uint32_t fst = fe + 1u;
-  uint2* fout = a.far[fe & 1];
-  uint32_t* fcnt_out = &c->fcnt[(fe & 1) * kCntStride];
-  uint2* qout = a.q[(k + 1) & 1];
-  uint32_t* cnt_out = &c->cnt[((k + 1) & 1) * kCntStride];
+  const uint32_t part = blockIdx.x % kParts, pwg = blockIdx.x / kParts;
+  const uint32_t seg = a.qoff[part], cap = a.qoff[part + 1] - seg;
+  uint2* fout = a.far[fe & 1] + seg;
+  uint32_t* fcnt_out = &c->fcnt[cidx(fe & 1, part)];
+  uint2* qout = a.q[(k + 1) & 1] + seg;
+  uint32_t* cnt_out = &c->cnt[cidx((k + 1) & 1, part)];
   const unsigned lane = threadIdx.x & (kWave - 1);
   const unsigned wv = threadIdx.x / kWave;
   uint2* nst = s_near[wv];
@@ -219,13 +241,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
   uint32_t n_staged = 0, f_staged = 0;
   uint32_t fmin_lane = W::kInfKey;
   unsigned long long edge_visits = 0;
-  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  const uint32_t nwaves = gridDim.x / kParts * kWavesPerBlock;  // of this part
 
   if (mode == kRoundFar) {
     // far pass over the pile of the epoch just closed
-    const uint2* fin = a.far[(fe - 1u) & 1];
-    const uint32_t nf = c->far_n, t_old = c->t_old, t_new = c->t_new;
-    for (uint32_t first = (blockIdx.x * kWavesPerBlock + wv) * kWave; first < nf; first += nwaves * kWave) {
+    const uint2* fin = a.far[(fe - 1u) & 1] + seg;
+    const uint32_t nf = c->far_n[part], t_old = c->t_old, t_new = c->t_new;
+    for (uint32_t first = (pwg * kWavesPerBlock + wv) * kWave; first < nf; first += nwaves * kWave) {
       const uint32_t i = first + lane;
       const bool ok = i < nf;
       const uint2 it = fin[ok ? i : 0u];
@@ -239,13 +261,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
       if (to_near && a.stamp[slot] != stamp) x = atomicExch(a.stamp + slot, stamp);
       if (keep && a.fstamp[slot] != fst) y = atomicExch(a.fstamp + slot, fst);
       if (keep) fmin_lane = min(fmin_lane, key);
-      stage_entry(x != stamp, it, nst, n_staged, kNearStage, qout, cnt_out, a.cap, &c->err);
-      stage_entry(y != fst, it, fst_stage, f_staged, kFarStage, fout, fcnt_out, a.cap, &c->err);
+      stage_entry(x != stamp, it, nst, n_staged, kNearStage, qout, cnt_out, cap, &c->err);
+      stage_entry(y != fst, it, fst_stage, f_staged, kFarStage, fout, fcnt_out, cap, &c->err);
     }
   } else {
-    const uint32_t n_in = c->n_in, tkey = c->tkey;
-    const uint2* qin = a.q[k & 1];
-    for (uint32_t first = (blockIdx.x * kWavesPerBlock + wv) * kWave; first < n_in; first += nwaves * kWave) {
+    const uint32_t n_in = c->n_in[part], tkey = c->tkey;
+    const uint2* qin = a.q[k & 1] + seg;
+    for (uint32_t first = (pwg * kWavesPerBlock + wv) * kWave; first < n_in; first += nwaves * kWave) {
       const uint32_t i = first + lane;
       int u = 0, beg = 0, deg = 0, qid = 0;
       uint32_t key = W::kInfKey;
@@ -356,26 +378,28 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           const uint2 it = make_uint2((uint32_t)qq[p], lv[p]);
-          stage_entry(x[p] != stamp, it, nst, n_staged, kNearStage, qout, cnt_out, a.cap, &c->err);
-          stage_entry(y[p] != fst, it, fst_stage, f_staged, kFarStage, fout, fcnt_out, a.cap, &c->err);
+          stage_entry(x[p] != stamp, it, nst, n_staged, kNearStage, qout, cnt_out, cap, &c->err);
+          stage_entry(y[p] != fst, it, fst_stage, f_staged, kFarStage, fout, fcnt_out, cap, &c->err);
         }
       }
       __builtin_amdgcn_wave_barrier();
     }
   }
-  batch_stage_flush(nst, n_staged, qout, cnt_out, a.cap, &c->err);
-  batch_stage_flush(fst_stage, f_staged, fout, fcnt_out, a.cap, &c->err);
+  batch_stage_flush(nst, n_staged, qout, cnt_out, cap, &c->err);
+  batch_stage_flush(fst_stage, f_staged, fout, fcnt_out, cap, &c->err);
   // smallest far key of the pile (one atomic per wave)
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) fmin_lane = min(fmin_lane, (uint32_t)__shfl_xor((int)fmin_lane, off, 64));
-  if (lane == 0 && fmin_lane != W::kInfKey) atomicMin(&c->fmin[(fe & 1) * kCntStride], fmin_lane);
+  if (lane == 0 && fmin_lane != W::kInfKey) atomicMin(&c->fmin[cidx(fe & 1, part)], fmin_lane);
   if (a.want_edges && lane == 0 && edge_visits) atomicAdd(&c->edges, edge_visits);
 }
 
 // End of a replay: the graph epoch (round stamps) and the entries left.
 __global__ void k_batch_epoch(BatchCtrl* c, int K) {
   c->epoch += 1u;
-  c->left = c->cnt[(K & 1) * kCntStride] + c->fcnt[(c->fepoch & 1) * kCntStride];
+  uint32_t left = 0;
+  for (uint32_t p = 0; p < kParts; ++p) left += c->cnt[cidx(K & 1, p)] + c->fcnt[cidx(c->fepoch & 1, p)];
+  c->left = left;
 }
 
 template <class W>
@@ -390,23 +414,32 @@ __global__ void k_batch_init(unsigned long long* __restrict__ labels, uint32_t* 
   }
 }
 
+// Sources of part p are src[soff[p] .. soff[p + 1]); they go to the start of
+// part p's segment of queue 0.
+struct PartOffsets {
+  uint32_t soff[kParts + 1];
+  uint32_t qoff[kParts + 1];
+};
+
 template <class W>
 __global__ void k_batch_sources(unsigned long long* __restrict__ labels, const QueryInfo* __restrict__ qinfo,
                                 const uint2* __restrict__ src, uint32_t nsrc, uint2* __restrict__ q0,
-                                BatchCtrl* __restrict__ c, uint32_t tkey0) {
+                                BatchCtrl* __restrict__ c, uint32_t tkey0, PartOffsets po) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nsrc) {
     const uint2 s = src[i];
     labels[qinfo[s.x].base + s.y] = ((unsigned long long)W::kZeroKey << 32) | kNoPred;
-    q0[i] = s;
+    uint32_t p = 0;
+    while (p + 1 < kParts && i >= po.soff[p + 1]) ++p;
+    q0[po.qoff[p] + (i - po.soff[p])] = s;
   }
   if (i == 0) {
-    for (int k = 0; k < 2; ++k) {
-      c->cnt[k * kCntStride] = 0;
-      c->fcnt[k * kCntStride] = 0;
-      c->fmin[k * kCntStride] = W::kInfKey;
-    }
-    c->cnt[0] = nsrc;
+    for (uint32_t k = 0; k < 2; ++k)
+      for (uint32_t p = 0; p < kParts; ++p) {
+        c->cnt[cidx(k, p)] = k == 0 ? po.soff[p + 1] - po.soff[p] : 0u;
+        c->fcnt[cidx(k, p)] = 0;
+        c->fmin[cidx(k, p)] = W::kInfKey;
+      }
     c->epoch = 0;
     c->err = 0;
     c->edges = 0;
@@ -509,21 +542,55 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
     acc += tiled_area(b[2] - b[0], b[3] - b[1]);
   }
   const int64_t slots = acc;  // label slots incl. tile padding
+  // parts of about equal area: largest query first to the lightest part
+  std::vector<int32_t> part((size_t)nq), by_area((size_t)nq);
+  std::iota(by_area.begin(), by_area.end(), 0);
+  std::stable_sort(by_area.begin(), by_area.end(), [&](int32_t x, int32_t y) {
+    return off[(size_t)x + 1] - off[(size_t)x] > off[(size_t)y + 1] - off[(size_t)y];
+  });
+  int64_t part_area[kParts] = {};
+  for (int32_t q : by_area) {
+    int b = 0;
+    for (int x = 1; x < kParts; ++x)
+      if (part_area[x] < part_area[b]) b = x;
+    part[(size_t)q] = b;
+    part_area[b] += off[(size_t)q + 1] - off[(size_t)q];
+  }
   std::vector<QueryInfo> qi((size_t)nq);
-  std::vector<uint2> src;
+  std::vector<std::vector<uint2>> psrc(kParts);
   for (int32_t q = 0; q < nq; ++q) {
     const int32_t* b = qb.boxes + 4 * (size_t)q;
     qi[(size_t)q] = QueryInfo{lbase[(size_t)q], off[(size_t)q], b[0], b[1], b[2] - b[0], b[3] - b[1]};
     for (int32_t i = qb.src_offsets[q]; i < qb.src_offsets[q + 1]; ++i) {
       const int64_t v = qb.sources[i];
       const int64_t r = v / qb.grid_cols - b[0], c = v % qb.grid_cols - b[1];
-      src.push_back(make_uint2((uint32_t)q, tiled_slot((uint32_t)r, (uint32_t)c, (uint32_t)(b[3] - b[1]))));
+      psrc[(size_t)part[(size_t)q]].push_back(
+          make_uint2((uint32_t)q, tiled_slot((uint32_t)r, (uint32_t)c, (uint32_t)(b[3] - b[1]))));
     }
   }
+  std::vector<uint2> src;
+  PartOffsets po{};
+  for (int x = 0; x < kParts; ++x) {
+    po.soff[x] = (uint32_t)src.size();
+    src.insert(src.end(), psrc[(size_t)x].begin(), psrc[(size_t)x].end());
+  }
+  po.soff[kParts] = (uint32_t)src.size();
   const int64_t nsrc = (int64_t)src.size();
+  // Queue capacity (entries of all parts), split in proportion to area; a
+  // part never gets less than its sources.
   int64_t cap = o.queue_capacity > 0 ? o.queue_capacity : std::max<int64_t>((total + nsrc) / 4, int64_t(1) << 24);
   cap = std::max<int64_t>(std::min<int64_t>(cap, total + nsrc), nsrc);
-  if (cap > (int64_t)UINT32_MAX) cap = UINT32_MAX;
+  int64_t qacc = 0;
+  for (int x = 0; x < kParts; ++x) {
+    const int64_t ns = (int64_t)psrc[(size_t)x].size();
+    int64_t cx = (int64_t)((double)cap * (double)part_area[x] / (double)std::max<int64_t>(total, 1));
+    cx = std::max<int64_t>(std::min<int64_t>(cx, part_area[x] + ns), ns);
+    po.qoff[x] = (uint32_t)qacc;
+    qacc += cx;
+    if (qacc > (int64_t)UINT32_MAX) fail(MI355_ECAPACITY, "batched frontier queues exceed 2^32 entries");
+  }
+  po.qoff[kParts] = (uint32_t)qacc;
+  const int64_t qtotal = qacc;
   // even: the queue and counter parity return to round 0's after K rounds
   const int K = std::max(2, (o.rounds_per_graph > 0 ? o.rounds_per_graph : 24) / 2 * 2);
   // near-far bucket width; Bellman-Ford rounds (threshold infinity) when
@@ -536,18 +603,20 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
     tkey0 = W::kType == MI355_W_F32 ? f32_to_key(delta)
                                     : (uint32_t)std::min(4294967294.0, std::max(1.0, std::ceil((double)delta)));
   const DeviceProps p = device_props();
-  const unsigned grid = (unsigned)std::max(1, (o.blocks_per_cu > 0 ? o.blocks_per_cu : 4) * p.cus);
+  // a whole number of workgroups per part
+  const unsigned grid = (unsigned)(div_up((uint64_t)std::max(1, (o.blocks_per_cu > 0 ? o.blocks_per_cu : 4) * p.cus),
+                                          kParts) * kParts);
 
   // state cached on the graph handle and grown as needed
   Workspace& ws = g.ws;
   ws.batch_labels.ensure((size_t)slots);
   ws.batch_stamp.ensure((size_t)slots);
-  ws.batch_q[0].ensure((size_t)cap);
-  ws.batch_q[1].ensure((size_t)cap);
+  ws.batch_q[0].ensure((size_t)std::max<int64_t>(qtotal, 1));
+  ws.batch_q[1].ensure((size_t)std::max<int64_t>(qtotal, 1));
   if (bucketed) {
     ws.batch_fstamp.ensure((size_t)slots);
-    ws.batch_far[0].ensure((size_t)cap);
-    ws.batch_far[1].ensure((size_t)cap);
+    ws.batch_far[0].ensure((size_t)std::max<int64_t>(qtotal, 1));
+    ws.batch_far[1].ensure((size_t)std::max<int64_t>(qtotal, 1));
   }
   ws.batch_qinfo.ensure((size_t)nq * sizeof(QueryInfo));
   ws.batch_src.ensure((size_t)std::max<int64_t>(nsrc, 1));
@@ -577,7 +646,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   a.q[1] = bs.q[1];
   a.far[0] = bucketed ? ws.batch_far[0].p : bs.q[0];
   a.far[1] = bucketed ? ws.batch_far[1].p : bs.q[1];
-  a.cap = (uint32_t)cap;
+  std::copy(po.qoff, po.qoff + kParts + 1, a.qoff);
   a.delta = bucketed ? delta : 0.0f;
   a.ctrl = ctrl;
   a.cols = qb.grid_cols;
@@ -591,7 +660,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   k_batch_init<W><<<(unsigned)std::min<int64_t>(div_up((uint64_t)slots, kBlock), 8192), kBlock, 0, s>>>(
       bs.labels, bs.stamp, a.fstamp, slots);
   k_batch_sources<W><<<(unsigned)div_up((uint64_t)std::max<int64_t>(nsrc, 1), kBlock), kBlock, 0, s>>>(
-      bs.labels, bs.qinfo, bs.src, (uint32_t)nsrc, bs.q[0], ctrl, tkey0);
+      bs.labels, bs.qinfo, bs.src, (uint32_t)nsrc, bs.q[0], ctrl, tkey0, po);
   MI_LAUNCH_CHECK();
 
   // K rounds (+ the epoch node) as one graph, or as plain launches
