@@ -12,7 +12,7 @@
 // frontier is a list of (query, local index) pairs from all queries at once,
 // so one round relaxes every query's frontier in the same launch:
 // D_q <- min(D_q, A^T (x) D_q) restricted to the rectangle, for all q.  A
-// round packs the edges of 64 entries per wave across lanes (two edges per
+// round packs the edges of 64 entries per wave across lanes (four edges per
 // lane in flight), skips out-of-rectangle targets and stages appends in LDS.
 //
 // Near-far frontier (bucketed label correcting): an improved slot whose new
