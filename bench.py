@@ -125,7 +125,9 @@ def routing_workloads(args, cpu_threads: int):
     # batched box-limited multi-source queries
     qs = B.routing_queries(side, side, args.queries, seed=args.seed)
     if args.batch_graph:
-        best, tune_ms = B.autotune(dg, qs, sample_stride=8)
+        # tuned on every 8th query, the best of each width confirmed on the whole
+        # batch (a router solves the same boxes in every iteration)
+        best, tune_ms = B.autotune(dg, qs, sample_stride=8, confirm=3)
         opts = dict(rounds_per_graph=best.rounds_per_graph, blocks_per_cu=best.blocks_per_cu,
                     order=best.order, delta_scale=best.delta_scale)
     else:

@@ -400,8 +400,7 @@ namespace {
 // One device slot per host thread for single-GEMM descriptors (kept alive for
 // the thread's lifetime so asynchronous launches never see a freed slot).
 thread_local mi355::DevBuf<mi355::MinplusItem> t_item;
-thread_local mi355::DevBuf<mi355::MinplusItem> t_table;
-thread_local mi355::DevBuf<int2> t_work;
+thread_local mi355::MinplusBatchCache t_batch;
 
 // Upload a dense matrix if it lives on the host.
 const float* stage_dense(const float* adj, int32_t memory, int n, mi355::DevBuf<float>& tmp,
@@ -444,9 +443,7 @@ int mi355_minplus_gemm_f32_batched(int count, const float* const* dA, const floa
         fail(MI355_EINVAL, "leading dimension smaller than the matrix");
       items[(size_t)i] = mi355::MinplusItem{dA[i], dB[i], dC[i], M[i], N[i], K[i], lda[i], ldb[i], ldc[i], 0};
     }
-    // Synchronise before reusing the descriptor table of a previous call.
-    MI_HIP(hipStreamSynchronize(as_stream(stream)));
-    mi355::minplus_gemm_batched(items, as_stream(stream), t_table, t_work);
+    mi355::minplus_gemm_batched(items, as_stream(stream), t_batch);
   });
 }
 

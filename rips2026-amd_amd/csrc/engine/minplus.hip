@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cstring>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(256) void k_minplus_tile(const MinplusItem* __restr
   for (int s = 0; s < nslices; ++s) {
     const int cur = s & 1;
     if (s + 1 < nslices) load_slice((s + 1) * BK);
-#pragma unroll (UK)
+#pragma unroll UK
     for (int k = 0; k < BK; k += 2) {
       float a0[TM], a1[TM], b0[TN], b1[TN];
 #pragma unroll
@@ -197,6 +198,7 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 std::mutex g_tune_mu;
 int g_tuned[32];
 bool g_tuned_init = false;
+uint64_t g_tune_gen = 0;  // bumped on every change of g_tuned
 
 // Static rule by the launch's 64x64 tile count: below 512 (two workgroups
 // per CU) the fully unrolled 64x64 tile, else the UK = 4 one.  Measured on
@@ -282,9 +284,10 @@ void minplus_gemm(const float* A, const float* B, float* C, int M, int N, int K,
   MI_LAUNCH_CHECK();
 }
 
-void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s,
-                          DevBuf<MinplusItem>& table, DevBuf<int2>& work) {
-  if (items.empty()) return;
+// Descriptors and work list of a batch, uploaded into the cache's buffers.
+static void build_batch(const std::vector<MinplusItem>& items, hipStream_t s, MinplusBatchCache& cache) {
+  DevBuf<MinplusItem>& table = cache.table;
+  DevBuf<int2>& work = cache.work;
   // One launch per tile shape (each product takes its size class's tuned
   // tile, else the static rule over the whole batch's tile count) over a flat
   // list of (product, tile) work entries: every workgroup owns a real tile,
@@ -319,13 +322,46 @@ void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s,
   for (auto& o : ordered) flat.insert(flat.end(), o.begin(), o.end());
   work.ensure(flat.size());
   MI_HIP(hipMemcpyAsync(work.p, flat.data(), flat.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+  cache.launches.clear();
+  for (int t = 0; t < kNumTiles; ++t)
+    if (!ordered[t].empty()) cache.launches.emplace_back(t, (uint32_t)ordered[t].size());
+}
+
+void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s, MinplusBatchCache& cache) {
+  if (items.empty()) return;
+  int dev = 0;
+  MI_HIP(hipGetDevice(&dev));
+  uint64_t gen;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    gen = g_tune_gen;
+  }
+  const bool same = cache.valid && cache.device == dev && cache.tune_gen == gen && cache.items.size() == items.size() &&
+                    std::memcmp(cache.items.data(), items.data(), items.size() * sizeof(MinplusItem)) == 0;
+  if (!same) {
+    // the previous call's kernels may still read the buffers being replaced
+    if (cache.valid) {
+      if (cache.device == dev) MI_HIP(hipStreamSynchronize(cache.stream));
+      else MI_HIP(hipDeviceSynchronize());
+    }
+    cache.valid = false;
+    if (cache.device != dev) {
+      cache.table.release();
+      cache.work.release();
+    }
+    build_batch(items, s, cache);
+    cache.items = items;
+    cache.tune_gen = gen;
+    cache.device = dev;
+    cache.valid = true;
+  }
+  cache.stream = s;
   size_t off = 0;
-  for (int t = 0; t < kNumTiles; ++t) {
-    if (ordered[t].empty()) continue;
-    hipLaunchKernelGGL(kTiles[t].fn, dim3((unsigned)ordered[t].size()), dim3(256), 0, s,
-                       static_cast<const MinplusItem*>(table.p), static_cast<const int2*>(work.p + off));
+  for (const auto& l : cache.launches) {
+    hipLaunchKernelGGL(kTiles[l.first].fn, dim3(l.second), dim3(256), 0, s,
+                       static_cast<const MinplusItem*>(cache.table.p), static_cast<const int2*>(cache.work.p + off));
     MI_LAUNCH_CHECK();
-    off += ordered[t].size();
+    off += l.second;
   }
 }
 
@@ -375,6 +411,7 @@ void minplus_autotune(const int* sizes, int count, hipStream_t s, double* report
       g_tuned_init = true;
     }
     g_tuned[cls] = best;
+    ++g_tune_gen;
   }
 }
 
@@ -388,6 +425,7 @@ void minplus_tuning_set(const int* table32) {
   for (int i = 0; i < 32; ++i)
     g_tuned[i] = (table32[i] >= 0 && table32[i] < kNumTiles) ? table32[i] : -1;
   g_tuned_init = true;
+  ++g_tune_gen;
 }
 
 }  // namespace mi355

@@ -21,10 +21,23 @@ const char* minplus_tile_name(int i);
 // Asynchronous on stream s; scratch_item is a device slot for the descriptor.
 void minplus_gemm(const float* A, const float* B, float* C, int M, int N, int K, int lda, int ldb,
                   int ldc, hipStream_t s, MinplusItem* scratch_item, int force_tile = -1);
-// table / work: grow-only device buffers for the descriptors and the
-// (product, tile) work list; the caller keeps them until the launches end.
-void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s,
-                          DevBuf<MinplusItem>& table, DevBuf<int2>& work);
+// A batch's device descriptors and (product, tile) work list, kept for the
+// next call: a call with the same products (pointers and sizes) on the same
+// device and tuning launches again without rebuilding or uploading anything.
+// Kernels only read these buffers, so reuse needs no synchronisation; a
+// rebuild first waits for the stream of the call that last used them.
+struct MinplusBatchCache {
+  DevBuf<MinplusItem> table;
+  DevBuf<int2> work;
+  std::vector<MinplusItem> items;
+  std::vector<std::pair<int, uint32_t>> launches;  // (tile, workgroups), in work-list order
+  uint64_t tune_gen = 0;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  bool valid = false;
+};
+
+void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s, MinplusBatchCache& cache);
 void minplus_autotune(const int* sizes, int count, hipStream_t s, double* report);
 void minplus_tuning_get(int* table32);
 void minplus_tuning_set(const int* table32);

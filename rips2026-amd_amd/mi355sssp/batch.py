@@ -98,12 +98,33 @@ def run(dg, qs: Queries, want_pred: bool = True, download: bool = True, **kw):
 TUNE_CANDIDATES = 18
 
 
-def autotune(dg, qs: Queries, sample_stride: int = 8, **kw):
-    """(best options, device ms of every candidate) on every sample_stride-th query."""
+def autotune(dg, qs: Queries, sample_stride: int = 8, confirm: int = 0, **kw):
+    """(best options, device ms of every candidate) on every sample_stride-th
+    query.  confirm > 0: for each bucket width, the fastest candidate on the
+    sample; the `confirm` fastest of those are timed once more on the whole
+    batch and the fastest there is returned (a sample has the batch's round
+    count but a fraction of its work, which favours wide buckets)."""
     best = options(**kw)
     t = np.zeros(TUNE_CANDIDATES, np.float64)
     check(engine().mi355_batch_autotune(dg.handle, C.byref(qs.c()), int(sample_stride), C.byref(best),
                                         t.ctypes.data, None))
+    if confirm > 0:
+        # candidate order of mi355_batch_autotune: order x width x workgroups per CU
+        cands = [(order, sc, bpc) for order in (0, 1) for sc in (0.5, 1.0, 2.0) for bpc in (2, 4, 8)]
+        per_width = [min((i for i in range(len(cands)) if cands[i][1] == sc), key=lambda i: t[i])
+                     for sc in (0.5, 1.0, 2.0)]
+        full = {}
+        d_dist = DeviceBuffer(4 * int(layout(dg, qs)[-1]))
+        for i in sorted(per_width, key=lambda i: t[i])[:confirm]:
+            order, sc, bpc = cands[int(i)]
+            o = options(**kw)
+            o.order, o.delta_scale, o.blocks_per_cu = order, sc, bpc
+            o.rounds_per_graph = best.rounds_per_graph
+            st = BatchStats()
+            check(engine().mi355_sssp_batched(dg.handle, C.byref(qs.c()), C.byref(o), d_dist.ptr, None,
+                                              C.byref(st), None))
+            full[int(i)] = (float(st.device_ms), o)
+        best = min(full.values(), key=lambda x: x[0])[1]
     return best, t
 
 

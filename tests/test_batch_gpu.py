@@ -146,6 +146,11 @@ def test_autotune_and_schedule_orders():
     best, t = B.autotune(dg, qs, sample_stride=4)
     assert np.all(t > 0) and best.rounds_per_graph % 2 == 0 and best.blocks_per_cu in (2, 4, 8)
     assert best.delta_scale in (0.5, 1.0, 2.0)
+    # confirmed on the whole batch: the fastest candidate of its width on the sample
+    best3, t3 = B.autotune(dg, qs, sample_stride=4, confirm=3)
+    cands = [(o, sc, b) for o in (0, 1) for sc in (0.5, 1.0, 2.0) for b in (2, 4, 8)]
+    i = cands.index((best3.order, best3.delta_scale, best3.blocks_per_cu))
+    assert t3[i] == min(t3[j] for j in range(len(cands)) if cands[j][1] == best3.delta_scale)
     ref, pref, off, _ = B.run(dg, qs)
     for order in (0, 1):
         d, p, _, _ = B.run(dg, qs, order=order, rounds_per_graph=best.rounds_per_graph,
