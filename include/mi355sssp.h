@@ -178,7 +178,10 @@ int mi355_minplus_gemm_f32(const float* dA, const float* dB, float* dC, int M, i
  * arrays are HOST arrays of device pointers / sizes.  Each item takes its size
  * class's (ceil(log2(max(M, N)))) autotuned tile, else the tile the static
  * rule picks for the whole batch's size; each tile shape is one launch over a
- * flat list of (item, tile) entries.  Asynchronous. */
+ * flat list of (item, tile) entries.  The calling thread keeps the last
+ * batch's plan on the device: a call with the same pointers and sizes (and
+ * tuning) only launches; any other call first waits for the kernels of the
+ * previous one.  Asynchronous. */
 int mi355_minplus_gemm_f32_batched(int count, const float* const* dA, const float* const* dB,
                                    float* const* dC, const int* M, const int* N, const int* K,
                                    const int* lda, const int* ldb, const int* ldc, void* stream);

@@ -340,22 +340,21 @@ void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s, 
                     std::memcmp(cache.items.data(), items.data(), items.size() * sizeof(MinplusItem)) == 0;
   if (!same) {
     // the previous call's kernels may still read the buffers being replaced
-    if (cache.valid) {
-      if (cache.device == dev) MI_HIP(hipStreamSynchronize(cache.stream));
-      else MI_HIP(hipDeviceSynchronize());
-    }
+    if (cache.done) MI_HIP(hipEventSynchronize(cache.done));
     cache.valid = false;
     if (cache.device != dev) {
+      if (cache.done) (void)hipEventDestroy(cache.done);
+      cache.done = nullptr;
       cache.table.release();
       cache.work.release();
+      MI_HIP(hipEventCreateWithFlags(&cache.done, hipEventDisableTiming));
+      cache.device = dev;
     }
     build_batch(items, s, cache);
     cache.items = items;
     cache.tune_gen = gen;
-    cache.device = dev;
     cache.valid = true;
   }
-  cache.stream = s;
   size_t off = 0;
   for (const auto& l : cache.launches) {
     hipLaunchKernelGGL(kTiles[l.first].fn, dim3(l.second), dim3(256), 0, s,
@@ -363,6 +362,7 @@ void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s, 
     MI_LAUNCH_CHECK();
     off += l.second;
   }
+  MI_HIP(hipEventRecord(cache.done, s));
 }
 
 // Times every tile variant on a sizes x sizes product per size class and keeps

@@ -25,7 +25,8 @@ void minplus_gemm(const float* A, const float* B, float* C, int M, int N, int K,
 // next call: a call with the same products (pointers and sizes) on the same
 // device and tuning launches again without rebuilding or uploading anything.
 // Kernels only read these buffers, so reuse needs no synchronisation; a
-// rebuild first waits for the stream of the call that last used them.
+// rebuild first waits for the event recorded after the last launches that
+// read them (whatever stream those ran on).
 struct MinplusBatchCache {
   DevBuf<MinplusItem> table;
   DevBuf<int2> work;
@@ -33,8 +34,14 @@ struct MinplusBatchCache {
   std::vector<std::pair<int, uint32_t>> launches;  // (tile, workgroups), in work-list order
   uint64_t tune_gen = 0;
   int device = -1;
-  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;  // recorded after the last launches, on the device of `device`
   bool valid = false;
+  MinplusBatchCache() = default;
+  MinplusBatchCache(const MinplusBatchCache&) = delete;
+  MinplusBatchCache& operator=(const MinplusBatchCache&) = delete;
+  ~MinplusBatchCache() {
+    if (done) (void)hipEventDestroy(done);
+  }
 };
 
 void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s, MinplusBatchCache& cache);

@@ -75,6 +75,43 @@ def test_minplus_gemm_batched_ragged_classes():
         M.minplus_tuning(saved)
 
 
+def test_minplus_gemm_batched_repeated_calls():
+    """The same products twice (the uploaded plan is reused) with new data in
+    the same buffers in between, then a different, smaller batch right after
+    (the plan is rebuilt), then the first batch again."""
+    import ctypes as C
+    rng = np.random.default_rng(17)
+    eng = M.engine()
+    sizes = [40, 300, 77, 129, 512, 64]
+
+    def batch(ns):
+        mats = []
+        for n in ns:
+            a, b = rng.random((n, n), dtype=np.float32), rng.random((n, n), dtype=np.float32)
+            mats.append([n, a, b, M.DeviceBuffer.from_array(a), M.DeviceBuffer.from_array(b),
+                         M.DeviceBuffer(4 * n * n)])
+        return mats
+
+    def run(mats):
+        cnt = len(mats)
+        ptr = lambda j: (C.c_void_p * cnt)(*[m[j].ptr.value for m in mats])  # noqa: E731
+        d = (C.c_int * cnt)(*[m[0] for m in mats])
+        M._abi.check(eng.mi355_minplus_gemm_f32_batched(cnt, ptr(3), ptr(4), ptr(5), d, d, d, d, d, d, None))
+        M._abi.check(eng.mi355_stream_sync(None))
+        for n, a, b, _, _, dc in mats:
+            want = orc.minplus_gemm(a.ravel(), b.ravel(), n, n, n)
+            assert dc.to_array(np.float32, n * n).tobytes() == want.tobytes(), n
+
+    first = batch(sizes)
+    run(first)
+    for m in first:  # new values, same buffers and sizes
+        m[1] = rng.random((m[0], m[0]), dtype=np.float32)
+        M._abi.check(eng.mi355_memcpy_h2d(m[3].ptr, m[1].ctypes.data, m[1].nbytes, None))
+    run(first)
+    run(batch([33, 90]))
+    run(first)
+
+
 def _rand_csr(rng, rows, cols, p):
     dense = np.where(rng.random((rows, cols)) < p, rng.random((rows, cols)).astype(np.float32), np.inf)
     rp, col, val = [0], [], []
