@@ -3,6 +3,7 @@ reference's tiny .gr fixture (GPU): output lines, INF, the fd 3 summary."""
 import json
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -75,3 +76,13 @@ def test_runs_env_and_stdout_output(tmp_path):
     assert len(fd3.read_text().splitlines()) == 1
     r = _run(["-s", "7", path])
     assert r.returncode == 1 and "outside" in r.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus above 1 exits with a message before
+    touching a device."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "single MI355X" in r.stderr and r.stdout == ""
