@@ -389,7 +389,8 @@ typedef struct {
 
 typedef struct {
   int32_t rounds_per_graph;    /* rounds per captured graph (even; 0 = 24) */
-  int32_t blocks_per_cu;       /* relaxation workgroups per CU (0 = 4) */
+  int32_t blocks_per_cu;       /* relaxation workgroups per CU (0 = 4; the total is rounded
+                                  up to a multiple of 8, one share per XCD part) */
   int32_t use_graph;           /* 1: replay a captured HIP graph, 0: plain launches */
   int32_t inf_mode;            /* MI355_INF_IEEE or MI355_INF_FLT_MAX (f32 outputs) */
   int32_t count_edges;         /* count edge visits (stats) */
@@ -397,7 +398,8 @@ typedef struct {
   float delta_scale;           /* near-far bucket width = delta_scale x the graph's heuristic
                                   delta0 (0 = 1); negative: Bellman-Ford rounds (no far pile) */
   int32_t reserved;
-  int64_t queue_capacity;      /* frontier entries per queue (0 = auto) */
+  int64_t queue_capacity;      /* frontier entries per queue, all parts together, split
+                                  in proportion to the parts' areas (0 = auto) */
 } mi355_batch_options;
 
 typedef struct {
@@ -408,7 +410,7 @@ typedef struct {
   int32_t rounds_per_graph;
   int32_t far_passes;          /* rounds that moved the far pile (threshold advances) */
   int64_t edges_relaxed;       /* when count_edges */
-  int64_t queue_capacity;
+  int64_t queue_capacity;      /* the capacity asked for (or chosen), before the split */
   float device_ms;             /* initialisation + all rounds (hipEvent) */
   float reserved_f;
 } mi355_batch_stats;
