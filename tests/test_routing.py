@@ -195,3 +195,56 @@ def test_route_rejects_a_sidecar_of_another_graph(tmp_path):
         R.route(small, meta, dg=object())
     with pytest.raises(ValueError):  # outgoing orientation: edge records would not line up
         R.route(g.transpose(), meta, dg=object())
+
+
+def _oracle_run(out):
+    """A stand-in for the batched engine in CPU tests: the oracle's
+    multi-source Dijkstra and canonical parents for every query."""
+    from oracle import orc
+
+    def run(dg, qs, want_pred=True, download=True, **kw):
+        dist, pred, off = [], [], [0]
+        for q in range(qs.count):
+            box = tuple(int(x) for x in qs.boxes[q])
+            srcs = qs.query_sources(q)
+            d = orc.dijkstra_box(out.rowptr, out.col, out.weights, qs.grid_cols, box, srcs)
+            dist.append(d)
+            pred.append(orc.canonical_pred_box(out.rowptr, out.col, out.weights, qs.grid_cols, box, srcs, d))
+            off.append(off[-1] + len(d))
+        return np.concatenate(dist), np.concatenate(pred), np.array(off), None
+    return run
+
+
+def test_route_paths_and_pips_on_the_cpu(tmp_path, monkeypatch):
+    """route()'s host side (chunking, path walk through the parents, edge
+    lookup, PIP records, names) with the oracle standing in for the device."""
+    g, meta = device_fixture(tmp_path, n=600, m=3000, nets=25, seed=11)
+    out = g.transpose()
+    monkeypatch.setattr(R.B, "run", _oracle_run(out))
+    nets = R.route(g, meta, dg=object())
+    chunked = R.route(g, meta, dg=object(), max_label_slots=4 * g.n)
+    bare = R.route(g, meta, dg=object(), resolve_names=False)
+    assert [n.request for n in nets] == [q for q in range(meta.request_count) if len(meta.request_sources(q))]
+    rowp, col = g.rowptr, g.col
+    reached = unreached = 0
+    for net, net2, net3 in zip(nets, chunked, bare):
+        assert net.name == meta.string(meta.net_string[net.request]) and net3.name == ""
+        assert np.array_equal(net.sources, np.unique(meta.request_sources(net.request)))
+        assert [s.node for s in net.sinks] == [int(x) for x in meta.request_sinks(net.request)[:, 0]]
+        for sk, sk2, sk3 in zip(net.sinks, net2.sinks, net3.sinks):
+            assert np.array_equal(sk.nodes, sk2.nodes) and np.array_equal(sk.edges, sk3.edges)
+            assert sk.pips == sk2.pips and sk3.pips == [] and sk3.site == ""
+            if not np.isfinite(sk.dist):
+                unreached += 1
+                assert len(sk.nodes) == 0 and len(sk.edges) == 0
+                continue
+            reached += 1
+            assert sk.nodes[0] in net.sources and sk.nodes[-1] == sk.node
+            assert len(sk.edges) == len(sk.nodes) - 1 == sk.dist  # unit weights
+            for i, e in enumerate(sk.edges):
+                u, v = int(sk.nodes[i]), int(sk.nodes[i + 1])
+                assert rowp[v] <= e < rowp[v + 1] and col[e] == u
+                pd = int(meta.edge_pip[e])
+                assert sk.pips[i] == (meta.string(meta.edge_tile[e]), meta.string(meta.pip_wire0[pd]),
+                                      meta.string(meta.pip_wire1[pd]), bool(meta.pip_forward[pd]))
+    assert reached > 20 and unreached > 0, (reached, unreached)
