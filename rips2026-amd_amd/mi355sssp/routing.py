@@ -133,7 +133,9 @@ def route(graph, meta: RouteMetadata, dg=None, max_label_slots: int = 1 << 30,
     """Route every request of `meta` on `graph` (the RIPSCSR1 host graph the
     sidecar belongs to).  dg: an existing DeviceGraph of graph (built when
     None).  Nets go to the batched engine in chunks of at most
-    max_label_slots labels (n per net)."""
+    max_label_slots labels (n per net).  The host side walks the parents of
+    all sinks of a chunk together and looks every hop's edge up in one
+    sorted search; names are resolved once per distinct string."""
     from .device import DeviceGraph
 
     n = int(graph.n)
@@ -147,8 +149,15 @@ def route(graph, meta: RouteMetadata, dg=None, max_label_slots: int = 1 << 30,
         raise ValueError("route() needs the RIPSCSR1 graph in incoming orientation")
     if dg is None:
         dg = DeviceGraph(graph)
-    rowptr, col = graph.rowptr, graph.col
-    keys, kidx = _edge_index(rowptr, col)
+    keys, kidx = _edge_index(graph.rowptr, graph.col)
+    names: dict[int, str] = {}
+
+    def name(i) -> str:
+        i = int(i)
+        if i not in names:
+            names[i] = meta.string(i)
+        return names[i]
+
     per_chunk = max(1, min(65535, max_label_slots // max(n, 1)))
     routed: list[RoutedNet] = []
     todo = [q for q in range(meta.request_count) if len(meta.request_sources(q)) > 0]
@@ -160,37 +169,67 @@ def route(graph, meta: RouteMetadata, dg=None, max_label_slots: int = 1 << 30,
         boxes = np.tile(np.array([0, 0, 1, n], np.int32), (len(chunk), 1))
         qs = B.Queries(n, off, np.concatenate(srcs), boxes)
         dist, pred, offs, _ = B.run(dg, qs, want_pred=True, **batch_options)
+        # every sink of the chunk: its query's block and node
+        sinks = [meta.request_sinks(q) for q in chunk]
+        counts = np.array([len(s) for s in sinks], np.int64)
+        rec = np.concatenate(sinks) if counts.sum() else np.zeros((0, 3), np.uint64)
+        node = rec[:, 0].astype(np.int64)
+        block = np.repeat(np.asarray(offs[:-1], np.int64), counts)
+        d = dist[block + node]
+        reached = np.isfinite(d)
+        # walk all reached sinks' parents together: (sink, node) per step
+        walk_sink, walk_node = [], []
+        act = np.flatnonzero(reached)
+        cur = node[act]
+        for _ in range(n + 1):
+            if len(act) == 0:
+                break
+            walk_sink.append(act)
+            walk_node.append(cur)
+            nxt = pred[block[act] + cur].astype(np.int64)
+            keep = nxt >= 0
+            act, cur = act[keep], nxt[keep]
+        if len(act):
+            q = chunk[int(np.searchsorted(np.cumsum(counts), act[0], side="right"))]
+            raise RuntimeError(f"net {q}: parent cycle at sink {int(node[act[0]])}")
+        if walk_sink:
+            ws, wn = np.concatenate(walk_sink), np.concatenate(walk_node)
+            step = np.concatenate([np.full(len(x), t, np.int64) for t, x in enumerate(walk_sink)])
+            # per sink, source first: sink ascending, step descending
+            order = np.lexsort((-step, ws))
+            ws, wn = ws[order], wn[order]
+        else:
+            ws = wn = np.zeros(0, np.int64)
+        bounds = np.searchsorted(ws, np.arange(len(node) + 1))
+        # every hop u -> v (row v, column u) in one sorted search
+        same = np.zeros(len(ws), bool)
+        if len(ws) > 1:
+            same[1:] = ws[1:] == ws[:-1]
+        hop_v, hop_u = wn[same], wn[np.flatnonzero(same) - 1]
+        want = hop_v * n + hop_u
+        pos = np.searchsorted(keys, want)
+        if np.any(pos >= len(keys)) or np.any(keys[np.minimum(pos, len(keys) - 1)] != want):
+            bad = int(ws[same][np.flatnonzero((pos >= len(keys)) | (keys[np.minimum(pos, len(keys) - 1)] != want))[0]])
+            q = chunk[int(np.searchsorted(np.cumsum(counts), bad, side="right"))]
+            raise RuntimeError(f"net {q}: path edge missing from the graph")
+        hop_e = kidx[pos].astype(np.int64)
+        hop_bounds = np.searchsorted(ws[same], np.arange(len(node) + 1))
+        k = 0
         for i, q in enumerate(chunk):
-            d, p = dist[offs[i]:offs[i + 1]], pred[offs[i]:offs[i + 1]]
-            sinks = []
-            for node, site, pin in meta.request_sinks(q):
-                node = int(node)
-                dv = float(d[node])
-                path = []
-                if np.isfinite(dv):
-                    v = node
-                    while v >= 0 and len(path) <= n:
-                        path.append(v)
-                        v = int(p[v])
-                    if len(path) > n:
-                        raise RuntimeError(f"net {q}: parent cycle at sink {node}")
-                path = np.array(path[::-1], np.int64)
-                if len(path) > 1:
-                    want = path[1:] * n + path[:-1]  # edges u -> v: row v, column u
-                    pos = np.searchsorted(keys, want)
-                    if np.any(pos >= len(keys)) or np.any(keys[np.minimum(pos, len(keys) - 1)] != want):
-                        raise RuntimeError(f"net {q}: path edge missing from the graph")
-                    eidx = kidx[pos].astype(np.int64)
-                else:
-                    eidx = np.zeros(0, np.int64)
+            out_sinks = []
+            for j in range(int(counts[i])):
+                _, site, pin = (int(x) for x in rec[k])
+                path = wn[bounds[k]:bounds[k + 1]].astype(np.int64)
+                eidx = hop_e[hop_bounds[k]:hop_bounds[k + 1]]
                 pips = []
                 if resolve_names:
                     for e in eidx:
                         pd = int(meta.edge_pip[e])
-                        pips.append((meta.string(meta.edge_tile[e]), meta.string(meta.pip_wire0[pd]),
-                                     meta.string(meta.pip_wire1[pd]), bool(meta.pip_forward[pd])))
-                sinks.append(RoutedSink(node, meta.string(site) if resolve_names else "",
-                                        meta.string(pin) if resolve_names else "", dv, path, eidx, pips))
-            routed.append(RoutedNet(q, meta.string(meta.net_string[q]) if resolve_names else "",
-                                    int(meta.net_logical[q]), srcs[i], sinks))
+                        pips.append((name(meta.edge_tile[e]), name(meta.pip_wire0[pd]), name(meta.pip_wire1[pd]),
+                                     bool(meta.pip_forward[pd])))
+                out_sinks.append(RoutedSink(int(node[k]), name(site) if resolve_names else "",
+                                            name(pin) if resolve_names else "", float(d[k]), path, eidx, pips))
+                k += 1
+            routed.append(RoutedNet(q, name(meta.net_string[q]) if resolve_names else "",
+                                    int(meta.net_logical[q]), srcs[i], out_sinks))
     return routed
