@@ -295,6 +295,15 @@ bool write_arr(FILE* f, const T* p, size_t n) {
   return n == 0 || std::fwrite(p, sizeof(T), n, f) == n;
 }
 
+// Bytes in the file (the position is restored); -1 when it cannot be sized.
+int64_t file_bytes(FILE* f) {
+  const long pos = std::ftell(f);
+  if (pos < 0 || std::fseek(f, 0, SEEK_END) != 0) return -1;
+  const long size = std::ftell(f);
+  if (std::fseek(f, pos, SEEK_SET) != 0) return -1;
+  return size;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -395,6 +404,14 @@ int mi355g_read_ripscsr1(const char* path, mi355g_graph** out) {
     if (h[1] != 1) return fail(MI355G_EFORMAT, "RIPSCSR1 orientation must be 1 (incoming)");
     if (h[7] != h[2] + 1 || h[8] != h[6] || h[9] != h[6])
       return fail(MI355G_EFORMAT, "inconsistent RIPSCSR1 array counts");
+    // the arrays must be in the file before anything is allocated for them
+    const int64_t size = file_bytes(f.f);
+    if (size < 0) return fail(MI355G_EIO, "cannot size the file");
+    const uint64_t avail = (uint64_t)size - 88u;  // header: magic + 10 words
+    if (h[7] > avail / 8 || h[6] > avail / 8 || 8 * h[7] + 8 * h[6] > avail)
+      return fail(MI355G_EFORMAT, "truncated RIPSCSR1 arrays");
+    if (h[3] > (uint64_t)std::numeric_limits<int32_t>::max() + 1u)
+      return fail(MI355G_EFORMAT, "RIPSCSR1 column count exceeds 32-bit indices");
     auto g = new mi355g_graph();
     g->rows = (int64_t)h[2];
     g->cols = (int64_t)h[3];
@@ -414,6 +431,16 @@ int mi355g_read_ripscsr1(const char* path, mi355g_graph** out) {
       delete g;
       return fail(MI355G_EFORMAT, "RIPSCSR1 rowptr must span [0, nnz]");
     }
+    for (int64_t r = 0; r < g->rows; ++r)
+      if (g->rowptr[r + 1] < g->rowptr[r]) {
+        delete g;
+        return fail(MI355G_EFORMAT, "RIPSCSR1 rowptr decreases at row " + std::to_string(r));
+      }
+    for (const int32_t c : g->col)
+      if (c < 0 || (int64_t)c >= g->cols) {
+        delete g;
+        return fail(MI355G_EFORMAT, "RIPSCSR1 column index out of range");
+      }
     *out = g;
     return 0;
   });
@@ -580,6 +607,10 @@ int mi355g_read_galois_gr(const char* path, mi355g_graph** out) {
         h[3] > (uint64_t)std::numeric_limits<int32_t>::max())
       return fail(MI355G_EFORMAT, ".gr graph exceeds 32-bit limits");
     const uint64_t n = h[2], m = h[3];
+    // the tables must be in the file before anything is allocated for them
+    const int64_t size = file_bytes(f.f);
+    if (size < 0) return fail(MI355G_EIO, "cannot size the file");
+    if (32 + 8 * n + 4 * m > (uint64_t)size) return fail(MI355G_EFORMAT, "truncated .gr tables");
     std::vector<uint64_t> ends(n);
     std::vector<uint32_t> dsts(m);
     if (!read_arr(f.f, ends.data(), n)) return fail(MI355G_EFORMAT, "truncated row-offset table");

@@ -122,6 +122,39 @@ def test_ripscsr1_roundtrip(tmp_path):
     assert (ver, orient, rows, cols) == (1, 1, 300, 300)
 
 
+def test_ripscsr1_reader_rejects_corrupt_files(tmp_path):
+    """Column indices, row offsets and array counts are checked (counts against
+    the file size, before anything is allocated)."""
+    g = M.chain_random(50, 3, 0.5, 2.0, M.W_F32, seed=9).transpose()
+    p = tmp_path / "g.csrbin"
+    g.write_ripscsr1(str(p))
+    good = p.read_bytes()
+    rows, nnz = 50, g.m
+    col_at = 8 + 80 + 8 * (rows + 1)
+    cases = {}
+    b = bytearray(good)
+    struct.pack_into("<i", b, col_at + 4 * 3, rows)  # column == cols
+    cases["column"] = b
+    b = bytearray(good)
+    struct.pack_into("<i", b, col_at, -1)
+    cases["negative column"] = b
+    b = bytearray(good)
+    _, second = struct.unpack_from("<2q", b, 8 + 80 + 8 * 10)
+    struct.pack_into("<q", b, 8 + 80 + 8 * 10, second + 1)  # rowptr[10] > rowptr[11]
+    cases["rowptr"] = b
+    b = bytearray(good)
+    for k, v in ((2, 1 << 40), (6, 1 << 40), (7, (1 << 40) + 1), (8, 1 << 40), (9, 1 << 40)):
+        struct.pack_into("<Q", b, 8 + 8 * k, v)  # counts far beyond the file
+    cases["huge counts"] = b
+    cases["short"] = bytearray(good[:-4])
+    for name, data in cases.items():
+        q = tmp_path / "bad.csrbin"
+        q.write_bytes(bytes(data))
+        with pytest.raises(M.Mi355Error):
+            M.read_ripscsr1(str(q))
+    assert nnz > 3
+
+
 def test_galois_gr_fixtures(tmp_path):
     ka = bf_cases.known_answers()
     t = ka["tiny_gr"]
