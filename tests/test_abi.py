@@ -3,6 +3,7 @@
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -51,3 +52,40 @@ def test_compat_headers_compile_against_the_abi(tmp_path):
                         "-I", os.path.join(REPO, "include"), "-I", compat, str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+STRUCTS = [("CsrDesc", "mi355_csr_desc", "mi355sssp.h"), ("GraphInfo", "mi355_graph_info", "mi355sssp.h"),
+           ("SsspOptions", "mi355_sssp_options", "mi355sssp.h"), ("SsspStats", "mi355_sssp_stats", "mi355sssp.h"),
+           ("MlpParams", "mi355_mlp_params", "mi355sssp.h"),
+           ("MlpTrainOptions", "mi355_mlp_train_options", "mi355sssp.h"),
+           ("QueryBatch", "mi355_query_batch", "mi355sssp.h"), ("BatchOptions", "mi355_batch_options", "mi355sssp.h"),
+           ("BatchStats", "mi355_batch_stats", "mi355sssp.h"), ("BfProgress", "mi355_bf_progress", "mi355sssp.h"),
+           ("BfCsrStats", "mi355_bf_csr_stats", "mi355sssp.h"), ("OneOptions", "mi355_one_options", "mi355sssp.h"),
+           ("OneResult", "mi355_one_result", "mi355sssp.h"), ("GView", "mi355g_view_t", "mi355graph.h"),
+           ("IfmHeader", "mi355g_ifm_header_t", "mi355graph.h")]
+
+
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    """Every ctypes mirror has the C struct's size and, field by field (same
+    names), the C offsets: compiled and printed by the host C compiler."""
+    import ctypes as C
+    from mi355sssp import _abi
+
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "mi355sssp.h"', '#include "mi355graph.h"',
+             "int main(void) {"]
+    want = []
+    for py, cname, _ in STRUCTS:
+        cls = getattr(_abi, py)
+        lines.append(f'  printf("%zu\\n", sizeof({cname}));')
+        want.append(C.sizeof(cls))
+        for f in cls._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({cname}, {f[0]}));')
+            want.append(getattr(cls, f[0]).offset)
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == want
