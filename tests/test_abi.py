@@ -89,3 +89,35 @@ def test_ctypes_structs_match_the_c_layout(tmp_path):
     subprocess.run(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert got == want
+
+
+REF_KERNELS = "/root/reference/HIP_kernel"
+REF_DRIVERS = ["bellman_ford/tests/test_correctness.cpp", "bellman_ford/tests/test_check_intervals.cpp",
+               "bellman_ford/tests/big_test.cpp", "delta_stepping/testing/verify_bf_delta_vs_dijkstra.cpp",
+               "delta_stepping/testing/speed_compare_delta_vs_bf.cpp"]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_KERNELS), reason="reference sources are not on this machine")
+@pytest.mark.parametrize("driver", REF_DRIVERS)
+def test_reference_test_drivers_build_against_the_engine(tmp_path, driver):
+    """The reference's own test drivers, unchanged, compile and link against
+    the compatibility headers and libmi355sssp.so.  Each driver is linked (not
+    copied) into a scratch tree whose `src/` holds the compatibility headers,
+    so its `../src/*.hpp` includes resolve to them (the reference ships no
+    `bf_hip_CSR_device_utils.hpp`; test_check_intervals.cpp needs it)."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    compat = os.path.join(root, "include", "compat")
+    family = driver.split("/")[0]
+    src = tmp_path / family / "src"
+    src.mkdir(parents=True)
+    for h in os.listdir(compat):
+        (src / h).symlink_to(os.path.join(compat, h))
+    link = tmp_path / driver
+    link.parent.mkdir(parents=True, exist_ok=True)
+    link.symlink_to(os.path.join(REF_KERNELS, driver))
+    lib = os.path.join(root, "rips2026-amd_amd", "lib")
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-std=c++17", "-O0", "-I", compat,
+                        "-I", os.path.join(root, "include"), str(link), "-L", lib, "-lmi355sssp",
+                        "-o", str(tmp_path / "driver")], capture_output=True, text=True, timeout=600,
+                       cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr[-2000:]
