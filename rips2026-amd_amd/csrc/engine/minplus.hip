@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "minplus.hpp"
+#include "xcd_deal.hpp"
 
 namespace mi355 {
 
@@ -234,42 +235,6 @@ void launch_single(int tile, const MinplusItem* d_item, int M, int N, hipStream_
   hipLaunchKernelGGL(t.fn, grid, dim3(256), 0, s, d_item, static_cast<const int2*>(nullptr));
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin by workgroup id, so the
-// entry at workgroup id j * 8 + x runs on XCD x.  The tiles of one tile shape
-// are cut into pieces (runs of tiles of one product, at most 1/64 of the
-// launch's tiles each), the pieces go longest-first (a tile's work is its
-// product's K) to the XCD with the least work so far, and each XCD runs its
-// pieces in that order: the XCDs finish together, long tiles start first, and
-// a product's tiles share an XCD and its L2, which holds the product's A rows
-// and B columns.  Short XCD lists are padded with item -1.
-constexpr int kXcds = 8;
-
-struct TileRun {
-  int item, first, count, k;
-};
-
-void xcd_order(std::vector<TileRun>& runs, uint64_t total_tiles, std::vector<int2>& out) {
-  const int piece = (int)std::max<uint64_t>(1, (total_tiles + 63) / 64);
-  std::vector<TileRun> pieces;
-  for (const TileRun& r : runs)
-    for (int f = 0; f < r.count; f += piece) pieces.push_back(TileRun{r.item, r.first + f, std::min(piece, r.count - f), r.k});
-  std::stable_sort(pieces.begin(), pieces.end(), [](const TileRun& x, const TileRun& y) { return x.k > y.k; });
-  std::vector<std::vector<int2>> bins(kXcds);
-  uint64_t work[kXcds] = {};
-  for (const TileRun& r : pieces) {
-    int b = 0;
-    for (int x = 1; x < kXcds; ++x)
-      if (work[x] < work[b]) b = x;
-    work[b] += (uint64_t)r.count * (uint64_t)std::max(r.k, 1);
-    for (int t = 0; t < r.count; ++t) bins[b].push_back(int2{r.item, r.first + t});
-  }
-  size_t len = 0;
-  for (auto& bn : bins) len = std::max(len, bn.size());
-  out.assign(len * kXcds, int2{-1, 0});
-  for (int x = 0; x < kXcds; ++x)
-    for (size_t j = 0; j < bins[x].size(); ++j) out[j * kXcds + x] = bins[x][j];
-}
-
 }  // namespace
 
 int minplus_tile_count() { return kNumTiles; }
@@ -312,7 +277,10 @@ static void build_batch(const std::vector<MinplusItem>& items, hipStream_t s, Mi
   size_t total = 0;
   for (int t = 0; t < kNumTiles; ++t) {
     if (lists[t].empty()) continue;
-    xcd_order(lists[t], list_tiles[t], ordered[t]);
+    std::vector<TileSlot> dealt;
+    xcd_deal(lists[t], list_tiles[t], dealt);
+    ordered[t].resize(dealt.size());
+    for (size_t j = 0; j < dealt.size(); ++j) ordered[t][j] = int2{dealt[j].item, dealt[j].tile};
     // a launch holds at most 2^32 - 1 threads
     if (ordered[t].size() > (size_t)UINT32_MAX / 256) fail(MI355_EINVAL, "min-plus batch too large");
     total += ordered[t].size();
