@@ -1,11 +1,14 @@
 // Drop-in for HIP_kernel/faster_GPU_algo/Charlotte_oneshot/sssp_hip.hpp
 // (:1-78) over the mi355sssp C ABI.  adds_sssp_hip runs the engine's
-// bucket-synchronous delta-stepping with the same initial-delta heuristic
-// (16 * mean positive weight / mean degree, sssp_hip.cpp:59-81);
-// bellman_ford_sssp_hip runs in-place frontier Bellman-Ford and reports a
-// negative cycle reachable from the source the way its extra detection pass
+// persistent ADDS kernel with the same initial-delta heuristic
+// (16 * mean positive weight / mean degree, sssp_hip.cpp:59-81); its
+// work_count counts processed vertices (sssp_hip.cpp:286), and a bucket whose
+// live entries exceed bucket_capacity_per_bucket fails the call with
+// std::runtime_error (sssp_hip.cpp:548-556).  bellman_ford_sssp_hip runs
+// in-place frontier Bellman-Ford (work_count: edges relaxed, :196) and reports
+// a negative cycle reachable from the source the way its extra detection pass
 // does (sssp_hip.cpp:204-220).  Distances are exact (fp32 left fold);
-// iterations_used / work_count describe the engine's own schedule.
+// iterations_used describes the engine's own schedule.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -71,25 +74,35 @@ inline SSSPResult run_one(int algorithm, const CsrGraph& g, int source, const SS
 }  // namespace mi355_compat
 
 // dense adj[u*n + v] = w(u -> v) (outgoing) -> CSR; the diagonal and
-// non-finite entries are skipped, negative weights rejected.
+// non-finite entries are not edges, a negative off-diagonal weight is an
+// error (the contract of ONE/sssp_hip.cpp:320-348).  Two passes: the first
+// validates and sizes every row, the second fills the arrays in place.
 inline CsrGraph dense_to_csr_outgoing(const std::vector<float>& adjacency, int n) {
   if (n <= 0) throw std::invalid_argument("n must be positive");
-  if (adjacency.size() != (size_t)n * (size_t)n)
-    throw std::invalid_argument("adjacency must contain n*n row-major values");
+  const size_t nn = (size_t)n * (size_t)n;
+  if (adjacency.size() != nn) throw std::invalid_argument("adjacency must contain n*n row-major values");
+  auto is_edge = [&](size_t i) { return i % ((size_t)n + 1) != 0 && std::isfinite(adjacency[i]); };
   CsrGraph g;
   g.num_nodes = n;
-  g.row_offsets.assign((size_t)n + 1, 0);
-  for (int u = 0; u < n; ++u) {
-    for (int v = 0; v < n; ++v) {
-      const float w = adjacency[(size_t)u * n + v];
-      if (u == v || !std::isfinite(w)) continue;
-      if (w < 0.0f) throw std::invalid_argument("negative edge weight");
-      g.col_indices.push_back(v);
-      g.weights.push_back(w);
-    }
-    g.row_offsets[(size_t)u + 1] = (int)g.col_indices.size();
+  g.row_offsets.resize((size_t)n + 1);
+  size_t total = 0;
+  for (size_t i = 0; i < nn; ++i) {
+    if (i % (size_t)n == 0) g.row_offsets[i / (size_t)n] = (int)total;
+    if (!is_edge(i)) continue;
+    if (adjacency[i] < 0.0f) throw std::invalid_argument("negative edge weight in dense adjacency");
+    ++total;
   }
-  g.num_edges = (int)g.col_indices.size();
+  g.row_offsets[(size_t)n] = (int)total;
+  g.col_indices.resize(total);
+  g.weights.resize(total);
+  size_t k = 0;
+  for (size_t i = 0; i < nn; ++i) {
+    if (!is_edge(i)) continue;
+    g.col_indices[k] = (int)(i % (size_t)n);
+    g.weights[k] = adjacency[i];
+    ++k;
+  }
+  g.num_edges = (int)total;
   return g;
 }
 

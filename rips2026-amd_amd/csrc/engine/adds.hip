@@ -160,6 +160,9 @@ struct AddsArgs {
   int* ring;          // kAddsSlots * (cap_mask + 1)
   int* hring;         // heavy lists, same shape
   uint32_t cap_mask;
+  // most live entries one bucket ring may hold before the run aborts with a
+  // capacity error: the ring size, or a caller's smaller per-bucket capacity
+  uint32_t ring_limit;
   uint32_t* sstamp;   // heavy-list stamp: bucket + 1 of the list holding v, 0 none
   const int* split;   // light edges per vertex (weight-sorted rows)
   HubChunk* hub;      // hub_mask + 1 records
@@ -227,7 +230,7 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
       const uint32_t h = ld_rlx(&HEAD(c, lane));
       // live entries after the push; head may already be past reserved but
       // unwritten positions, so the difference is signed
-      if ((int)(pos0 + k - h) > (int)(a.cap_mask + 1u)) {
+      if ((int)(pos0 + k - h) > (int)a.ring_limit) {
         atomicOr(&c->err, kErrRing);
         atomicExch(&c->done, 1u);
       }
@@ -1042,6 +1045,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.ring = ws.adds_ring.p;
   a.hring = ws.adds_ring.p + (size_t)kAddsSlots * cap;
   a.cap_mask = cap - 1;
+  a.ring_limit = req.bucket_capacity > 0 && req.bucket_capacity < (int64_t)cap ? (uint32_t)req.bucket_capacity : cap;
   // light/heavy split at the starting bucket width (MLP mode: at the
   // network's first width, set below; one cached split per graph either way)
   a.split = req.use_mlp ? nullptr : light_split(g, delta, s);
@@ -1104,7 +1108,9 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
                  h.diag[3], h.diag[4], h.diag[5], h.diag[6], h.diag[7], h.diag[8] * 1e-5 / workers,
                  h.diag[9] * 1e-5 / workers, h.diag[10] * 1e-5 / workers, h.diag[11] * 1e-5 / workers,
                  h.diag[12] * 1e-5 / workers, h.diag[13] * 1e-5 / workers, h.diag[14] * 1e-5);
-  if (h.err & kErrRing) fail(MI355_ECAPACITY, "ADDS bucket queue overflow");
+  if (h.err & kErrRing)
+    fail(MI355_ECAPACITY, a.ring_limit < cap ? "ADDS bucket overflow: a bucket exceeded the requested per-bucket capacity"
+                                             : "ADDS bucket queue overflow");
   if (h.err & kErrHub) fail(MI355_ECAPACITY, "ADDS hub chunk queue overflow");
   if (h.err & kErrStall) fail(MI355_EDEVICE, "ADDS workers stalled (aborted by the spin bound)");
   res.iterations_used = h.phases;

@@ -598,11 +598,19 @@ int mi355_one_sssp_f32(int32_t algorithm, int n, int m, const int* row_offsets,
       req.max_iters = opt.max_iters > 0 ? opt.max_iters : -1;
       req.dynamic_delta = opt.dynamic_delta;
       req.want_work = true;
+      // a caller's per-bucket capacity bounds every bucket's live entries;
+      // exceeding it fails the call (ONE/sssp_hip.cpp:548-556)
+      req.bucket_capacity = opt.bucket_capacity_per_bucket > 0 ? opt.bucket_capacity_per_bucket : 0;
       mi355::run_sssp(*g, (uint32_t)source, req, res, s);
       download_dist(*g, (uint32_t)source, MI355_INF_IEEE, dist, s);
       out.iterations_used = (int)res.iterations_used;
       out.converged = res.converged ? 1 : 0;
       out.final_delta = res.final_delta;
+      // work_count counts processed vertices (ONE/sssp_hip.cpp:286)
+      out.milliseconds = res.device_ms;
+      out.work_count = (unsigned long long)res.vertices;
+      if (result) *result = out;
+      return;
     } else {
       fail(MI355_EINVAL, "algorithm must be MI355_ALG_BF or MI355_ALG_ADDS");
     }

@@ -22,6 +22,7 @@ struct RunRequest {
   // labels start from them (source forced to 0), only the source is active.
   const uint32_t* init_dist = nullptr;
   int active_buckets = 0;  // ADDS priority window (0: 1)
+  int64_t bucket_capacity = 0;  // ADDS: most live entries per bucket (0: the ring size)
   float* trace = nullptr;  // ADDS phase trace (device, trace_cap x kTraceStride)
   uint32_t trace_cap = 0;
 };
