@@ -91,13 +91,27 @@ typedef struct {
   int64_t large_chunks;     /* 2048-edge chunks those vertices split into */
   int32_t has_negative;     /* any negative weight */
   int32_t device;
+  int32_t vertex_order;     /* MI355_ORDER_INPUT or MI355_ORDER_DEGREE (the order used) */
+  int32_t reserved;
 } mi355_graph_info;
+
+/* Vertex order of a graph handle.  DEGREE renumbers the vertices internally
+ * by descending out-degree (ties keep input order) so that the labels of the
+ * vertices most edges point to share cache lines; sources, outputs and
+ * parents stay in input ids and results are identical.  AUTO picks DEGREE
+ * for graphs of at least 4096 vertices whose maximum degree is at least 32x
+ * the mean (R-MAT), INPUT otherwise (grids keep their geometry). */
+enum { MI355_ORDER_AUTO = 0, MI355_ORDER_INPUT = 1, MI355_ORDER_DEGREE = 2 };
 
 int mi355_graph_create(const mi355_csr_desc* desc, int32_t allow_negative, void* stream,
                        mi355_graph** out);
+/* Same with an explicit vertex order (mi355_graph_create uses MI355_ORDER_AUTO). */
+int mi355_graph_create_ex(const mi355_csr_desc* desc, int32_t allow_negative, int32_t vertex_order,
+                          void* stream, mi355_graph** out);
 int mi355_graph_destroy(mi355_graph* g);
 int mi355_graph_info_get(const mi355_graph* g, mi355_graph_info* info);
-/* Copy the engine's outgoing CSR back (int64 rowptr, int32 col, 32-bit w). */
+/* Copy the engine's outgoing CSR back (int64 rowptr, int32 col, 32-bit w), in
+ * the input vertex numbering. */
 int mi355_graph_download(const mi355_graph* g, int64_t* rowptr, int32_t* col, uint32_t* w);
 
 /* ---- single-source shortest paths on a graph handle -------------------- */

@@ -14,10 +14,10 @@
 //    base + floor((d - T_base) / delta), clamped into the window (far items
 //    are clipped into the last slot and re-queued when they come up) and
 //    never below the bucket of the vertex that produced it.
-//  * Workers.  Every workgroup is a worker: wave 0 claims up to 256 items
-//    (512 from a bucket holding 64K or more: fewer claims when work is
-//    plentiful, more claimers when it is scarce) and, 256 at a time,
-//    the four waves relax them, their edges packed across the workgroup's
+//  * Workers.  Every workgroup (8 waves) is a worker: wave 0 claims up to
+//    512 items (1024 from a bucket holding 64K or more: fewer claims when
+//    work is plentiful, more claimers when it is scarce) and, 512 at a time,
+//    the eight waves relax them, their edges packed across the workgroup's
 //    lanes (four edges per lane in flight); vertices with more than
 //    kHubDegree edges in the range being relaxed are cut into 8192-edge hub
 //    chunks any workgroup can claim (R-MAT scale 22: 2048-edge chunks cost
@@ -85,7 +85,12 @@ namespace mi355 {
 namespace {
 
 constexpr uint32_t kNoStamp = 0xFFFFFFFFu;
-constexpr uint32_t kClaim = 512;    // most items per workgroup claim (two passes of kBlock)
+// An ADDS workgroup is 8 waves: its claims are shared by more waves (more
+// memory accesses in flight per claimer) while the number of claimers, which
+// contend on the worklist's control words, stays that of 4-wave workgroups.
+constexpr int kAB = 512;            // threads per ADDS workgroup
+constexpr int kAW = kAB / kWave;    // waves per ADDS workgroup
+constexpr uint32_t kClaim = 2 * kAB;  // most items per workgroup claim (two passes of kAB)
 constexpr uint32_t kBigClaimAvail = 65536;  // slot size above which claims take kClaim
 constexpr int kHubChunk = 8192;     // edges per hub chunk
 constexpr int kHubDegree = 4096;  // rows with more edges become hub chunks
@@ -163,12 +168,14 @@ struct AddsArgs {
   // most live entries one bucket ring may hold before the run aborts with a
   // capacity error: the ring size, or a caller's smaller per-bucket capacity
   uint32_t ring_limit;
+  uint32_t local_keep;  // most vertices a workgroup keeps in its local queue (0: none)
   uint32_t* sstamp;   // heavy-list stamp: bucket + 1 of the list holding v, 0 none
   const int* split;   // light edges per vertex (weight-sorted rows)
   HubChunk* hub;      // hub_mask + 1 records
   uint32_t hub_mask;
   AddsCtrl* c;
   const AddsPolicy* policy;
+  const int* ids;     // input id of every internal vertex, the parent in labels (nullptr: the same)
   uint32_t n;
   int live_edges;     // keep the live edge counter (phase traces read it)
   // Upper bound on claims by one workgroup: far above any real run; reaching
@@ -192,13 +199,85 @@ __device__ inline void ring_store(const AddsArgs& a, uint32_t slot, uint32_t pos
   ring_put(a.ring + (size_t)slot * (a.cap_mask + 1u) + (pos & a.cap_mask), v, &a.c->err, &a.c->done);
 }
 
+// Workgroup-local queue (an LDS ring) of the bucket a claim holds.  Improved
+// vertices that fall into that bucket are processed by the same workgroup in
+// further passes of its claim instead of going through the global ring
+// (publish, then someone's claim): a chain of light edges then advances by
+// one pass per hop, not one publish-claim round trip per hop, which is what
+// bounds a high-diameter graph.  The entries stay inside the claim (pending
+// and the slot's inflight count already cover them); the stamp protocol is
+// the ring's.  Entries [free, head) are being read by the current pass,
+// [head, tail) are queued; appends need tail - free < kLocal.
+constexpr uint32_t kLocal = 4096;
+// Most vertices a workgroup keeps locally at a time: one pass.  Measured on
+// the 4900 x 4900 routing grid / R-MAT scale 22 (plain ADDS, kernel ms):
+// none 226 / 7.05, 32: 213 / 6.95, 128: 191 / 6.92, 512: 159 / 6.33, 4096:
+// 167 / 10.8 (with 4096 a few workgroups hoard R-MAT's ramp-up while the
+// others idle).
+constexpr uint32_t kLocalKeep = 512;
+constexpr uint32_t kNoLocal = 0xFFFFFFFFu;
+constexpr uint32_t kTakenLocal = 0xFFFFu;  // stage entry moved to the local queue
+static_assert((kLocal & (kLocal - 1)) == 0, "local queue is a power-of-two ring");
+struct LocalQueue {
+  int* q;          // kLocal entries
+  uint32_t* free;  // LDS counters (monotone)
+  uint32_t* head;
+  uint32_t* tail;
+  uint32_t slot;   // ring slot of the held bucket; kNoLocal: keep nothing local
+  uint32_t keep;   // most entries queued locally at a time (<= kLocal)
+};
+
 // Per-wave push stage in LDS: vertex and target slot of each queued entry.
 struct PushStage {
   int* v;          // kPushStage
   uint32_t* slot;  // kPushStage; reused for the entry offset during a flush
   uint32_t* hist;  // kAddsSlots counters
   uint32_t n;      // wave-uniform
+  LocalQueue lq;   // where entries of the held bucket go first
 };
+
+// Move the staged entries of the held bucket to the local queue (as many as
+// it has room for); returns how many moved (wave-uniform).
+__device__ inline uint32_t stage_to_local(PushStage& st) {
+  const LocalQueue& lq = st.lq;
+  const unsigned lane = lane_id();
+  uint32_t want = 0;
+  for (uint32_t i0 = 0; i0 < st.n; i0 += kWave) {
+    const uint32_t i = i0 + lane;
+    want += (uint32_t)__popcll(__ballot(i < st.n && st.slot[i] == lq.slot));
+  }
+  if (want == 0) return 0;
+  uint32_t pos0 = 0, take = 0;
+  if (lane == 0) {
+    uint32_t t = __hip_atomic_load(lq.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+      const uint32_t used = t - __hip_atomic_load(lq.free, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      take = used < lq.keep ? min(want, lq.keep - used) : 0u;
+      if (take == 0) break;
+      const uint32_t seen = atomicCAS(lq.tail, t, t + take);
+      if (seen == t) break;
+      t = seen;
+    }
+    pos0 = t;
+  }
+  take = __shfl(take, 0, kWave);
+  pos0 = __shfl(pos0, 0, kWave);
+  if (take == 0) return 0;
+  uint32_t rank = 0;
+  for (uint32_t i0 = 0; i0 < st.n; i0 += kWave) {
+    const uint32_t i = i0 + lane;
+    const bool mine = i < st.n && st.slot[i] == lq.slot;
+    const unsigned long long m = __ballot(mine);
+    const uint32_t r = rank + prefix_count(m);
+    if (mine && r < take) {
+      lq.q[(pos0 + r) & (kLocal - 1u)] = st.v[i];
+      st.slot[i] = kTakenLocal;
+    }
+    rank += (uint32_t)__popcll(m);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  return take;
+}
 
 __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   if (st.n == 0) return;
@@ -207,16 +286,21 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // count first: pending covers the entries before anyone can claim them
+  const uint32_t n_local = st.lq.slot != kNoLocal ? stage_to_local(st) : 0u;
+  if (n_local == st.n) {
+    st.n = 0;
+    return;
+  }
+  __builtin_amdgcn_wave_barrier();
   // pending covers the entries before they are offered (the wait before the
   // avail update below); the producer's own claim keeps pending above zero
   // meanwhile
-  if (lane == 0) atomicAdd(&c->pending, (unsigned long long)st.n);
+  if (lane == 0) atomicAdd(&c->pending, (unsigned long long)(st.n - n_local));
   if (lane < (unsigned)kAddsSlots) st.hist[lane] = 0;
   __builtin_amdgcn_wave_barrier();
   for (uint32_t i = lane; i < st.n; i += kWave) {
     const uint32_t s = st.slot[i];
-    st.slot[i] = (s << 16) | atomicAdd(st.hist + s, 1u);  // LDS atomic: offset in slot group
+    if (s != kTakenLocal) st.slot[i] = (s << 16) | atomicAdd(st.hist + s, 1u);  // LDS atomic: offset in slot group
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -242,6 +326,7 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   for (uint32_t i = lane; i < st.n; i += kWave) {
     const uint32_t so = st.slot[i];
+    if (so == kTakenLocal) continue;
     const uint32_t s = so >> 16;
     ring_store(a, s, st.hist[s] + (so & 0xFFFFu), st.v[i]);
   }
@@ -449,7 +534,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   if (mode == kPolicyMlp || (a.policy && a.policy->trace))
     mlp_features(a.policy, c, occ, window_items, delta, lane, &feat, &raw);
   if (mode == kPolicyHeuristic) {
-    const float capacity = (float)c->workers * (float)kBlock;
+    const float capacity = (float)c->workers * (float)kAB;
     if ((float)window_items < c->util_low * capacity) delta = fminf(delta * c->growth, c->max_delta);
     else if ((float)window_items > c->util_high * capacity) delta = fmaxf(delta * c->shrink, c->min_delta);
   } else if (mode == kPolicyMlp) {
@@ -480,12 +565,12 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   return true;
 }
 
-// Relax the packed edges of up to kBlock vertices held one per thread (deg = 0
-// for threads without work) and stage the improved targets; lo = producer
-// bucket.  The edge list is packed across the whole workgroup and its waves
+// Relax the packed edges of up to kAB vertices held one per thread (deg = 0
+// for threads without work) and stage the improved targets; u = the vertex's
+// input id (the parent written into the labels), lo = producer bucket.  The edge list is packed across the whole workgroup and its waves
 // take 128-edge steps of it in turn (two edges per lane), so one high-degree
 // vertex does not keep the other waves waiting at the next barrier.  The
-// arrays are kBlock entries in LDS; s_wsum holds kWavesPerBlock totals.
+// arrays are kAB entries in LDS; s_wsum holds kAW totals.
 template <class W>
 __device__ inline void relax_block(const AddsArgs& a, PushStage& st, int* s_incl, int* s_base,
                                    uint32_t* s_key, int* s_u, uint32_t* s_lo, int* s_wsum, int u,
@@ -504,7 +589,7 @@ __device__ inline void relax_block(const AddsArgs& a, PushStage& st, int* s_incl
   __syncthreads();
   int total = 0;
 #pragma unroll
-  for (int w = 0; w < kWavesPerBlock; ++w) {
+  for (int w = 0; w < kAW; ++w) {
     const int ws = s_wsum[w];
     if (w < (int)wv) incl += ws;
     total += ws;
@@ -520,8 +605,8 @@ __device__ inline void relax_block(const AddsArgs& a, PushStage& st, int* s_incl
   // edge j belongs to the vertex p with s_incl[p - 1] <= j < s_incl[p];
   // a wave takes 256-edge steps, four edges per lane (j, j + 64, j + 128,
   // j + 192)
-  static_assert(kBlock == 256, "binary search over 256 packed vertices");
-  for (int t = (int)wv * 4 * kWave; t < total; t += 4 * kBlock) {
+  static_assert((kAB & (kAB - 1)) == 0, "binary search over a power-of-two block");
+  for (int t = (int)wv * 4 * kWave; t < total; t += 4 * kAB) {
     int pos[4];
     bool ok[4];
 #pragma unroll
@@ -530,7 +615,7 @@ __device__ inline void relax_block(const AddsArgs& a, PushStage& st, int* s_incl
       ok[p] = t + p * kWave + (int)lane < total;
     }
 #pragma unroll
-    for (int s2 = kBlock / 2; s2 > 0; s2 >>= 1) {
+    for (int s2 = kAB / 2; s2 > 0; s2 >>= 1) {
 #pragma unroll
       for (int p = 0; p < 4; ++p)
         if (s_incl[pos[p] + s2 - 1] <= t + p * kWave + (int)lane) pos[p] += s2;
@@ -587,7 +672,7 @@ __device__ inline void make_hub_chunks(const AddsArgs& a, uint32_t u, uint32_t k
 // Claim from a (ring, head, avail) set through its counting semaphore while
 // holding inflight on slot `pin`; returns the count and sets *begin.
 __device__ inline uint32_t sem_claim(int* avail, uint32_t* head, int* infl, uint32_t* begin,
-                                     int want = (int)kBlock) {
+                                     int want = (int)kAB) {
   atomic_add_done(infl, 1);  // before the claim is visible
   const int old = atomic_add_done(avail, -want);
   const int k = min(old, want);
@@ -600,20 +685,32 @@ __device__ inline uint32_t sem_claim(int* avail, uint32_t* head, int* infl, uint
   return (uint32_t)k;
 }
 
-template <class W>
-__global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
+// kDiag: keep the per-workgroup counters and clocks of the MI355_ADDS_DEBUG
+// report (a separate instantiation, so the default kernel carries none of
+// their registers).
+#define DIAG(...)            \
+  do {                       \
+    if constexpr (kDiag) {   \
+      __VA_ARGS__;           \
+    }                        \
+  } while (0)
+
+template <class W, bool kDiag>
+__global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k_adds(AddsArgs a) {
   __shared__ uint32_t s_slot, s_qslot, s_begin, s_count, s_kind, s_done, s_base, s_active, s_hub_ok;
   __shared__ float s_tbase, s_inv;
-  __shared__ int s_incl[kWavesPerBlock][kWave];
-  __shared__ int s_beg[kWavesPerBlock][kWave];
-  __shared__ uint32_t s_key[kWavesPerBlock][kWave];
-  __shared__ int s_u[kWavesPerBlock][kWave];
-  __shared__ uint32_t s_lo[kWavesPerBlock][kWave];
-  __shared__ int s_wsum[kWavesPerBlock];
-  __shared__ int s_pv[kWavesPerBlock][kPushStage];
-  __shared__ uint32_t s_ps[kWavesPerBlock][kPushStage];
-  __shared__ uint32_t s_ph[kWavesPerBlock][kAddsSlots];
-  __shared__ int s_hv[kWavesPerBlock][kHeavyStage];
+  __shared__ int s_incl[kAW][kWave];
+  __shared__ int s_beg[kAW][kWave];
+  __shared__ uint32_t s_key[kAW][kWave];
+  __shared__ int s_u[kAW][kWave];
+  __shared__ uint32_t s_lo[kAW][kWave];
+  __shared__ int s_wsum[kAW];
+  __shared__ int s_pv[kAW][kPushStage];
+  __shared__ uint32_t s_ps[kAW][kPushStage];
+  __shared__ uint32_t s_ph[kAW][kAddsSlots];
+  __shared__ int s_hv[kAW][kHeavyStage];
+  __shared__ int s_lq[kLocal];
+  __shared__ uint32_t s_lfree, s_lhead, s_ltail, s_lnum, s_lbeg;
   AddsCtrl* c = a.c;
   const unsigned lane = threadIdx.x & 63;
   const unsigned wv = threadIdx.x >> 6;
@@ -622,7 +719,9 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0)
     __hip_atomic_store(&c->diag[15], (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
-  PushStage st{s_pv[wv], s_ps[wv], s_ph[wv], 0};
+  if (threadIdx.x == 0) s_lfree = s_lhead = s_ltail = 0;  // ordered by the first barrier below
+  PushStage st{s_pv[wv], s_ps[wv], s_ph[wv], 0,
+               LocalQueue{s_lq, &s_lfree, &s_lhead, &s_ltail, kNoLocal, min(a.local_keep, kLocal)}};
   uint32_t hv_n = 0;
   unsigned long long edges_local = 0, pushed_local = 0, items_local = 0;
   uint32_t idle = 0, backoff = 1;
@@ -635,7 +734,8 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
   bool dormant = false;
 
   for (;;) {
-    const long long t_top = wall_clock64();
+    long long t_top = 0;
+    DIAG(t_top = wall_clock64());
     if (wv == 0) {
       uint32_t got = 0, slot = 0, qslot = 0, begin = 0, kind = kClaimItems, done = 0;
       if (dormant) {
@@ -664,7 +764,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         const uint32_t A = max(1u, min((uint32_t)__shfl(probe, 3, 64), (uint32_t)kAddsSlots));
         const unsigned long long ready = __ballot(lane >= 4 && lane < 4 + A && probe > 0) >> 4;
         // slots holding many items are claimed kClaim at a time (fewer
-        // claims); otherwise kBlock, so that scarce work spreads over more
+        // claims); otherwise kAB, so that scarce work spreads over more
         // workgroups
         const unsigned long long big = __ballot(lane >= 4 && lane < 4 + A && probe >= (int)kBigClaimAvail) >> 4;
         // Workgroups beyond the policy's worker share do not claim: fewer
@@ -687,7 +787,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
               begin = atomicAdd(&c->hub_head, 1u);
             } else {
               atomicAdd(&c->hub_avail, 1);
-              ++d_cas;
+              DIAG(++d_cas);
             }
           }
           if (!got && base0 > 0 && hv_n0 > 0) {
@@ -701,18 +801,18 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
               slot = ps;
               qslot = hs;
             } else {
-              ++d_cas;
+              DIAG(++d_cas);
             }
           }
           for (unsigned long long m = ready; m && !got; m &= m - 1) {
             const uint32_t s = (base0 + (uint32_t)__ffsll((long long)m) - 1) % kAddsSlots;
-            const int want = (big >> ((uint32_t)__ffsll((long long)m) - 1)) & 1ull ? (int)kClaim : (int)kBlock;
+            const int want = (big >> ((uint32_t)__ffsll((long long)m) - 1)) & 1ull ? (int)kClaim : (int)kAB;
             got = sem_claim(&AVAIL(c, s), &HEAD(c, s), &INFL(c, s), &begin, want);
             if (got) {
               slot = s;
               qslot = s;
             } else {
-              ++d_cas;
+              DIAG(++d_cas);
             }
           }
         }
@@ -742,7 +842,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
             } else {
               // exponential back-off keeps idle workgroups from flooding the
               // control words that the busy ones update
-              ++d_idle;
+              DIAG(++d_idle);
               for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(8);
               if (backoff < 32) backoff <<= 1;
               if (++idle > kIdleLimit) {  // no progress for seconds: abort, do not hang
@@ -774,9 +874,8 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
     }
     __syncthreads();
     const uint32_t count = s_count;
-    const long long t_claimed = wall_clock64();
-    if (s_count) t_claim += t_claimed - t_top;
-    else t_idle += t_claimed - t_top;
+    long long t_claimed = 0;
+    DIAG(t_claimed = wall_clock64(); if (s_count) t_claim += t_claimed - t_top; else t_idle += t_claimed - t_top);
     if (s_done) break;
     if (count == 0) {
       __syncthreads();
@@ -806,8 +905,8 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       const uint32_t hu = ld_rlx(&hc->u), hkey = ld_rlx(&hc->key), hfirst = ld_rlx(&hc->first);
       const uint32_t hcnt = min(ld_rlx(&hc->cnt), (uint32_t)kHubChunk);
       const uint32_t hslot = ld_rlx(&hc->slot) % kAddsSlots, hbucket = ld_rlx(&hc->bucket);
-      // wave wv takes edges [wv * 256, wv * 256 + 256) of every 1024, four per lane
-      for (uint32_t t0 = wv * 4 * kWave; t0 < hcnt; t0 += 4 * kBlock) {
+      // wave wv takes edges [wv * 256, wv * 256 + 256) of every 4 * kAB, four per lane
+      for (uint32_t t0 = wv * 4 * kWave; t0 < hcnt; t0 += 4 * kAB) {
         uint32_t v[4], ck[4], uu[4], lo[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
@@ -823,29 +922,60 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       stage_publish(a, st);
       if (a.live_edges && wv == 0 && lane == 0) atomicAdd(&c->edges, (unsigned long long)hcnt);
       edges_local += hcnt;
-      d_hub += threadIdx.x == 0;
+      DIAG(d_hub += threadIdx.x == 0);
       __syncthreads();
       if (threadIdx.x == 0) {
         atomic_add_done(&INFL(c, hslot), -1);
         atomicAdd(&c->pending, ~0ull);
       }
-      t_hubs += wall_clock64() - t_claimed;
+      DIAG(t_hubs += wall_clock64() - t_claimed);
       continue;
     }
 
-    // items of a bucket ring or a heavy list, in passes of kBlock: in pass p
-    // wave w takes claim entries [p*256 + w*64, p*256 + w*64 + 64)
+    // items of a bucket ring or a heavy list, in passes of kAB: in pass p
+    // wave w takes claim entries [p*kAB + w*64, p*kAB + w*64 + 64)
     const bool heavy = kind == kClaimHeavy;
     const uint32_t slot = s_slot, begin = s_begin;
     int* ring = (heavy ? a.hring : a.ring) + (size_t)s_qslot * (a.cap_mask + 1u);
-    for (uint32_t pass = 0; pass * kBlock < count; ++pass) {
-      const uint32_t i = pass * kBlock + wv * kWave + lane;
+    // improved vertices of the held bucket stay in this workgroup (item
+    // claims), up to local_keep at a time; the rest go to the ring
+    st.lq.slot = (!heavy && a.local_keep > 0) ? slot : kNoLocal;
+    const uint32_t ring_passes = (count + kAB - 1) / kAB;
+    for (uint32_t pass = 0;; ++pass) {
+      // passes over the claimed ring entries, then over the local queue until
+      // it is empty
+      const bool from_ring = pass < ring_passes;
+      uint32_t local_n = 0, local_beg = 0;
+      if (!from_ring) {
+        __syncthreads();  // the previous pass has read its entries and appended
+        if (threadIdx.x == 0) {
+          const uint32_t h = s_lhead;
+          s_lfree = h;
+          s_lnum = min(s_ltail - h, (uint32_t)kAB);
+          s_lbeg = h;
+          s_lhead = h + s_lnum;
+        }
+        __syncthreads();
+        local_n = s_lnum;
+        local_beg = s_lbeg;
+        if (local_n == 0) break;
+      }
+      const uint32_t i = from_ring ? pass * kAB + wv * kWave + lane : wv * kWave + lane;
+      const bool has = from_ring ? i < count : i < local_n;
       int u = 0, beg = 0, deg = 0;
       uint32_t key = W::kInfKey, bk = 0;
       bool entry_ok = true;
       bool requeue = false, to_heavy = false;
       uint32_t want = 0;
-      if (i < count) {
+      if (has && !from_ring) {
+        u = s_lq[(local_beg + i) & (kLocal - 1u)];
+        entry_ok = u >= 0 && (uint32_t)u < a.n;
+        if (!entry_ok) {
+          atomicOr(&c->err, kErrRing);
+          atomicExch(&c->done, 1u);
+          u = 0;
+        }
+      } else if (has) {
         // The producer reserved this position before writing it: wait for the
         // write (bounded), then hand the entry back to the ring as empty.
         int* ep = ring + ((begin + i) & a.cap_mask);
@@ -853,7 +983,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
         while ((u = __hip_atomic_load(ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == kEmptyEntry &&
                ++spins < kSpinLimit)
           __builtin_amdgcn_s_sleep(1);
-        d_spin += spins;
+        DIAG(d_spin += spins);
         entry_ok = u != kEmptyEntry && u >= 0 && (uint32_t)u < a.n;
         if (!entry_ok) {
           atomicOr(&c->err, kErrRing);
@@ -865,13 +995,14 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       }
       // row bounds and light-prefix length of u depend only on u: issued now,
       // they arrive while the stamp exchange and the label read complete
-      int r_beg = 0, r_end = 0, r_split = 0;
-      if (i < count && entry_ok) {
+      int r_beg = 0, r_end = 0, r_split = 0, pu = u;
+      if (has && entry_ok) {
         r_beg = a.rowptr[u];
         r_end = a.rowptr[u + 1];
         r_split = a.split[u];
+        if (a.ids) pu = a.ids[u];
       }
-      if (i < count && entry_ok) {
+      if (has && entry_ok) {
         if (heavy) {
           // Clearing the stamp first means a later light pass of u (after an
           // improvement) lists u again, or this pass reads the label it set.
@@ -879,7 +1010,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           vm_wait();  // the label is read after the stamp is cleared
           key = atomic_load_key(a.labels + u);
           bk = base;
-          ++d_heavy;
+          DIAG(++d_heavy);
         } else {
           // Whoever takes the stamp processes the vertex.  (Putting back a
           // stamp that belongs to an entry in another ring would race with that
@@ -895,10 +1026,10 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
               // clipped or re-mapped far item: queue it where it belongs now
               key = W::kInfKey;
               requeue = true;
-              ++d_req;
+              DIAG(++d_req);
             }
           } else {
-            ++d_drop;
+            DIAG(++d_drop);
           }
         }
         if (key != W::kInfKey) {
@@ -917,7 +1048,7 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
           }
           deg = end - beg;
           if (deg > kHubDegree) {
-            make_hub_chunks(a, (uint32_t)u, key, beg, end, slot, bk);
+            make_hub_chunks(a, (uint32_t)pu, key, beg, end, slot, bk);
             deg = 0;
           }
         }
@@ -934,38 +1065,41 @@ __global__ __launch_bounds__(kBlock) void k_adds(AddsArgs a) {
       }
       // re-queued items go through the stage like any push
       push_improved(a, st, requeue, (uint32_t)u, want, pushed_local);
-      const long long t_r0 = wall_clock64();
+      long long t_r0 = 0, t_r1 = 0;
+      DIAG(t_r0 = wall_clock64());
       relax_block<W>(a, st, &s_incl[0][0], &s_beg[0][0], &s_key[0][0], &s_u[0][0], &s_lo[0][0], s_wsum,
-                     u, beg, deg, key, bk, base, tbase, inv, edges_local, pushed_local);
-      const long long t_r1 = wall_clock64();
+                     pu, beg, deg, key, bk, base, tbase, inv, edges_local, pushed_local);
+      DIAG(t_r1 = wall_clock64());
       stage_publish(a, st);
       heavy_publish(a, s_hv[wv], hv_n, slot);
-      t_relax += t_r1 - t_r0;
-      t_pub += wall_clock64() - t_r1;
+      DIAG(t_relax += t_r1 - t_r0; t_pub += wall_clock64() - t_r1);
     }
+    st.lq.slot = kNoLocal;
     __syncthreads();
     if (threadIdx.x == 0) {
       atomic_add_done(&INFL(c, slot), -1);
       atomicAdd(&c->pending, 0ull - (unsigned long long)count);
     }
-    t_items += wall_clock64() - t_claimed;
+    DIAG(t_items += wall_clock64() - t_claimed);
   }
-  if (threadIdx.x == 0) {
-    atomicAdd(&c->diag[8], t_claim);
-    atomicAdd(&c->diag[9], t_idle);
-    atomicAdd(&c->diag[10], t_items);
-    atomicAdd(&c->diag[11], t_hubs);
-    atomicAdd(&c->diag[12], t_pub);
-    atomicAdd(&c->diag[13], t_relax);
+  if constexpr (kDiag) {
+    if (threadIdx.x == 0) {
+      atomicAdd(&c->diag[8], t_claim);
+      atomicAdd(&c->diag[9], t_idle);
+      atomicAdd(&c->diag[10], t_items);
+      atomicAdd(&c->diag[11], t_hubs);
+      atomicAdd(&c->diag[12], t_pub);
+      atomicAdd(&c->diag[13], t_relax);
+    }
+    if (threadIdx.x == 0 && claims) atomicAdd(&c->diag[0], claims);
+    if (d_cas) atomicAdd(&c->diag[1], d_cas);
+    if (threadIdx.x == 0 && d_idle) atomicAdd(&c->diag[2], d_idle);
+    if (d_spin) atomicAdd(&c->diag[3], d_spin);
+    if (d_hub) atomicAdd(&c->diag[4], d_hub);
+    if (d_drop) atomicAdd(&c->diag[5], d_drop);
+    if (d_req) atomicAdd(&c->diag[6], d_req);
+    if (d_heavy) atomicAdd(&c->diag[7], d_heavy);
   }
-  if (threadIdx.x == 0 && claims) atomicAdd(&c->diag[0], claims);
-  if (d_cas) atomicAdd(&c->diag[1], d_cas);
-  if (threadIdx.x == 0 && d_idle) atomicAdd(&c->diag[2], d_idle);
-  if (d_spin) atomicAdd(&c->diag[3], d_spin);
-  if (d_hub) atomicAdd(&c->diag[4], d_hub);
-  if (d_drop) atomicAdd(&c->diag[5], d_drop);
-  if (d_req) atomicAdd(&c->diag[6], d_req);
-  if (d_heavy) atomicAdd(&c->diag[7], d_heavy);
   if (lane == 0) {
     if (!a.live_edges && edges_local) atomicAdd(&c->edges, edges_local);
     if (pushed_local) atomicAdd(&c->pushes, pushed_local);
@@ -981,7 +1115,7 @@ __global__ void k_adds_init(AddsArgs a, int64_t n, uint32_t src, const uint32_t*
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t v = tid; v < n; v += stride) {
     unsigned long long l = inf_label;
-    if (init) l = ((unsigned long long)W::output_to_key(init[v]) << 32) | kNoPred;
+    if (init) l = ((unsigned long long)W::output_to_key(init[a.ids ? a.ids[v] : v]) << 32) | kNoPred;
     a.labels[v] = v == (int64_t)src ? (((unsigned long long)W::kZeroKey << 32) | kNoPred) : l;
     a.qstamp[v] = v == (int64_t)src ? 0u : kNoStamp;
     a.sstamp[v] = 0u;
@@ -1045,7 +1179,10 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.ring = ws.adds_ring.p;
   a.hring = ws.adds_ring.p + (size_t)kAddsSlots * cap;
   a.cap_mask = cap - 1;
+  // a caller's per-bucket capacity counts every queued entry: with one, all
+  // entries go through the rings
   a.ring_limit = req.bucket_capacity > 0 && req.bucket_capacity < (int64_t)cap ? (uint32_t)req.bucket_capacity : cap;
+  a.local_keep = a.ring_limit < cap ? 0u : kLocalKeep;
   // light/heavy split at the starting bucket width (MLP mode: at the
   // network's first width, set below; one cached split per graph either way)
   a.split = req.use_mlp ? nullptr : light_split(g, delta, s);
@@ -1053,6 +1190,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.hub_mask = hub_cap - 1;
   a.c = c;
   a.policy = d_policy;
+  a.ids = input_ids(g);
   a.n = (uint32_t)n;
   a.live_edges = req.trace ? 1 : 0;
   a.claim_limit = 16ull * (unsigned long long)(n + g.m) + (1ull << 20);
@@ -1086,7 +1224,8 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
     a.split = light_split(g, first_delta > 0.0f && std::isfinite(first_delta) ? first_delta : delta, s);
   }
   MI_HIP(hipEventRecord(evk, s));
-  k_adds<W><<<workers, kBlock, 0, s>>>(a);
+  if (std::getenv("MI355_ADDS_DEBUG")) k_adds<W, true><<<workers, kAB, 0, s>>>(a);
+  else k_adds<W, false><<<workers, kAB, 0, s>>>(a);
   MI_LAUNCH_CHECK();
   MI_HIP(hipEventRecord(ev1, s));
   MI_HIP(hipEventSynchronize(ev1));

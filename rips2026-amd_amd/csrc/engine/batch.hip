@@ -67,20 +67,30 @@ __host__ __device__ constexpr int cidx(uint32_t parity, uint32_t part) {
   return (int)((parity * kParts + part) * kCntStride);
 }
 
-// Local slot of rectangle cell (dr, dc) in a block of width bw: 4 x 4 tiles,
-// tile-major, row-major inside the tile.
-__host__ __device__ inline uint32_t tiled_slot(uint32_t dr, uint32_t dc, uint32_t bw) {
+// Local slot of rectangle cell (dr, dc) in a bh x bw block: 4 x 4 tiles,
+// tile-major, row-major inside the tile; a block less than 4 rows high (a
+// 1 x n "rectangle" is how a general graph's query spans all its vertices)
+// is plain row-major, so it carries no tile padding.
+__host__ __device__ inline uint32_t tiled_slot(uint32_t dr, uint32_t dc, uint32_t bh, uint32_t bw) {
+  if (bh < 4u) return dr * bw + dc;
   const uint32_t tw = (bw + 3u) >> 2;
   return ((((dr >> 2) * tw) + (dc >> 2)) << 4) | ((dr & 3u) << 2) | (dc & 3u);
 }
-__host__ __device__ inline void tiled_cell(uint32_t l, uint32_t bw, uint32_t& dr, uint32_t& dc) {
+__host__ __device__ inline void tiled_cell(uint32_t l, uint32_t bh, uint32_t bw, uint32_t& dr, uint32_t& dc) {
+  if (bh < 4u) {
+    dr = l / bw;
+    dc = l - dr * bw;
+    return;
+  }
   const uint32_t tw = (bw + 3u) >> 2;
   const uint32_t t = l >> 4, tr = t / tw;
   dr = (tr << 2) | ((l >> 2) & 3u);
   dc = ((t - tr * tw) << 2) | (l & 3u);
 }
-// Slots of a bh x bw block (padded to whole tiles).
-__host__ __device__ inline int64_t tiled_area(int64_t bh, int64_t bw) { return ((bh + 3) & ~3LL) * ((bw + 3) & ~3LL); }
+// Slots of a bh x bw block (padded to whole tiles when tiled).
+__host__ __device__ inline int64_t tiled_area(int64_t bh, int64_t bw) {
+  return bh < 4 ? bh * bw : ((bh + 3) & ~3LL) * ((bw + 3) & ~3LL);
+}
 constexpr uint32_t kRoundIdle = 0, kRoundRelax = 1, kRoundFar = 2;
 constexpr uint32_t kNearStage = 256;  // staged near appends per wave
 constexpr uint32_t kFarStage = 128;   // staged far appends per wave
@@ -276,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
         qid = (int)it.x;
         const QueryInfo qi = a.qinfo[qid];
         uint32_t r, col;
-        tiled_cell(it.y, (uint32_t)qi.bw, r, col);
+        tiled_cell(it.y, (uint32_t)qi.bh, (uint32_t)qi.bw, r, col);
         u = (qi.r0 + (int)r) * a.cols + qi.c0 + (int)col;
         key = atomic_load_key(a.labels + qi.base + it.y);
         beg = a.rowptr[u];
@@ -337,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
           const int rv = (int)e[p].x / a.cols, cv = (int)e[p].x - rv * a.cols;
           const uint32_t dr = (uint32_t)(rv - qi[p].r0), dc = (uint32_t)(cv - qi[p].c0);
           const bool in = ok[p] && dr < (uint32_t)qi[p].bh && dc < (uint32_t)qi[p].bw;
-          lv[p] = in ? tiled_slot(dr, dc, (uint32_t)qi[p].bw) : 0u;
+          lv[p] = in ? tiled_slot(dr, dc, (uint32_t)qi[p].bh, (uint32_t)qi[p].bw) : 0u;
           slot[p] = in ? qi[p].base + lv[p] : 0;
           ck[p] = in ? W::relax(s_key[wv][pos[p]], e[p].y) : W::kInfKey;
           cand[p] = ck[p] != W::kInfKey ? ((unsigned long long)ck[p] << 32) | (uint32_t)s_u[wv][pos[p]] : kNone;
@@ -461,7 +471,7 @@ __global__ void k_batch_output(const unsigned long long* __restrict__ labels, co
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < area; i += stride) {
     const uint32_t dr = (uint32_t)(i / qi.bw), dc = (uint32_t)(i - (int64_t)dr * qi.bw);
-    const unsigned long long l = labels[qi.base + tiled_slot(dr, dc, (uint32_t)qi.bw)];
+    const unsigned long long l = labels[qi.base + tiled_slot(dr, dc, (uint32_t)qi.bh, (uint32_t)qi.bw)];
     const uint32_t key = (uint32_t)(l >> 32);
     const bool reached = key != W::kInfKey;
     if (dist) {
@@ -483,7 +493,7 @@ __global__ void k_batch_source_preds(const QueryInfo* __restrict__ qinfo, const 
   if (i < nsrc && pred) {
     const QueryInfo qi = qinfo[src[i].x];
     uint32_t dr, dc;
-    tiled_cell(src[i].y, (uint32_t)qi.bw, dr, dc);
+    tiled_cell(src[i].y, (uint32_t)qi.bh, (uint32_t)qi.bw, dr, dc);
     pred[qi.out_base + (int64_t)dr * qi.bw + dc] = -1;
   }
 }
@@ -565,7 +575,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
       const int64_t v = qb.sources[i];
       const int64_t r = v / qb.grid_cols - b[0], c = v % qb.grid_cols - b[1];
       psrc[(size_t)part[(size_t)q]].push_back(
-          make_uint2((uint32_t)q, tiled_slot((uint32_t)r, (uint32_t)c, (uint32_t)(b[3] - b[1]))));
+          make_uint2((uint32_t)q, tiled_slot((uint32_t)r, (uint32_t)c, (uint32_t)(b[2] - b[0]), (uint32_t)(b[3] - b[1]))));
     }
   }
   std::vector<uint2> src;
@@ -634,8 +644,8 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   MI_HIP(hipMemcpyAsync(bs.src, src.data(), src.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
 
   BatchArgs a{};
-  a.rowptr = g.rowptr.p;
-  a.edges = g.edges.p;
+  // the rectangles are in the input numbering's grid geometry
+  input_order_arrays(g, &a.rowptr, &a.edges, s);
   a.labels = bs.labels;
   a.stamp = bs.stamp;
   // Bellman-Ford rounds never touch the far stamps or piles: they alias the

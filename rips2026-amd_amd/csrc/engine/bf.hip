@@ -51,15 +51,19 @@ __global__ void k_init_bf(unsigned long long* __restrict__ labels, int64_t n, ui
   }
 }
 
+// Output element x (input numbering) comes from internal vertex inv[x]
+// (inv == nullptr: the same id); parents are stored as input ids.  src is the
+// internal id of the source.
 template <class W>
 __global__ void k_labels_to_output(const unsigned long long* __restrict__ labels, int64_t n,
                                    uint32_t src, int inf_mode, uint32_t* __restrict__ dist,
                                    int32_t* __restrict__ pred, const int* __restrict__ rowptr,
-                                   unsigned long long* __restrict__ reach) {
+                                   const int* __restrict__ inv, unsigned long long* __restrict__ reach) {
   __shared__ unsigned long long s_r[kBlock], s_e[kBlock];
   unsigned long long r = 0, e = 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += stride) {
+    const int64_t v = inv ? (int64_t)inv[x] : x;
     const unsigned long long l = labels[v];
     const uint32_t key = (uint32_t)(l >> 32);
     const bool reached = key != W::kInfKey;
@@ -70,10 +74,10 @@ __global__ void k_labels_to_output(const unsigned long long* __restrict__ labels
     } else {
       out = key;
     }
-    if (dist) dist[v] = out;
+    if (dist) dist[x] = out;
     if (pred) {
       const uint32_t p = (uint32_t)l;
-      pred[v] = (v == (int64_t)src || !reached || p == kNoPred) ? -1 : (int32_t)p;
+      pred[x] = (v == (int64_t)src || !reached || p == kNoPred) ? -1 : (int32_t)p;
     }
     if (reached) {
       ++r;
@@ -167,10 +171,12 @@ void run_bf_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
                            r < (int64_t)ws.log.n ? ws.log.p + r : nullptr};
       if (jacobi)
         k_relax_frontier<W, true, BfSink>
-            <<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, ws.labels.p, in, cq, sink, hk, work);
+            <<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, ws.labels.p, in, cq, sink, hk, work,
+                                     input_ids(g));
       else
         k_relax_frontier<W, false, BfSink>
-            <<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, ws.labels.p, in, cq, sink, hk, work);
+            <<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, ws.labels.p, in, cq, sink, hk, work,
+                                     input_ids(g));
       MI_LAUNCH_CHECK();
       if (g.info.large_vertices > 0) {
         k_relax_chunks<W, BfSink>
@@ -231,9 +237,10 @@ float heuristic_delta(const mi355_graph& g) {
   return (float)std::max(1.0e-6, d);
 }
 
-void run_sssp(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
+void run_sssp(mi355_graph& g, uint32_t input_source, const RunRequest& req, RunResult& res,
               hipStream_t stream) {
-  if ((int64_t)source >= g.n) fail(MI355_ERANGE, "source vertex is outside the graph");
+  if ((int64_t)input_source >= g.n) fail(MI355_ERANGE, "source vertex is outside the graph");
+  const uint32_t source = internal_vertex(g, input_source);
   switch (req.algorithm) {
     case MI355_ALG_BF:
     case MI355_ALG_BF_JACOBI:
@@ -257,8 +264,9 @@ void run_sssp(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult&
   }
 }
 
-void run_bf_continue(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
+void run_bf_continue(mi355_graph& g, uint32_t input_source, const RunRequest& req, RunResult& res,
                      hipStream_t stream) {
+  const uint32_t source = internal_vertex(g, input_source);
   if (g.weight_type == MI355_W_F32) run_bf_typed<WF32>(g, source, req, res, stream, true);
   else run_bf_typed<WU32>(g, source, req, res, stream, true);
 }
@@ -266,14 +274,16 @@ void run_bf_continue(mi355_graph& g, uint32_t source, const RunRequest& req, Run
 void labels_to_output(mi355_graph& g, uint32_t source, int inf_mode, void* d_dist, int32_t* d_pred,
                       unsigned long long* reach, hipStream_t s) {
   const unsigned grid = std::min<unsigned>(div_up((uint64_t)g.n, kBlock), 4096u);
+  const uint32_t src = internal_vertex(g, source);
+  const int* inv = g.relabeled ? g.inv.p : nullptr;
   if (g.weight_type == MI355_W_F32)
-    k_labels_to_output<WF32><<<grid, kBlock, 0, s>>>(g.ws.labels.p, g.n, source, inf_mode,
+    k_labels_to_output<WF32><<<grid, kBlock, 0, s>>>(g.ws.labels.p, g.n, src, inf_mode,
                                                      static_cast<uint32_t*>(d_dist), d_pred,
-                                                     g.rowptr.p, reach);
+                                                     g.rowptr.p, inv, reach);
   else
-    k_labels_to_output<WU32><<<grid, kBlock, 0, s>>>(g.ws.labels.p, g.n, source, inf_mode,
+    k_labels_to_output<WU32><<<grid, kBlock, 0, s>>>(g.ws.labels.p, g.n, src, inf_mode,
                                                      static_cast<uint32_t*>(d_dist), d_pred,
-                                                     g.rowptr.p, reach);
+                                                     g.rowptr.p, inv, reach);
   MI_LAUNCH_CHECK();
 }
 

@@ -135,6 +135,13 @@ int mi355_graph_create(const mi355_csr_desc* desc, int32_t allow_negative, void*
     *out = mi355::build_graph(*desc, allow_negative ? 1 : 0, as_stream(stream));
   });
 }
+int mi355_graph_create_ex(const mi355_csr_desc* desc, int32_t allow_negative, int32_t vertex_order,
+                          void* stream, mi355_graph** out) {
+  return guard([&] {
+    if (!desc || !out) fail(MI355_EINVAL, "null argument");
+    *out = mi355::build_graph(*desc, allow_negative ? 1 : 0, as_stream(stream), vertex_order);
+  });
+}
 int mi355_graph_destroy(mi355_graph* g) {
   return guard([&] {
     if (g) {
@@ -152,10 +159,15 @@ int mi355_graph_info_get(const mi355_graph* g, mi355_graph_info* info) {
 int mi355_graph_download(const mi355_graph* g, int64_t* rowptr, int32_t* col, uint32_t* w) {
   return guard([&] {
     if (!g || !rowptr || (g->m > 0 && (!col || !w))) fail(MI355_EINVAL, "null argument");
+    MI_HIP(hipSetDevice(g->device));
     std::vector<int> rp((size_t)g->n + 1);
     std::vector<uint2> e((size_t)g->m);
-    MI_HIP(hipMemcpy(rp.data(), g->rowptr.p, rp.size() * 4, hipMemcpyDeviceToHost));
-    if (g->m) MI_HIP(hipMemcpy(e.data(), g->edges.p, e.size() * 8, hipMemcpyDeviceToHost));
+    // the input-order arrays are a cache built on first use
+    const int* drp = nullptr;
+    const uint2* de = nullptr;
+    mi355::input_order_arrays(*const_cast<mi355_graph*>(g), &drp, &de, nullptr);
+    MI_HIP(hipMemcpy(rp.data(), drp, rp.size() * 4, hipMemcpyDeviceToHost));
+    if (g->m) MI_HIP(hipMemcpy(e.data(), de, e.size() * 8, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < rp.size(); ++i) rowptr[i] = rp[i];
     for (size_t i = 0; i < e.size(); ++i) {
       col[i] = (int32_t)e[i].x;
@@ -220,15 +232,16 @@ int mi355_sssp(mi355_graph* g, int32_t source, const mi355_sssp_options* opt, vo
     const hipStream_t s = as_stream(stream);
     mi355::RunResult res;
     mi355::run_sssp(*g, (uint32_t)source, make_request(opt, stats != nullptr), res, s);
-    mi355::DevBuf<unsigned long long> reach;
+    unsigned long long* reach = nullptr;
     if (stats) {
-      reach.reset(2);
-      MI_HIP(hipMemsetAsync(reach.p, 0, 16, s));
+      g->ws.reach.ensure(2);
+      reach = g->ws.reach.p;
+      MI_HIP(hipMemsetAsync(reach, 0, 16, s));
     }
     mi355::labels_to_output(*g, (uint32_t)source, opt ? opt->inf_mode : MI355_INF_IEEE, d_dist,
-                            d_pred, stats ? reach.p : nullptr, s);
+                            d_pred, reach, s);
     unsigned long long hr[2] = {0, 0};
-    if (stats) MI_HIP(hipMemcpyAsync(hr, reach.p, 16, hipMemcpyDeviceToHost, s));
+    if (stats) MI_HIP(hipMemcpyAsync(hr, reach, 16, hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
     if (stats) {
       *stats = mi355_sssp_stats{};

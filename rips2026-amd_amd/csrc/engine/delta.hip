@@ -88,6 +88,7 @@ struct DsArrays {
   uint32_t delta_bits;
   int want_work;
   const int* split;    // light edges per vertex (rows are sorted by weight)
+  const int* ids;      // input id of every internal vertex (nullptr: the same)
 };
 
 __device__ inline int bucket_of(float d, float inv_delta) {
@@ -299,7 +300,7 @@ __global__ void k_ds_init(DsArrays a, BucketCtrl* __restrict__ c, int64_t n, uin
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
     unsigned long long l = inf_label;
-    if (init) l = ((unsigned long long)W::output_to_key(init[v]) << 32) | kNoPred;
+    if (init) l = ((unsigned long long)W::output_to_key(init[a.ids ? a.ids[v] : v]) << 32) | kNoPred;
     a.labels[v] = (v == (int64_t)src) ? (((unsigned long long)W::kZeroKey << 32) | kNoPred) : l;
     a.bstamp[v] = 0xFFFFFFFFu;
     a.sstamp[v] = 0u;
@@ -339,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_ds_work(DsArrays a, BucketCtrl* __re
     const int* qin = kind == kStepLight ? a.lq[in_sel] : a.settled;
     relax_frontier_body<W, false>(a.rowptr, a.edges, a.labels, qin, nullptr, c->first, c->count, cq,
                                   kind == kStepLight ? kLightEdges : kHeavyEdges, a.delta_bits, sink,
-                                  work, a.split);
+                                  work, a.ids, a.split);
     sink.flush();
     return;
   }
@@ -464,6 +465,7 @@ void run_ds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
   a.delta_bits = __builtin_bit_cast(uint32_t, delta);
   a.want_work = req.want_work ? 1 : 0;
   a.split = light_split(g, delta, s);
+  a.ids = input_ids(g);
 
   const int64_t max_buckets = req.max_iters < 0 ? (int64_t)INT64_MAX : req.max_iters;
 

@@ -53,7 +53,8 @@ struct BfSink {
 // lanes.  Frontier entries [first, n_in) are processed; cls selects all,
 // light or heavy edges.  With `split` (rows sorted by weight, split[u] = light
 // edges of u) a class is a prefix or suffix of the row and only that range is
-// read.
+// read.  ids[u] is the input id of internal vertex u, the parent written into
+// the labels (nullptr: internal ids are input ids).
 template <class W, bool JACOBI, class Sink>
 __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
                                            const uint2* __restrict__ edges,
@@ -63,6 +64,7 @@ __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
                                            uint32_t n_in, ChunkQueue cq, int cls,
                                            uint32_t delta_bits, Sink& sink,
                                            unsigned long long* __restrict__ work,
+                                           const int* __restrict__ ids,
                                            const int* __restrict__ split = nullptr) {
   __shared__ int s_incl[kWavesPerBlock][kWave];
   __shared__ int s_base[kWavesPerBlock][kWave];
@@ -77,10 +79,11 @@ __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
   for (uint32_t base = first + (blockIdx.x * kWavesPerBlock + wv) * kWave; base < n_in;
        base += nwaves * kWave) {
     const uint32_t i = base + lane;
-    int u = 0, beg = 0, deg = 0;
+    int u = 0, beg = 0, deg = 0, pu = 0;
     uint32_t key = W::kInfKey;
     if (i < n_in) {
       u = qin[i];
+      pu = ids ? ids[u] : u;
       key = JACOBI ? qkeys[i] : atomic_load_key(labels + u);
       beg = rowptr[u];
       int end = rowptr[u + 1];
@@ -97,7 +100,7 @@ __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
           for (uint32_t c = 0; c < nch; ++c) {
             const int a = beg + (int)c * kChunk;
             const int b = min(end, a + kChunk);
-            if (pos + c < cq.cap) cq.q[pos + c] = make_uint4((uint32_t)u, key, (uint32_t)a, (uint32_t)b);
+            if (pos + c < cq.cap) cq.q[pos + c] = make_uint4((uint32_t)pu, key, (uint32_t)a, (uint32_t)b);
           }
           deg = 0;
         }
@@ -114,7 +117,7 @@ __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
     s_incl[wv][lane] = incl;
     s_base[wv][lane] = beg - (incl - deg);
     s_key[wv][lane] = key;
-    s_u[wv][lane] = u;
+    s_u[wv][lane] = pu;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -147,7 +150,8 @@ __device__ inline void relax_frontier_body(const int* __restrict__ rowptr,
   if (work && lane == 0 && edge_visits) atomicAdd(work, edge_visits);
 }
 
-// Chunks of high-degree vertices: one chunk per wave, lanes stream its edges.
+// Chunks of high-degree vertices: one chunk per wave, lanes stream its edges
+// ({parent id, key, first edge, end edge}).
 template <class W, class Sink>
 __device__ inline void relax_chunks_body(const uint2* __restrict__ edges,
                                          unsigned long long* __restrict__ labels, ChunkQueue cq,
@@ -182,7 +186,7 @@ template <class W, bool JACOBI, class Sink>
 __global__ __launch_bounds__(kBlock) void k_relax_frontier(
     const int* __restrict__ rowptr, const uint2* __restrict__ edges,
     unsigned long long* __restrict__ labels, FrontierIn in, ChunkQueue cq, Sink sink,
-    RoundHousekeeping hk, unsigned long long* __restrict__ work) {
+    RoundHousekeeping hk, unsigned long long* __restrict__ work, const int* __restrict__ ids) {
   __shared__ int s_stage[kWavesPerBlock][kStageCap];
   const uint32_t n_in = *in.cnt;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -192,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_frontier(
   }
   sink.stage = s_stage[threadIdx.x / kWave];
   relax_frontier_body<W, JACOBI>(rowptr, edges, labels, in.q, in.keys, 0u, n_in, cq, kAllEdges,
-                                 0u, sink, work);
+                                 0u, sink, work, ids);
   sink.flush();
 }
 

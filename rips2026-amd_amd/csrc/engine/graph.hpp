@@ -24,6 +24,7 @@ struct BucketCtrl;
 
 struct Workspace {
   DevBuf<unsigned long long> labels;  // n packed labels
+  DevBuf<unsigned long long> reach;   // reached vertices / edges of the last output conversion
   DevBuf<uint32_t> stamp;             // n round stamps (frontier dedup)
   DevBuf<int> q[2];                   // frontier queues (n each)
   DevBuf<uint32_t> qkey[2];           // Jacobi snapshot keys (n each)
@@ -71,6 +72,20 @@ struct mi355_graph {
   int32_t weight_type = MI355_W_F32;
   mi355::DevBuf<int> rowptr;   // n+1 int32 offsets
   mi355::DevBuf<uint2> edges;  // m records {dst, weight bits}
+  // Vertex order.  A graph with a skewed degree distribution is renumbered by
+  // descending degree at build time (stable by input id), so the labels,
+  // stamps and row offsets of the vertices that most edges point to share a
+  // few cache lines; internal vertex v is input vertex perm[v], input vertex
+  // x is internal inv[x].  Labels store input ids as parents (so the
+  // smallest-parent tie rule refers to input ids) and outputs are written in
+  // input order.  Empty perm: internal ids are the input ids.
+  bool relabeled = false;
+  mi355::DevBuf<int> perm, inv;
+  std::vector<int32_t> inv_host;
+  // input-order copy of the arrays (built on first use by the paths that
+  // read the input numbering's geometry: batched grid queries, download)
+  mi355::DevBuf<int> in_rowptr;
+  mi355::DevBuf<uint2> in_edges;
   mi355_graph_info info{};
   std::vector<int32_t> out_degree_host;  // kept only when built from host data
   mi355::Workspace ws;
@@ -83,7 +98,25 @@ struct mi355_graph {
 };
 
 namespace mi355 {
+// Vertex order of a graph handle (mi355_graph_create_ex).
+enum VertexOrder : int32_t { kOrderAuto = 0, kOrderInput = 1, kOrderDegree = 2 };
+
 // Build a graph handle (graph_build.hip).
 // allow_negative: bit 0 admits negative weights, bit 1 admits +inf weights.
-mi355_graph* build_graph(const mi355_csr_desc& d, int allow_negative, hipStream_t stream);
+mi355_graph* build_graph(const mi355_csr_desc& d, int allow_negative, hipStream_t stream,
+                         int32_t order = kOrderAuto);
+
+// The 14 graph features of the learned delta policy (policy.hip), sampled in
+// the input numbering (computed once per graph, before any renumbering).
+void graph_features(mi355_graph& g, hipStream_t s, float* out14);
+
+// Internal id of input vertex v.
+inline uint32_t internal_vertex(const mi355_graph& g, int64_t v) {
+  return g.relabeled ? (uint32_t)g.inv_host[(size_t)v] : (uint32_t)v;
+}
+// Input id of every internal vertex (nullptr when the input order is kept).
+inline const int* input_ids(const mi355_graph& g) { return g.relabeled ? g.perm.p : nullptr; }
+// Row offsets and edge records in the input numbering (the graph's own
+// arrays, or a copy built on first use when the graph is renumbered).
+void input_order_arrays(mi355_graph& g, const int** rowptr, const uint2** edges, hipStream_t stream);
 }  // namespace mi355

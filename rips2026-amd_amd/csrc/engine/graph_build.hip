@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -235,9 +236,121 @@ __global__ void k_stats(const int* __restrict__ rowptr, const uint2* __restrict_
   }
 }
 
+// Sort key of a vertex for the degree order: ascending keys = descending
+// degree (a stable sort keeps input order among equal degrees).
+__global__ void k_degree_keys(const int* __restrict__ rowptr, uint32_t* __restrict__ keys,
+                              uint32_t* __restrict__ ids, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    keys[v] = ~(uint32_t)(rowptr[v + 1] - rowptr[v]);
+    ids[v] = (uint32_t)v;
+  }
+}
+
+__global__ void k_inverse(const int* __restrict__ perm, int* __restrict__ inv, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    inv[perm[i]] = (int)i;
+}
+
+// Degree of output row i = input row rows[i].
+__global__ void k_gathered_degrees(const int* __restrict__ rowptr, const int* __restrict__ rows,
+                                   int* __restrict__ deg, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    deg[i] = rowptr[rows[i] + 1] - rowptr[rows[i]];
+}
+
+// Output edge j of row r (binary search over the output offsets) is edge
+// j - out_rp[r] of input row rows[r], its destination renamed by ids[].
+__global__ void k_permute_edges(const int* __restrict__ in_rp, const uint2* __restrict__ in_e,
+                                const int* __restrict__ out_rp, const int* __restrict__ rows,
+                                const int* __restrict__ ids, uint2* __restrict__ out_e, int64_t n,
+                                int64_t m) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += stride) {
+    int64_t lo = 0, hi = n;  // out_rp[lo] <= j < out_rp[hi]
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (out_rp[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    const uint2 e = in_e[in_rp[rows[lo]] + (j - out_rp[lo])];
+    out_e[j] = make_uint2((uint32_t)ids[e.x], e.y);
+  }
+}
+
+// Renumbered copy of a CSR: output row i is input row rows[i], and every
+// destination x becomes ids[x].  Row contents keep their order.
+void permute_csr(const int* in_rp, const uint2* in_e, const int* rows, const int* ids, int64_t n,
+                 int64_t m, int* out_rp, uint2* out_e, hipStream_t stream) {
+  DevBuf<int> deg((size_t)n);
+  k_gathered_degrees<<<grid_for(n), kBlock, 0, stream>>>(in_rp, rows, deg.p, n);
+  MI_LAUNCH_CHECK();
+  MI_HIP(hipMemsetAsync(out_rp, 0, sizeof(int), stream));
+  size_t temp_bytes = 0;
+  MI_HIP(rocprim::inclusive_scan(nullptr, temp_bytes, deg.p, out_rp + 1, (size_t)n, rocprim::plus<int>(),
+                                 stream));
+  DevBuf<unsigned char> temp(std::max<size_t>(temp_bytes, 1));
+  MI_HIP(rocprim::inclusive_scan(temp.p, temp_bytes, deg.p, out_rp + 1, (size_t)n, rocprim::plus<int>(),
+                                 stream));
+  if (m > 0) k_permute_edges<<<grid_for(m), kBlock, 0, stream>>>(in_rp, in_e, out_rp, rows, ids, out_e, n, m);
+  MI_LAUNCH_CHECK();
+  MI_HIP(hipStreamSynchronize(stream));
+}
+
+// Renumber g by descending degree (ties keep input order).
+void renumber_by_degree(mi355_graph& g, hipStream_t stream) {
+  const int64_t n = g.n, m = g.m;
+  DevBuf<uint32_t> keys_in((size_t)n), keys_out((size_t)n), ids((size_t)n);
+  g.perm.reset((size_t)n);
+  g.inv.reset((size_t)n);
+  k_degree_keys<<<grid_for(n), kBlock, 0, stream>>>(g.rowptr.p, keys_in.p, ids.p, n);
+  MI_LAUNCH_CHECK();
+  size_t temp_bytes = 0;
+  uint32_t* perm = reinterpret_cast<uint32_t*>(g.perm.p);
+  MI_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys_in.p, keys_out.p, ids.p, perm, (size_t)n, 0, 32,
+                                   stream));
+  DevBuf<unsigned char> temp(std::max<size_t>(temp_bytes, 1));
+  MI_HIP(rocprim::radix_sort_pairs(temp.p, temp_bytes, keys_in.p, keys_out.p, ids.p, perm, (size_t)n, 0, 32,
+                                   stream));
+  k_inverse<<<grid_for(n), kBlock, 0, stream>>>(g.perm.p, g.inv.p, n);
+  MI_LAUNCH_CHECK();
+  DevBuf<int> rp((size_t)n + 1);
+  DevBuf<uint2> e((size_t)std::max<int64_t>(m, 1));
+  permute_csr(g.rowptr.p, g.edges.p, g.perm.p, g.inv.p, n, m, rp.p, e.p, stream);
+  std::swap(g.rowptr, rp);
+  std::swap(g.edges, e);
+  g.inv_host.resize((size_t)n);
+  MI_HIP(hipMemcpyAsync(g.inv_host.data(), g.inv.p, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+  MI_HIP(hipStreamSynchronize(stream));
+  g.relabeled = true;
+}
+
 }  // namespace
 
-mi355_graph* build_graph(const mi355_csr_desc& d, int allow_negative, hipStream_t stream) {
+void input_order_arrays(mi355_graph& g, const int** rowptr, const uint2** edges, hipStream_t stream) {
+  if (!g.relabeled) {
+    *rowptr = g.rowptr.p;
+    *edges = g.edges.p;
+    return;
+  }
+  if (!g.in_rowptr.p) {
+    // input row x is internal row inv[x]; internal destinations become perm[]
+    DevBuf<int> rp((size_t)g.n + 1);
+    DevBuf<uint2> e((size_t)std::max<int64_t>(g.m, 1));
+    permute_csr(g.rowptr.p, g.edges.p, g.inv.p, g.perm.p, g.n, g.m, rp.p, e.p, stream);
+    std::swap(g.in_rowptr, rp);
+    std::swap(g.in_edges, e);
+  }
+  *rowptr = g.in_rowptr.p;
+  *edges = g.in_edges.p;
+}
+
+mi355_graph* build_graph(const mi355_csr_desc& d, int allow_negative, hipStream_t stream,
+                         int32_t order) {
+  if (order != kOrderAuto && order != kOrderInput && order != kOrderDegree)
+    fail(MI355_EINVAL, "unknown vertex order");
   if (d.rows <= 0) fail(MI355_EINVAL, "CSR graph must contain at least one vertex");
   if (d.rows != d.cols) fail(MI355_EINVAL, "SSSP expects a square CSR adjacency matrix");
   if (d.nnz < 0) fail(MI355_EINVAL, "CSR nnz must be nonnegative");
@@ -378,6 +491,16 @@ mi355_graph* build_graph(const mi355_csr_desc& d, int allow_negative, hipStream_
     inf.large_chunks = (int64_t)chunks;
     inf.has_negative = (hflags & kNegWeight) ? 1 : 0;
     inf.device = g->device;
+    // Degree order for skewed graphs: most relaxations then target the few
+    // high-degree vertices, whose labels and stamps share cache lines.
+    const bool skewed = n >= 4096 && (double)maxdeg >= 32.0 * inf.avg_degree;
+    if (m > 0 && (order == kOrderDegree || (order == kOrderAuto && skewed))) {
+      // the delta policy's features are sampled in the input numbering
+      float feat[14];
+      graph_features(*g, stream, feat);
+      renumber_by_degree(*g, stream);
+    }
+    inf.vertex_order = g->relabeled ? kOrderDegree : kOrderInput;
   } catch (...) {
     delete g;
     throw;

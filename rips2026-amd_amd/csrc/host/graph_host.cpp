@@ -402,6 +402,9 @@ int mi355g_read_ripscsr1(const char* path, mi355g_graph** out) {
     // h: version, orientation, rows, cols, declared, loaded, nnz, rowptr_n, colind_n, values_n
     if (h[0] != 1) return fail(MI355G_EFORMAT, "unsupported RIPSCSR1 version");
     if (h[1] != 1) return fail(MI355G_EFORMAT, "RIPSCSR1 orientation must be 1 (incoming)");
+    // vertex ids are 32-bit: this also keeps rows + 1 below wrap-around
+    if (h[2] > (uint64_t)std::numeric_limits<int32_t>::max())
+      return fail(MI355G_EFORMAT, "RIPSCSR1 row count exceeds 32-bit vertex ids");
     if (h[7] != h[2] + 1 || h[8] != h[6] || h[9] != h[6])
       return fail(MI355G_EFORMAT, "inconsistent RIPSCSR1 array counts");
     // the arrays must be in the file before anything is allocated for them

@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import (ALG_ADDS, ALG_BF, ALG_BF_JACOBI, ALG_DELTA, INF_FLT_MAX, INF_IEEE,
-                   W_F32, W_U32, OUT, IN, HOST, DEVICE, SsspOptions, SsspStats, CsrDesc,
+                   W_F32, W_U32, OUT, IN, HOST, DEVICE, ORDER_AUTO, ORDER_INPUT, ORDER_DEGREE, SsspOptions, SsspStats, CsrDesc,
                    BfCsrStats, BfProgress, PROGRESS_CB, Mi355Error, engine, graphlib)
 from .graph import HostGraph, grid4, rmat, routing_grid, chain_random, from_edges, \
     from_csr, from_dense, read_ripscsr1, write_ripsadj1, read_ripsadj1, read_dimacs, \
@@ -26,7 +26,7 @@ from .device import DeviceGraph, DeviceBuffer, bf_csr, delta_stepping_csr
 
 __all__ = [
     "ALG_ADDS", "ALG_BF", "ALG_BF_JACOBI", "ALG_DELTA", "INF_FLT_MAX", "INF_IEEE", "W_F32",
-    "W_U32", "OUT", "IN", "HOST", "DEVICE", "SsspOptions", "SsspStats", "CsrDesc", "BfCsrStats",
+    "W_U32", "OUT", "IN", "ORDER_AUTO", "ORDER_INPUT", "ORDER_DEGREE", "HOST", "DEVICE", "SsspOptions", "SsspStats", "CsrDesc", "BfCsrStats",
     "BfProgress", "PROGRESS_CB", "Mi355Error", "engine", "graphlib", "HostGraph", "grid4", "rmat",
     "routing_grid", "chain_random", "from_edges", "from_csr", "from_dense", "read_ripscsr1",
     "write_ripsadj1", "read_ripsadj1", "read_dimacs",

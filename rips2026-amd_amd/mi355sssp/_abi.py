@@ -10,6 +10,7 @@ LIB_DIR = os.path.normpath(os.path.join(_HERE, "..", "lib"))
 ALG_BF, ALG_BF_JACOBI, ALG_DELTA, ALG_ADDS = 0, 1, 2, 3
 INF_IEEE, INF_FLT_MAX = 0, 1
 W_F32, W_U32 = 0, 1
+ORDER_AUTO, ORDER_INPUT, ORDER_DEGREE = 0, 1, 2
 OUT, IN = 0, 1
 HOST, DEVICE = 0, 1
 
@@ -36,7 +37,8 @@ class GraphInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("m", C.c_int64), ("weight_type", C.c_int32),
                 ("max_degree", C.c_int32), ("avg_degree", C.c_double),
                 ("avg_weight", C.c_double), ("large_vertices", C.c_int64),
-                ("large_chunks", C.c_int64), ("has_negative", C.c_int32), ("device", C.c_int32)]
+                ("large_chunks", C.c_int64), ("has_negative", C.c_int32), ("device", C.c_int32),
+                ("vertex_order", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SsspOptions(C.Structure):
@@ -144,6 +146,7 @@ _ENGINE_SIGS = {
     "mi355_stream_destroy": (C.c_int, [P]),
     "mi355_stream_sync": (C.c_int, [P]),
     "mi355_graph_create": (C.c_int, [C.POINTER(CsrDesc), I32, P, C.POINTER(P)]),
+    "mi355_graph_create_ex": (C.c_int, [C.POINTER(CsrDesc), I32, I32, P, C.POINTER(P)]),
     "mi355_graph_destroy": (C.c_int, [P]),
     "mi355_graph_info_get": (C.c_int, [P, C.POINTER(GraphInfo)]),
     "mi355_graph_download": (C.c_int, [P, P, P, P]),

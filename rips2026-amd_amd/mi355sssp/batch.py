@@ -64,6 +64,19 @@ def routing_queries(rows: int, cols: int, count: int, seed: int, side_min: int =
     return Queries(cols, np.array(off), np.concatenate(srcs), boxes)
 
 
+def block_slots(bh: int, bw: int) -> int:
+    """Label slots the engine stores for a bh x bw rectangle: 4 x 4 tiles
+    padded to whole tiles, or row-major (no padding) below 4 rows."""
+    bh, bw = int(bh), int(bw)
+    return bh * bw if bh < 4 else ((bh + 3) & ~3) * ((bw + 3) & ~3)
+
+
+def storage_slots(qs: "Queries") -> int:
+    """Label slots the engine allocates for a batch (tile padding included)."""
+    b = qs.boxes.astype(np.int64)
+    return int(sum(block_slots(h, w) for h, w in zip(b[:, 2] - b[:, 0], b[:, 3] - b[:, 1])))
+
+
 def options(**kw) -> BatchOptions:
     o = BatchOptions()
     engine().mi355_batch_options_default(C.byref(o))

@@ -6,8 +6,8 @@ import ctypes as C
 import numpy as np
 
 from ._abi import (ALG_DELTA, BfCsrStats, BfProgress, CsrDesc, DEVICE, GraphInfo, HOST, IN,
-                   INF_IEEE, OUT, PROGRESS_CB, SsspOptions, SsspStats, W_F32, W_U32, check,
-                   engine)
+                   INF_IEEE, ORDER_AUTO, OUT, PROGRESS_CB, SsspOptions, SsspStats, W_F32, W_U32,
+                   check, engine)
 from .graph import HostGraph
 
 
@@ -59,11 +59,14 @@ def desc_from_arrays(rowptr: np.ndarray, col: np.ndarray, wbits: np.ndarray, wei
 class DeviceGraph:
     """A device-resident graph handle (mi355_graph)."""
 
-    def __init__(self, g: HostGraph, allow_negative: bool = False):
+    def __init__(self, g: HostGraph, allow_negative: bool = False, order: int = ORDER_AUTO):
+        """order: ORDER_AUTO (degree order for skewed graphs), ORDER_INPUT or
+        ORDER_DEGREE (mi355_graph_create_ex)."""
         self.host = g
         d = desc_from_arrays(g.rowptr, g.col, g.wbits, g.weight_type, g.orientation)
         self._h = C.c_void_p()
-        check(engine().mi355_graph_create(C.byref(d), int(allow_negative), None, C.byref(self._h)))
+        check(engine().mi355_graph_create_ex(C.byref(d), int(allow_negative), int(order), None,
+                                             C.byref(self._h)))
         info = GraphInfo()
         check(engine().mi355_graph_info_get(self._h, C.byref(info)))
         self.info = info
@@ -104,6 +107,14 @@ class DeviceGraph:
         dist = self.dist_buf.to_array(dt, self.n)
         pred = self.pred_buf.to_array(np.int32, self.n) if want_pred else None
         return dist, pred, st
+
+    def sssp_device(self, source: int, opts: SsspOptions, stats: bool = True):
+        """One full SSSP leaving distances and parents on the device (in this
+        graph's dist_buf / pred_buf); returns the stats (None without)."""
+        st = SsspStats() if stats else None
+        check(engine().mi355_sssp(self._h, source, C.byref(opts), self.dist_buf.ptr, self.pred_buf.ptr,
+                                  C.byref(st) if st is not None else None, None))
+        return st
 
     def sssp_labels(self, source: int, opts: SsspOptions) -> SsspStats:
         st = SsspStats()

@@ -90,6 +90,30 @@ def spgemm(a_rowptr, a_col, a_val, a_cols, b_rowptr, b_col, b_val, b_cols, hash_
     return rp, col, val, (ccol.value is None, cval.value is None)
 
 
+def sparse_vector_min_merge(a_rowptr, a_val, b_rowptr, b_val):
+    """Element-wise min of two sparse n x 1 vectors on the device
+    (mi355_sparse_vector_min_merge); returns (rowptr, col, val, changed)."""
+    ins = [DeviceBuffer.from_array(np.ascontiguousarray(x, dt)) for x, dt in
+           ((a_rowptr, np.int64), (a_val, np.float32), (b_rowptr, np.int64), (b_val, np.float32))]
+    rows = len(a_rowptr) - 1
+    crp, ccol, cval = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    nnz, changed = C.c_int64(), C.c_int()
+    check(engine().mi355_sparse_vector_min_merge(rows, ins[0].ptr, ins[1].ptr, ins[2].ptr, ins[3].ptr,
+                                                 C.byref(crp), C.byref(ccol), C.byref(cval),
+                                                 C.byref(nnz), C.byref(changed), None))
+    rp = np.empty(rows + 1, np.int64)
+    col = np.empty(nnz.value, np.int32)
+    val = np.empty(nnz.value, np.float32)
+    check(engine().mi355_memcpy_d2h(rp.ctypes.data, crp, rp.nbytes, None))
+    if nnz.value:
+        check(engine().mi355_memcpy_d2h(col.ctypes.data, ccol, col.nbytes, None))
+        check(engine().mi355_memcpy_d2h(val.ctypes.data, cval, val.nbytes, None))
+    for p in (crp, ccol, cval):
+        if p.value:
+            engine().mi355_device_free(p)
+    return rp, col, val, bool(changed.value)
+
+
 def bf_dense(A, n, source, max_iters=-1, on_device=False):
     A = _f32(A)
     dist = np.empty(n, np.float32)

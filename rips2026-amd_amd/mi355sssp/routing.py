@@ -132,8 +132,8 @@ def route(graph, meta: RouteMetadata, dg=None, max_label_slots: int = 1 << 30,
           resolve_names: bool = True, **batch_options) -> list[RoutedNet]:
     """Route every request of `meta` on `graph` (the RIPSCSR1 host graph the
     sidecar belongs to).  dg: an existing DeviceGraph of graph (built when
-    None).  Nets go to the batched engine in chunks of at most
-    max_label_slots labels (n per net).  The host side walks the parents of
+    None).  Nets go to the batched engine in chunks whose label storage
+    (B.storage_slots: n per net) stays within max_label_slots.  The host side walks the parents of
     all sinks of a chunk together and looks every hop's edge up in one
     sorted search; names are resolved once per distinct string."""
     from .device import DeviceGraph
@@ -158,7 +158,8 @@ def route(graph, meta: RouteMetadata, dg=None, max_label_slots: int = 1 << 30,
             names[i] = meta.string(i)
         return names[i]
 
-    per_chunk = max(1, min(65535, max_label_slots // max(n, 1)))
+    # every net is a 1 x n rectangle, stored row-major (n slots, no padding)
+    per_chunk = max(1, min(65535, max_label_slots // max(B.block_slots(1, n), 1)))
     routed: list[RoutedNet] = []
     todo = [q for q in range(meta.request_count) if len(meta.request_sources(q)) > 0]
     for c0 in range(0, len(todo), per_chunk):

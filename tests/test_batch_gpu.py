@@ -175,3 +175,41 @@ def test_one_query_at_a_time_adds_matches_the_batch():
         gp = np.where((p[:area] < 0) | (p[:area] == area), -1,
                       (r0 + p[:area] // bw) * side + c0 + p[:area] % bw)
         assert np.array_equal(gp, pred[off[q]:off[q + 1]])
+
+
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_short_rectangles_row_major_blocks(use_graph):
+    """Rectangles of 1-3 rows are stored row-major (no tile padding); mixed
+    with tiled ones in one batch they give the oracle's answers."""
+    side = 120
+    g = M.routing_grid(side, seed=41)
+    dg = M.DeviceGraph(g)
+    rng = np.random.default_rng(3)
+    boxes, srcs, off = [], [], [0]
+    for q in range(36):
+        h = (1, 2, 3, 4, 5, 9)[q % 6]
+        w = int(rng.integers(1, side + 1))
+        r0, c0 = int(rng.integers(0, side - h + 1)), int(rng.integers(0, side - w + 1))
+        boxes.append((r0, c0, r0 + h, c0 + w))
+        k = int(rng.integers(1, 4))
+        s = (rng.integers(r0, r0 + h, k) * side + rng.integers(c0, c0 + w, k)).astype(np.int32)
+        srcs.append(s)
+        off.append(off[-1] + k)
+    qs = B.Queries(side, np.array(off), np.concatenate(srcs), np.array(boxes))
+    assert B.storage_slots(qs) < sum(B.block_slots(max(4, b[2] - b[0]), b[3] - b[1]) for b in boxes)
+    print(f"batch: {qs.count} queries, use_graph={use_graph}", flush=True)
+    dist, pred, off, st = B.run(dg, qs, use_graph=use_graph)
+    assert st.label_slots == off[-1] == qs.areas().sum()
+    _check(g, qs, dist, pred, off)
+
+
+def test_batch_on_a_degree_ordered_graph():
+    """The batch reads the input numbering's grid geometry: on a graph handle
+    renumbered by degree it runs on an input-order copy, same answers."""
+    side = 64
+    g = M.routing_grid(side, seed=43)
+    qs = B.routing_queries(side, side, 12, seed=9, side_min=4, side_max=64)
+    dg = M.DeviceGraph(g, order=M.ORDER_DEGREE)
+    assert dg.info.vertex_order == M.ORDER_DEGREE
+    dist, pred, off, _ = B.run(dg, qs)
+    _check(g, qs, dist, pred, off)

@@ -147,6 +147,16 @@ def test_ripscsr1_reader_rejects_corrupt_files(tmp_path):
         struct.pack_into("<Q", b, 8 + 8 * k, v)  # counts far beyond the file
     cases["huge counts"] = b
     cases["short"] = bytearray(good[:-4])
+    b = bytearray(good)
+    # rows = 2^64 - 1 makes rows + 1 wrap to 0: with an empty rowptr count the
+    # count checks alone would pass
+    for k, v in ((2, (1 << 64) - 1), (7, 0)):
+        struct.pack_into("<Q", b, 8 + 8 * k, v)
+    cases["rows wrap"] = b
+    b = bytearray(good)
+    for k, v in ((2, 1 << 31), (7, (1 << 31) + 1)):
+        struct.pack_into("<Q", b, 8 + 8 * k, v)
+    cases["rows beyond 32-bit ids"] = b
     for name, data in cases.items():
         q = tmp_path / "bad.csrbin"
         q.write_bytes(bytes(data))

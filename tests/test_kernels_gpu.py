@@ -167,6 +167,58 @@ def _dense(n, edges):
     return A
 
 
+
+def _sparse_vec(rng, rows, density, dup):
+    """Random sparse n x 1 vector: row counts 0, 1 or (dup) 2; some entries
+    NaN or +inf (both ignored), the rest in (0, 8]."""
+    cnt = (rng.random(rows) < density).astype(np.int64)
+    if dup:
+        cnt += (rng.random(rows) < 0.2) & (cnt > 0)
+    rp = np.concatenate([[0], np.cumsum(cnt)])
+    val = (rng.integers(1, 1 << 24, int(rp[-1])) * 2.0 ** -21).astype(np.float32)
+    odd = rng.random(len(val))
+    val[odd < 0.03] = np.nan
+    val[(odd >= 0.03) & (odd < 0.06)] = np.inf
+    return rp, val
+
+
+def _dense_of(rp, val):
+    rows = len(rp) - 1
+    d = np.full(rows, np.inf, np.float32)
+    r = np.repeat(np.arange(rows), np.diff(rp))
+    ok = ~np.isnan(val)
+    np.minimum.at(d, r[ok], val[ok])
+    return d
+
+
+@pytest.mark.parametrize("rows,dup", [(1, False), (777, False), (50000, True)])
+def test_sparse_vector_min_merge_matches_elementwise_min(rows, dup):
+    """The bf_csr_detail merge (mi355_sparse_vector_min_merge): per row the
+    smaller entry (absent, NaN = +inf), +inf results dropped, every output row
+    one entry in column 0; changed when some entry of b beats a's."""
+    rng = np.random.default_rng(rows)
+    arp, aval = _sparse_vec(rng, rows, 0.5, dup)
+    brp, bval = _sparse_vec(rng, rows, 0.5, dup)
+    print(f"merge rows={rows} a_nnz={len(aval)} b_nnz={len(bval)}", flush=True)
+    rp, col, val, changed = M.kernels.sparse_vector_min_merge(arp, aval, brp, bval)
+    da, db = _dense_of(arp, aval), _dense_of(brp, bval)
+    want = np.minimum(da, db)
+    keep = np.isfinite(want)
+    assert np.array_equal(np.diff(rp), keep.astype(np.int64))
+    assert np.all(col == 0)
+    assert val.tobytes() == want[keep].tobytes()
+    assert changed == bool(np.any(db < da))
+    # b never below a: unchanged, and merging a vector with itself as well
+    _, _, _, ch2 = M.kernels.sparse_vector_min_merge(arp, aval, arp, aval)
+    assert not ch2
+
+
+def test_sparse_vector_min_merge_empty_vectors():
+    z = np.zeros(11, np.int64)
+    rp, col, val, changed = M.kernels.sparse_vector_min_merge(z, np.zeros(0, np.float32), z,
+                                                             np.zeros(0, np.float32))
+    assert np.all(rp == 0) and len(val) == 0 and not changed
+
 @pytest.mark.parametrize("case", bf_cases.all_cases(include_large=True), ids=lambda c: c[0])
 def test_dense_bellman_ford_known_answers(case):
     name, n, src, edges, expected, _ = case

@@ -220,9 +220,19 @@ def test_route_paths_and_pips_on_the_cpu(tmp_path, monkeypatch):
     lookup, PIP records, names) with the oracle standing in for the device."""
     g, meta = device_fixture(tmp_path, n=600, m=3000, nets=25, seed=11)
     out = g.transpose()
-    monkeypatch.setattr(R.B, "run", _oracle_run(out))
+    run = _oracle_run(out)
+    chunk_slots = []
+
+    def recording_run(dg, qs, *a, **kw):
+        chunk_slots.append(R.B.storage_slots(qs))
+        return run(dg, qs, *a, **kw)
+
+    monkeypatch.setattr(R.B, "run", recording_run)
     nets = R.route(g, meta, dg=object())
+    chunk_slots.clear()
     chunked = R.route(g, meta, dg=object(), max_label_slots=4 * g.n)
+    # each chunk's label storage (tile padding included) stays within the budget
+    assert len(chunk_slots) > 1 and max(chunk_slots) <= 4 * g.n
     bare = R.route(g, meta, dg=object(), resolve_names=False)
     assert [n.request for n in nets] == [q for q in range(meta.request_count) if len(meta.request_sources(q))]
     rowp, col = g.rowptr, g.col

@@ -261,3 +261,53 @@ def test_engine_rows_sorted_by_weight(wtype):
         if orient == M.OUT:
             order = np.argsort(key_in, kind="stable")  # ties keep the input column order
             assert np.array_equal(col, go.col[order])
+
+
+@pytest.mark.parametrize("alg", [M.ALG_BF, M.ALG_BF_JACOBI, M.ALG_DELTA, M.ALG_ADDS])
+def test_vertex_order_gives_identical_results(alg):
+    """A skewed graph is renumbered by degree internally (auto), or kept in
+    input order on request: distances, canonical parents and counts are the
+    same either way, in input ids."""
+    g = M.rmat(13, 16, seed=99)
+    srcs = g.pick_sources(3, seed=4)
+    sts = {}
+    for order in (M.ORDER_INPUT, M.ORDER_DEGREE, M.ORDER_AUTO):
+        dg = M.DeviceGraph(g, order=order)
+        assert dg.info.vertex_order == (M.ORDER_DEGREE if order == M.ORDER_AUTO else order)
+        sts[order] = [(_check(g, dg, int(s), alg).iterations_used if alg in (M.ALG_BF_JACOBI, M.ALG_DELTA)
+                       else 0) for s in srcs]
+    assert sts[M.ORDER_INPUT] == sts[M.ORDER_DEGREE] == sts[M.ORDER_AUTO]
+
+
+def test_degree_order_download_is_in_input_order():
+    g = M.rmat(12, 16, seed=5)
+    a = M.DeviceGraph(g, order=M.ORDER_INPUT).download()
+    b = M.DeviceGraph(g, order=M.ORDER_DEGREE).download()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert np.array_equal(a[0], g.rowptr)
+
+
+def test_hip_adds_warm_start_on_a_renumbered_graph():
+    """initialize_distances = false on a graph renumbered by degree: the
+    caller's distances (input order) seed the labels.  One vertex v starts
+    below its true distance: it keeps that value and is never expanded, so
+    the rest equals Dijkstra with v's out-edges removed."""
+    g = M.rmat(13, 16, seed=12)
+    src = int(g.pick_sources(1, seed=3)[0])
+    true = orc.dijkstra(g.rowptr, g.col, g.weights, src)
+    deg = np.diff(g.rowptr)
+    v = int(np.flatnonzero((deg > 50) & np.isfinite(true) & (true > 0))[0])
+    init = np.full(g.n, np.inf, np.float32)
+    init[v] = np.float32(true[v] / 2)
+    o = M.AddsOptions.defaults()
+    o.initialize_distances = 0
+    rc, dist, _ = M.hip_adds(g.rowptr, g.col, g.weights, src, M.W_F32, o, init)
+    assert rc == 0
+    keep = np.ones(g.m, bool)
+    keep[g.rowptr[v]:g.rowptr[v + 1]] = False
+    deg_kept = deg.copy()
+    deg_kept[v] = 0
+    rp = np.concatenate([[0], np.cumsum(deg_kept)])
+    want = orc.dijkstra(rp, g.col[keep], g.weights[keep], src)
+    want[v] = init[v]
+    assert dist.tobytes() == want.tobytes()
