@@ -5,14 +5,18 @@ Workload (the reference's headline configuration that fits one GPU): R-MAT
 scale 22, edge factor 16, (a,b,c,d) = (.57,.19,.19,.05), self loops and
 duplicate pairs removed, symmetrised, fp32 weights in (0,1], 64 seeded
 sources with out-degree > 0.  The graph is generated on the host, uploaded
-once, and stays resident in HBM; a step is one full SSSP (label init
-included) from the next source with ADDS (persistent worklist kernel) under the
-shipped learned delta policy; plain ADDS (fixed heuristic delta),
-delta-stepping and Bellman-Ford are timed beside it on the same sources.  The
+once, and stays resident in HBM; a step is one full SSSP from the next source
+(label initialisation, the SSSP kernel, and the conversion of the labels to
+distances and parents in HBM) with ADDS (persistent worklist kernel) under the
+shipped learned delta policy.  The timed steps are spread evenly over the 64
+sources, and all 64 are timed once more when the steps cover fewer.  Plain
+ADDS (fixed heuristic delta), delta-stepping and Bellman-Ford are timed beside
+it on the same sources.  The 256 x 256 grid (every algorithm), the
 routing-grid legs (single source, and 4096 box-limited multi-source queries as
 one batch vs plain ADDS one query at a time) and the batched min-plus GEMM leg
 report into config.  Distances of the first --check sources are checked bit
-for bit against the CPU oracle outside the timed region.
+for bit against the CPU oracle outside the timed region; the CPU baseline
+runs pinned to one core.
 
 Prints one JSON line (the bench contract) with roofline and cpu_baseline.
 """
@@ -57,6 +61,58 @@ def cpu_model() -> str:
     except Exception:
         pass
     return platform.processor() or "unknown"
+
+
+class pinned_to_one_core:
+    """Run the enclosed CPU work on one core (the first this process may use);
+    the previous affinity is restored afterwards."""
+
+    def __enter__(self):
+        self.old = os.sched_getaffinity(0)
+        self.core = min(self.old)
+        os.sched_setaffinity(0, {self.core})
+        return self.core
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.old)
+        return False
+
+
+def b_alg(m_r, n_r, n):
+    """Algorithmic bytes of one SSSP: 12 per reached edge, 20 per reached
+    vertex, 8 per vertex of label initialisation."""
+    return 12.0 * m_r + 20.0 * n_r + 8.0 * n
+
+
+def grid256_rows(args):
+    """The 256 x 256 4-neighbour grid (uint32 weights in [1, 100], source 0):
+    every algorithm's ms per SSSP, GTEPS and fraction of the HBM roofline."""
+    import mi355sssp as M
+    from oracle import orc
+
+    g = M.grid4(256, 256, 1, 100, seed=args.seed)
+    dg = M.DeviceGraph(g)
+    want = orc.dijkstra(g.rowptr, g.col, g.weights, 0)
+    rows = {"n": g.n, "m": g.m, "source": 0}
+    for name in ("bf", "bf_jacobi", "delta", "adds"):
+        o = dg.options(algorithm_id(name))
+        d, _, st = dg.sssp(0, algorithm_id(name))
+        if d.tobytes() != want.tobytes():
+            raise RuntimeError(f"256 x 256 grid: {name} differs from the oracle")
+        for _ in range(2):
+            dg.sssp_device(0, o)
+        reps = 20
+        t0 = time.perf_counter()
+        dev = 0.0
+        for _ in range(reps):
+            dev += dg.sssp_device(0, o).device_ms
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        rows[name] = {"ms_per_sssp": ms, "device_ms_per_sssp": dev / reps,
+                      "gteps": st.reached_edges / (ms * 1e-3) / 1e9,
+                      "roofline_frac": b_alg(st.reached_edges, st.reached, g.n) / (dev / reps * 1e-3) / 1e9
+                      / HBM_PEAK_GBS, "bit_exact": True}
+    log(f"256 x 256 grid: {json.dumps(rows)}")
+    return rows
 
 
 def profile_traffic(path: str):
@@ -115,8 +171,10 @@ def routing_workloads(args, cpu_threads: int):
         log(f"routing grid source {srcs[0]}: dist bit-exact={ok}")
         if not ok:
             raise RuntimeError("routing grid parity failed")
+    ba = b_alg(st.reached_edges, st.reached, g.n)
     for name in res:
         res[name]["gteps"] = st.reached_edges / (res[name]["ms_per_sssp"] * 1e-3) / 1e9
+        res[name]["roofline_frac"] = ba / (res[name]["ms_per_sssp"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     if "learned_adds" in res:
         res["learned_speedup"] = res["plain_adds"]["ms_per_sssp"] / res["learned_adds"]["ms_per_sssp"]
     out["single_source"] = dict(res, sources=len(srcs), n=g.n, m=g.m, reached_edges=int(st.reached_edges))
@@ -292,6 +350,8 @@ def main() -> int:
                     help="0: --policy auto falls back to plain ADDS even when a model is shipped")
     ap.add_argument("--routing", type=int, default=1,
                     help="also run the routing-grid single-source and batched-query workloads")
+    ap.add_argument("--grid256", type=int, default=1,
+                    help="also time every algorithm on the 256 x 256 4-neighbour grid")
     ap.add_argument("--grid-side", type=int, default=4900)
     ap.add_argument("--grid-sources", type=int, default=16)
     ap.add_argument("--queries", type=int, default=4096)
@@ -376,50 +436,73 @@ def main() -> int:
                 print("parity check failed", file=sys.stderr)
                 return 1
 
-    def timed(o, steps, warmup):
+    def step_sources(steps):
+        # steps spread evenly over the seeded sources (distinct while steps <=
+        # their count; cycling beyond)
+        if steps <= len(sources):
+            return np.linspace(0, len(sources) - 1, steps).round().astype(int)
+        return np.arange(steps) % len(sources)
+
+    def timed(o, idx, warmup):
+        # one step = one full SSSP: label initialisation, the SSSP kernels and
+        # the conversion of labels to distances and parents (device outputs)
         for i in range(warmup):
-            dg.sssp_labels(int(sources[i % len(sources)]), o)
+            dg.sssp_device(int(sources[idx[i % len(idx)]]), o)
         M.engine().mi355_stream_sync(None)
         dev_ms = kern_ms = 0.0
         t0 = time.perf_counter()
-        for i in range(steps):
-            st = dg.sssp_labels(int(sources[i % len(sources)]), o)
+        for i in idx:
+            st = dg.sssp_device(int(sources[i]), o)
             dev_ms += st.device_ms
             kern_ms += st.kernel_ms
         M.engine().mi355_stream_sync(None)
         return time.perf_counter() - t0, dev_ms, kern_ms
 
-    wall, dev_ms, kern_ms = timed(opts, args.steps, args.warmup)
-    idx = np.arange(args.steps) % len(sources)
+    idx = step_sources(args.steps)
+    wall, dev_ms, kern_ms = timed(opts, idx, args.warmup)
     edges = float(mr[idx].sum())
     bytes_alg = float((12 * mr[idx] + 20 * nr[idx] + 8 * g.n).sum())
-    ms_per = wall * 1e3 / args.steps
+    ms_per = wall * 1e3 / len(idx)
     gteps = edges / wall / 1e9
     # dominant kernel: the persistent worklist kernel for ADDS (one launch per
     # SSSP); the whole HIP-event span of the run for the multi-launch ones
     kernel_ms = kern_ms if kern_ms > 0 else dev_ms
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
-    log(f"{args.algorithm}: {ms_per:.3f} ms/SSSP wall, {dev_ms / args.steps:.3f} ms device, "
-        f"{kernel_ms / args.steps:.3f} ms kernel, {gteps:.2f} GTEPS")
+    log(f"{args.algorithm}: {ms_per:.3f} ms/SSSP wall, {dev_ms / len(idx):.3f} ms device, "
+        f"{kernel_ms / len(idx):.3f} ms kernel, {gteps:.2f} GTEPS")
+
+    # every seeded source once (when the timed steps covered fewer)
+    all_sources = None
+    if len(np.unique(idx)) < len(sources):
+        aidx = np.arange(len(sources))
+        aw, ad, ak = timed(opts, aidx, 2)
+        all_sources = {"sources": len(sources), "ms_per_sssp": aw * 1e3 / len(sources),
+                       "kernel_ms_per_sssp": (ak if ak > 0 else ad) / len(sources),
+                       "gteps": float(mr.sum()) / aw / 1e9}
+        log(f"all {len(sources)} sources: {json.dumps(all_sources)}")
 
     # plain ADDS (fixed heuristic delta, every workgroup claiming) on the same
     # sources: the comparator of the learned policy
     plain = None
     if args.algorithm == "adds" and policy != "plain" and args.compare_plain:
         po, _ = opts_for("adds", "plain")
-        pw, pd, pk = timed(po, args.steps, 2)
-        plain = {"ms_per_sssp": pw * 1e3 / args.steps, "kernel_ms_per_sssp": pk / args.steps,
+        pw, pd, pk = timed(po, idx, 2)
+        plain = {"ms_per_sssp": pw * 1e3 / len(idx), "kernel_ms_per_sssp": pk / len(idx),
                  "gteps": edges / pw / 1e9, "speedup_of_timed_policy": pw / wall}
+        if all_sources is not None:
+            aw, _, ak = timed(po, np.arange(len(sources)), 2)
+            plain["all_sources_ms_per_sssp"] = aw * 1e3 / len(sources)
+            plain["all_sources_speedup_of_timed_policy"] = aw * 1e3 / len(sources) / all_sources["ms_per_sssp"]
         log(f"plain adds: {json.dumps(plain)}")
 
     others = {}
     if args.compare_steps > 0:
-        cidx = np.arange(args.compare_steps) % len(sources)
+        cidx = step_sources(args.compare_steps)
         for name in ("adds", "delta", "bf"):
             if name == args.algorithm:
                 continue
             o, _ = opts_for(name)
-            w2, d2, _ = timed(o, args.compare_steps, 2)
+            w2, d2, _ = timed(o, cidx, 2)
             others[name] = {"ms_per_sssp": w2 * 1e3 / args.compare_steps,
                             "device_ms_per_sssp": d2 / args.compare_steps,
                             "gteps": float(mr[cidx].sum()) / w2 / 1e9}
@@ -431,17 +514,19 @@ def main() -> int:
         from oracle import orc
 
         cs = sources[: args.cpu_sources]
-        t0 = time.perf_counter()
-        for s in cs:
-            orc.dijkstra(g.rowptr, g.col, g.weights, int(s))
-        cwall = time.perf_counter() - t0
+        with pinned_to_one_core() as core:
+            t0 = time.perf_counter()
+            for s in cs:
+                orc.dijkstra(g.rowptr, g.col, g.weights, int(s))
+            cwall = time.perf_counter() - t0
         cedges = float(mr[: len(cs)].sum())
         cpu = {"value": cedges / cwall / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
                "sample": f"{len(cs)} sources of the same graph, {cwall / len(cs) * 1e3:.0f} ms/SSSP "
-                         f"(oracle fp32 binary-heap Dijkstra, 1 thread, {cpu_model()}, "
-                         f"{os.cpu_count()} host CPUs)",
+                         f"(oracle fp32 binary-heap Dijkstra, 1 thread pinned to core {core}, "
+                         f"{cpu_model()}, {os.cpu_count()} host CPUs)",
                "ms_per_sssp": cwall / len(cs) * 1e3}
 
+    grid = grid256_rows(args) if args.grid256 else None
     routing = routing_workloads(args, min(16, os.cpu_count() or 1)) if args.routing else None
     gemm = batched_gemm(args) if args.gemm else None
     traffic = profile_traffic(TRAFFIC_FILE) if args.algorithm == "adds" and args.scale == 22 else None
@@ -464,7 +549,12 @@ def main() -> int:
                               "plain": "plain fixed-heuristic delta",
                               "feedback": "feedback delta"}[policy] if args.algorithm == "adds" else None,
                    "n": g.n, "m": g.m,
-                   "sources": len(sources), "graph_sha256": g.sha256(),
+                   "sources": len(sources), "sources_timed": int(len(np.unique(idx))),
+                   "step": "one full SSSP from the next source: label initialisation, the SSSP "
+                           "kernel(s) and the label-to-output conversion (distances and parents "
+                           "left in HBM)",
+                   "all_sources": all_sources,
+                   "graph_sha256": g.sha256(),
                    "avg_reached_edges": float(mr.mean()),
                    "graph500_gteps": gteps / 2.0,
                    "device_ms_per_sssp": dev_ms / args.steps,
@@ -472,6 +562,7 @@ def main() -> int:
                    "policy_used": policy_used if args.algorithm == "adds" else None,
                    "plain_adds": plain,
                    "others": others,
+                   "grid_256x256_u32": grid,
                    "routing_grid": routing,
                    "batched_minplus_gemm": gemm},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "mi355graph.h"
@@ -81,14 +82,34 @@ void parse_args(int argc, char** argv) {
   }
 }
 
-int env_int(const char* name, int def, int lo, int hi) {
+// Integer environment knob in [lo, hi]; unset or empty gives def.  A value
+// that is not a decimal integer in range ends the program, as the reference
+// driver does (ADDSF/kernel.hip:34-51).
+unsigned env_unsigned(const char* name, unsigned def, unsigned lo, unsigned hi) {
   const char* v = std::getenv(name);
   if (!v || !*v) return def;
+  errno = 0;
+  char* end = nullptr;
+  const unsigned long long x = std::strtoull(v, &end, 10);
+  const bool digits = std::strspn(v, "0123456789") == std::strlen(v);
+  if (errno != 0 || !digits || end == v || *end != '\0' || x < lo || x > hi) {
+    std::fprintf(stderr, "Invalid %s='%s'; expected an integer in [%u, %u]\n", name, v, lo, hi);
+    std::exit(EXIT_FAILURE);
+  }
+  return (unsigned)x;
+}
+
+// ADDS_RUNS: repetitions in [1, 1000]; an invalid value is reported and the
+// default used (the reference's behaviour for this knob, ADDSF/kernel.hip:365-379).
+int runs_from_env() {
+  constexpr int kDefaultRuns = 8;
+  const char* v = std::getenv("ADDS_RUNS");
+  if (!v || !*v) return kDefaultRuns;
   char* end = nullptr;
   const long x = std::strtol(v, &end, 10);
-  if (end == v || *end != '\0' || x < lo || x > hi) {
-    std::fprintf(stderr, "warning: ignoring invalid %s='%s'; using %d\n", name, v, def);
-    return def;
+  if (end == v || *end != '\0' || x < 1 || x > 1000) {
+    std::fprintf(stderr, "warning: ignoring invalid ADDS_RUNS='%s'; using %d\n", v, kDefaultRuns);
+    return kDefaultRuns;
   }
   return (int)x;
 }
@@ -156,8 +177,26 @@ int main(int argc, char** argv) {
   mi355_sssp_options_default(&o);
   o.algorithm = MI355_ALG_ADDS;
   o.inf_mode = MI355_INF_FLT_MAX;
-  o.worker_blocks = env_int("ADDS_WORKER_BLOCKS", 0, 0, 1 << 20);
-  const int runs = env_int("ADDS_RUNS", 8, 1, 1000);
+  // Worker pool: ADDS_RESERVED_CUS CUs are left out of it (the reference
+  // keeps them for its manager kernel, ADDSF/kernel.hip:468-480; this engine
+  // has no manager, so they simply stay idle), ADDS_WORKER_BLOCKS picks the
+  // number of persistent workgroups up to what the remaining CUs hold.
+  int32_t default_workers = 0;
+  if (mi355_adds_default_workers(&default_workers) != MI355_OK || default_workers < 4)
+    die_engine("device query failed");
+  const unsigned per_cu = 2, cus = (unsigned)default_workers / per_cu;
+  const unsigned reserved = env_unsigned("ADDS_RESERVED_CUS", 1u, 1u, cus - 1);
+  const unsigned max_workers = per_cu * (cus - reserved);
+  o.worker_blocks = (int32_t)env_unsigned("ADDS_WORKER_BLOCKS", max_workers, 1u, max_workers);
+  // ADDS_WL_ENTRIES: the reference's worklist size (ADDSF/kernel.hip:496-510),
+  // checked against the same range; the engine sizes its bucket rings from
+  // the vertex count itself, so the value does not bound them.
+  const uint64_t wl_min = 32ull * 65536ull * 8ull, wl_max = 536870912ull;
+  const uint64_t wl_def = std::min<uint64_t>((uint64_t)std::ceil(1.5 * (double)v.nnz) + wl_min, wl_max);
+  const unsigned wl_entries = env_unsigned("ADDS_WL_ENTRIES", (unsigned)wl_def, (unsigned)wl_min, (unsigned)wl_max);
+  const int runs = runs_from_env();
+  std::printf("workers: %d workgroup(s) of 512 threads (8 waves), reserved CUs=%u, worklist entries=%u\n",
+              o.worker_blocks, reserved, wl_entries);
   void* d_dist = nullptr;
   if (mi355_device_malloc(&d_dist, 4 * (size_t)v.rows) != MI355_OK) die_engine("allocation failed");
 

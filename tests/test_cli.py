@@ -86,3 +86,31 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120, cwd=root)
     assert r.returncode == 2
     assert "single MI355X" in r.stderr and r.stdout == ""
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,value", [("ADDS_WORKER_BLOCKS", "0"), ("ADDS_WORKER_BLOCKS", "12x"),
+                                        ("ADDS_WORKER_BLOCKS", "99999999"), ("ADDS_RESERVED_CUS", "0"),
+                                        ("ADDS_RESERVED_CUS", "-1"), ("ADDS_WL_ENTRIES", "5")])
+def test_invalid_knobs_end_the_program(tmp_path, name, value):
+    # invalid ADDS_WORKER_BLOCKS / ADDS_RESERVED_CUS / ADDS_WL_ENTRIES exit
+    # with the accepted range (ADDSF/kernel.hip:34-51, 468-510)
+    path, _ = _tiny_gr(tmp_path)
+    r = _run(["-s", "0", path], env=dict(os.environ, **{name: value}))
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert f"Invalid {name}='{value}'; expected an integer in [" in r.stderr
+
+
+@pytest.mark.gpu
+def test_valid_knobs_and_invalid_runs(tmp_path):
+    path, ka = _tiny_gr(tmp_path)
+    out = tmp_path / "d.txt"
+    env = dict(os.environ, ADDS_WORKER_BLOCKS="16", ADDS_RESERVED_CUS="8", ADDS_WL_ENTRIES="20000000",
+               ADDS_RUNS="many")
+    r = _run(["-s", str(ka["source"]), "-o", str(out), path], env=env)
+    assert r.returncode == 0, r.stderr
+    assert "warning: ignoring invalid ADDS_RUNS='many'; using 8" in r.stderr
+    assert r.stdout.count("processed vertices=") == 8
+    assert "workers: 16 workgroup(s)" in r.stdout and "reserved CUs=8" in r.stdout
+    want = [f"{i} INF" if d is None else f"{i} {d:.9g}" for i, d in enumerate(ka["expected"])]
+    assert out.read_text().splitlines() == want
