@@ -200,6 +200,19 @@ int mi355_minplus_gemm_f32_batched(int count, const float* const* dA, const floa
                                    float* const* dC, const int* M, const int* N, const int* K,
                                    const int* lda, const int* ldb, const int* ldc, void* stream);
 
+/* No counterpart in the reference: the batched product above for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  Same arguments and results; the batch's descriptors and work list
+ * go into a fixed 4 MiB array per GPU that the library holds until the
+ * process ends, so no hipMalloc / hipFree runs.  At most 4096 products and
+ * about 500K output tiles per call (MI355_ECAPACITY beyond).  Every call on a
+ * GPU reuses that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be ordered by the caller.  No plan is kept
+ * between calls.  Asynchronous. */
+int mi355_minplus_gemm_f32_batched_noalloc(int count, const float* const* dA, const float* const* dB,
+                                           float* const* dC, const int* M, const int* N, const int* K,
+                                           const int* lda, const int* ldb, const int* ldc, void* stream);
+
 /* Project autotuner for the min-plus tile (no reference counterpart; the
  * reference hard-codes 64x64x32 tiles, MP/src/minplus_hip.cpp:144-151).  Times
  * every tile variant on an n x n x n product for each n in sizes and keeps the

@@ -460,6 +460,28 @@ int mi355_minplus_gemm_f32_batched(int count, const float* const* dA, const floa
   });
 }
 
+int mi355_minplus_gemm_f32_batched_noalloc(int count, const float* const* dA, const float* const* dB,
+                                           float* const* dC, const int* M, const int* N, const int* K,
+                                           const int* lda, const int* ldb, const int* ldc, void* stream) {
+  return guard([&] {
+    if (count < 0) fail(MI355_EINVAL, "negative batch count");
+    if (count == 0) return;
+    if (!dA || !dB || !dC || !M || !N || !K || !lda || !ldb || !ldc) fail(MI355_EINVAL, "null array");
+    if ((size_t)count > mi355::minplus_noalloc_max_items())
+      fail(MI355_ECAPACITY, "no-allocation min-plus batch holds at most 4096 products");
+    thread_local std::vector<mi355::MinplusItem> items;
+    items.resize((size_t)count);
+    for (int i = 0; i < count; ++i) {
+      if (!dA[i] || !dB[i] || !dC[i]) fail(MI355_EINVAL, "null matrix pointer in batch");
+      if (M[i] <= 0 || N[i] <= 0 || K[i] <= 0) fail(MI355_EINVAL, "matrix sizes must be positive");
+      if (lda[i] < K[i] || ldb[i] < N[i] || ldc[i] < N[i])
+        fail(MI355_EINVAL, "leading dimension smaller than the matrix");
+      items[(size_t)i] = mi355::MinplusItem{dA[i], dB[i], dC[i], M[i], N[i], K[i], lda[i], ldb[i], ldc[i], 0};
+    }
+    mi355::minplus_gemm_batched_noalloc(items, as_stream(stream));
+  });
+}
+
 int mi355_minplus_tile_count(void) { return mi355::minplus_tile_count(); }
 const char* mi355_minplus_tile_name(int index) { return mi355::minplus_tile_name(index); }
 

@@ -249,18 +249,14 @@ void minplus_gemm(const float* A, const float* B, float* C, int M, int N, int K,
   MI_LAUNCH_CHECK();
 }
 
-// Descriptors and work list of a batch, uploaded into the cache's buffers.
-static void build_batch(const std::vector<MinplusItem>& items, hipStream_t s, MinplusBatchCache& cache) {
-  DevBuf<MinplusItem>& table = cache.table;
-  DevBuf<int2>& work = cache.work;
-  // One launch per tile shape (each product takes its size class's tuned
-  // tile, else the static rule over the whole batch's tile count) over a flat
-  // list of (product, tile) work entries: every workgroup owns a real tile,
-  // whatever the spread of sizes.  512 products of n in [32, 512] on the
-  // 64x64 full-unroll tile: one grid sized for the largest product, 0.93 ms;
-  // one grid per size class, 0.75 ms; the work list, 0.54 ms.
-  table.ensure(items.size());
-  MI_HIP(hipMemcpyAsync(table.p, items.data(), items.size() * sizeof(MinplusItem), hipMemcpyHostToDevice, s));
+// Work list of a batch: one launch per tile shape (each product takes its
+// size class's tuned tile, else the static rule over the whole batch's tile
+// count) over a flat list of (product, tile) entries, so every workgroup owns
+// a real tile whatever the spread of sizes.  512 products of n in [32, 512]
+// on the 64x64 full-unroll tile: one grid sized for the largest product,
+// 0.93 ms; one grid per size class, 0.75 ms; the work list, 0.54 ms.
+void minplus_plan(const std::vector<MinplusItem>& items, std::vector<int2>& flat,
+                  std::vector<std::pair<int, uint32_t>>& launches) {
   uint64_t all_tiles64 = 0;
   for (const MinplusItem& it : items) all_tiles64 += tiles64(it.M, it.N);
   std::vector<std::vector<TileRun>> lists(kNumTiles);
@@ -285,14 +281,35 @@ static void build_batch(const std::vector<MinplusItem>& items, hipStream_t s, Mi
     if (ordered[t].size() > (size_t)UINT32_MAX / 256) fail(MI355_EINVAL, "min-plus batch too large");
     total += ordered[t].size();
   }
-  std::vector<int2> flat;
+  flat.clear();
   flat.reserve(total);
   for (auto& o : ordered) flat.insert(flat.end(), o.begin(), o.end());
+  launches.clear();
+  for (int t = 0; t < kNumTiles; ++t)
+    if (!ordered[t].empty()) launches.emplace_back(t, (uint32_t)ordered[t].size());
+}
+
+// The launches of a plan whose descriptors and work list are on the device.
+void minplus_launch_plan(const std::vector<std::pair<int, uint32_t>>& launches, const MinplusItem* d_table,
+                         const int2* d_work, hipStream_t s) {
+  size_t off = 0;
+  for (const auto& l : launches) {
+    hipLaunchKernelGGL(kTiles[l.first].fn, dim3(l.second), dim3(256), 0, s, d_table, d_work + off);
+    MI_LAUNCH_CHECK();
+    off += l.second;
+  }
+}
+
+// Descriptors and work list of a batch, uploaded into the cache's buffers.
+static void build_batch(const std::vector<MinplusItem>& items, hipStream_t s, MinplusBatchCache& cache) {
+  DevBuf<MinplusItem>& table = cache.table;
+  DevBuf<int2>& work = cache.work;
+  table.ensure(items.size());
+  MI_HIP(hipMemcpyAsync(table.p, items.data(), items.size() * sizeof(MinplusItem), hipMemcpyHostToDevice, s));
+  std::vector<int2> flat;
+  minplus_plan(items, flat, cache.launches);
   work.ensure(flat.size());
   MI_HIP(hipMemcpyAsync(work.p, flat.data(), flat.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-  cache.launches.clear();
-  for (int t = 0; t < kNumTiles; ++t)
-    if (!ordered[t].empty()) cache.launches.emplace_back(t, (uint32_t)ordered[t].size());
 }
 
 void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s, MinplusBatchCache& cache) {
@@ -323,13 +340,7 @@ void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s, 
     cache.tune_gen = gen;
     cache.valid = true;
   }
-  size_t off = 0;
-  for (const auto& l : cache.launches) {
-    hipLaunchKernelGGL(kTiles[l.first].fn, dim3(l.second), dim3(256), 0, s,
-                       static_cast<const MinplusItem*>(cache.table.p), static_cast<const int2*>(cache.work.p + off));
-    MI_LAUNCH_CHECK();
-    off += l.second;
-  }
+  minplus_launch_plan(cache.launches, cache.table.p, cache.work.p, s);
   MI_HIP(hipEventRecord(cache.done, s));
 }
 

@@ -45,6 +45,19 @@ struct MinplusBatchCache {
 };
 
 void minplus_gemm_batched(const std::vector<MinplusItem>& items, hipStream_t s, MinplusBatchCache& cache);
+// The (product, tile) work list of a batch and its launches (tile shape,
+// workgroups), in work-list order (host only).
+void minplus_plan(const std::vector<MinplusItem>& items, std::vector<int2>& work,
+                  std::vector<std::pair<int, uint32_t>>& launches);
+// Launch a plan whose descriptors and work list are in device memory.
+void minplus_launch_plan(const std::vector<std::pair<int, uint32_t>>& launches, const MinplusItem* d_table,
+                         const int2* d_work, hipStream_t s);
+// The batch with its plan in the library's fixed per-GPU array
+// (minplus_noalloc.hip): no allocation; at most minplus_noalloc_max_items()
+// products and minplus_noalloc_max_tiles() work entries.
+void minplus_gemm_batched_noalloc(const std::vector<MinplusItem>& items, hipStream_t s);
+size_t minplus_noalloc_max_items();
+size_t minplus_noalloc_max_tiles();
 void minplus_autotune(const int* sizes, int count, hipStream_t s, double* report);
 void minplus_tuning_get(int* table32);
 void minplus_tuning_set(const int* table32);

@@ -309,6 +309,7 @@ class AddsStats(C.Structure):
 _ENGINE_SIGS.update({
     "mi355_minplus_gemm_f32": (C.c_int, [P, P, P] + [C.c_int] * 6 + [P]),
     "mi355_minplus_gemm_f32_batched": (C.c_int, [C.c_int] + [P] * 9 + [P]),
+    "mi355_minplus_gemm_f32_batched_noalloc": (C.c_int, [C.c_int] + [P] * 9 + [P]),
     "mi355_minplus_tile_count": (C.c_int, []),
     "mi355_minplus_tile_name": (C.c_char_p, [C.c_int]),
     "mi355_minplus_autotune": (C.c_int, [P, C.c_int, P, P]),
