@@ -115,15 +115,16 @@ def grid256_rows(args):
     return rows
 
 
-def profile_traffic(path: str):
-    """HBM bytes per launch of the persistent ADDS kernel on this workload,
-    from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE per
-    dispatch, scaled by the copy-kernel calibration it records); None if the
-    summary is absent."""
+def profile_traffic(path: str, key: str = "traffic_bytes_per_launch"):
+    """Memory traffic per launch of the persistent ADDS kernel on this
+    workload, from the committed rocprofv3 --pmc summary (FETCH_SIZE +
+    WRITE_SIZE per dispatch, scaled by the calibration it records: the copy
+    kernel's factors for `traffic_bytes_per_launch`, the random-gather
+    kernel's for `traffic_bytes_per_launch_random`); None if absent."""
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -180,6 +181,8 @@ def routing_workloads(args, cpu_threads: int):
     out["single_source"] = dict(res, sources=len(srcs), n=g.n, m=g.m, reached_edges=int(st.reached_edges))
     log(f"routing grid single source: {json.dumps(out['single_source'])}")
 
+    if not args.routing_batch:
+        return out
     # batched box-limited multi-source queries
     qs = B.routing_queries(side, side, args.queries, seed=args.seed)
     if args.batch_graph:
@@ -346,6 +349,11 @@ def main() -> int:
                     help="0: skip the plain ADDS comparator (profiling runs of one policy)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size (counter calibration runs)")
+    ap.add_argument("--calib-gather-mib", default="",
+                    help="random 8-byte gathers from arrays of these sizes, comma-separated, in order "
+                         "(counter calibration runs)")
+    ap.add_argument("--routing-batch", type=int, default=1,
+                    help="0: the routing-grid legs stop after the single-source one")
     ap.add_argument("--learned", type=int, default=1,
                     help="0: --policy auto falls back to plain ADDS even when a model is shipped")
     ap.add_argument("--routing", type=int, default=1,
@@ -381,6 +389,20 @@ def main() -> int:
     log(f"uploaded in {time.perf_counter() - t0:.1f}s; max degree {dg.info.max_degree}, "
         f"hub chunks {dg.info.large_chunks}")
 
+    for mib in [int(x) for x in args.calib_gather_mib.split(",") if x]:
+        # random 8-byte gather from an array of the given size (64M records
+        # at uniform random indices): what a random access moves per counter
+        nbytes = mib << 20
+        count = 1 << 26
+        src = M.DeviceBuffer(nbytes)
+        M.engine().mi355_memset_d8(src.ptr, 1, nbytes, None)
+        idx = np.random.default_rng(args.seed).integers(0, nbytes // 8, count, dtype=np.uint64).astype(np.uint32)
+        d_idx = M.DeviceBuffer.from_array(idx)
+        dst = M.DeviceBuffer(8 * count)
+        M._abi.check(M.engine().mi355_device_gather_u64(dst.ptr, src.ptr, d_idx.ptr, count, None))
+        M.engine().mi355_stream_sync(None)
+        log(f"calibration gather of {count} random 8-byte records from {nbytes} bytes done")
+        del src, d_idx, dst
     if args.calib_copy_mib > 0:
         nbytes = args.calib_copy_mib << 20
         a = M.DeviceBuffer(nbytes)
@@ -529,7 +551,9 @@ def main() -> int:
     grid = grid256_rows(args) if args.grid256 else None
     routing = routing_workloads(args, min(16, os.cpu_count() or 1)) if args.routing else None
     gemm = batched_gemm(args) if args.gemm else None
-    traffic = profile_traffic(TRAFFIC_FILE) if args.algorithm == "adds" and args.scale == 22 else None
+    headline = args.algorithm == "adds" and args.scale == 22
+    traffic = profile_traffic(TRAFFIC_FILE) if headline else None
+    traffic_random = profile_traffic(TRAFFIC_FILE, "traffic_bytes_per_launch_random") if headline else None
     out = {
         "metric": "SSSP GTEPS + ms/query (learned-delta & batched vs plain ADDS), bit-exact dist",
         "value": gteps,
@@ -562,6 +586,8 @@ def main() -> int:
                    "policy_used": policy_used if args.algorithm == "adds" else None,
                    "plain_adds": plain,
                    "others": others,
+                   "traffic_random_calibrated": traffic_random,
+                   "algorithmic_bytes_per_sssp": bytes_alg / len(idx),
                    "grid_256x256_u32": grid,
                    "routing_grid": routing,
                    "batched_minplus_gemm": gemm},

@@ -48,6 +48,10 @@ int mi355_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int mi355_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int mi355_memset_d8(void* dst, int value, size_t bytes, void* stream);
 int mi355_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+/* dst[i] = src[idx[i]] for count 8-byte records (device pointers; idx 32-bit).
+ * No reference counterpart: the random-access reference point of the HBM
+ * traffic counters.  Asynchronous. */
+int mi355_device_gather_u64(void* dst, const void* src, const uint32_t* idx, int64_t count, void* stream);
 int mi355_stream_create(void** stream);
 int mi355_stream_destroy(void* stream);
 int mi355_stream_sync(void* stream);

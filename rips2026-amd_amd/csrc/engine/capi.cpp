@@ -106,6 +106,9 @@ int mi355_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
     MI_HIP(hipStreamSynchronize(as_stream(stream)));
   });
 }
+int mi355_device_gather_u64(void* dst, const void* src, const uint32_t* idx, int64_t count, void* stream) {
+  return guard([&] { mi355::device_gather8(dst, src, idx, count, as_stream(stream)); });
+}
 int mi355_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
   return guard([&] { mi355::device_copy(dst, src, bytes, as_stream(stream)); });
 }

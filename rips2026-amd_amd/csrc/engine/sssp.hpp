@@ -59,6 +59,8 @@ void labels_to_output(mi355_graph& g, uint32_t source, int inf_mode, void* d_dis
 // Delta-stepping driver (delta.hip).
 // Streaming device-to-device copy (copy.hip).
 void device_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+// dst[i] = src[idx[i]] for 8-byte records, i < count (copy.hip).
+void device_gather8(void* dst, const void* src, const uint32_t* idx, int64_t count, hipStream_t stream);
 
 // Light edges (w <= delta) of every vertex, a prefix of its weight-sorted row;
 // cached on the graph for the last delta asked for.

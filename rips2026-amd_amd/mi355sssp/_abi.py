@@ -141,6 +141,7 @@ _ENGINE_SIGS = {
     "mi355_memcpy_h2d": (C.c_int, [P, P, C.c_size_t, P]),
     "mi355_memcpy_d2h": (C.c_int, [P, P, C.c_size_t, P]),
     "mi355_memset_d8": (C.c_int, [P, C.c_int, C.c_size_t, P]),
+    "mi355_device_gather_u64": (C.c_int, [P, P, P, C.c_int64, P]),
     "mi355_memcpy_d2d": (C.c_int, [P, P, C.c_size_t, P]),
     "mi355_stream_create": (C.c_int, [C.POINTER(P)]),
     "mi355_stream_destroy": (C.c_int, [P]),
