@@ -94,6 +94,16 @@ __host__ __device__ inline int64_t tiled_area(int64_t bh, int64_t bw) {
 constexpr uint32_t kRoundIdle = 0, kRoundRelax = 1, kRoundFar = 2;
 constexpr uint32_t kNearStage = 256;  // staged near appends per wave
 constexpr uint32_t kFarStage = 128;   // staged far appends per wave
+// Wave-local queue of a relax round: a slot that a wave improves below the
+// threshold is relaxed again by the same wave later in the round, instead of
+// waiting for the next round (its label line is still in this XCD's L2).  Such
+// entries take no round stamp, so an improvement that arrives after the wave
+// has relaxed the slot is pushed by its own producer as usual.  At most
+// kLocalBudget local entries per wave and round; beyond that, and when the
+// ring is full, improved slots go to the next round.
+constexpr uint32_t kLocalQ = 512;
+constexpr uint32_t kLocalBudget = 4096;
+static_assert((kLocalQ & (kLocalQ - 1)) == 0, "local queue is a power-of-two ring");
 
 struct BatchCtrl {
   alignas(128) uint32_t cnt[2 * kParts * kCntStride];   // near queue sizes (round parity, part)
@@ -117,8 +127,9 @@ struct BatchArgs {
   const int* rowptr;
   const uint2* edges;
   unsigned long long* labels;
-  uint32_t* stamp;    // round stamp of the slot's last near append
-  uint32_t* fstamp;   // far stamp of the slot's last far append
+  // per slot: round stamp of its last near append (.x) and far stamp of its
+  // last far append (.y), side by side so that one 8-byte probe reads both
+  uint2* stamps;
   const QueryInfo* qinfo;
   uint2* q[2];        // near queues of (query, local index)
   uint2* far[2];      // far piles
@@ -232,6 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
   __shared__ int s_q[kWavesPerBlock][kWave];
   __shared__ uint2 s_near[kWavesPerBlock][kNearStage];
   __shared__ uint2 s_far[kWavesPerBlock][kFarStage];
+  __shared__ uint2 s_loc[kWavesPerBlock][kLocalQ];
   BatchCtrl* c = a.ctrl;
   const uint32_t mode = c->mode;
   if (mode == kRoundIdle) return;
@@ -268,8 +280,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
       const bool to_near = ok && key >= t_old && key < t_new;
       const bool keep = ok && key >= t_new;
       uint32_t x = stamp, y = fst;
-      if (to_near && a.stamp[slot] != stamp) x = atomicExch(a.stamp + slot, stamp);
-      if (keep && a.fstamp[slot] != fst) y = atomicExch(a.fstamp + slot, fst);
+      const uint2 sf = (to_near || keep) ? a.stamps[slot] : make_uint2(stamp, fst);
+      if (to_near && sf.x != stamp) x = atomicExch(&a.stamps[slot].x, stamp);
+      if (keep && sf.y != fst) y = atomicExch(&a.stamps[slot].y, fst);
       if (keep) fmin_lane = min(fmin_lane, key);
       stage_entry(x != stamp, it, nst, n_staged, kNearStage, qout, cnt_out, cap, &c->err);
       stage_entry(y != fst, it, fst_stage, f_staged, kFarStage, fout, fcnt_out, cap, &c->err);
@@ -277,12 +290,26 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
   } else {
     const uint32_t n_in = c->n_in[part], tkey = c->tkey;
     const uint2* qin = a.q[k & 1] + seg;
-    for (uint32_t first = (pwg * kWavesPerBlock + wv) * kWave; first < n_in; first += nwaves * kWave) {
-      const uint32_t i = first + lane;
+    uint2* loc = s_loc[wv];
+    uint32_t lhead = 0, ltail = 0, lbudget = kLocalBudget;  // wave-uniform
+    uint32_t first = (pwg * kWavesPerBlock + wv) * kWave;
+    for (;;) {
+      // the wave's local queue first, then its next chunk of the round's input
+      const bool from_local = lhead != ltail;
+      if (!from_local && first >= n_in) break;
+      uint32_t take = 0;
+      if (from_local) {
+        take = min(ltail - lhead, (uint32_t)kWave);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      const uint32_t i = from_local ? lane : first + lane;
+      const bool has = from_local ? lane < take : i < n_in;
       int u = 0, beg = 0, deg = 0, qid = 0;
       uint32_t key = W::kInfKey;
-      if (i < n_in) {
-        const uint2 it = qin[i];
+      if (has) {
+        const uint2 it = from_local ? loc[(lhead + lane) & (kLocalQ - 1u)] : qin[i];
         qid = (int)it.x;
         const QueryInfo qi = a.qinfo[qid];
         uint32_t r, col;
@@ -299,6 +326,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
         if ((int)lane >= off) incl += t;
       }
       const int total = __shfl(incl, kWave - 1, kWave);
+      if (from_local) lhead += take;  // entries read; their ring cells may be reused
+      else first += nwaves * kWave;
       s_incl[wv][lane] = incl;
       s_beg[wv][lane] = beg - (incl - deg);
       s_key[wv][lane] = key;
@@ -354,8 +383,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
           // label probe and both stamps (the stamp reads are only filters:
           // the exchanges below decide)
           cur[p] = a.labels[slot[p]];
-          so[p] = a.stamp[slot[p]];
-          fo[p] = a.fstamp[slot[p]];
+          const uint2 sf = a.stamps[slot[p]];
+          so[p] = sf.x;
+          fo[p] = sf.y;
         }
         asm volatile("" : "+v"(cur[0]), "+v"(cur[1]), "+v"(cur[2]), "+v"(cur[3]));
         asm volatile("" : "+v"(so[0]), "+v"(so[1]), "+v"(so[2]), "+v"(so[3]), "+v"(fo[0]), "+v"(fo[1]),
@@ -378,10 +408,25 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
           x[p] = stamp;  // already stamped: no push
           y[p] = fst;
         }
+        // near improvements this wave relaxes again later in the round, while
+        // its ring has room and its budget lasts
+        bool lc[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-          if (nr[p] && so[p] != stamp) x[p] = atomicExch(a.stamp + slot[p], stamp);
-          if (fr[p] && fo[p] != fst) y[p] = atomicExch(a.fstamp + slot[p], fst);
+          const unsigned long long m = __ballot(nr[p]);
+          const uint32_t kk = (uint32_t)__popcll(m);
+          lc[p] = false;
+          if (kk && ltail - lhead + kk <= kLocalQ && kk <= lbudget) {
+            if (nr[p]) loc[(ltail + prefix_count(m)) & (kLocalQ - 1u)] = make_uint2((uint32_t)qq[p], lv[p]);
+            lc[p] = nr[p];
+            ltail += kk;
+            lbudget -= kk;
+          }
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          if (nr[p] && !lc[p] && so[p] != stamp) x[p] = atomicExch(&a.stamps[slot[p]].x, stamp);
+          if (fr[p] && fo[p] != fst) y[p] = atomicExch(&a.stamps[slot[p]].y, fst);
         }
         asm volatile("" : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(y[0]), "+v"(y[1]),
                      "+v"(y[2]), "+v"(y[3]));
@@ -413,14 +458,13 @@ __global__ void k_batch_epoch(BatchCtrl* c, int K) {
 }
 
 template <class W>
-__global__ void k_batch_init(unsigned long long* __restrict__ labels, uint32_t* __restrict__ stamp,
-                             uint32_t* __restrict__ fstamp, int64_t total) {
+__global__ void k_batch_init(unsigned long long* __restrict__ labels, uint2* __restrict__ stamps,
+                             int64_t total) {
   const unsigned long long inf_label = ((unsigned long long)W::kInfKey << 32) | kNoPred;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
     labels[i] = inf_label;
-    stamp[i] = 0u;
-    fstamp[i] = 0u;
+    stamps[i] = make_uint2(0u, 0u);
   }
 }
 
@@ -620,11 +664,10 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   // state cached on the graph handle and grown as needed
   Workspace& ws = g.ws;
   ws.batch_labels.ensure((size_t)slots);
-  ws.batch_stamp.ensure((size_t)slots);
+  ws.batch_stamp.ensure((size_t)slots * 2);  // uint2 per slot
   ws.batch_q[0].ensure((size_t)std::max<int64_t>(qtotal, 1));
   ws.batch_q[1].ensure((size_t)std::max<int64_t>(qtotal, 1));
   if (bucketed) {
-    ws.batch_fstamp.ensure((size_t)slots);
     ws.batch_far[0].ensure((size_t)std::max<int64_t>(qtotal, 1));
     ws.batch_far[1].ensure((size_t)std::max<int64_t>(qtotal, 1));
   }
@@ -647,10 +690,9 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   // the rectangles are in the input numbering's grid geometry
   input_order_arrays(g, &a.rowptr, &a.edges, s);
   a.labels = bs.labels;
-  a.stamp = bs.stamp;
-  // Bellman-Ford rounds never touch the far stamps or piles: they alias the
-  // round stamps and the queues (never read there)
-  a.fstamp = bucketed ? ws.batch_fstamp.p : bs.stamp;
+  a.stamps = reinterpret_cast<uint2*>(bs.stamp);
+  // Bellman-Ford rounds never touch the far piles: they alias the queues
+  // (never read there)
   a.qinfo = bs.qinfo;
   a.q[0] = bs.q[0];
   a.q[1] = bs.q[1];
@@ -668,7 +710,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   MI_HIP(hipEventCreate(&ev1));
   MI_HIP(hipEventRecord(ev0, s));
   k_batch_init<W><<<(unsigned)std::min<int64_t>(div_up((uint64_t)slots, kBlock), 8192), kBlock, 0, s>>>(
-      bs.labels, bs.stamp, a.fstamp, slots);
+      bs.labels, a.stamps, slots);
   k_batch_sources<W><<<(unsigned)div_up((uint64_t)std::max<int64_t>(nsrc, 1), kBlock), kBlock, 0, s>>>(
       bs.labels, bs.qinfo, bs.src, (uint32_t)nsrc, bs.q[0], ctrl, tkey0, po);
   MI_LAUNCH_CHECK();

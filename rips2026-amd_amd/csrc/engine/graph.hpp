@@ -52,8 +52,7 @@ struct Workspace {
   DevBuf<AddsPolicy> adds_policy;
   // batched multi-source queries (batch.hip)
   DevBuf<unsigned long long> batch_labels;
-  DevBuf<uint32_t> batch_stamp;
-  DevBuf<uint32_t> batch_fstamp;  // near-far: far stamp per label slot
+  DevBuf<uint32_t> batch_stamp;   // per slot: round stamp, far stamp (interleaved)
   DevBuf<uint2> batch_q[2];
   DevBuf<uint2> batch_far[2];     // near-far: the two far piles
   DevBuf<unsigned char> batch_qinfo;

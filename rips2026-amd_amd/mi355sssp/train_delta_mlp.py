@@ -38,6 +38,9 @@ def main() -> int:
                     help="collect only the training specs of this graph kind")
     ap.add_argument("--large-grids", type=int, default=0,
                     help="collect this many 4e6-3e7 vertex grids instead (learn.large_grid_specs)")
+    ap.add_argument("--max-rmat-scale", type=int, default=None,
+                    help="leave out R-MAT specs above this scale (21: the training set of scales 16-21 "
+                         "and grids of 1e5-2e6 vertices, evaluated on held-out scale 22 / 24M graphs)")
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--budget-s", type=float, default=None, help="stop collecting after this long")
     ap.add_argument("--shard-out", default=None, help="write the collected shard here (.npz)")
@@ -61,6 +64,8 @@ def main() -> int:
             specs = L.training_specs(args.total, args.seed)[args.first:args.first + args.graphs]
         if args.kind:
             specs = [s for s in specs if s["kind"] == args.kind]
+        if args.max_rmat_scale is not None:
+            specs = [s for s in specs if s["kind"] != "rmat" or s["scale"] <= args.max_rmat_scale]
         t0 = time.time()
         x, y, w, metas = L.collect(specs, log=log, reps=args.reps, time_budget_s=args.budget_s)
         log(f"collected {len(metas)} graphs, {len(x)} samples in {time.time() - t0:.0f}s")
