@@ -321,3 +321,23 @@ def test_device_copy(nbytes, off):
     got = b.to_array(np.uint8, nbytes + 64)
     assert np.array_equal(got[off:off + nbytes], src[off:off + nbytes])
     assert np.all(got[:off] == 0xA5) and np.all(got[off + nbytes:] == 0xA5)
+
+
+def test_minplus_signed_zero_ties():
+    """The one documented divergence from the reference's k-ordered select:
+    where -0.0 (from -0.0 + -0.0) ties with +0.0 the zero's sign may differ;
+    the value, and every other entry, is exact."""
+    rng = np.random.default_rng(5)
+    n = 64
+    A = (rng.integers(1, 9, (n, n)) * 0.25).astype(np.float32)
+    B = (rng.integers(1, 9, (n, n)) * 0.25).astype(np.float32)
+    A[:8, :] = 0.0
+    B[:, :8] = 0.0
+    A[:8, ::2] = -0.0
+    B[::2, :8] = -0.0
+    got = M.minplus_gemm(A, B)
+    want = np.asarray(orc.minplus_gemm(A, B, n, n, n), np.float32).reshape(n, n)
+    assert np.array_equal(got, want)  # equal values (+0.0 == -0.0)
+    zero = want == 0.0
+    assert zero[:8, :8].all() and zero.sum() == 64
+    assert got[~zero].tobytes() == want[~zero].tobytes()
