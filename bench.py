@@ -201,6 +201,20 @@ def routing_workloads(args, cpu_threads: int):
     _, _, off, bst = B.run(dg, qs, want_pred=True, download=False, **opts)
     wall = time.perf_counter() - t0
     qps = qs.count / wall
+    # the same batch with 1-8 sinks per net: each query stops once its sinks
+    # are settled (what a router reads back)
+    sinks = None
+    if args.routing_sinks > 0:
+        tg = B.routing_sinks(qs, seed=args.seed + 1, max_sinks=args.routing_sinks)
+        B.run(dg, qs, want_pred=False, download=False, targets=tg, **opts)  # warm
+        t0 = time.perf_counter()
+        _, _, _, sst = B.run(dg, qs, want_pred=True, download=False, targets=tg, **opts)
+        swall = time.perf_counter() - t0
+        sinks = {"max_sinks_per_query": args.routing_sinks, "sinks": int(len(tg[1])), "wall_s": swall,
+                 "device_ms": float(sst.device_ms), "queries_per_s": qs.count / swall,
+                 "stopped_queries": int(sst.stopped_queries), "rounds": int(sst.rounds),
+                 "far_passes": int(sst.far_passes), "speedup_vs_full": wall / swall}
+        log(f"batched queries with sinks: {json.dumps(sinks)}")
     # one at a time: plain ADDS on each sampled query's induced subgraph
     # (kernel time only; building the subgraph is not counted)
     sample = np.linspace(0, qs.count - 1, args.one_at_a_time).astype(int)
@@ -264,6 +278,7 @@ def routing_workloads(args, cpu_threads: int):
                                      "estimated_total_s": one_total_s,
                                      "queries_per_s": qs.count / one_total_s if one_total_s else None},
         "speedup_vs_one_at_a_time": one_total_s / wall if one_total_s else None,
+        "with_sinks": sinks,
         "cpu_baseline": cpu,
     }
     log(f"batched: {qs.count} queries in {wall * 1e3:.1f} ms ({qps:.0f} queries/s)" +
@@ -352,6 +367,9 @@ def main() -> int:
     ap.add_argument("--calib-gather-mib", default="",
                     help="random 8-byte gathers from arrays of these sizes, comma-separated, in order "
                          "(counter calibration runs)")
+    ap.add_argument("--routing-sinks", type=int, default=8,
+                    help="also time the batch with 1..N sinks per query, each query stopping once its "
+                         "sinks are settled (0: off)")
     ap.add_argument("--routing-batch", type=int, default=1,
                     help="0: the routing-grid legs stop after the single-source one")
     ap.add_argument("--learned", type=int, default=1,

@@ -443,7 +443,8 @@ typedef struct {
   int64_t edges_relaxed;       /* when count_edges */
   int64_t queue_capacity;      /* the capacity asked for (or chosen), before the split */
   float device_ms;             /* initialisation + all rounds (hipEvent) */
-  float reserved_f;
+  int32_t stopped_queries;     /* mi355_sssp_batched_targets: queries that settled all their
+                                  targets and stopped before the batch ended */
 } mi355_batch_stats;
 
 void mi355_batch_options_default(mi355_batch_options* o);
@@ -454,6 +455,20 @@ int mi355_batch_layout(const mi355_graph* g, const mi355_query_batch* qb, int64_
  * unreached) or NULL.  Blocking. */
 int mi355_sssp_batched(mi355_graph* g, const mi355_query_batch* qb, const mi355_batch_options* o,
                        void* d_dist, int32_t* d_pred, mi355_batch_stats* stats, void* stream);
+/* mi355_sssp_batched with targets: query q stops once every vertex in
+ * targets[target_offsets[q] .. target_offsets[q+1]) (global ids, inside q's
+ * rectangle) is settled, i.e. once the batch's near-far threshold has passed
+ * all of their distances.  target_offsets: host array of count + 1 (int32,
+ * starting at 0); targets: host int32.  A query with no targets runs to the
+ * end.  Outputs of a query that stopped hold the exact distance and parent for
+ * every vertex whose distance is below the threshold it stopped at (its
+ * targets among them) and the unreached value (INF, parent -1) for the rest.
+ * Same batching and options as mi355_sssp_batched; no reference counterpart
+ * (the reference's routing_api runs every query to completion and reads the
+ * sinks afterwards, GRAPH/routing_api.cpp). */
+int mi355_sssp_batched_targets(mi355_graph* g, const mi355_query_batch* qb, const int32_t* target_offsets,
+                               const int32_t* targets, const mi355_batch_options* o, void* d_dist,
+                               int32_t* d_pred, mi355_batch_stats* stats, void* stream);
 /* Autotuner: times every (order, delta_scale, blocks_per_cu) candidate, order
  * in {0, 1} x delta_scale in {0.5, 1, 2} x blocks_per_cu in {2, 4, 8}, on every
  * sample_stride-th query of the batch and writes the fastest into *best (other

@@ -139,7 +139,35 @@ struct BatchArgs {
   int32_t rounds_per_graph;
   int want_edges;
   float delta;        // near-far bucket width (value domain); <= 0: Bellman-Ford rounds
+  // per query: kNoStop while it runs, else the threshold below which its labels
+  // are final when it stopped (all its targets settled); nullptr: no targets
+  const uint32_t* qstop;
 };
+
+constexpr uint32_t kNoStop = 0xFFFFFFFFu;
+
+// Targets of a far pass: at a far pass the near frontier is empty, so every
+// label below the closing threshold t_old is final.  A target found below it
+// is settled (once: its flag); the query whose last target settles stops with
+// threshold t_old: its entries are dropped from then on, and its outputs at or
+// above t_old read as unreached.  Runs between the decision and the work
+// kernel of every round, and does nothing in relax rounds.
+__global__ void k_batch_targets(const unsigned long long* __restrict__ labels, const QueryInfo* __restrict__ qinfo,
+                                const BatchCtrl* __restrict__ c, const uint2* __restrict__ tgt,
+                                uint32_t* __restrict__ tflag, uint32_t ntgt, uint32_t* __restrict__ qleft,
+                                uint32_t* __restrict__ qstop) {
+  if (c->mode != kRoundFar) return;
+  const uint32_t t_old = c->t_old;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntgt; i += gridDim.x * blockDim.x) {
+    if (tflag[i]) continue;
+    const uint2 t = tgt[i];
+    const uint32_t key = (uint32_t)(labels[qinfo[t.x].base + t.y] >> 32);
+    if (key < t_old) {
+      tflag[i] = 1u;
+      if (atomicSub(qleft + t.x, 1u) == 1u) qstop[t.x] = t_old;
+    }
+  }
+}
 
 __device__ inline void batch_stage_flush(uint2* stage, uint32_t& staged, uint2* q, uint32_t* cnt,
                                          uint32_t cap, uint32_t* err) {
@@ -276,9 +304,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
       const int64_t slot = a.qinfo[it.x].base + it.y;
       const uint32_t key = (uint32_t)(a.labels[slot] >> 32);
       // below t_old: stale (the slot improved below the threshold since and
-      // was relaxed as near); [t_old, t_new): next frontier; above: stays far
-      const bool to_near = ok && key >= t_old && key < t_new;
-      const bool keep = ok && key >= t_new;
+      // was relaxed as near); [t_old, t_new): next frontier; above: stays far.
+      // Entries of a stopped query are dropped.
+      const bool live = ok && (!a.qstop || a.qstop[it.x] == kNoStop);
+      const bool to_near = live && key >= t_old && key < t_new;
+      const bool keep = live && key >= t_new;
       uint32_t x = stamp, y = fst;
       const uint2 sf = (to_near || keep) ? a.stamps[slot] : make_uint2(stamp, fst);
       if (to_near && sf.x != stamp) x = atomicExch(&a.stamps[slot].x, stamp);
@@ -317,6 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
         u = (qi.r0 + (int)r) * a.cols + qi.c0 + (int)col;
         key = atomic_load_key(a.labels + qi.base + it.y);
         beg = a.rowptr[u];
+        if (a.qstop && a.qstop[qid] != kNoStop) key = W::kInfKey;  // stopped query
         if (key != W::kInfKey) deg = a.rowptr[u + 1] - beg;
       }
       int incl = deg;
@@ -509,14 +540,17 @@ __global__ void k_batch_sources(unsigned long long* __restrict__ labels, const Q
 // Labels -> outputs, one query per blockIdx.y (its rectangle in chunks).
 template <class W>
 __global__ void k_batch_output(const unsigned long long* __restrict__ labels, const QueryInfo* __restrict__ qinfo,
-                               int inf_mode, uint32_t* __restrict__ dist, int32_t* __restrict__ pred) {
+                               int inf_mode, uint32_t* __restrict__ dist, int32_t* __restrict__ pred,
+                               const uint32_t* __restrict__ qstop) {
   const QueryInfo qi = qinfo[blockIdx.y];
+  const uint32_t stop = qstop ? qstop[blockIdx.y] : kNoStop;
   const int64_t area = (int64_t)qi.bh * qi.bw;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < area; i += stride) {
     const uint32_t dr = (uint32_t)(i / qi.bw), dc = (uint32_t)(i - (int64_t)dr * qi.bw);
     const unsigned long long l = labels[qi.base + tiled_slot(dr, dc, (uint32_t)qi.bh, (uint32_t)qi.bw)];
-    const uint32_t key = (uint32_t)(l >> 32);
+    // a stopped query's labels at or above its threshold may not be final
+    const uint32_t key = (uint32_t)(l >> 32) >= stop ? W::kInfKey : (uint32_t)(l >> 32);
     const bool reached = key != W::kInfKey;
     if (dist) {
       uint32_t out = key;
@@ -572,7 +606,8 @@ void batch_layout(const mi355_query_batch& qb, int64_t n, std::vector<int64_t>& 
 
 template <class W>
 static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_options& o,
-                            uint32_t* d_dist, int32_t* d_pred, mi355_batch_stats& st, hipStream_t s) {
+                            const int32_t* t_off, const int32_t* targets, uint32_t* d_dist, int32_t* d_pred,
+                            mi355_batch_stats& st, hipStream_t s) {
   std::vector<int64_t> off;
   batch_layout(qb, g.n, off);
   const int64_t total = off.back();
@@ -622,6 +657,30 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
           make_uint2((uint32_t)q, tiled_slot((uint32_t)r, (uint32_t)c, (uint32_t)(b[2] - b[0]), (uint32_t)(b[3] - b[1]))));
     }
   }
+  // targets of each query (local slots); a query without any runs to the end
+  std::vector<uint2> tgt;
+  std::vector<uint32_t> qleft;
+  if (t_off) {
+    if (t_off[0] != 0) fail(MI355_EINVAL, "target offsets must start at 0");
+    qleft.assign((size_t)nq, 0xFFFFFFFFu);
+    for (int32_t q = 0; q < nq; ++q) {
+      const int32_t* b = qb.boxes + 4 * (size_t)q;
+      const int32_t t0 = t_off[q], t1 = t_off[q + 1];
+      if (t1 < t0) fail(MI355_EINVAL, "target offsets must not decrease");
+      if (t1 > t0) qleft[(size_t)q] = (uint32_t)(t1 - t0);
+      for (int32_t i = t0; i < t1; ++i) {
+        const int64_t v = targets[i];
+        if (v < 0 || v >= g.n) fail(MI355_ERANGE, "target vertex is outside the graph");
+        const int64_t r = v / qb.grid_cols - b[0], c = v % qb.grid_cols - b[1];
+        if (r < 0 || r >= b[2] - b[0] || c < 0 || c >= b[3] - b[1]) fail(MI355_EINVAL, "target outside its query rectangle");
+        tgt.push_back(make_uint2((uint32_t)q, tiled_slot((uint32_t)r, (uint32_t)c, (uint32_t)(b[2] - b[0]),
+                                                        (uint32_t)(b[3] - b[1]))));
+      }
+    }
+    if (tgt.size() > (size_t)UINT32_MAX) fail(MI355_EINVAL, "too many targets");
+  }
+  const uint32_t ntgt = (uint32_t)tgt.size();
+  const bool stopping = t_off && ntgt > 0;
   std::vector<uint2> src;
   PartOffsets po{};
   for (int x = 0; x < kParts; ++x) {
@@ -672,6 +731,10 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
     ws.batch_far[1].ensure((size_t)std::max<int64_t>(qtotal, 1));
   }
   ws.batch_qinfo.ensure((size_t)nq * sizeof(QueryInfo));
+  if (stopping) {
+    ws.batch_tgt.ensure(ntgt);
+    ws.batch_tstate.ensure((size_t)ntgt + 2 * (size_t)nq);  // target flags | queries left | stop
+  }
   ws.batch_src.ensure((size_t)std::max<int64_t>(nsrc, 1));
   ws.batch_ctrl.ensure(sizeof(BatchCtrl));
   struct {
@@ -700,6 +763,29 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   a.far[1] = bucketed ? ws.batch_far[1].p : bs.q[1];
   std::copy(po.qoff, po.qoff + kParts + 1, a.qoff);
   a.delta = bucketed ? delta : 0.0f;
+  uint32_t* tflag = nullptr;
+  uint32_t* d_qleft = nullptr;
+  uint32_t* d_qstop = nullptr;
+  if (stopping) {
+    tflag = ws.batch_tstate.p;
+    d_qleft = tflag + ntgt;
+    d_qstop = d_qleft + nq;
+    std::vector<uint32_t> init((size_t)ntgt + 2 * (size_t)nq, 0u);
+    std::copy(qleft.begin(), qleft.end(), init.begin() + ntgt);
+    std::fill(init.begin() + ntgt + nq, init.end(), kNoStop);
+    MI_HIP(hipMemcpyAsync(ws.batch_tgt.p, tgt.data(), tgt.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
+    MI_HIP(hipMemcpyAsync(tflag, init.data(), init.size() * 4, hipMemcpyHostToDevice, s));
+    MI_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+    a.qstop = d_qstop;
+  }
+  const unsigned tgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(div_up((uint64_t)std::max(ntgt, 1u), kBlock), 1024));
+  auto round_nodes = [&](int k) {
+    k_batch_decide<W><<<1, 1, 0, s>>>(a, k);
+    if (stopping)
+      k_batch_targets<<<tgrid, kBlock, 0, s>>>(bs.labels, bs.qinfo, ctrl, ws.batch_tgt.p, tflag, ntgt, d_qleft,
+                                                d_qstop);
+    k_batch_work<W><<<grid, kBlock, 0, s>>>(a, k);
+  };
   a.ctrl = ctrl;
   a.cols = qb.grid_cols;
   a.rounds_per_graph = K;
@@ -720,10 +806,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   hipGraph_t graph = nullptr;
   if (o.use_graph) {
     MI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < K; ++k) {
-      k_batch_decide<W><<<1, 1, 0, s>>>(a, k);
-      k_batch_work<W><<<grid, kBlock, 0, s>>>(a, k);
-    }
+    for (int k = 0; k < K; ++k) round_nodes(k);
     k_batch_epoch<<<1, 1, 0, s>>>(ctrl, K);
     MI_HIP(hipStreamEndCapture(s, &graph));
     MI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -740,10 +823,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
         if (exec) {
           MI_HIP(hipGraphLaunch(exec, s));
         } else {
-          for (int k = 0; k < K; ++k) {
-            k_batch_decide<W><<<1, 1, 0, s>>>(a, k);
-            k_batch_work<W><<<grid, kBlock, 0, s>>>(a, k);
-          }
+          for (int k = 0; k < K; ++k) round_nodes(k);
           k_batch_epoch<<<1, 1, 0, s>>>(ctrl, K);
           MI_LAUNCH_CHECK();
         }
@@ -770,7 +850,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
     int64_t max_area = 0;
     for (const QueryInfo& x : qi) max_area = std::max<int64_t>(max_area, (int64_t)x.bh * x.bw);
     const dim3 og((unsigned)std::min<int64_t>(div_up((uint64_t)max_area, kBlock), 64), (unsigned)nq);
-    k_batch_output<W><<<og, kBlock, 0, s>>>(bs.labels, bs.qinfo, o.inf_mode, d_dist, d_pred);
+    k_batch_output<W><<<og, kBlock, 0, s>>>(bs.labels, bs.qinfo, o.inf_mode, d_dist, d_pred, d_qstop);
   }
   if (d_pred && nsrc)
     k_batch_source_preds<<<(unsigned)div_up((uint64_t)nsrc, kBlock), kBlock, 0, s>>>(bs.qinfo, bs.src,
@@ -796,10 +876,15 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   st.edges_relaxed = (int64_t)hc.edges;
   st.device_ms = ms;
   st.queue_capacity = cap;
+  if (stopping) {
+    std::vector<uint32_t> hs((size_t)nq);
+    MI_HIP(hipMemcpy(hs.data(), d_qstop, (size_t)nq * 4, hipMemcpyDeviceToHost));
+    st.stopped_queries = (int32_t)std::count_if(hs.begin(), hs.end(), [](uint32_t x) { return x != kNoStop; });
+  }
 }
 
-void run_batch(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_options& o, void* d_dist,
-               int32_t* d_pred, mi355_batch_stats& st, hipStream_t caller) {
+void run_batch(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_options& o, const int32_t* t_off,
+               const int32_t* targets, void* d_dist, int32_t* d_pred, mi355_batch_stats& st, hipStream_t caller) {
   if (g.info.has_negative) fail(MI355_EINVAL, "batched queries require nonnegative edge weights");
   // Graph capture needs a stream of its own (not the legacy default stream);
   // it is ordered after the caller's work and the call is blocking.
@@ -811,9 +896,9 @@ void run_batch(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_op
     MI_HIP(hipEventRecord(after, caller));
     MI_HIP(hipStreamWaitEvent(s, after, 0));
     if (g.weight_type == MI355_W_F32)
-      run_batch_typed<WF32>(g, qb, o, static_cast<uint32_t*>(d_dist), d_pred, st, s);
+      run_batch_typed<WF32>(g, qb, o, t_off, targets, static_cast<uint32_t*>(d_dist), d_pred, st, s);
     else
-      run_batch_typed<WU32>(g, qb, o, static_cast<uint32_t*>(d_dist), d_pred, st, s);
+      run_batch_typed<WU32>(g, qb, o, t_off, targets, static_cast<uint32_t*>(d_dist), d_pred, st, s);
     MI_HIP(hipStreamSynchronize(s));
   } catch (...) {
     (void)hipStreamSynchronize(s);
@@ -856,7 +941,7 @@ void batch_autotune(mi355_graph& g, const mi355_query_batch& qb, int32_t stride,
         o.use_graph = 1;
         o.count_edges = 0;
         mi355_batch_stats st{};
-        run_batch(g, sub, o, dist.p, nullptr, st, caller);
+        run_batch(g, sub, o, nullptr, nullptr, dist.p, nullptr, st, caller);
         if (timings) timings[idx] = st.device_ms;
         if (st.device_ms < best_ms) {
           best_ms = st.device_ms;

@@ -844,7 +844,23 @@ int mi355_sssp_batched(mi355_graph* g, const mi355_query_batch* qb, const mi355_
     mi355_batch_options d;
     mi355_batch_options_default(&d);
     mi355_batch_stats st{};
-    mi355::run_batch(*g, *qb, o ? *o : d, d_dist, d_pred, st, as_stream(stream));
+    mi355::run_batch(*g, *qb, o ? *o : d, nullptr, nullptr, d_dist, d_pred, st, as_stream(stream));
+    if (stats) *stats = st;
+  });
+}
+
+int mi355_sssp_batched_targets(mi355_graph* g, const mi355_query_batch* qb, const int32_t* target_offsets,
+                               const int32_t* targets, const mi355_batch_options* o, void* d_dist,
+                               int32_t* d_pred, mi355_batch_stats* stats, void* stream) {
+  return guard([&] {
+    if (!g || !qb || !target_offsets) fail(MI355_EINVAL, "null argument");
+    if (!d_dist && !d_pred) fail(MI355_EINVAL, "no output requested");
+    if (qb->count > 0 && target_offsets[qb->count] > 0 && !targets) fail(MI355_EINVAL, "null targets");
+    MI_HIP(hipSetDevice(g->device));
+    mi355_batch_options d;
+    mi355_batch_options_default(&d);
+    mi355_batch_stats st{};
+    mi355::run_batch(*g, *qb, o ? *o : d, target_offsets, targets, d_dist, d_pred, st, as_stream(stream));
     if (stats) *stats = st;
   });
 }

@@ -58,6 +58,8 @@ struct Workspace {
   DevBuf<unsigned char> batch_qinfo;
   DevBuf<uint2> batch_src;
   DevBuf<unsigned char> batch_ctrl;
+  DevBuf<uint2> batch_tgt;        // (query, local slot) of every target
+  DevBuf<uint32_t> batch_tstate;  // target settled flags, targets left per query, stop thresholds
   int64_t slot_cap = 0;
   int nslots = 0;
 };

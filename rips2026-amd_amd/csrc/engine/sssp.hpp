@@ -77,8 +77,10 @@ const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hip
 
 // Batched multi-source queries (batch.hip).
 void batch_layout(const mi355_query_batch& qb, int64_t n, std::vector<int64_t>& offsets);
-void run_batch(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_options& o, void* d_dist,
-               int32_t* d_pred, mi355_batch_stats& st, hipStream_t stream);
+// t_off / targets (host, may be null): per-query target vertices; a query
+// stops once all of its targets are settled
+void run_batch(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_options& o, const int32_t* t_off,
+               const int32_t* targets, void* d_dist, int32_t* d_pred, mi355_batch_stats& st, hipStream_t stream);
 void batch_autotune(mi355_graph& g, const mi355_query_batch& qb, int32_t stride, mi355_batch_options& best,
                     double* timings, hipStream_t stream);
 

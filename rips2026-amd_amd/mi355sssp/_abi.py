@@ -89,7 +89,7 @@ class BatchStats(C.Structure):
     _fields_ = [("label_slots", C.c_int64), ("sources", C.c_int64), ("rounds", C.c_int32),
                 ("graph_replays", C.c_int32), ("rounds_per_graph", C.c_int32),
                 ("far_passes", C.c_int32), ("edges_relaxed", C.c_int64),
-                ("queue_capacity", C.c_int64), ("device_ms", C.c_float), ("reserved_f", C.c_float)]
+                ("queue_capacity", C.c_int64), ("device_ms", C.c_float), ("stopped_queries", C.c_int32)]
 
 
 class BfProgress(C.Structure):
@@ -164,6 +164,8 @@ _ENGINE_SIGS = {
     "mi355_batch_layout": (C.c_int, [P, C.POINTER(QueryBatch), P]),
     "mi355_sssp_batched": (C.c_int, [P, C.POINTER(QueryBatch), C.POINTER(BatchOptions), P, P,
                                      C.POINTER(BatchStats), P]),
+    "mi355_sssp_batched_targets": (C.c_int, [P, C.POINTER(QueryBatch), P, P, C.POINTER(BatchOptions), P, P,
+                                             C.POINTER(BatchStats), P]),
     "mi355_batch_autotune": (C.c_int, [P, C.POINTER(QueryBatch), C.c_int32, C.POINTER(BatchOptions),
                                        P, P]),
     "mi355_mlp_train_options_default": (None, [C.POINTER(MlpTrainOptions)]),

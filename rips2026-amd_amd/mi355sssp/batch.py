@@ -64,6 +64,22 @@ def routing_queries(rows: int, cols: int, count: int, seed: int, side_min: int =
     return Queries(cols, np.array(off), np.concatenate(srcs), boxes)
 
 
+def routing_sinks(qs: Queries, seed: int, max_sinks: int = 8):
+    """Net-like sinks for the queries of `qs`: 1..max_sinks cells uniform in
+    each query's rectangle, as (target offsets, target vertices) for run()."""
+    rng = np.random.default_rng(seed)
+    off, tv = [0], []
+    cols = qs.grid_cols
+    for q in range(qs.count):
+        r0, c0, r1, c1 = (int(x) for x in qs.boxes[q])
+        k = int(rng.integers(1, max_sinks + 1))
+        rr = rng.integers(r0, r1, k)
+        cc = rng.integers(c0, c1, k)
+        tv.append((rr * cols + cc).astype(np.int32))
+        off.append(off[-1] + k)
+    return np.array(off, np.int32), (np.concatenate(tv) if tv else np.zeros(0, np.int32))
+
+
 def block_slots(bh: int, bw: int) -> int:
     """Label slots the engine stores for a bh x bw rectangle: 4 x 4 tiles
     padded to whole tiles, or row-major (no padding) below 4 rows."""
@@ -91,16 +107,33 @@ def layout(dg, qs: Queries) -> np.ndarray:
     return off
 
 
-def run(dg, qs: Queries, want_pred: bool = True, download: bool = True, **kw):
+def run(dg, qs: Queries, want_pred: bool = True, download: bool = True, targets=None, **kw):
     """Solve all queries; returns (dist, pred, offsets, stats) with dist/pred
-    concatenated per query (None when not downloaded)."""
+    concatenated per query (None when not downloaded).
+
+    targets: None, or (target_offsets, target_vertices) with count + 1 offsets:
+    query q stops once all of its targets (global vertex ids inside its
+    rectangle) are settled; its outputs are then exact below the threshold it
+    stopped at (its targets among them) and unreached above.  A query with no
+    targets runs to the end (mi355_sssp_batched_targets)."""
     off = layout(dg, qs)
     total = int(off[-1])
     d_dist = DeviceBuffer(4 * total)
     d_pred = DeviceBuffer(4 * total) if want_pred else None
     st = BatchStats()
-    check(engine().mi355_sssp_batched(dg.handle, C.byref(qs.c()), C.byref(options(**kw)), d_dist.ptr,
-                                      d_pred.ptr if d_pred else None, C.byref(st), None))
+    if targets is None:
+        check(engine().mi355_sssp_batched(dg.handle, C.byref(qs.c()), C.byref(options(**kw)), d_dist.ptr,
+                                          d_pred.ptr if d_pred else None, C.byref(st), None))
+    else:
+        t_off = np.ascontiguousarray(targets[0], np.int32)
+        t_v = np.ascontiguousarray(targets[1], np.int32)
+        if t_off.shape != (qs.count + 1,):
+            raise ValueError(f"target offsets: expected {qs.count + 1} entries, got {t_off.shape}")
+        if int(t_off[-1]) != len(t_v):
+            raise ValueError("target offsets do not match the target list")
+        check(engine().mi355_sssp_batched_targets(dg.handle, C.byref(qs.c()), t_off.ctypes.data,
+                                                  t_v.ctypes.data if len(t_v) else None, C.byref(options(**kw)),
+                                                  d_dist.ptr, d_pred.ptr if d_pred else None, C.byref(st), None))
     if not download:
         return None, None, off, st
     dist = d_dist.to_array(np.float32 if dg.weight_type == W_F32 else np.uint32, total)

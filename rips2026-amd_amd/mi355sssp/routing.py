@@ -129,13 +129,15 @@ def _edge_index(rowptr: np.ndarray, col: np.ndarray):
 
 
 def route(graph, meta: RouteMetadata, dg=None, max_label_slots: int = 1 << 30,
-          resolve_names: bool = True, **batch_options) -> list[RoutedNet]:
+          resolve_names: bool = True, early_stop: bool = True, **batch_options) -> list[RoutedNet]:
     """Route every request of `meta` on `graph` (the RIPSCSR1 host graph the
     sidecar belongs to).  dg: an existing DeviceGraph of graph (built when
     None).  Nets go to the batched engine in chunks whose label storage
     (B.storage_slots: n per net) stays within max_label_slots.  The host side walks the parents of
     all sinks of a chunk together and looks every hop's edge up in one
-    sorted search; names are resolved once per distinct string."""
+    sorted search; names are resolved once per distinct string.  early_stop:
+    each net's query stops once all of its sinks are settled (the sinks' paths
+    only read labels below theirs, which are final by then)."""
     from .device import DeviceGraph
 
     n = int(graph.n)
@@ -169,12 +171,17 @@ def route(graph, meta: RouteMetadata, dg=None, max_label_slots: int = 1 << 30,
         off[1:] = np.cumsum([len(s) for s in srcs])
         boxes = np.tile(np.array([0, 0, 1, n], np.int32), (len(chunk), 1))
         qs = B.Queries(n, off, np.concatenate(srcs), boxes)
-        dist, pred, offs, _ = B.run(dg, qs, want_pred=True, **batch_options)
         # every sink of the chunk: its query's block and node
         sinks = [meta.request_sinks(q) for q in chunk]
         counts = np.array([len(s) for s in sinks], np.int64)
         rec = np.concatenate(sinks) if counts.sum() else np.zeros((0, 3), np.uint64)
         node = rec[:, 0].astype(np.int64)
+        tgt = None
+        if early_stop:
+            t_off = np.zeros(len(chunk) + 1, np.int32)
+            t_off[1:] = np.cumsum(counts)
+            tgt = (t_off, node.astype(np.int32))
+        dist, pred, offs, _ = B.run(dg, qs, want_pred=True, targets=tgt, **batch_options)
         block = np.repeat(np.asarray(offs[:-1], np.int64), counts)
         d = dist[block + node]
         reached = np.isfinite(d)

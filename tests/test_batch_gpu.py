@@ -213,3 +213,72 @@ def test_batch_on_a_degree_ordered_graph():
     assert dg.info.vertex_order == M.ORDER_DEGREE
     dist, pred, off, _ = B.run(dg, qs)
     _check(g, qs, dist, pred, off)
+
+
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_queries_stop_once_their_targets_are_settled(use_graph):
+    """mi355_sssp_batched_targets: every target's distance and parent equal the
+    full run's; a query that stopped keeps exact outputs below its threshold and
+    reads unreached above it (a cut in distance order); queries without
+    targets equal the full run."""
+    side = 140
+    g = M.routing_grid(side, seed=41)
+    dg = M.DeviceGraph(g)
+    qs = B.routing_queries(side, side, 40, seed=17, side_min=20, side_max=140)
+    rng = np.random.default_rng(3)
+    t_off, t_v = [0], []
+    for q in range(qs.count):
+        r0, c0, r1, c1 = (int(x) for x in qs.boxes[q])
+        k = 0 if q % 5 == 4 else int(rng.integers(1, 4))  # every fifth query has no targets
+        for _ in range(k):
+            # near a source: the query should stop long before its rectangle is done
+            s = int(qs.query_sources(q)[0])
+            r = min(max(s // side + int(rng.integers(-3, 4)), r0), r1 - 1)
+            c = min(max(s % side + int(rng.integers(-3, 4)), c0), c1 - 1)
+            t_v.append(r * side + c)
+        t_off.append(len(t_v))
+    full_d, full_p, off, _ = B.run(dg, qs, use_graph=use_graph)
+    print(f"batch: {qs.count} queries, {len(t_v)} targets, use_graph={use_graph}", flush=True)
+    d, p, off2, st = B.run(dg, qs, use_graph=use_graph, targets=(np.array(t_off), np.array(t_v)))
+    assert np.array_equal(off, off2)
+    with_targets = sum(1 for q in range(qs.count) if t_off[q + 1] > t_off[q])
+    assert 0 < st.stopped_queries <= with_targets, (st.stopped_queries, with_targets)
+    stopped_cells = 0
+    for q in range(qs.count):
+        r0, c0, r1, c1 = (int(x) for x in qs.boxes[q])
+        a, b = off[q], off[q + 1]
+        fd, fp, gd, gp = full_d[a:b], full_p[a:b], d[a:b], p[a:b]
+        for v in t_v[t_off[q]:t_off[q + 1]]:
+            i = (v // side - r0) * (c1 - c0) + v % side - c0
+            assert gd[i].tobytes() == fd[i].tobytes() and gp[i] == fp[i], (q, v)
+        same = (gd.view(np.uint32) == fd.view(np.uint32)) & (gp == fp)
+        cut = ~same
+        if t_off[q + 1] == t_off[q]:
+            assert not cut.any(), f"query {q} without targets differs from the full run"
+            continue
+        # cells that differ read unreached (uint32 weights: 0xFFFFFFFF) and lie
+        # at or beyond every kept distance
+        unreached = gd == np.uint32(0xFFFFFFFF)
+        assert np.all(unreached[cut]) and np.all(gp[cut] == -1), q
+        if cut.any():
+            stopped_cells += int(cut.sum())
+            assert fd[cut].min() >= gd[~unreached].max(), q
+    assert stopped_cells > 0
+
+
+def test_invalid_targets_are_rejected():
+    side = 30
+    g = M.routing_grid(side, seed=4)
+    dg = M.DeviceGraph(g)
+    qs = B.Queries(side, [0, 1], [3 * side + 3], [[2, 2, 10, 10]])
+    for t_off, t_v in (([0, 1], [0]),                  # outside the rectangle
+                       ([0, 1], [side * side]),        # outside the graph
+                       ([1, 1], [3 * side + 4])):      # offsets not starting at 0
+        with pytest.raises((M.Mi355Error, ValueError)):
+            B.run(dg, qs, targets=(np.array(t_off), np.array(t_v)))
+    with pytest.raises(ValueError):                     # offsets and list disagree
+        B.run(dg, qs, targets=(np.array([0, 2]), np.array([3 * side + 4])))
+    # no targets at all: the query runs to the end
+    d, p, off, st = B.run(dg, qs, targets=(np.array([0, 0]), np.array([], np.int32)))
+    full, fp, _, _ = B.run(dg, qs)
+    assert d.tobytes() == full.tobytes() and np.array_equal(p, fp) and st.stopped_queries == 0

@@ -160,6 +160,12 @@ def test_route_requests_against_the_oracle(tmp_path):
     nets = R.route(g, meta)
     assert len(nets) == meta.request_count
     again = R.route(g, meta, max_label_slots=3 * g.n)  # 3 nets per chunk
+    # nets run to the end (no per-net stop at the last sink) route the same
+    full = R.route(g, meta, early_stop=False)
+    for net, net3 in zip(nets, full):
+        for sk, sk3 in zip(net.sinks, net3.sinks):
+            assert np.float32(sk.dist).tobytes() == np.float32(sk3.dist).tobytes()
+            assert np.array_equal(sk.nodes, sk3.nodes) and np.array_equal(sk.edges, sk3.edges)
     rowp, col = g.rowptr, g.col
     reached = unreached = 0
     for net, net2 in zip(nets, again):
@@ -221,14 +227,24 @@ def test_route_paths_and_pips_on_the_cpu(tmp_path, monkeypatch):
     g, meta = device_fixture(tmp_path, n=600, m=3000, nets=25, seed=11)
     out = g.transpose()
     run = _oracle_run(out)
-    chunk_slots = []
+    chunk_slots, chunk_targets = [], []
 
     def recording_run(dg, qs, *a, **kw):
         chunk_slots.append(R.B.storage_slots(qs))
+        chunk_targets.append(kw.get("targets"))
         return run(dg, qs, *a, **kw)
 
     monkeypatch.setattr(R.B, "run", recording_run)
     nets = R.route(g, meta, dg=object())
+    # the targets handed to the engine: every net's sinks, per query
+    todo = [q for q in range(meta.request_count) if len(meta.request_sources(q))]
+    t_counts = np.concatenate([np.diff(t[0]) for t in chunk_targets])
+    assert np.array_equal(t_counts, [len(meta.request_sinks(q)) for q in todo])
+    assert np.array_equal(np.concatenate([t[1] for t in chunk_targets]),
+                          np.concatenate([meta.request_sinks(q)[:, 0] for q in todo]).astype(np.int32))
+    chunk_targets.clear()
+    R.route(g, meta, dg=object(), early_stop=False)
+    assert all(t is None for t in chunk_targets)
     chunk_slots.clear()
     chunked = R.route(g, meta, dg=object(), max_label_slots=4 * g.n)
     # each chunk's label storage (tile padding included) stays within the budget
