@@ -135,7 +135,9 @@ typedef struct {
   float delta;              /* DS / ADDS bucket width; <= 0: heuristic 16*wbar/dbar */
   int32_t inf_mode;         /* MI355_INF_IEEE or MI355_INF_FLT_MAX (f32 outputs) */
   int32_t dynamic_delta;    /* ADDS: feedback delta per phase (0 = plain ADDS: fixed delta) */
-  int32_t use_mlp;          /* ADDS: the loaded delta model picks delta and active buckets */
+  int32_t use_mlp;          /* ADDS: 1: the loaded delta model picks delta, active buckets and
+                               worker share per phase; 2: the phase schedule of
+                               mi355_adds_schedule_set does */
   int32_t worker_blocks;    /* ADDS: persistent workgroups (0 = auto) */
   int32_t rounds_per_sync;  /* host sync cadence (0 = adaptive) */
   int32_t active_buckets;   /* ADDS: buckets served at once, 1..32 (0 = 1) */
@@ -159,7 +161,7 @@ typedef struct {
   int32_t has_negative_cycle;
   int32_t policy;           /* ADDS phase policy used: 0 fixed delta, 1 feedback, 2 learned
                                network (0 when use_mlp is set but the graph lies outside the
-                               model's training range) */
+                               model's training range), 3 explicit schedule */
 } mi355_sssp_stats;
 
 void mi355_sssp_options_default(mi355_sssp_options* opt);
@@ -469,6 +471,21 @@ int mi355_sssp_batched(mi355_graph* g, const mi355_query_batch* qb, const mi355_
 int mi355_sssp_batched_targets(mi355_graph* g, const mi355_query_batch* qb, const int32_t* target_offsets,
                                const int32_t* targets, const mi355_batch_options* o, void* d_dist,
                                int32_t* d_pred, mi355_batch_stats* stats, void* stream);
+/* Paths out of a batched run's outputs, read on the device: target i is
+ * vertex target[i] (global id) of query query[i] (host arrays); d_dist /
+ * d_pred and out_offsets are that run's outputs and mi355_batch_layout's
+ * offsets.  path_offsets (host, count + 1) receives the offsets of each path:
+ * its length is the number of vertices from the source to the target, 0 when
+ * the target is unreached.  target_dist (host, count, may be NULL) receives
+ * each target's 32-bit distance.  path (host, may be NULL): the vertices of
+ * every path, source first; path_capacity must hold path_offsets[count]
+ * (MI355_ECAPACITY otherwise; call with path = NULL first to size it).
+ * Blocking.  No reference counterpart: the routing consumer walks each sink's
+ * parents without downloading whole output blocks. */
+int mi355_batch_paths(mi355_graph* g, const mi355_query_batch* qb, const int64_t* out_offsets,
+                      const void* d_dist, const int32_t* d_pred, const int32_t* query, const int32_t* target,
+                      int64_t count, int64_t* path_offsets, uint32_t* target_dist, int32_t* path,
+                      int64_t path_capacity, void* stream);
 /* Autotuner: times every (order, delta_scale, blocks_per_cu) candidate, order
  * in {0, 1} x delta_scale in {0.5, 1, 2} x blocks_per_cu in {2, 4, 8}, on every
  * sample_stride-th query of the batch and writes the fastest into *best (other
@@ -525,6 +542,13 @@ int mi355_adds_default_workers(int32_t* workers);
 /* Load / drop the model used by mi355_sssp_options.use_mlp (process-wide). */
 int mi355_adds_mlp_set(const mi355_mlp_params* p);
 int mi355_adds_mlp_clear(void);
+/* Explicit ADDS phase schedule for use_mlp = 2 (process-wide): phase i (0 =
+ * the source's bucket) runs with entry min(i, count - 1) of table (count x 3
+ * floats: bucket width as a multiple of the heuristic delta0, buckets served
+ * at once in [1, 8], share of the persistent workgroups that claim in (0, 1]).
+ * count 1..64; 0 clears it.  No reference counterpart: the settings the
+ * learned policy chooses among, fixed per phase (training-label search). */
+int mi355_adds_schedule_set(const float* table, int32_t count);
 
 /* One ADDS run that records, at every phase boundary, MI355_ADDS_TRACE_STRIDE
  * floats: [0,32) the phase features (before the model's normalisation), 32

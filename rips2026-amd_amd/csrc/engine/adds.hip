@@ -460,6 +460,16 @@ __device__ inline void mlp_decide(const AddsPolicy* p, const AddsCtrl* c, float 
   active = (uint32_t)fminf(fmaxf(o[1] + 0.5f, 1.0f), 8.0f);
 }
 
+// Schedule mode: the settings of phase `phase` from the table.
+__device__ inline void sched_decide(const AddsPolicy* p, const AddsCtrl* c, uint32_t phase, float& delta,
+                                    uint32_t& active, uint32_t& workers_now) {
+  const float* e = p->sched + 3u * min(phase, p->sched_len - 1u);
+  delta = fminf(fmaxf(c->base_delta * e[0], c->min_delta), c->max_delta);
+  active = (uint32_t)fminf(fmaxf(e[1], 1.0f), 8.0f);
+  const float share = fminf(fmaxf(e[2], 1.0f / 32.0f), 1.0f);
+  workers_now = min(c->workers, (uint32_t)fmaxf((float)c->workers * share, 16.0f));
+}
+
 // MLP mode, before the first claim: the network also sets the first phase
 // (the source alone in bucket 0, delta = delta0), so the run does not start
 // with the heuristic width on every workgroup.  One wave.
@@ -470,7 +480,8 @@ __global__ void k_adds_first_phase(AddsArgs a) {
   mlp_features(a.policy, c, lane == 0 ? 1u : 0u, 1u, c->delta, lane, &feat, &raw);
   float delta;
   uint32_t active, workers_now;
-  mlp_decide(a.policy, c, feat, lane, delta, active, workers_now);
+  if (a.policy->mode == kPolicySchedule) sched_decide(a.policy, c, 0u, delta, active, workers_now);
+  else mlp_decide(a.policy, c, feat, lane, delta, active, workers_now);
   if (lane == 0) {
     c->delta = delta;
     c->active = active;
@@ -539,6 +550,9 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
     else if ((float)window_items > c->util_high * capacity) delta = fmaxf(delta * c->shrink, c->min_delta);
   } else if (mode == kPolicyMlp) {
     mlp_decide(a.policy, c, feat, lane, delta, active, workers_now);
+  } else if (mode == kPolicySchedule) {
+    // only the wave that moved the base counts phases: the next phase's index
+    sched_decide(a.policy, c, ld_rlx(&c->phases) + 1u, delta, active, workers_now);
   }
   // the record index comes from the phase counter itself: two waves that
   // advance one after the other never share a record

@@ -12,7 +12,8 @@ constexpr int kMlpHidden = 64;
 constexpr int kMlpOut = 3;         // log2 delta scale, active buckets, log2 worker share
 constexpr int kTraceStride = 40;   // floats per traced phase
 
-enum PolicyMode { kPolicyFixed = 0, kPolicyHeuristic = 1, kPolicyMlp = 2 };
+enum PolicyMode { kPolicyFixed = 0, kPolicyHeuristic = 1, kPolicyMlp = 2, kPolicySchedule = 3 };
+constexpr int kSchedMax = 64;      // phases of an explicit schedule (the last entry repeats)
 
 // Device control block of one ADDS run.  Every counter that many waves hit
 // with atomics sits on its own 256-byte line (kPad words apart) so unrelated
@@ -79,6 +80,10 @@ struct AddsPolicy {
   float b1[kMlpHidden];
   uint16_t w2[kMlpOut * kMlpHidden];
   float b2[kMlpOut];
+  // schedule mode: phase i runs with entry min(i, sched_len - 1): width as a
+  // multiple of delta0, buckets served at once, share of claiming workgroups
+  uint32_t sched_len;
+  float sched[kSchedMax * 3];
 };
 
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }

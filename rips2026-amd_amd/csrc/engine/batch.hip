@@ -578,6 +578,128 @@ __global__ void k_batch_source_preds(const QueryInfo* __restrict__ qinfo, const 
 
 }  // namespace
 
+// One target of a path read: its query's output block and rectangle.
+struct PathTarget {
+  int64_t out_base;
+  int32_t r0, c0, bh, bw;
+  int32_t target;
+  int32_t pad;
+};
+
+// Output index of global vertex v in target t's block; -1 outside its rectangle.
+__device__ inline int64_t path_index(const PathTarget& t, int64_t v, int32_t cols) {
+  const int64_t r = v / cols - t.r0, c = v % cols - t.c0;
+  if (v < 0 || r < 0 || r >= t.bh || c < 0 || c >= t.bw) return -1;
+  return t.out_base + r * t.bw + c;
+}
+
+// Pass 1 (write == false): each target's path length (vertices from its
+// source to it; 0 when unreached: parent -1 and a nonzero distance) and its
+// distance bits.  Pass 2: the vertices, source first, from off[i].  A walk
+// longer than its rectangle's area, or one that leaves the rectangle, sets err.
+__global__ void k_batch_paths(const PathTarget* __restrict__ tg, int64_t count, int32_t cols,
+                              const int32_t* __restrict__ pred, const uint32_t* __restrict__ dist,
+                              int64_t* __restrict__ len, uint32_t* __restrict__ tdist,
+                              const int64_t* __restrict__ off, int32_t* __restrict__ path,
+                              uint32_t* __restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const PathTarget t = tg[i];
+  const int64_t i0 = path_index(t, t.target, cols);
+  if (i0 < 0) {
+    atomicOr(err, 1u);
+    return;
+  }
+  const uint32_t d0 = dist[i0];
+  int64_t n = 0;
+  if (!(pred[i0] < 0 && d0 != 0u)) {
+    const int64_t limit = (int64_t)t.bh * t.bw;
+    int64_t cur = t.target, at = i0;
+    const int64_t L = path ? len[i] : 0;  // pass 2: the length pass 1 found
+    for (;;) {
+      if (path && n < L) path[off[i] + L - 1 - n] = (int32_t)cur;
+      ++n;
+      const int32_t p = pred[at];
+      if (p < 0) break;
+      cur = p;
+      at = path_index(t, cur, cols);
+      if (at < 0 || n >= limit) {
+        atomicOr(err, 1u);
+        break;
+      }
+    }
+  }
+  if (!path) {
+    len[i] = n;
+    tdist[i] = d0;
+  }
+}
+
+void batch_paths(const mi355_query_batch& qb, const int64_t* out_offsets, const int32_t* d_pred,
+                 const uint32_t* d_dist, const int32_t* query, const int32_t* target, int64_t count,
+                 int64_t* path_offsets, uint32_t* target_dist, int32_t* path, int64_t path_capacity,
+                 hipStream_t s) {
+  if (count < 0) fail(MI355_EINVAL, "negative target count");
+  path_offsets[0] = 0;
+  if (count == 0) return;
+  if (qb.grid_cols <= 0 || !qb.boxes) fail(MI355_EINVAL, "query batch without rectangles");
+  std::vector<PathTarget> h((size_t)count);
+  for (int64_t i = 0; i < count; ++i) {
+    const int32_t q = query[i];
+    if (q < 0 || q >= qb.count) fail(MI355_ERANGE, "target's query index outside the batch");
+    const int32_t* b = qb.boxes + 4 * (size_t)q;
+    PathTarget& t = h[(size_t)i];
+    t.out_base = out_offsets[q];
+    t.r0 = b[0];
+    t.c0 = b[1];
+    t.bh = b[2] - b[0];
+    t.bw = b[3] - b[1];
+    t.target = target[i];
+    t.pad = 0;
+    if (out_offsets[q + 1] - out_offsets[q] != (int64_t)t.bh * t.bw)
+      fail(MI355_EINVAL, "output offsets do not match the query rectangles");
+    const int64_t r = target[i] / qb.grid_cols - b[0], c = target[i] % qb.grid_cols - b[1];
+    if (target[i] < 0 || r < 0 || r >= t.bh || c < 0 || c >= t.bw)
+      fail(MI355_EINVAL, "target outside its query rectangle");
+  }
+  DevBuf<PathTarget> d_t;
+  DevBuf<int64_t> d_len, d_off;
+  DevBuf<uint32_t> d_td, d_err;
+  d_t.ensure((size_t)count);
+  d_len.ensure((size_t)count);
+  d_td.ensure((size_t)count);
+  d_err.ensure(1);
+  MI_HIP(hipMemcpyAsync(d_t.p, h.data(), h.size() * sizeof(PathTarget), hipMemcpyHostToDevice, s));
+  MI_HIP(hipMemsetAsync(d_err.p, 0, 4, s));
+  const unsigned grid = (unsigned)div_up((uint64_t)count, kBlock);
+  k_batch_paths<<<grid, kBlock, 0, s>>>(d_t.p, count, qb.grid_cols, d_pred, d_dist, d_len.p, d_td.p, nullptr,
+                                        nullptr, d_err.p);
+  MI_LAUNCH_CHECK();
+  std::vector<int64_t> len((size_t)count);
+  uint32_t err = 0;
+  MI_HIP(hipMemcpyAsync(len.data(), d_len.p, len.size() * 8, hipMemcpyDeviceToHost, s));
+  MI_HIP(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, s));
+  if (target_dist) MI_HIP(hipMemcpyAsync(target_dist, d_td.p, (size_t)count * 4, hipMemcpyDeviceToHost, s));
+  MI_HIP(hipStreamSynchronize(s));
+  if (err) fail(MI355_EINVAL, "parent chain leaves its rectangle or does not end (not a batched run's output?)");
+  for (int64_t i = 0; i < count; ++i) path_offsets[i + 1] = path_offsets[i] + len[(size_t)i];
+  const int64_t total = path_offsets[count];
+  if (!path) return;
+  if (path_capacity < total) fail(MI355_ECAPACITY, "path buffer smaller than path_offsets[count]");
+  if (total == 0) return;
+  DevBuf<int32_t> d_path;
+  d_off.ensure((size_t)count);
+  d_path.ensure((size_t)total);
+  MI_HIP(hipMemcpyAsync(d_off.p, path_offsets, (size_t)count * 8, hipMemcpyHostToDevice, s));
+  k_batch_paths<<<grid, kBlock, 0, s>>>(d_t.p, count, qb.grid_cols, d_pred, d_dist, d_len.p, d_td.p, d_off.p,
+                                        d_path.p, d_err.p);
+  MI_LAUNCH_CHECK();
+  MI_HIP(hipMemcpyAsync(path, d_path.p, (size_t)total * 4, hipMemcpyDeviceToHost, s));
+  MI_HIP(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, s));
+  MI_HIP(hipStreamSynchronize(s));
+  if (err) fail(MI355_EINVAL, "parent chain leaves its rectangle or does not end");
+}
+
 void batch_layout(const mi355_query_batch& qb, int64_t n, std::vector<int64_t>& offsets) {
   if (qb.count <= 0) fail(MI355_EINVAL, "batch needs at least one query");
   if (qb.count > 65535) fail(MI355_EINVAL, "at most 65535 queries per batch");

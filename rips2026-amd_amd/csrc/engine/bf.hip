@@ -254,7 +254,7 @@ void run_sssp(mi355_graph& g, uint32_t input_source, const RunRequest& req, RunR
       int mode = kPolicyFixed;
       const AddsPolicy* pol = prepare_adds_policy(g, req, stream, &mode);
       RunRequest r = req;
-      r.use_mlp = mode == kPolicyMlp;
+      r.use_mlp = mode == kPolicyMlp || mode == kPolicySchedule;  // the policy sets the first phase
       run_adds(g, source, r, res, stream, pol);
       res.policy = mode;
       break;

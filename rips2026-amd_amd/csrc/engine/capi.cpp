@@ -766,6 +766,10 @@ int mi355_adds_mlp_set(const mi355_mlp_params* p) {
   return guard([&] { mi355::set_mlp_params(p); });
 }
 
+int mi355_adds_schedule_set(const float* table, int32_t count) {
+  return guard([&] { mi355::set_schedule(table, count); });
+}
+
 int mi355_adds_mlp_clear(void) {
   return guard([&] { mi355::clear_mlp(); });
 }
@@ -823,6 +827,19 @@ int mi355_batch_layout(const mi355_graph* g, const mi355_query_batch* qb, int64_
     std::vector<int64_t> off;
     mi355::batch_layout(*qb, g->n, off);
     std::copy(off.begin(), off.end(), out_offsets);
+  });
+}
+
+int mi355_batch_paths(mi355_graph* g, const mi355_query_batch* qb, const int64_t* out_offsets,
+                      const void* d_dist, const int32_t* d_pred, const int32_t* query, const int32_t* target,
+                      int64_t count, int64_t* path_offsets, uint32_t* target_dist, int32_t* path,
+                      int64_t path_capacity, void* stream) {
+  return guard([&] {
+    if (!g || !qb || !out_offsets || !d_dist || !d_pred || !path_offsets) fail(MI355_EINVAL, "null argument");
+    if (count > 0 && (!query || !target)) fail(MI355_EINVAL, "null target arrays");
+    MI_HIP(hipSetDevice(g->device));
+    mi355::batch_paths(*qb, out_offsets, d_pred, static_cast<const uint32_t*>(d_dist), query, target, count,
+                       path_offsets, target_dist, path, path_capacity, as_stream(stream));
   });
 }
 

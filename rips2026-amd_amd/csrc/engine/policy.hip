@@ -15,6 +15,7 @@ std::mutex g_mlp_mu;
 bool g_mlp_loaded = false;
 AddsPolicy g_mlp{};  // weights + feature normalisation (mode/trace unused)
 float g_feat_lo[14], g_feat_hi[14];  // training range of the graph features
+std::vector<float> g_sched;           // explicit phase schedule, 3 floats per phase
 
 // True when every bounded graph feature lies in the model's training range
 // (widened by 1 % of the range on each side).
@@ -78,6 +79,18 @@ void graph_features(mi355_graph& g, hipStream_t s, float* out14) {
   std::copy(g.graph_feat, g.graph_feat + 14, out14);
 }
 
+void set_schedule(const float* table, int32_t count) {
+  if (count < 0 || count > kSchedMax) fail(MI355_EINVAL, "a phase schedule holds 1..64 phases (0 clears it)");
+  if (count > 0 && !table) fail(MI355_EINVAL, "null schedule");
+  for (int32_t i = 0; i < count; ++i) {
+    const float sc = table[3 * i], ac = table[3 * i + 1], sh = table[3 * i + 2];
+    if (!(sc > 0.0f) || !std::isfinite(sc) || !(ac >= 1.0f && ac <= 8.0f) || !(sh > 0.0f && sh <= 1.0f))
+      fail(MI355_EINVAL, "schedule entry: width scale > 0, active buckets in [1, 8], worker share in (0, 1]");
+  }
+  std::lock_guard<std::mutex> lk(g_mlp_mu);
+  g_sched.assign(table, table + 3 * (size_t)count);
+}
+
 void set_mlp(const AddsPolicy& p, const float* feat_lo, const float* feat_hi) {
   std::lock_guard<std::mutex> lk(g_mlp_mu);
   g_mlp = p;
@@ -104,7 +117,13 @@ const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hip
   AddsPolicy h{};
   float lo[14], hi[14];
   int mode = req.dynamic_delta ? kPolicyHeuristic : kPolicyFixed;
-  if (req.use_mlp) {
+  if (req.use_mlp == 2) {
+    std::lock_guard<std::mutex> lk(g_mlp_mu);
+    if (g_sched.empty()) fail(MI355_EINVAL, "no phase schedule set (mi355_adds_schedule_set)");
+    h.sched_len = (uint32_t)(g_sched.size() / 3);
+    std::copy(g_sched.begin(), g_sched.end(), h.sched);
+    mode = kPolicySchedule;
+  } else if (req.use_mlp) {
     if (!get_mlp(&h, lo, hi)) fail(MI355_EINVAL, "no delta model loaded (mi355_adds_mlp_set)");
     mode = kPolicyMlp;
   }

@@ -77,6 +77,11 @@ const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hip
 
 // Batched multi-source queries (batch.hip).
 void batch_layout(const mi355_query_batch& qb, int64_t n, std::vector<int64_t>& offsets);
+// Parent chains of targets out of a batched run's outputs (see mi355_batch_paths)
+void batch_paths(const mi355_query_batch& qb, const int64_t* out_offsets, const int32_t* d_pred,
+                 const uint32_t* d_dist, const int32_t* query, const int32_t* target, int64_t count,
+                 int64_t* path_offsets, uint32_t* target_dist, int32_t* path, int64_t path_capacity,
+                 hipStream_t s);
 // t_off / targets (host, may be null): per-query target vertices; a query
 // stops once all of its targets are settled
 void run_batch(mi355_graph& g, const mi355_query_batch& qb, const mi355_batch_options& o, const int32_t* t_off,
@@ -86,6 +91,7 @@ void batch_autotune(mi355_graph& g, const mi355_query_batch& qb, int32_t stride,
 
 // Process-wide delta model (policy.hip) and its training kernels (mlp.hip).
 void set_mlp(const AddsPolicy& p, const float* feat_lo, const float* feat_hi);
+void set_schedule(const float* table, int32_t count);
 bool get_mlp(AddsPolicy* p, float* feat_lo = nullptr, float* feat_hi = nullptr);
 void clear_mlp();
 void set_mlp_params(const mi355_mlp_params* p);

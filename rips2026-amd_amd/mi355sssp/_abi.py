@@ -166,6 +166,7 @@ _ENGINE_SIGS = {
                                      C.POINTER(BatchStats), P]),
     "mi355_sssp_batched_targets": (C.c_int, [P, C.POINTER(QueryBatch), P, P, C.POINTER(BatchOptions), P, P,
                                              C.POINTER(BatchStats), P]),
+    "mi355_batch_paths": (C.c_int, [P, C.POINTER(QueryBatch), P, P, P, P, P, I64, P, P, P, I64, P]),
     "mi355_batch_autotune": (C.c_int, [P, C.POINTER(QueryBatch), C.c_int32, C.POINTER(BatchOptions),
                                        P, P]),
     "mi355_mlp_train_options_default": (None, [C.POINTER(MlpTrainOptions)]),
@@ -176,6 +177,7 @@ _ENGINE_SIGS = {
     "mi355_adds_mlp_set": (C.c_int, [C.POINTER(MlpParams)]),
     "mi355_adds_default_workers": (C.c_int, [C.POINTER(C.c_int32)]),
     "mi355_adds_mlp_clear": (C.c_int, []),
+    "mi355_adds_schedule_set": (C.c_int, [P, I32]),
     "mi355_adds_trace": (C.c_int, [P, I32, C.POINTER(SsspOptions), P, I32, C.POINTER(I32),
                                    C.POINTER(SsspStats), P]),
 }

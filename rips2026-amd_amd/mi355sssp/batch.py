@@ -107,9 +107,11 @@ def layout(dg, qs: Queries) -> np.ndarray:
     return off
 
 
-def run(dg, qs: Queries, want_pred: bool = True, download: bool = True, targets=None, **kw):
+def run(dg, qs: Queries, want_pred: bool = True, download: bool = True, targets=None, device_out: bool = False,
+        **kw):
     """Solve all queries; returns (dist, pred, offsets, stats) with dist/pred
-    concatenated per query (None when not downloaded).
+    concatenated per query (None when not downloaded; the device buffers
+    themselves when device_out, for paths()).
 
     targets: None, or (target_offsets, target_vertices) with count + 1 offsets:
     query q stops once all of its targets (global vertex ids inside its
@@ -134,11 +136,35 @@ def run(dg, qs: Queries, want_pred: bool = True, download: bool = True, targets=
         check(engine().mi355_sssp_batched_targets(dg.handle, C.byref(qs.c()), t_off.ctypes.data,
                                                   t_v.ctypes.data if len(t_v) else None, C.byref(options(**kw)),
                                                   d_dist.ptr, d_pred.ptr if d_pred else None, C.byref(st), None))
+    if device_out:
+        return d_dist, d_pred, off, st
     if not download:
         return None, None, off, st
     dist = d_dist.to_array(np.float32 if dg.weight_type == W_F32 else np.uint32, total)
     pred = d_pred.to_array(np.int32, total) if d_pred else None
     return dist, pred, off, st
+
+
+def paths(dg, qs: Queries, off, d_dist, d_pred, query, target):
+    """Parent chains of targets out of run(..., device_out=True)'s buffers,
+    read on the device (mi355_batch_paths): target i is global vertex
+    target[i] of query query[i].  Returns (path offsets [count + 1], target
+    distances [count], path vertices, source first); an unreached target has
+    an empty path."""
+    query = np.ascontiguousarray(query, np.int32)
+    target = np.ascontiguousarray(target, np.int32)
+    if query.shape != target.shape:
+        raise ValueError("query and target arrays differ in length")
+    off = np.ascontiguousarray(off, np.int64)
+    count = len(target)
+    poff = np.zeros(count + 1, np.int64)
+    tdist = np.zeros(count, np.float32 if dg.weight_type == W_F32 else np.uint32)
+    args = (dg.handle, C.byref(qs.c()), off.ctypes.data, d_dist.ptr, d_pred.ptr, query.ctypes.data,
+            target.ctypes.data, count, poff.ctypes.data, tdist.ctypes.data)
+    check(engine().mi355_batch_paths(*args, None, 0, None))
+    nodes = np.zeros(max(int(poff[-1]), 1), np.int32)
+    check(engine().mi355_batch_paths(*args, nodes.ctypes.data, int(poff[-1]), None))
+    return poff, tdist, nodes[:int(poff[-1])]
 
 
 TUNE_CANDIDATES = 18

@@ -157,6 +157,13 @@ def clear_delta_model() -> None:
     check(engine().mi355_adds_mlp_clear())
 
 
+def set_schedule(table) -> None:
+    """Phase schedule for use_mlp=2: rows (width scale, active buckets, worker
+    share); phase i runs with row min(i, len - 1).  None or [] clears it."""
+    t = np.ascontiguousarray(np.zeros((0, 3)) if table is None else table, np.float32).reshape(-1, 3)
+    check(engine().mi355_adds_schedule_set(t.ctypes.data if len(t) else None, len(t)))
+
+
 def load_default_delta_model(install: bool = True) -> DeltaModel | None:
     if not os.path.exists(DEFAULT_MODEL):
         return None

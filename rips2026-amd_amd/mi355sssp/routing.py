@@ -181,34 +181,17 @@ def route(graph, meta: RouteMetadata, dg=None, max_label_slots: int = 1 << 30,
             t_off = np.zeros(len(chunk) + 1, np.int32)
             t_off[1:] = np.cumsum(counts)
             tgt = (t_off, node.astype(np.int32))
-        dist, pred, offs, _ = B.run(dg, qs, want_pred=True, targets=tgt, **batch_options)
-        block = np.repeat(np.asarray(offs[:-1], np.int64), counts)
-        d = dist[block + node]
-        reached = np.isfinite(d)
-        # walk all reached sinks' parents together: (sink, node) per step
-        walk_sink, walk_node = [], []
-        act = np.flatnonzero(reached)
-        cur = node[act]
-        for _ in range(n + 1):
-            if len(act) == 0:
-                break
-            walk_sink.append(act)
-            walk_node.append(cur)
-            nxt = pred[block[act] + cur].astype(np.int64)
-            keep = nxt >= 0
-            act, cur = act[keep], nxt[keep]
-        if len(act):
-            q = chunk[int(np.searchsorted(np.cumsum(counts), act[0], side="right"))]
-            raise RuntimeError(f"net {q}: parent cycle at sink {int(node[act[0]])}")
-        if walk_sink:
-            ws, wn = np.concatenate(walk_sink), np.concatenate(walk_node)
-            step = np.concatenate([np.full(len(x), t, np.int64) for t, x in enumerate(walk_sink)])
-            # per sink, source first: sink ascending, step descending
-            order = np.lexsort((-step, ws))
-            ws, wn = ws[order], wn[order]
-        else:
-            ws = wn = np.zeros(0, np.int64)
-        bounds = np.searchsorted(ws, np.arange(len(node) + 1))
+        d_dist, d_pred, offs, _ = B.run(dg, qs, want_pred=True, targets=tgt, device_out=True, **batch_options)
+        # every sink's path, read on the device (source first; empty when unreached)
+        qidx = np.repeat(np.arange(len(chunk), dtype=np.int32), counts)
+        poff, tdist, wn = B.paths(dg, qs, offs, d_dist, d_pred, qidx, node.astype(np.int32))
+        del d_dist, d_pred
+        wn = wn.astype(np.int64)
+        plen = np.diff(poff)
+        d = tdist.astype(np.float64)
+        d[plen == 0] = np.inf
+        ws = np.repeat(np.arange(len(node), dtype=np.int64), plen)
+        bounds = poff
         # every hop u -> v (row v, column u) in one sorted search
         same = np.zeros(len(ws), bool)
         if len(ws) > 1:

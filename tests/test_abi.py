@@ -37,6 +37,18 @@ def test_engine_reports_errors_without_a_device():
     assert b"null" in lib.mi355_last_error()
 
 
+def test_phase_schedules_are_checked_on_the_host():
+    """mi355_adds_schedule_set validates every row before it is kept."""
+    from mi355sssp import learn as L
+
+    for bad in ([(0.0, 1, 0.5)], [(1.0, 9, 0.5)], [(1.0, 1, 0.0)], [(1.0, 1, 1.5)], [(float("nan"), 1, 1)],
+                [(1.0, 1, 1.0)] * 65):
+        with pytest.raises(M.Mi355Error):
+            L.set_schedule(bad)
+    L.set_schedule([(0.25, 1, 0.5), (2.0, 4, 0.25)])
+    L.set_schedule(None)
+
+
 def test_compat_headers_compile_against_the_abi(tmp_path):
     """The C++ compatibility headers restating the reference signatures must
     compile (syntax only) with hipcc for gfx950."""

@@ -26,7 +26,9 @@ def _check(g, qs, dist, pred, off, sample=None):
             assert badp.size == 0, f"query {q}: {badp.size} predecessors differ"
 
 
-def _f32_grid(rows, cols, seed):
+def _f32_grid(rows, cols, seed, holes=0):
+    """4-neighbour grid with fp32 weights, 8 % of the edges missing; holes > 0:
+    no edge into a vertex v with v % holes == 1 (unreachable unless a source)."""
     rng = np.random.default_rng(seed)
     src, dst = [], []
     for r in range(rows):
@@ -34,6 +36,8 @@ def _f32_grid(rows, cols, seed):
             v = r * cols + c
             for dr, dc in ((-1, 0), (0, -1), (0, 1), (1, 0)):
                 rr, cc = r + dr, c + dc
+                if holes and (rr * cols + cc) % holes == 1:
+                    continue
                 if 0 <= rr < rows and 0 <= cc < cols and rng.random() > 0.08:
                     src.append(v)
                     dst.append(rr * cols + cc)
@@ -282,3 +286,54 @@ def test_invalid_targets_are_rejected():
     d, p, off, st = B.run(dg, qs, targets=(np.array([0, 0]), np.array([], np.int32)))
     full, fp, _, _ = B.run(dg, qs)
     assert d.tobytes() == full.tobytes() and np.array_equal(p, fp) and st.stopped_queries == 0
+
+
+def test_paths_read_on_the_device_match_the_parent_outputs():
+    """mi355_batch_paths: every target's path is its parent chain in the
+    downloaded outputs (source first), its distance the output distance, and
+    an unreached target has an empty path; also on a stopped batch."""
+    side = 120
+    g = _f32_grid(side, side, seed=9, holes=13)  # every 13th vertex unreachable
+    dg = M.DeviceGraph(g)
+    qs = B.routing_queries(side, side, 24, seed=6, side_min=3, side_max=120)
+    rng = np.random.default_rng(1)
+    query, target = [], []
+    for q in range(qs.count):
+        r0, c0, r1, c1 = (int(x) for x in qs.boxes[q])
+        for _ in range(5):
+            query.append(q)
+            target.append(int(rng.integers(r0, r1)) * side + int(rng.integers(c0, c1)))
+    for stop in (False, True):
+        tg = None
+        if stop:
+            t_off = np.searchsorted(np.array(query), np.arange(qs.count + 1))
+            tg = (t_off, np.array(target))
+        d_dist, d_pred, off, _ = B.run(dg, qs, targets=tg, device_out=True)
+        dist = d_dist.to_array(np.float32, int(off[-1]))
+        pred = d_pred.to_array(np.int32, int(off[-1]))
+        print(f"paths: {len(target)} targets, stop={stop}", flush=True)
+        poff, tdist, nodes = B.paths(dg, qs, off, d_dist, d_pred, query, target)
+        assert poff[0] == 0 and len(poff) == len(target) + 1 and poff[-1] == len(nodes)
+        unreached = 0
+        for i, (q, t) in enumerate(zip(query, target)):
+            r0, c0, r1, c1 = (int(x) for x in qs.boxes[q])
+            loc = lambda v: off[q] + (v // side - r0) * (c1 - c0) + v % side - c0  # noqa: E731
+            assert tdist[i].tobytes() == dist[loc(t)].tobytes()
+            want = []
+            if np.isfinite(dist[loc(t)]):
+                v = t
+                while True:
+                    want.append(v)
+                    if pred[loc(v)] < 0:
+                        break
+                    v = int(pred[loc(v)])
+                want = want[::-1]
+                assert want[0] in qs.query_sources(q)
+            else:
+                unreached += 1
+            assert np.array_equal(nodes[poff[i]:poff[i + 1]], want), (q, t)
+        assert unreached > 0
+    with pytest.raises(M.Mi355Error):  # target outside its query's rectangle
+        r0, c0, r1, c1 = (int(x) for x in qs.boxes[0])
+        outside = (r1 % side) * side + c0 if r1 < side else (r0 - 1) * side + c0
+        B.paths(dg, qs, off, d_dist, d_pred, [0], [outside])

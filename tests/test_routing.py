@@ -203,6 +203,24 @@ def test_route_rejects_a_sidecar_of_another_graph(tmp_path):
         R.route(g.transpose(), meta, dg=object())
 
 
+def _host_paths(dg, qs, off, dist, pred, query, target):
+    """A stand-in for B.paths over host arrays (1 x n rectangles): each
+    target's parent chain, source first, empty when unreached."""
+    poff, tdist, nodes = [0], [], []
+    for q, t in zip(query, target):
+        b = int(off[q])
+        tdist.append(dist[b + t])
+        if not np.isfinite(dist[b + t]):
+            poff.append(poff[-1])
+            continue
+        chain = [int(t)]
+        while pred[b + chain[-1]] >= 0:
+            chain.append(int(pred[b + chain[-1]]))
+        nodes.extend(chain[::-1])
+        poff.append(len(nodes))
+    return np.array(poff, np.int64), np.array(tdist, np.float32), np.array(nodes, np.int32)
+
+
 def _oracle_run(out):
     """A stand-in for the batched engine in CPU tests: the oracle's
     multi-source Dijkstra and canonical parents for every query."""
@@ -235,6 +253,7 @@ def test_route_paths_and_pips_on_the_cpu(tmp_path, monkeypatch):
         return run(dg, qs, *a, **kw)
 
     monkeypatch.setattr(R.B, "run", recording_run)
+    monkeypatch.setattr(R.B, "paths", _host_paths)
     nets = R.route(g, meta, dg=object())
     # the targets handed to the engine: every net's sinks, per query
     todo = [q for q in range(meta.request_count) if len(meta.request_sources(q))]
