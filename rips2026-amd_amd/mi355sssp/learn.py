@@ -7,10 +7,16 @@ width relative to the heuristic delta0, number of buckets served at once,
 log2 of the share of the persistent workgroups that claim work).  It is
 trained by regression on traces that ADDS itself records: for each training
 graph ADDS is timed under static settings (first the worker share, then the
-width scale x active buckets grid at the best share), every timed setting is
-also run once with phase records, the fastest setting (kernel time) becomes
-the target of every record of that graph, and the work done and relaxations
-wasted of each setting are kept in the dataset's metadata.
+width scale x active buckets grid at the best share), and every timed setting
+is also run once with phase records.  Then two-phase schedules are timed (the
+first phase, the source's bucket, at a width of HEADS; every later phase at
+the best static setting or that setting at a width of TAIL_SCALES): on R-MAT
+graphs a narrow first bucket (the ramp-up, where a wide bucket relaxes
+vertices that are not settled yet) and wide later buckets beat every static
+setting.  The fastest schedule gives two targets: its first phase for one
+record with the input the engine gives the network before the first claim,
+and its tail for every phase-boundary record of the graph.  The work done and
+relaxations wasted of each static setting are kept in the metadata.
 """
 from __future__ import annotations
 
@@ -37,6 +43,10 @@ SHARES = (1.0, 0.5, 0.25, 0.125)
 SCALES = (0.25, 0.5, 1.0, 2.0, 4.0)
 ACTIVE = (1, 2, 4)
 SETTINGS = [(s, a) for a in ACTIVE for s in SCALES]
+# two-phase schedules searched per training graph: first-phase width scales,
+# and the tail widths tried besides the best static setting's
+HEADS = (0.0625, 0.125, 0.25)
+TAIL_SCALES = (1.0, 2.0, 4.0)
 
 
 def _f32(a, shape=None) -> np.ndarray:
@@ -301,12 +311,64 @@ def collect_graph(dg, source: int, reps: int = 2, max_records: int = 24):
     return x, y, meta
 
 
-def collect(specs, log=print, reps: int = 2, time_budget_s: float | None = None):
-    """Dataset over training specs: features, targets, per-sample weights
-    (each graph weighs the same in total) and per-graph metadata."""
+def first_phase_features(graph_feat) -> np.ndarray:
+    """The 32 inputs the engine gives the network before the first claim
+    (adds.hip, k_adds_first_phase): the 14 graph features, a window of one
+    queued item (log2(1 + 1) = 1), the heuristic width (log2 ratio 0) and that
+    one item in the first window slot (log2(1 + 1) = 1), other slots empty."""
+    x = np.zeros(N_IN, np.float32)
+    x[:N_GRAPH] = _f32(graph_feat, (N_GRAPH,))
+    x[14], x[15], x[16] = 1.0, 0.0, 1.0
+    return x
+
+
+def collect_head(dg, source: int, graph_feat, static, delta0: float, reps: int = 2):
+    """Search two-phase schedules on one graph: the first phase at width h x
+    delta0 (1 bucket) for every h in HEADS, then a tail setting (scale,
+    active, share) from the best static setting and the same setting at widths
+    1, 2 and 4 x delta0 (a narrow first bucket leaves the later, shallower
+    phases free to run wide); against the best static setting alone.  Returns
+    (the first-phase record, its target, the tail target for the graph's
+    phase-boundary records, metadata)."""
+    ss, sa, sw = (float(static[0]), int(static[1]), float(static[2]))
+    tails = [(ss, sa, sw)] + [(s, sa, sw) for s in TAIL_SCALES if s != ss]
+    o_static = dg.options(ALG_ADDS, delta=ss * delta0, active_buckets=sa, worker_blocks=workers_for(sw))
+    o_sched = dg.options(ALG_ADDS, use_mlp=2)
+
+    def best_of(o):
+        return min(dg.sssp_labels(int(source), o).kernel_ms for _ in range(reps))
+
+    costs = {"static": best_of(o_static)}
+    try:
+        for h in HEADS:
+            for t in tails:
+                set_schedule([(h, 1, t[2]), t])
+                costs[(h,) + t] = best_of(o_sched)
+    finally:
+        set_schedule(None)
+    best = min(costs, key=costs.get)
+    if best == "static":
+        head = tail = (ss, sa, sw)
+    else:
+        head, tail = (best[0], 1, best[3]), best[1:]
+
+    def target(s):
+        return np.array([math.log2(s[0]), float(s[1]), math.log2(s[2])], np.float32)
+
+    meta = {"head": list(head), "tail": list(tail), "head_ms": costs[best], "static_ms": costs["static"],
+            "head_costs": {"x".join(str(v) for v in k) if k != "static" else k: c for k, c in costs.items()}}
+    return first_phase_features(graph_feat), target(head), target(tail), meta
+
+
+def collect(specs, log=print, reps: int = 2, time_budget_s: float | None = None, heads: bool = True):
+    """Dataset over training specs: phase-boundary features, targets,
+    per-sample weights (each graph weighs the same in total), per-graph
+    metadata, and (heads) one first-phase record and target per graph
+    (hx, hy; hidx: its graph's index in metas)."""
     from .device import DeviceGraph
 
     X, Y, W, metas = [], [], [], []
+    HX, HY, HI = [], [], []
     t0 = time.time()
     for i, spec in enumerate(specs):
         if time_budget_s is not None and time.time() - t0 > time_budget_s:
@@ -316,22 +378,87 @@ def collect(specs, log=print, reps: int = 2, time_budget_s: float | None = None)
         dg = DeviceGraph(g)
         src = int(g.pick_sources(1, seed=spec["seed"] & 0xFFFFFFFF)[0])
         x, y, meta = collect_graph(dg, src, reps=reps)
+        if heads and len(x):
+            hx, hy, ty, hmeta = collect_head(dg, src, x[0, :N_GRAPH], meta["best"], meta["delta0"], reps=reps)
+            meta.update(hmeta)
+            y[:] = ty  # the phase-boundary records' target: the schedule's tail
+            HX.append(hx)
+            HY.append(hy)
+            HI.append(len(metas))
         meta.update(spec)
         meta["n"], meta["m"], meta["source"] = g.n, g.m, src
+        meta["records"] = int(len(x))
         X.append(x)
         Y.append(y)
         W.append(np.full(len(x), 1.0 / max(len(x), 1), np.float32))
         metas.append(meta)
         if i % 10 == 0:
+            hd = f" head={meta['head']} gain={meta['plain_ms'] / meta['head_ms']:.2f}" if "head" in meta else ""
             log(f"collect {i + 1}/{len(specs)} {spec['kind']} n={g.n} best={meta['best']} "
-                f"gain={meta['plain_ms'] / meta['best_ms']:.2f} ({time.time() - t0:.0f}s)")
+                f"gain={meta['plain_ms'] / meta['best_ms']:.2f}{hd} ({time.time() - t0:.0f}s)")
         del dg, g
     x = np.concatenate(X) if X else np.zeros((0, N_IN), np.float32)
     y = np.concatenate(Y) if Y else np.zeros((0, N_OUT), np.float32)
     w = np.concatenate(W) if W else np.zeros(0, np.float32)
     if len(w):
         w *= len(w) / w.sum()
-    return x, y, w, metas
+    hx = np.array(HX, np.float32).reshape(-1, N_IN)
+    hy = np.array(HY, np.float32).reshape(-1, N_OUT)
+    return x, y, w, metas, hx, hy, np.array(HI, np.int64)
+
+
+def collect_heads(metas, x, y, w, log=print, reps: int = 2, time_budget_s: float | None = None):
+    """First-phase records for the graphs of an existing dataset (metas, its
+    phase-boundary features x, targets y and weights w, in collect()'s order):
+    each graph is built again from its spec and collect_head() runs from its
+    best static setting; y is rewritten in place with each searched graph's
+    tail target.  Returns (hx, hy, hidx, per-graph head metadata)."""
+    from .device import DeviceGraph
+
+    counts = record_counts(metas, w)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    HX, HY, HI, HM = [], [], [], []
+    t0 = time.time()
+    for i, meta in enumerate(metas):
+        if time_budget_s is not None and time.time() - t0 > time_budget_s:
+            log(f"heads: time budget reached after {i} graphs")
+            break
+        if counts[i] == 0:
+            continue
+        g = make_graph(meta)
+        dg = DeviceGraph(g)
+        hx, hy, ty, hmeta = collect_head(dg, int(meta["source"]), x[first[i], :N_GRAPH], meta["best"],
+                                         float(meta["delta0"]), reps=reps)
+        y[first[i]:first[i] + counts[i]] = ty
+        HX.append(hx)
+        HY.append(hy)
+        HI.append(i)
+        HM.append(hmeta)
+        if i % 20 == 0:
+            log(f"heads {i + 1}/{len(metas)} {meta['kind']} n={g.n} static={meta['best']} head={hmeta['head']} "
+                f"tail={hmeta['tail']} {hmeta['static_ms']:.2f} -> {hmeta['head_ms']:.2f} ms "
+                f"({time.time() - t0:.0f}s)")
+        del dg, g
+    return (np.array(HX, np.float32).reshape(-1, N_IN), np.array(HY, np.float32).reshape(-1, N_OUT),
+            np.array(HI, np.int64), HM)
+
+
+def record_counts(metas, w) -> np.ndarray:
+    """Phase-boundary records per graph of a dataset, in the order of metas:
+    from the metadata (records) where collect() stored it, else from the
+    weights (collect() gives each graph the same total c, spread evenly over
+    its records, so a graph's count is c / w of its first record)."""
+    w = np.asarray(w, np.float32)
+    c = float(w.astype(np.float64).sum()) / max(len(metas), 1)
+    out = np.zeros(len(metas), np.int64)
+    i = 0
+    for j, m in enumerate(metas):
+        k = int(m["records"]) if "records" in m else (int(round(c / float(w[i]))) if i < len(w) else 0)
+        out[j] = k
+        i += k
+    if i != len(w):
+        raise ValueError("sample weights do not match the graphs' record counts")
+    return out
 
 
 def time_weights(w, metas, power: float = 1.0):

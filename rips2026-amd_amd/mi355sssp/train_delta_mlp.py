@@ -4,8 +4,10 @@ network on the device, and write the model (and a summary) for the engine.
     python -m mi355sssp.train_delta_mlp --graphs 2000 --out mi355sssp/data/delta_mlp.json
 
 Collection can be split over runs (--first/--graphs write one .npz shard
-each); --train-only trains on the shards given with --shards.  Training
-graphs come from learn.training_specs(seed); evaluation graphs use other seeds.
+each); --train-only trains on the shards given with --shards.  --heads-for
+adds the first-phase records (learn.collect_heads) to a shard collected
+without them.  Training graphs come from learn.training_specs(seed);
+evaluation graphs use other seeds.
 """
 from __future__ import annotations
 
@@ -46,6 +48,12 @@ def main() -> int:
     ap.add_argument("--shard-out", default=None, help="write the collected shard here (.npz)")
     ap.add_argument("--shards", nargs="*", default=[], help="train on these shards")
     ap.add_argument("--train-only", action="store_true")
+    ap.add_argument("--heads-for", default=None,
+                    help="search the first phase of every graph of this shard and write the shard "
+                         "with its first-phase records to --shard-out")
+    ap.add_argument("--head-weight", type=float, default=1.0,
+                    help="weight of a graph's first-phase record relative to all of its "
+                         "phase-boundary records together")
     ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -57,6 +65,21 @@ def main() -> int:
     args = ap.parse_args()
 
     shards = list(args.shards)
+    if args.heads_for:
+        z = np.load(args.heads_for, allow_pickle=False)
+        metas = json.loads(str(z["meta"]))
+        t0 = time.time()
+        y = z["y"].copy()
+        hx, hy, hidx, hm = L.collect_heads(metas, z["x"], y, z["w"], log=log, reps=args.reps,
+                                           time_budget_s=args.budget_s)
+        for i, m in zip(hidx, hm):
+            metas[int(i)].update(m)
+        gains = [metas[int(i)]["plain_ms"] / metas[int(i)]["head_ms"] for i in hidx]
+        log(f"first phases of {len(hidx)} graphs in {time.time() - t0:.0f}s; plain/best two-phase "
+            f"mean {np.mean(gains) if gains else 0:.3f}")
+        np.savez_compressed(args.shard_out, x=z["x"], y=y, w=z["w"], hx=hx, hy=hy, hidx=hidx,
+                            meta=np.array(json.dumps(metas)))
+        return 0
     if not args.train_only:
         if args.large_grids > 0:
             specs = L.large_grid_specs(args.large_grids, args.seed)
@@ -67,20 +90,22 @@ def main() -> int:
         if args.max_rmat_scale is not None:
             specs = [s for s in specs if s["kind"] != "rmat" or s["scale"] <= args.max_rmat_scale]
         t0 = time.time()
-        x, y, w, metas = L.collect(specs, log=log, reps=args.reps, time_budget_s=args.budget_s)
+        x, y, w, metas, hx, hy, hidx = L.collect(specs, log=log, reps=args.reps, time_budget_s=args.budget_s)
         log(f"collected {len(metas)} graphs, {len(x)} samples in {time.time() - t0:.0f}s")
         if args.shard_out:
-            np.savez_compressed(args.shard_out, x=x, y=y, w=w,
+            np.savez_compressed(args.shard_out, x=x, y=y, w=w, hx=hx, hy=hy, hidx=hidx,
                                 meta=np.array(json.dumps(metas)))
             shards.append(args.shard_out)
         else:
             shards = []
-            data = [(x, y, w, metas)]
+            data = [(x, y, w, metas, hx, hy, hidx)]
     if shards:
         data = []
         for p in shards:
             z = np.load(p, allow_pickle=False)
-            data.append((z["x"], z["y"], z["w"], json.loads(str(z["meta"]))))
+            empty = (np.zeros((0, L.N_IN), np.float32), np.zeros((0, L.N_OUT), np.float32), np.zeros(0, np.int64))
+            h = (z["hx"], z["hy"], z["hidx"]) if "hx" in z.files else empty
+            data.append((z["x"], z["y"], z["w"], json.loads(str(z["meta"]))) + h)
     x = np.concatenate([d[0] for d in data])
     y = np.concatenate([d[1] for d in data])
     metas = [m for d in data for m in d[3]]
@@ -93,13 +118,31 @@ def main() -> int:
         w = L.class_weights(w, metas)
     elif args.graph_weight != "equal":
         w = L.time_weights(w, metas, 1.0 if args.graph_weight == "time" else 0.5)
+    # first-phase records: each weighs head_weight x its graph's total
+    counts = np.concatenate([L.record_counts(d[3], d[2]) for d in data])
+    bounds = np.concatenate([[0], np.cumsum(counts)])
+    per_graph = np.array([w[bounds[i]:bounds[i + 1]].sum() for i in range(len(metas))])
+    base = np.cumsum([0] + [len(d[3]) for d in data])[:-1]
+    hidx = np.concatenate([d[6] + b for d, b in zip(data, base)]).astype(np.int64)
+    hx = np.concatenate([d[4] for d in data]).reshape(-1, L.N_IN)
+    hy = np.concatenate([d[5] for d in data]).reshape(-1, L.N_OUT)
+    hw = args.head_weight * per_graph[hidx] if len(hidx) else np.zeros(0)
+    # a boundary record with at most one queued item has the first phase's
+    # input (the end of a run): left out, its decision does not matter
+    keep = x[:, 14] > 1.0 + 1e-6
+    x = np.concatenate([x[keep], hx])
+    y = np.concatenate([y[keep], hy])
+    w = np.concatenate([w[keep], hw]).astype(np.float32)
     w = w * (len(w) / w.sum())
+    log(f"{int(keep.sum())} phase-boundary records ({int((~keep).sum())} end-of-run records left out), "
+        f"{len(hidx)} first-phase records")
     t0 = time.time()
     model, losses = L.fit(x, y, w, epochs=args.epochs, batch=args.batch, lr=args.lr)
     log(f"trained on {len(x)} samples from {len(metas)} graphs in {time.time() - t0:.1f}s: "
         f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     pred = model.forward(x)
     best_gain = [m["plain_ms"] / m["best_ms"] for m in metas]
+    head_gain = [m["plain_ms"] / m["head_ms"] for m in metas if "head_ms" in m]
     summary = {
         "graphs": len(metas), "samples": int(len(x)), "epochs": args.epochs, "batch": args.batch,
         "lr": args.lr, "train_seed": args.seed, "graph_weight": args.graph_weight,
@@ -110,6 +153,8 @@ def main() -> int:
         "mean_abs_err_log2_share": float(np.abs(pred[:, 2] - y[:, 2]).mean()),
         "oracle_static_gain_mean": float(np.mean(best_gain)),
         "oracle_static_gain_median": float(np.median(best_gain)),
+        "oracle_two_phase_gain_mean": float(np.mean(head_gain)) if head_gain else None,
+        "first_phase_records": int(len(hidx)), "head_weight": args.head_weight, "heads": list(L.HEADS),
         "worker_shares": list(L.SHARES), "settings": [list(s) for s in L.SETTINGS],
     }
     model.save(args.out, meta=summary)

@@ -44,6 +44,28 @@ def test_time_weights_scale_each_graph_by_its_plain_time():
         assert tot / tot[0] == pytest.approx(want)
 
 
+def test_record_counts_from_metadata_or_weights():
+    records = [3, 4, 2, 5]
+    w = _equal_per_graph(records)
+    metas = [dict(kind="rmat") for _ in records]
+    assert L.record_counts(metas, w).tolist() == records
+    metas[1]["records"] = 4
+    assert L.record_counts(metas, w).tolist() == records
+    metas[1]["records"] = 5
+    with pytest.raises(ValueError):
+        L.record_counts(metas, w)
+
+
+def test_first_phase_features():
+    """The input the engine builds before the first claim: graph features,
+    a window of one item in the first slot, the heuristic width."""
+    g = np.arange(L.N_GRAPH, dtype=np.float32) * 0.5
+    x = L.first_phase_features(g)
+    assert x.shape == (L.N_IN,) and np.array_equal(x[:L.N_GRAPH], g)
+    assert x[14] == np.log2(2.0) and x[15] == 0.0 and x[16] == np.log2(2.0)
+    assert not x[17:].any()
+
+
 def test_graph_class_and_feature_range():
     assert L.graph_class(dict(kind="rmat", scale=22, n=1 << 22)) == "rmat22"
     assert L.graph_class(dict(kind="grid", n=50_000)) == "grid5"        # clamped up

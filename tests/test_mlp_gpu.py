@@ -102,8 +102,43 @@ def test_adds_with_network_is_exact_and_follows_the_network():
             share = np.exp2(np.clip(out[:, 2], -5, 0)).astype(np.float32)
             want_workers = np.minimum(np.floor(np.maximum(wk * share, np.float32(16))), wk)
             assert np.all(np.abs(rec[:, 37] - want_workers) <= 1)
+            # the first phase's width is the network's output for the input the
+            # training set uses for it (learn.first_phase_features); record 0
+            # holds log2(that width / delta0) as its input 15
+            first = m.forward(L.first_phase_features(rec[0, :14])[None, :])[0]
+            want_first = np.clip(d0 * np.float32(np.exp2(np.clip(first[0], -4, 4))), d0 / 64, d0 * 64)
+            assert rec[0, 15] == pytest.approx(float(np.log2(want_first / d0)), abs=1e-4)
     finally:
         L.clear_delta_model()
+
+
+def test_adds_follows_a_phase_schedule():
+    """use_mlp = 2: phase i runs with row min(i, len - 1) of the schedule
+    (width, active buckets, share of claiming workgroups); distances and
+    parents stay exact."""
+    sched = [(0.125, 1, 0.5), (0.5, 2, 1.0), (2.0, 4, 0.25)]
+    L.set_schedule(sched)
+    try:
+        for g in (M.rmat(14, 16, seed=12), M.grid4(150, 150, 1, 100, seed=13)):
+            dg = M.DeviceGraph(g)
+            for src in g.pick_sources(2, seed=5):
+                st = _check_adds(g, dg, int(src), use_mlp=2)
+                assert st.policy == 3
+            rec, st = L.trace(dg, int(g.pick_sources(1, seed=5)[0]), use_mlp=2)
+            assert len(rec) > 3
+            d0 = np.float32(st.initial_delta)
+            assert rec[0, 15] == pytest.approx(math.log2(sched[0][0]), abs=1e-5)
+            row = np.minimum(np.arange(len(rec)) + 1, len(sched) - 1)  # record i sets phase i + 1
+            want_delta = np.array([d0 * np.float32(sched[r][0]) for r in row], np.float32)
+            assert np.allclose(rec[:, 33], want_delta, rtol=2e-6)
+            assert np.array_equal(rec[:, 34], [sched[r][1] for r in row])
+            wk = L.default_workers()
+            want_workers = np.array([min(wk, max(int(wk * sched[r][2]), 16)) for r in row])
+            assert np.all(np.abs(rec[:, 37] - want_workers) <= 1)
+    finally:
+        L.set_schedule(None)
+    with pytest.raises(M.Mi355Error):  # no schedule set
+        dg.sssp(0, M.ALG_ADDS, use_mlp=2)
 
 
 def test_graph_outside_the_training_range_runs_the_fixed_delta():
