@@ -141,7 +141,15 @@ typedef struct {
   int32_t worker_blocks;    /* ADDS: persistent workgroups (0 = auto) */
   int32_t rounds_per_sync;  /* host sync cadence (0 = adaptive) */
   int32_t active_buckets;   /* ADDS: buckets served at once, 1..32 (0 = 1) */
-  int32_t reserved[6];
+  int32_t pull;             /* ADDS: how a finished bucket's heavy edges are relaxed.
+                               0: pushed from the bucket's vertices (the reference's ADDS);
+                               1: pulled by the unsettled vertices instead when the estimated
+                               push work exceeds 2^pull_margin x the estimated pull work
+                               (use_mlp = 1 with a model that has a pull output: the
+                               network's margin for the phase) */
+  float pull_margin;        /* ADDS, pull = 1: log2 of the push / pull work ratio above which
+                               a bucket is pulled (0: whenever pulling is estimated cheaper) */
+  int32_t reserved[4];
 } mi355_sssp_options;
 
 typedef struct {
@@ -162,6 +170,8 @@ typedef struct {
   int32_t policy;           /* ADDS phase policy used: 0 fixed delta, 1 feedback, 2 learned
                                network (0 when use_mlp is set but the graph lies outside the
                                model's training range), 3 explicit schedule */
+  int32_t pulled_phases;    /* ADDS: finished buckets whose heavy edges were pulled */
+  int32_t reserved;
 } mi355_sssp_stats;
 
 void mi355_sssp_options_default(mi355_sssp_options* opt);

@@ -94,7 +94,10 @@ constexpr uint32_t kClaim = 2 * kAB;  // most items per workgroup claim (two pas
 constexpr uint32_t kBigClaimAvail = 65536;  // slot size above which claims take kClaim
 constexpr int kHubChunk = 8192;     // edges per hub chunk
 constexpr int kHubDegree = 4096;  // rows with more edges become hub chunks
-static_assert(5 + kAddsSlots <= 64, "claim probe: lanes 0-3, the window's semaphores, the worker share");
+static_assert(8 + kAddsSlots <= 64,
+              "claim probe: lanes 0-3, the window's semaphores, the worker share, the pull words");
+// vertices per pull chunk (four passes of kAB)
+constexpr uint32_t kPullChunk = 4 * 512;
 constexpr int kEmptyEntry = -1;     // ring sentinel: reserved, not yet written
 constexpr uint32_t kPushStage = 512;  // staged pushes per wave
 constexpr uint32_t kHeavyStage = 256; // staged heavy-list entries per wave
@@ -113,6 +116,8 @@ constexpr uint32_t kErrRing = 1u, kErrHub = 2u, kErrStall = 4u;
 #define HV_HEAD(c, s) ((c)->hv_head[(s) * kPad])
 #define HV_TAIL(c, s) ((c)->hv_tail[(s) * kPad])
 #define HV_AVAIL(c, s) ((c)->hv_avail[(s) * kPad])
+#define HV_EDGES(c, s) ((c)->hv_edges[(s) * (kPad / 2)])
+#define HV_KEY(c, s) ((c)->hv_key[(s) * kPad])
 
 // Wait until this wave's outstanding vector memory operations have completed
 // (s_waitcnt vmcnt(0)).  Device-scope atomics, and loads and stores with the
@@ -176,6 +181,13 @@ struct AddsArgs {
   AddsCtrl* c;
   const AddsPolicy* policy;
   const int* ids;     // input id of every internal vertex, the parent in labels (nullptr: the same)
+  // in-edge rows for pulls (weight-sorted; nullptr: pulls off) and their
+  // light prefixes at the split width (a pull reads only heavy in-edges: the
+  // light edges of a bucket's vertices were relaxed when they were processed)
+  const int* trow;
+  const uint2* tedges;
+  const int* tsplit;
+  unsigned long long m_edges;  // edges of the graph (the pull work estimate)
   uint32_t n;
   int live_edges;     // keep the live edge counter (phase traces read it)
   // Upper bound on claims by one workgroup: far above any real run; reaching
@@ -217,6 +229,7 @@ constexpr uint32_t kLocal = 4096;
 constexpr uint32_t kLocalKeep = 512;
 constexpr uint32_t kNoLocal = 0xFFFFFFFFu;
 constexpr uint32_t kTakenLocal = 0xFFFFu;  // stage entry moved to the local queue
+constexpr uint32_t kFirstReach = 0x80000000u;  // stage entry: the vertex's first finite label
 static_assert((kLocal & (kLocal - 1)) == 0, "local queue is a power-of-two ring");
 struct LocalQueue {
   int* q;          // kLocal entries
@@ -279,6 +292,7 @@ __device__ inline uint32_t stage_to_local(PushStage& st) {
   return take;
 }
 
+template <bool P>
 __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   if (st.n == 0) return;
   const unsigned lane = lane_id();
@@ -286,6 +300,25 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if constexpr (P) {
+    // the out-degree of every newly reached vertex goes into the reached-edge
+    // total that the pull decision weighs (each lane reads back only the
+    // entries it flags and clears)
+    uint32_t fr = 0;
+    for (uint32_t i = lane; i < st.n; i += kWave) {
+      const uint32_t sl = st.slot[i];
+      if (sl & kFirstReach) {
+        st.slot[i] = sl & ~kFirstReach;
+        const int v = st.v[i];
+        fr += (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
+      }
+    }
+    if (__ballot(fr != 0)) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) fr += __shfl_xor(fr, off, 64);
+      if (lane == 0) atomicAdd(&c->reached_edges, (unsigned long long)fr);
+    }
+  }
   const uint32_t n_local = st.lq.slot != kNoLocal ? stage_to_local(st) : 0u;
   if (n_local == st.n) {
     st.n = 0;
@@ -348,26 +381,29 @@ __device__ inline bool claim_stamp(const AddsArgs& a, bool imp, uint32_t v, uint
   return __hip_atomic_fetch_min(a.qstamp + v, bkt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > bkt;
 }
 
-// Stage the lanes that claimed a stamp.
+// Stage the lanes that claimed a stamp (P, pulls on: first = the push follows
+// the vertex's first finite label).
+template <bool P>
 __device__ inline void stage_wanted(const AddsArgs& a, PushStage& st, bool want, uint32_t v,
-                                    uint32_t bkt, unsigned long long& pushed) {
+                                    uint32_t bkt, unsigned long long& pushed, bool first = false) {
   const unsigned long long mask = __ballot(want);
   if (mask == 0) return;
   const uint32_t k = (uint32_t)__popcll(mask);
-  if (st.n + k > kPushStage) stage_publish(a, st);
+  if (st.n + k > kPushStage) stage_publish<P>(a, st);
   if (want) {
     const uint32_t i = st.n + prefix_count(mask);
     st.v[i] = (int)v;
-    st.slot[i] = bkt % kAddsSlots;
+    st.slot[i] = (bkt % kAddsSlots) | (P && first ? kFirstReach : 0u);
   }
   st.n += k;
   pushed += k;
 }
 
 // Stage the lanes' improved vertices (dedup by qstamp).
+template <bool P>
 __device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp, uint32_t v,
                                      uint32_t bkt, unsigned long long& pushed) {
-  stage_wanted(a, st, claim_stamp(a, imp, v, bkt), v, bkt, pushed);
+  stage_wanted<P>(a, st, claim_stamp(a, imp, v, bkt), v, bkt, pushed);
 }
 
 // Relax four edges per lane (edge p: u[p] -> v[p], candidate key ck[p],
@@ -378,7 +414,7 @@ __device__ inline void push_improved(const AddsArgs& a, PushStage& st, bool imp,
 // lane keep four times the accesses in flight.  A lane without an edge probes
 // label 0 and discards it (a guarded load would wait inside its branch).  Per
 // edge this is the step of relax_edge_pair followed by push_improved.
-template <class W>
+template <class W, bool P>
 __device__ inline void relax_quad(const AddsArgs& a, PushStage& st, const uint32_t (&v)[4],
                                   const uint32_t (&ck)[4], const uint32_t (&u)[4],
                                   const uint32_t (&lo)[4], uint32_t base, float tbase, float inv,
@@ -400,9 +436,11 @@ __device__ inline void relax_quad(const AddsArgs& a, PushStage& st, const uint32
     if (cand[p] < cur[p]) old[p] = atomicMin(a.labels + v[p], cand[p]);
   }
   uint32_t bk[4], q[4];
+  bool first[4];
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     imp[p] = cand[p] < cur[p] && ck[p] < (uint32_t)(old[p] >> 32);
+    first[p] = P && imp[p] && (uint32_t)(old[p] >> 32) == W::kInfKey;
     bk[p] = imp[p] ? map_bucket(W::bucket_value(ck[p]), base, tbase, inv, lo[p]) : 0u;
   }
 #pragma unroll
@@ -413,7 +451,7 @@ __device__ inline void relax_quad(const AddsArgs& a, PushStage& st, const uint32
   // keep the four claims in flight: the comparisons stay after the last issue
   asm volatile("" : "+v"(q[0]), "+v"(q[1]), "+v"(q[2]), "+v"(q[3]));
 #pragma unroll
-  for (int p = 0; p < 4; ++p) stage_wanted(a, st, q[p] > bk[p], v[p], bk[p], pushed);
+  for (int p = 0; p < 4; ++p) stage_wanted<P>(a, st, q[p] > bk[p], v[p], bk[p], pushed, first[p]);
 }
 
 // Publish a wave's staged heavy-list entries (all of one bucket, list slot
@@ -489,14 +527,26 @@ __global__ void k_adds_first_phase(AddsArgs a) {
   }
 }
 
-// Wave-level phase advance; true when this wave moved the base.
+// Wave-level phase advance; true when this wave moved the base.  P: pulls on
+// (the boundary decides push or pull, and the next advance waits for both).
+template <bool P>
 __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   AddsCtrl* c = a.c;
   const unsigned lane = lane_id();
   const uint32_t base = base_of(bw);
   const uint32_t s = base % kAddsSlots;
   bool ok = false;
+  // The boundary work of `base` (policy, pull registration) is published and
+  // the pull it registered, if any, is finished; the emptiness reads below
+  // follow these loads.
+  bool ready = false;
   if (lane == 0) {
+    ready = true;
+    if constexpr (P)
+      ready = ld_rlx(&c->decided) == base &&
+              (ld_rlx(&c->pull_base) != base || ld_rlx(&c->pull_done) >= ld_rlx(&c->pull_chunks));
+  }
+  if (ready) {
     // Only holders of a claim (or hub chunk) of bucket `base` push into it,
     // and a claimer raises inflight before its head CAS.  head == tail (slot
     // empty at the tail read), then inflight == 0 (no holder), then an
@@ -574,6 +624,45 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
     __hip_atomic_store(&c->delta, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&c->active, active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&c->active_workers, workers_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (P) {
+      // Heavy edges of the bucket just finished: pushed from its heavy list, or
+      // pulled by the vertices they could still improve when the estimated push
+      // work (the listed heavy edges) exceeds 2^margin x the estimated pull work
+      // (the edges of the vertices not reached yet, which are read whole, one
+      // label read per vertex, and about two row reads per queued vertex).  Its
+      // claims are all finished (the base moved), so the slot's statistics are
+      // final; they are reset for the bucket that reuses the slot.
+      const uint32_t fs = base % kAddsSlots;
+      const unsigned long long heavy_edges = ld_rlx(&HV_EDGES(c, fs));
+      const uint32_t kmin = ld_rlx(&HV_KEY(c, fs));
+      st_rlx(&HV_EDGES(c, fs), 0ull);
+      st_rlx(&HV_KEY(c, fs), kNoStamp);
+      bool pull = false;
+      if (ld_rlx(&c->pull_mode) != 0u && heavy_edges > 0 && kmin != kNoStamp) {
+        const unsigned long long reached = ld_rlx(&c->reached_edges);
+        const float unreached = a.m_edges > reached ? (float)(a.m_edges - reached) : 0.0f;
+        const float est = unreached + 0.125f * (float)a.n + 2.0f * (float)window_items;
+        pull = (float)heavy_edges > exp2f(ld_rlx(&c->pull_margin)) * est;
+      }
+      if (pull) {
+        // the pull holds the new phase open (pending, and pull_done before the
+        // next advance); its chunks are offered last
+        const uint32_t chunks = (a.n + kPullChunk - 1) / kPullChunk;
+        st_rlx(&c->pull_key, kmin);
+        st_rlx(&c->pull_chunks, chunks);
+        st_rlx(&c->pull_head, 0u);
+        st_rlx(&c->pull_done, 0u);
+        st_rlx(&c->pull_base, nb);
+        atomicAdd(&c->pending, (unsigned long long)chunks);
+        atomicAdd(&c->pulls, 1u);
+        vm_wait();
+        atomicAdd(&c->pull_avail, (int)chunks);
+      }
+      if (a.policy && a.policy->trace && ph < a.policy->trace_cap)
+        a.policy->trace[(size_t)ph * kTraceStride + 39] = pull ? 1.0f : 0.0f;
+      vm_wait();
+      st_rlx(&c->decided, nb);
+    }
     atomicAdd(&c->diag[14], (unsigned long long)(wall_clock64() - t_adv));
   }
   return true;
@@ -585,7 +674,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
 // take 128-edge steps of it in turn (two edges per lane), so one high-degree
 // vertex does not keep the other waves waiting at the next barrier.  The
 // arrays are kAB entries in LDS; s_wsum holds kAW totals.
-template <class W>
+template <class W, bool P>
 __device__ inline void relax_block(const AddsArgs& a, PushStage& st, int* s_incl, int* s_base,
                                    uint32_t* s_key, int* s_u, uint32_t* s_lo, int* s_wsum, int u,
                                    int beg, int deg, uint32_t key, uint32_t lo, uint32_t base,
@@ -646,12 +735,97 @@ __device__ inline void relax_block(const AddsArgs& a, PushStage& st, int* s_incl
       u[p] = (uint32_t)s_u[pos[p]];
       lo[p] = s_lo[pos[p]];
     }
-    relax_quad<W>(a, st, v, ck, u, lo, base, tbase, inv, pushed);
+    relax_quad<W, P>(a, st, v, ck, u, lo, base, tbase, inv, pushed);
   }
   __builtin_amdgcn_wave_barrier();
 }
 
-enum ClaimKind : uint32_t { kClaimItems = 0, kClaimHub = 1, kClaimHeavy = 2 };
+// One pass of a pull chunk: thread t holds vertex v (has), the label it read
+// (lab: a stale read is only larger, which widens the range below) and the
+// in-edge range [beg, beg + len) whose edges can still lower or tie its label.
+// The ranges are packed across the workgroup as in relax_block; every edge
+// u -> v reads u's label with a device-coherent load (the candidate is built
+// from it, so a stale cached copy could lose the improvement) and keeps the
+// smallest (candidate key, input id of u) per vertex in LDS.  Afterwards each
+// thread applies its vertex's best through the label atomicMin and queues an
+// improved vertex like any push (producer bucket: the base).
+template <class W, bool kDiag>
+__device__ inline void pull_pass(const AddsArgs& a, PushStage& st, int* s_incl, int* s_base, uint32_t* s_key,
+                                 unsigned long long* s_best, int* s_wsum, bool has, uint32_t v,
+                                 unsigned long long lab, int beg, int len, uint32_t base, float tbase,
+                                 float inv, unsigned long long& edges, unsigned long long& pushed) {
+  const unsigned lane = lane_id();
+  const unsigned wv = threadIdx.x / kWave;
+  int incl = len;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int t = __shfl_up(incl, off, kWave);
+    if ((int)lane >= off) incl += t;
+  }
+  if (lane == kWave - 1) s_wsum[wv] = incl;
+  __syncthreads();
+  int total = 0;
+#pragma unroll
+  for (int w = 0; w < kAW; ++w) {
+    const int ws = s_wsum[w];
+    if (w < (int)wv) incl += ws;
+    total += ws;
+  }
+  s_incl[threadIdx.x] = incl;
+  s_base[threadIdx.x] = beg - (incl - len);
+  s_key[threadIdx.x] = (uint32_t)(lab >> 32);
+  s_best[threadIdx.x] = lab;
+  __syncthreads();
+  if (a.live_edges && threadIdx.x == 0 && total > 0) atomicAdd(&a.c->edges, (unsigned long long)total);
+  if (wv == 0) edges += (unsigned long long)total;
+  if (kDiag && threadIdx.x == 0 && total > 0) atomicAdd(&a.c->pdiag[1], (unsigned long long)total);
+  for (int t = (int)wv * 4 * kWave; t < total; t += 4 * kAB) {
+    int pos[4];
+    bool ok[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      pos[p] = 0;
+      ok[p] = t + p * kWave + (int)lane < total;
+    }
+#pragma unroll
+    for (int s2 = kAB / 2; s2 > 0; s2 >>= 1) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        if (s_incl[pos[p] + s2 - 1] <= t + p * kWave + (int)lane) pos[p] += s2;
+    }
+    uint2 e[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) e[p] = a.tedges[ok[p] ? s_base[pos[p]] + t + p * kWave + (int)lane : 0];
+    unsigned long long lu[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      lu[p] = __hip_atomic_load(a.labels + (ok[p] ? e[p].x : 0u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const uint32_t ck = ok[p] ? W::relax((uint32_t)(lu[p] >> 32), e[p].y) : W::kInfKey;
+      if (ck != W::kInfKey && ck <= s_key[pos[p]]) {
+        const uint32_t pid = a.ids ? (uint32_t)a.ids[e[p].x] : e[p].x;
+        const unsigned long long cand = ((unsigned long long)ck << 32) | pid;
+        if (cand < s_best[pos[p]]) atomicMin(s_best + pos[p], cand);
+      }
+    }
+  }
+  __syncthreads();
+  const unsigned long long best = s_best[threadIdx.x];
+  bool imp = false, first = false;
+  uint32_t bk = 0;
+  if (has && best < lab) {
+    const unsigned long long old = atomicMin(a.labels + v, best);
+    const uint32_t bkey = (uint32_t)(best >> 32);
+    imp = bkey < (uint32_t)(old >> 32);
+    first = imp && (uint32_t)(old >> 32) == W::kInfKey;
+    if (imp) bk = map_bucket(W::bucket_value(bkey), base, tbase, inv, base);
+  }
+  stage_wanted<true>(a, st, claim_stamp(a, imp, v, bk), v, bk, pushed, first);
+  __syncthreads();  // the LDS arrays are refilled by the next pass
+}
+
+enum ClaimKind : uint32_t { kClaimItems = 0, kClaimHub = 1, kClaimHeavy = 2, kClaimPull = 3, kClaimHeavyDrop = 4 };
 
 // Cut the row range [beg, end) of u into hub chunks (pending work held on
 // inflight slot `slot`, pushes clamped to bucket >= lo).
@@ -709,7 +883,7 @@ __device__ inline uint32_t sem_claim(int* avail, uint32_t* head, int* infl, uint
     }                        \
   } while (0)
 
-template <class W, bool kDiag>
+template <class W, bool kDiag, bool kPull>
 __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k_adds(AddsArgs a) {
   __shared__ uint32_t s_slot, s_qslot, s_begin, s_count, s_kind, s_done, s_base, s_active, s_hub_ok;
   __shared__ float s_tbase, s_inv;
@@ -723,8 +897,15 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
   __shared__ uint32_t s_ps[kAW][kPushStage];
   __shared__ uint32_t s_ph[kAW][kAddsSlots];
   __shared__ int s_hv[kAW][kHeavyStage];
-  __shared__ int s_lq[kLocal];
+  // the local queue; a pull claim (never holding local entries) uses its
+  // first kAB x 8 bytes for the per-vertex best candidates
+  __shared__ __align__(16) int s_lq[kLocal];
   __shared__ uint32_t s_lfree, s_lhead, s_ltail, s_lnum, s_lbeg;
+  // a claim's heavy-list statistics (pulls on): heavy edges listed, smallest
+  // key of a processed vertex with heavy edges
+  __shared__ unsigned long long s_hve;
+  __shared__ uint32_t s_hvk;
+  static_assert(kLocal * sizeof(int) >= kAB * sizeof(unsigned long long), "pull candidates fit the local queue");
   AddsCtrl* c = a.c;
   const unsigned lane = threadIdx.x & 63;
   const unsigned wv = threadIdx.x >> 6;
@@ -742,6 +923,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
   unsigned long long claims = 0;
   unsigned long long d_cas = 0, d_idle = 0, d_spin = 0, d_hub = 0, d_drop = 0, d_req = 0, d_heavy = 0;
   unsigned long long t_claim = 0, t_idle = 0, t_items = 0, t_hubs = 0, t_pub = 0, t_relax = 0;
+  unsigned long long t_pull = 0, d_psearch = 0;
   // A workgroup beyond the policy's worker share is dormant: it sleeps and
   // reads only the end flag and the share, so it adds no traffic on the
   // control words that the claimers update (wave 0's copy is the one used).
@@ -773,6 +955,9 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
         else if (lane == 3) probe = (int)ld_rlx(&c->active);
         else if (lane >= 4 && lane < 4u + kAddsSlots) probe = ld_rlx(&AVAIL(c, (base0 + lane - 4) % kAddsSlots));
         else if (lane == 4u + kAddsSlots) probe = (int)ld_rlx(&c->active_workers);
+        else if (kPull && lane == 5u + kAddsSlots) probe = (int)ld_rlx(&c->decided);
+        else if (kPull && lane == 6u + kAddsSlots) probe = (int)ld_rlx(&c->pull_base);
+        else if (kPull && lane == 7u + kAddsSlots) probe = ld_rlx(&c->pull_avail);
         done = (uint32_t)__shfl(probe, 0, 64);
         const int hub_n = __shfl(probe, 1, 64), hv_n0 = __shfl(probe, 2, 64);
         const uint32_t A = max(1u, min((uint32_t)__shfl(probe, 3, 64), (uint32_t)kAddsSlots));
@@ -787,6 +972,12 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
         // includes them again or the run ends; the claimers advance the
         // phases and detect the end.
         const uint32_t share = (uint32_t)__shfl(probe, 4 + kAddsSlots, 64);
+        // the heavy list of bucket base0 - 1 is claimed once the boundary
+        // decided whether it is pushed or pulled (a read of an older word only
+        // delays the claim, or pushes edges a pull also covers)
+        const bool decided = !kPull || (uint32_t)__shfl(probe, 5 + kAddsSlots, 64) == base0;
+        const bool pulled = kPull && decided && (uint32_t)__shfl(probe, 6 + kAddsSlots, 64) == base0;
+        const int pull_n = kPull ? __shfl(probe, 7 + kAddsSlots, 64) : 0;
         const bool claimer = blockIdx.x < max(share, 1u);
         dormant = !claimer;
         if (lane == 0 && !done && claimer) {
@@ -804,18 +995,27 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
               DIAG(++d_cas);
             }
           }
-          if (!got && base0 > 0 && hv_n0 > 0) {
+          if (!got && base0 > 0 && hv_n0 > 0 && decided) {
             const uint32_t hs = (base0 - 1) % kAddsSlots;
             // pinned on slot base: bucket base cannot finish before the heavy
-            // edges of bucket base - 1 are relaxed
+            // edges of bucket base - 1 are relaxed (or, pulled, dropped)
             const uint32_t ps = base0 % kAddsSlots;
             got = sem_claim(&HV_AVAIL(c, hs), &HV_HEAD(c, hs), &INFL(c, ps), &begin);
             if (got) {
-              kind = kClaimHeavy;
+              kind = pulled ? kClaimHeavyDrop : kClaimHeavy;
               slot = ps;
               qslot = hs;
             } else {
               DIAG(++d_cas);
+            }
+          }
+          if (kPull && !got && pulled && pull_n > 0) {
+            const uint32_t ps = base0 % kAddsSlots;
+            got = sem_claim(&c->pull_avail, &c->pull_head, &INFL(c, ps), &begin, 1);
+            if (got) {
+              kind = kClaimPull;
+              slot = ps;
+              qslot = ps;
             }
           }
           for (unsigned long long m = ready; m && !got; m &= m - 1) {
@@ -850,7 +1050,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
             done = 1;
           } else {
             const unsigned long long bw = __shfl(lane == 0 ? ld_rlx(&c->base_word) : 0ull, 0, 64);
-            if (try_advance(a, bw)) {
+            if (try_advance<kPull>(a, bw)) {
               idle = 0;
               backoff = 1;
             } else {
@@ -873,6 +1073,8 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       if (lane == 0) {
         s_done = done;
         s_count = got;
+        s_hve = 0;
+        s_hvk = kNoStamp;
         s_slot = slot;
         s_qslot = qslot;
         s_begin = begin;
@@ -931,9 +1133,9 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
           uu[p] = hu;
           lo[p] = hbucket;
         }
-        relax_quad<W>(a, st, v, ck, uu, lo, base, tbase, inv, pushed_local);
+        relax_quad<W, kPull>(a, st, v, ck, uu, lo, base, tbase, inv, pushed_local);
       }
-      stage_publish(a, st);
+      stage_publish<kPull>(a, st);
       if (a.live_edges && wv == 0 && lane == 0) atomicAdd(&c->edges, (unsigned long long)hcnt);
       edges_local += hcnt;
       DIAG(d_hub += threadIdx.x == 0);
@@ -946,9 +1148,65 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       continue;
     }
 
+    if (kPull && kind == kClaimPull) {
+      // One chunk of the pull of bucket base - 1.  Vertex v reads the in-edges
+      // u -> v of its weight-sorted row with pull_key (+) w <= dist(v) (every
+      // vertex of the pulled bucket has a key >= pull_key, and the rows are
+      // ordered by weight, so the edges past that prefix can neither lower
+      // nor tie v's label): all of them while v is unreached, none when
+      // dist(v) < pull_key.  The prefix ends are found by binary search.
+      const uint32_t pkey = ld_rlx(&c->pull_key);
+      const uint32_t q = s_begin;
+      unsigned long long* s_best = reinterpret_cast<unsigned long long*>(s_lq);
+      if (q < ld_rlx(&c->pull_chunks)) {
+        const uint32_t v0 = q * kPullChunk, v1 = min(a.n, v0 + kPullChunk);
+        for (uint32_t p0 = v0; p0 < v1; p0 += kAB) {
+          const uint32_t v = p0 + threadIdx.x;
+          const bool has = v < v1;
+          const unsigned long long lab = has ? a.labels[v] : ~0ull;
+          const uint32_t key_v = (uint32_t)(lab >> 32);
+          int beg = 0, len = 0;
+          if (has && key_v >= pkey) {
+            const int rb = a.trow[v] + a.tsplit[v], re = a.trow[v + 1];
+            int lo = rb;
+            if (key_v == W::kInfKey) {
+              lo = re;
+            } else if (rb < re && W::relax(pkey, a.tedges[rb].y) <= key_v) {
+              // the first heavy in-edge can reach: search for the end
+              int hi = re;
+              ++lo;
+              while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (W::relax(pkey, a.tedges[mid].y) <= key_v) lo = mid + 1;
+                else hi = mid;
+              }
+            }
+            beg = rb;
+            len = lo - rb;
+            DIAG(++d_psearch);
+          }
+          pull_pass<W, kDiag>(a, st, &s_incl[0][0], &s_beg[0][0], &s_key[0][0], s_best, s_wsum, has, v, lab, beg,
+                       len, base, tbase, inv, edges_local, pushed_local);
+        }
+      }
+      stage_publish<true>(a, st);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        // the chunk's pushes have completed (each wave waited in its publish)
+        atomicAdd(&c->pull_done, 1u);
+        atomic_add_done(&INFL(c, s_slot), -1);
+        atomicAdd(&c->pending, ~0ull);
+      }
+      DIAG(t_pull += wall_clock64() - t_claimed);
+      continue;
+    }
+
     // items of a bucket ring or a heavy list, in passes of kAB: in pass p
     // wave w takes claim entries [p*kAB + w*64, p*kAB + w*64 + 64)
-    const bool heavy = kind == kClaimHeavy;
+    const bool heavy = kind == kClaimHeavy || kind == kClaimHeavyDrop;
+    // the bucket of a dropped heavy list was pulled: its entries are consumed
+    // and their list stamps cleared, their edges not relaxed
+    const bool drop = kPull && kind == kClaimHeavyDrop;
     const uint32_t slot = s_slot, begin = s_begin;
     int* ring = (heavy ? a.hring : a.ring) + (size_t)s_qslot * (a.cap_mask + 1u);
     // improved vertices of the held bucket stay in this workgroup (item
@@ -1010,6 +1268,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       // row bounds and light-prefix length of u depend only on u: issued now,
       // they arrive while the stamp exchange and the label read complete
       int r_beg = 0, r_end = 0, r_split = 0, pu = u;
+      uint32_t hv_k = kNoStamp, hv_e = 0;  // pull statistics of this item
       if (has && entry_ok) {
         r_beg = a.rowptr[u];
         r_end = a.rowptr[u + 1];
@@ -1021,10 +1280,12 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
           // Clearing the stamp first means a later light pass of u (after an
           // improvement) lists u again, or this pass reads the label it set.
           __hip_atomic_exchange(a.sstamp + u, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          vm_wait();  // the label is read after the stamp is cleared
-          key = atomic_load_key(a.labels + u);
-          bk = base;
-          DIAG(++d_heavy);
+          if (!drop) {
+            vm_wait();  // the label is read after the stamp is cleared
+            key = atomic_load_key(a.labels + u);
+            bk = base;
+            DIAG(++d_heavy);
+          }
         } else {
           // Whoever takes the stamp processes the vertex.  (Putting back a
           // stamp that belongs to an entry in another ring would race with that
@@ -1055,9 +1316,12 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
             beg = split;
           } else {
             // light prefix now; the heavy suffix once bucket bk is finished
-            if (split < end)
+            if (split < end) {
               to_heavy = __hip_atomic_exchange(a.sstamp + u, bk + 1u, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT) != bk + 1u;
+              hv_k = key;
+              if (to_heavy) hv_e = (uint32_t)(end - split);
+            }
             end = split;
           }
           deg = end - beg;
@@ -1077,20 +1341,39 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
           hv_n += k;
         }
       }
+      if (kPull && __ballot(hv_k != kNoStamp)) {
+        unsigned long long e = hv_e;
+        uint32_t k = hv_k;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          e += __shfl_xor(e, off, 64);
+          k = min(k, (uint32_t)__shfl_xor(k, off, 64));
+        }
+        if (lane == 0) {
+          atomicAdd(&s_hve, e);
+          atomicMin(&s_hvk, k);
+        }
+      }
       // re-queued items go through the stage like any push
-      push_improved(a, st, requeue, (uint32_t)u, want, pushed_local);
+      push_improved<kPull>(a, st, requeue, (uint32_t)u, want, pushed_local);
       long long t_r0 = 0, t_r1 = 0;
       DIAG(t_r0 = wall_clock64());
-      relax_block<W>(a, st, &s_incl[0][0], &s_beg[0][0], &s_key[0][0], &s_u[0][0], &s_lo[0][0], s_wsum,
+      relax_block<W, kPull>(a, st, &s_incl[0][0], &s_beg[0][0], &s_key[0][0], &s_u[0][0], &s_lo[0][0], s_wsum,
                      pu, beg, deg, key, bk, base, tbase, inv, edges_local, pushed_local);
       DIAG(t_r1 = wall_clock64());
-      stage_publish(a, st);
+      stage_publish<kPull>(a, st);
       heavy_publish(a, s_hv[wv], hv_n, slot);
       DIAG(t_relax += t_r1 - t_r0; t_pub += wall_clock64() - t_r1);
     }
     st.lq.slot = kNoLocal;
     __syncthreads();
     if (threadIdx.x == 0) {
+      // the statistics land before the claim's release, so the wave that
+      // finishes the bucket reads them whole
+      if (kPull && s_hvk != kNoStamp) {
+        atomicAdd(&HV_EDGES(c, slot), s_hve);
+        atomicMin(&HV_KEY(c, slot), s_hvk);
+      }
       atomic_add_done(&INFL(c, slot), -1);
       atomicAdd(&c->pending, 0ull - (unsigned long long)count);
     }
@@ -1113,6 +1396,8 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (d_drop) atomicAdd(&c->diag[5], d_drop);
     if (d_req) atomicAdd(&c->diag[6], d_req);
     if (d_heavy) atomicAdd(&c->diag[7], d_heavy);
+    if (threadIdx.x == 0 && t_pull) atomicAdd(&c->pdiag[0], t_pull);
+    if (d_psearch) atomicAdd(&c->pdiag[2], d_psearch);
   }
   if (lane == 0) {
     if (!a.live_edges && edges_local) atomicAdd(&c->edges, edges_local);
@@ -1178,6 +1463,13 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   h.util_high = 0.85f;
   h.growth = 1.5f;
   h.shrink = 0.75f;
+  if (req.pull != 0 && req.pull != 1) fail(MI355_EINVAL, "pull must be 0 (push) or 1 (pull when cheaper)");
+  if (!std::isfinite(req.pull_margin)) fail(MI355_EINVAL, "pull_margin must be finite");
+  h.decided = 0;
+  h.pull_mode = (uint32_t)req.pull;
+  h.pull_margin = req.pull_margin;
+  h.pull_base = kNoStamp;
+  for (int i = 0; i < kAddsSlots; ++i) h.hv_key[i * kPad] = kNoStamp;
   h.active = (uint32_t)std::min(std::max(req.active_buckets, 1), kAddsSlots);
   h.workers = (uint32_t)workers;
   h.active_workers = (uint32_t)workers;
@@ -1200,11 +1492,17 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   // light/heavy split at the starting bucket width (MLP mode: at the
   // network's first width, set below; one cached split per graph either way)
   a.split = req.use_mlp ? nullptr : light_split(g, delta, s);
+  float split_delta = delta;
   a.hub = ws.adds_hub.p;
   a.hub_mask = hub_cap - 1;
   a.c = c;
   a.policy = d_policy;
   a.ids = input_ids(g);
+  a.trow = nullptr;
+  a.tedges = nullptr;
+  a.tsplit = nullptr;
+  if (req.pull) pull_rows(g, &a.trow, &a.tedges, s);
+  a.m_edges = (unsigned long long)g.m;
   a.n = (uint32_t)n;
   a.live_edges = req.trace ? 1 : 0;
   a.claim_limit = 16ull * (unsigned long long)(n + g.m) + (1ull << 20);
@@ -1235,11 +1533,20 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
     float first_delta = delta;
     MI_HIP(hipMemcpyAsync(&first_delta, &c->delta, sizeof(float), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
-    a.split = light_split(g, first_delta > 0.0f && std::isfinite(first_delta) ? first_delta : delta, s);
+    split_delta = first_delta > 0.0f && std::isfinite(first_delta) ? first_delta : delta;
+    a.split = light_split(g, split_delta, s);
   }
+  // the pull reads the heavy in-edges at the same split width
+  if (a.trow) a.tsplit = pull_light_split(g, split_delta, s);
   MI_HIP(hipEventRecord(evk, s));
-  if (std::getenv("MI355_ADDS_DEBUG")) k_adds<W, true><<<workers, kAB, 0, s>>>(a);
-  else k_adds<W, false><<<workers, kAB, 0, s>>>(a);
+  const bool dbg = std::getenv("MI355_ADDS_DEBUG") != nullptr;
+  if (a.trow) {
+    if (dbg) k_adds<W, true, true><<<workers, kAB, 0, s>>>(a);
+    else k_adds<W, false, true><<<workers, kAB, 0, s>>>(a);
+  } else {
+    if (dbg) k_adds<W, true, false><<<workers, kAB, 0, s>>>(a);
+    else k_adds<W, false, false><<<workers, kAB, 0, s>>>(a);
+  }
   MI_LAUNCH_CHECK();
   MI_HIP(hipEventRecord(ev1, s));
   MI_HIP(hipEventSynchronize(ev1));
@@ -1253,14 +1560,15 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   if (h.err) ws.adds_ring_clean = false;
   if (std::getenv("MI355_ADDS_DEBUG"))
     std::fprintf(stderr,
-                 "adds: %.3f ms phases %u items %llu edges %llu pushes %llu | claims %llu cas_fail %llu "
+                 "adds: %.3f ms phases %u pulls %u items %llu edges %llu pushes %llu | claims %llu cas_fail %llu "
                  "idle %llu ring_wait %llu hub %llu dropped %llu requeued %llu heavy %llu\n"
-                 "  per-workgroup ms: claim %.2f idle %.2f items %.2f hubs %.2f (publish %.2f relax %.2f)"
-                 " | phase advances %.3f ms in total\n",
-                 res.device_ms, h.phases, h.items, h.edges, h.pushes, h.diag[0], h.diag[1], h.diag[2],
+                 "  per-workgroup ms: claim %.2f idle %.2f items %.2f hubs %.2f pulls %.2f (publish %.2f relax %.2f)"
+                 " | phase advances %.3f ms in total | pulled in-edges %llu searched vertices %llu\n",
+                 res.device_ms, h.phases, h.pulls, h.items, h.edges, h.pushes, h.diag[0], h.diag[1], h.diag[2],
                  h.diag[3], h.diag[4], h.diag[5], h.diag[6], h.diag[7], h.diag[8] * 1e-5 / workers,
                  h.diag[9] * 1e-5 / workers, h.diag[10] * 1e-5 / workers, h.diag[11] * 1e-5 / workers,
-                 h.diag[12] * 1e-5 / workers, h.diag[13] * 1e-5 / workers, h.diag[14] * 1e-5);
+                 h.pdiag[0] * 1e-5 / workers, h.diag[12] * 1e-5 / workers, h.diag[13] * 1e-5 / workers,
+                 h.diag[14] * 1e-5, h.pdiag[1], h.pdiag[2]);
   if (h.err & kErrRing)
     fail(MI355_ECAPACITY, a.ring_limit < cap ? "ADDS bucket overflow: a bucket exceeded the requested per-bucket capacity"
                                              : "ADDS bucket queue overflow");
@@ -1274,6 +1582,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   res.vertices = (int64_t)h.items;
   res.initial_delta = delta;
   res.final_delta = h.delta;
+  res.pulls = h.pulls;
 }
 
 }  // namespace

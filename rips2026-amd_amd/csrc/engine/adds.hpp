@@ -51,6 +51,36 @@ struct AddsCtrl {
   // polling, item claims, hub chunks, publishing, relaxing
   unsigned long long diag[16];
   uint32_t pad7[kPad - 32];
+  // pull diagnostics (debug runs): workgroup-0-wave ticks in pull chunks,
+  // in-edges read, vertices whose in-edge range was searched
+  unsigned long long pdiag[4];
+  uint32_t pad7b[kPad - 8];
+  // Pulls.  At a phase boundary the heavy edges of the bucket just finished
+  // are either pushed from its vertices (its heavy list) or pulled by every
+  // vertex that an edge from it could still improve; the pull runs as
+  // kPullChunk-vertex chunks while base == pull_base.  `decided` is the last
+  // base whose boundary work (policy, pull registration) is published: the
+  // base does not move past it before, and the heavy list is not claimed.
+  uint32_t decided;
+  uint32_t pull_mode;            // 0: always push; 1: pull when estimated cheaper
+  float pull_margin;             // log2(push / pull work estimate) above which to pull
+  uint32_t pull_base;            // base during which the registered pull runs (kNoStamp: none)
+  uint32_t pull_chunks;          // chunks of the registered pull
+  uint32_t pull_key;             // smallest key a vertex of the pulled bucket was processed with
+  uint32_t pulls;                // buckets pulled
+  uint32_t pad8[kPad - 7];
+  int pull_avail;                // unclaimed chunks of the registered pull (may dip below 0)
+  uint32_t pad9[kPad - 1];
+  uint32_t pull_head;            // next chunk index
+  uint32_t pad10[kPad - 1];
+  uint32_t pull_done;            // finished chunks
+  uint32_t pad11[kPad - 1];
+  unsigned long long reached_edges;  // out-degree sum of the vertices reached so far
+  uint32_t pad12[kPad - 2];
+  // per bucket slot: heavy edges listed, smallest key of a processed vertex
+  // with heavy edges (read and reset when the bucket finishes)
+  unsigned long long hv_edges[kAddsSlots * (kPad / 2)];
+  uint32_t hv_key[kAddsSlots * kPad];
   uint32_t head[kAddsSlots * kPad];  // pop counter of slot s at [s * kPad] (monotone)
   uint32_t tail[kAddsSlots * kPad];  // push counter of slot s
   int inflight[kAddsSlots * kPad];   // claims of slot s still being processed

@@ -203,6 +203,8 @@ static mi355::RunRequest make_request(const mi355_sssp_options* opt, bool want_w
     r.use_mlp = opt->use_mlp;
     r.worker_blocks = opt->worker_blocks;
     r.active_buckets = opt->active_buckets;
+    r.pull = opt->pull;
+    r.pull_margin = opt->pull_margin;
   } else {
     r.algorithm = MI355_ALG_DELTA;
   }
@@ -224,6 +226,7 @@ static void fill_stats(const mi355::RunResult& res, mi355_sssp_stats* st) {
   st->kernel_ms = res.kernel_ms;
   st->policy = res.policy;
   st->has_negative_cycle = res.negative_cycle ? 1 : 0;
+  st->pulled_phases = (int32_t)std::min<int64_t>(res.pulls, INT32_MAX);
 }
 
 int mi355_sssp(mi355_graph* g, int32_t source, const mi355_sssp_options* opt, void* d_dist,

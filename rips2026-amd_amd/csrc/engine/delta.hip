@@ -510,21 +510,38 @@ void run_ds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
   res.initial_delta = res.final_delta = delta;
 }
 
+namespace {
+void light_count(const int* rowptr, const uint2* edges, int64_t n, int32_t wtype, uint32_t bits, int* out,
+                 hipStream_t s) {
+  const unsigned grid = std::min<unsigned>(div_up((uint64_t)n, kBlock), 8192u);
+  if (wtype == MI355_W_F32) k_light_count<WF32><<<grid, kBlock, 0, s>>>(rowptr, edges, n, bits, out);
+  else k_light_count<WU32><<<grid, kBlock, 0, s>>>(rowptr, edges, n, bits, out);
+  MI_LAUNCH_CHECK();
+}
+}  // namespace
+
 const int* light_split(mi355_graph& g, float delta, hipStream_t s) {
   Workspace& ws = g.ws;
   const uint32_t bits = __builtin_bit_cast(uint32_t, delta);
   if (!ws.split_valid || ws.split_delta_bits != bits) {
     ws.split.ensure((size_t)g.n);
-    const unsigned grid = std::min<unsigned>(div_up((uint64_t)g.n, kBlock), 8192u);
-    if (g.weight_type == MI355_W_F32)
-      k_light_count<WF32><<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, g.n, bits, ws.split.p);
-    else
-      k_light_count<WU32><<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, g.n, bits, ws.split.p);
-    MI_LAUNCH_CHECK();
+    light_count(g.rowptr.p, g.edges.p, g.n, g.weight_type, bits, ws.split.p, s);
     ws.split_valid = true;
     ws.split_delta_bits = bits;
   }
   return ws.split.p;
+}
+
+const int* pull_light_split(mi355_graph& g, float delta, hipStream_t s) {
+  if (g.pull_state != 2) return light_split(g, delta, s);
+  const uint32_t bits = __builtin_bit_cast(uint32_t, delta);
+  if (!g.t_split_valid || g.t_split_bits != bits) {
+    g.t_split.ensure((size_t)g.n);
+    light_count(g.t_rowptr.p, g.t_edges.p, g.n, g.weight_type, bits, g.t_split.p, s);
+    g.t_split_valid = true;
+    g.t_split_bits = bits;
+  }
+  return g.t_split.p;
 }
 
 void run_delta_stepping(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,

@@ -96,6 +96,17 @@ struct mi355_graph {
   // graph features of the learned delta policy (computed on first use)
   bool graph_feat_ready = false;
   float graph_feat[14] = {};
+  // In-edge rows for ADDS pulls, built on first use: row v lists {u, w} for
+  // every edge u -> v, weight-sorted.  pull_state 1: the graph is symmetric
+  // (every row holds the same (neighbour, weight) multiset in both
+  // orientations) and its own rows serve; 2: t_rowptr / t_edges hold the
+  // transpose.
+  int pull_state = 0;
+  mi355::DevBuf<int> t_rowptr;
+  mi355::DevBuf<uint2> t_edges;
+  mi355::DevBuf<int> t_split;  // light in-edges per row of the transpose (for t_split_bits)
+  uint32_t t_split_bits = 0;
+  bool t_split_valid = false;
 };
 
 namespace mi355 {
@@ -120,4 +131,7 @@ inline const int* input_ids(const mi355_graph& g) { return g.relabeled ? g.perm.
 // Row offsets and edge records in the input numbering (the graph's own
 // arrays, or a copy built on first use when the graph is renumbered).
 void input_order_arrays(mi355_graph& g, const int** rowptr, const uint2** edges, hipStream_t stream);
+// In-edge rows of the internal numbering, weight-sorted (the graph's own rows
+// when it is symmetric; a transpose built on first use otherwise).
+void pull_rows(mi355_graph& g, const int** rowptr, const uint2** edges, hipStream_t stream);
 }  // namespace mi355

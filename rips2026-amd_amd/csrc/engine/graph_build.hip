@@ -187,6 +187,76 @@ void sort_rows_by_weight(mi355_graph& g, hipStream_t stream) {
   MI_HIP(hipStreamSynchronize(stream));
 }
 
+// Sort key of an edge u -> v for the transpose: target row in the high word,
+// order-preserving weight key in the low word; the value is the source row.
+__global__ void k_col_weight_keys(const uint2* __restrict__ edges, int32_t wtype,
+                                  unsigned long long* __restrict__ keys, int64_t m) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride) {
+    const uint2 ed = edges[e];
+    const uint32_t wk = wtype == MI355_W_F32 ? f32_to_key(__uint_as_float(ed.y)) : ed.y;
+    keys[e] = ((unsigned long long)ed.x << 32) | wk;
+  }
+}
+
+__global__ void k_key_rows(const unsigned long long* __restrict__ keys, uint32_t* __restrict__ rows,
+                           int64_t m) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride)
+    rows[e] = (uint32_t)(keys[e] >> 32);
+}
+
+// flag = 1 when the two CSRs differ anywhere (offsets or records, bit for bit).
+__global__ void k_csr_differ(const int* __restrict__ rp_a, const uint2* __restrict__ e_a,
+                             const int* __restrict__ rp_b, const uint2* __restrict__ e_b, int64_t n,
+                             int64_t m, uint32_t* __restrict__ flag) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  bool differ = false;
+  const int64_t total = n + 1 > m ? n + 1 : m;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    if (i <= n && rp_a[i] != rp_b[i]) differ = true;
+    if (i < m) {
+      const uint2 a = e_a[i], b = e_b[i];
+      if (a.x != b.x || a.y != b.y) differ = true;
+    }
+  }
+  if (differ) atomicOr(flag, 1u);
+}
+
+// Transpose of a CSR (n x n, m records): row v of the result lists {u, w} for
+// every record u -> v, ordered by (weight, u) (a stable radix sort over a
+// source-ordered input).
+void transpose_sorted(const int* rowptr, const uint2* edges, int64_t n, int64_t m, int32_t wtype,
+                      DevBuf<int>& out_rp, DevBuf<uint2>& out_e, hipStream_t stream) {
+  out_rp.reset((size_t)n + 1);
+  out_e.reset((size_t)std::max<int64_t>(m, 1));
+  if (m == 0) {
+    MI_HIP(hipMemsetAsync(out_rp.p, 0, (size_t)(n + 1) * 4, stream));
+    return;
+  }
+  DevBuf<uint32_t> rowid((size_t)m), vals_out((size_t)m);
+  DevBuf<unsigned long long> keys_in((size_t)m), keys_out((size_t)m);
+  k_rowid_search<<<grid_for(m), kBlock, 0, stream>>>(rowptr, rowid.p, n, m);
+  k_col_weight_keys<<<grid_for(m), kBlock, 0, stream>>>(edges, wtype, keys_in.p, m);
+  MI_LAUNCH_CHECK();
+  int bits = 1;
+  while (bits < 32 && ((int64_t)1 << bits) < n) ++bits;
+  size_t temp_bytes = 0;
+  MI_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys_in.p, keys_out.p, rowid.p, vals_out.p,
+                                   (size_t)m, 0, 32 + bits, stream));
+  {
+    DevBuf<unsigned char> temp(temp_bytes);
+    MI_HIP(rocprim::radix_sort_pairs(temp.p, temp_bytes, keys_in.p, keys_out.p, rowid.p, vals_out.p,
+                                     (size_t)m, 0, 32 + bits, stream));
+    MI_HIP(hipStreamSynchronize(stream));
+  }
+  k_key_rows<<<grid_for(m), kBlock, 0, stream>>>(keys_out.p, rowid.p, m);
+  k_offsets_from_sorted<<<grid_for(m + 1), kBlock, 0, stream>>>(rowid.p, out_rp.p, m, n);
+  k_row_weight_gather<<<grid_for(m), kBlock, 0, stream>>>(keys_out.p, vals_out.p, wtype, out_e.p, m);
+  MI_LAUNCH_CHECK();
+  MI_HIP(hipStreamSynchronize(stream));
+}
+
 // Per-block partial statistics: [0] sum of positive finite weights (double),
 // [1] their count, [2] max degree, [3] large vertices, [4] large chunks.
 __global__ void k_stats(const int* __restrict__ rowptr, const uint2* __restrict__ edges, int64_t n,
@@ -345,6 +415,37 @@ void input_order_arrays(mi355_graph& g, const int** rowptr, const uint2** edges,
   }
   *rowptr = g.in_rowptr.p;
   *edges = g.in_edges.p;
+}
+
+// The transpose T of the graph, ordered by (row, weight, column), and T's
+// transpose, which is the graph itself in that order: the graph is symmetric
+// exactly when the two are equal, and then its own rows are its in-edge rows
+// (a pull reads a row by weight only, so the order of equal weights in it does
+// not matter).  Otherwise T is kept.
+void pull_rows(mi355_graph& g, const int** rowptr, const uint2** edges, hipStream_t stream) {
+  if (g.pull_state == 0) {
+    DevBuf<int> t_rp, tt_rp;
+    DevBuf<uint2> t_e, tt_e;
+    transpose_sorted(g.rowptr.p, g.edges.p, g.n, g.m, g.weight_type, t_rp, t_e, stream);
+    transpose_sorted(t_rp.p, t_e.p, g.n, g.m, g.weight_type, tt_rp, tt_e, stream);
+    DevBuf<uint32_t> flag(1);
+    MI_HIP(hipMemsetAsync(flag.p, 0, 4, stream));
+    k_csr_differ<<<grid_for(std::max(g.n + 1, g.m)), kBlock, 0, stream>>>(t_rp.p, t_e.p, tt_rp.p, tt_e.p,
+                                                                          g.n, g.m, flag.p);
+    MI_LAUNCH_CHECK();
+    uint32_t differ = 0;
+    MI_HIP(hipMemcpyAsync(&differ, flag.p, 4, hipMemcpyDeviceToHost, stream));
+    MI_HIP(hipStreamSynchronize(stream));
+    if (differ) {
+      g.t_rowptr = std::move(t_rp);
+      g.t_edges = std::move(t_e);
+      g.pull_state = 2;
+    } else {
+      g.pull_state = 1;
+    }
+  }
+  *rowptr = g.pull_state == 2 ? g.t_rowptr.p : g.rowptr.p;
+  *edges = g.pull_state == 2 ? g.t_edges.p : g.edges.p;
 }
 
 mi355_graph* build_graph(const mi355_csr_desc& d, int allow_negative, hipStream_t stream,

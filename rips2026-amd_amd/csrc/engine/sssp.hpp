@@ -25,6 +25,10 @@ struct RunRequest {
   int64_t bucket_capacity = 0;  // ADDS: most live entries per bucket (0: the ring size)
   float* trace = nullptr;  // ADDS phase trace (device, trace_cap x kTraceStride)
   uint32_t trace_cap = 0;
+  // ADDS: 0 push every finished bucket's heavy edges; 1 pull them instead when
+  // the push work estimate exceeds 2^pull_margin x the pull work estimate
+  int pull = 0;
+  float pull_margin = 0.0f;
 };
 
 struct RunResult {
@@ -39,6 +43,7 @@ struct RunResult {
   float device_ms = 0.f;
   float kernel_ms = 0.f;   // the persistent kernel alone (ADDS)
   int policy = 0;          // ADDS: PolicyMode used
+  int64_t pulls = 0;       // ADDS: finished buckets whose heavy edges were pulled
   bool negative_cycle = false;
   std::vector<uint32_t> frontier_log;  // BF: |F_r| for r = 0..rounds
 };
@@ -65,6 +70,8 @@ void device_gather8(void* dst, const void* src, const uint32_t* idx, int64_t cou
 // Light edges (w <= delta) of every vertex, a prefix of its weight-sorted row;
 // cached on the graph for the last delta asked for.
 const int* light_split(mi355_graph& g, float delta, hipStream_t stream);
+// The same for the in-edge rows of pulls (pull_rows must have run).
+const int* pull_light_split(mi355_graph& g, float delta, hipStream_t stream);
 void run_delta_stepping(mi355_graph& g, uint32_t source, const RunRequest& req, RunResult& res,
                         hipStream_t stream);
 

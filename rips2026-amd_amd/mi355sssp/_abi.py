@@ -46,7 +46,8 @@ class SsspOptions(C.Structure):
                 ("check_interval", C.c_int32), ("delta", C.c_float), ("inf_mode", C.c_int32),
                 ("dynamic_delta", C.c_int32), ("use_mlp", C.c_int32),
                 ("worker_blocks", C.c_int32), ("rounds_per_sync", C.c_int32),
-                ("active_buckets", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("active_buckets", C.c_int32), ("pull", C.c_int32), ("pull_margin", C.c_float),
+                ("reserved", C.c_int32 * 4)]
 
 
 class SsspStats(C.Structure):
@@ -55,7 +56,8 @@ class SsspStats(C.Structure):
                 ("vertices_processed", C.c_int64), ("reached", C.c_int64),
                 ("reached_edges", C.c_int64), ("initial_delta", C.c_float),
                 ("final_delta", C.c_float), ("device_ms", C.c_float), ("kernel_ms", C.c_float),
-                ("has_negative_cycle", C.c_int32), ("policy", C.c_int32)]
+                ("has_negative_cycle", C.c_int32), ("policy", C.c_int32),
+                ("pulled_phases", C.c_int32), ("reserved", C.c_int32)]
 
 
 class MlpParams(C.Structure):
