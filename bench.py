@@ -155,18 +155,20 @@ def routing_workloads(args, cpu_threads: int):
     plain_o = dg.options(M.ALG_ADDS)
     model = L.load_default_delta_model()
     res = {}
-    for name, o in (("plain_adds", plain_o),) + ((("learned_adds", dg.options(M.ALG_ADDS, use_mlp=1)),)
-                                                  if model else ()):
+    learned_o = dg.options(M.ALG_ADDS, use_mlp=1, pull=args.pull)
+    for name, o in (("plain_adds", plain_o),) + ((("learned_adds", learned_o),) if model else ()):
         dg.sssp_labels(int(srcs[0]), o)
         ms, mr = [], []
         for s_ in srcs:
             st = dg.sssp_labels(int(s_), o)
             ms.append(st.kernel_ms)
         res[name] = {"ms_per_sssp": float(np.mean(ms)), "phases_last": int(st.iterations_used),
+                     "pulled_phases_last": int(st.pulled_phases),
                      # 2: the network ran; 0: fixed delta (also when the grid lies
                      # outside the model's training range)
                      "policy_used": int(st.policy)}
-    d, _, st = dg.sssp(int(srcs[0]), M.ALG_ADDS, use_mlp=int(model is not None))
+    d, _, st = dg.sssp(int(srcs[0]), M.ALG_ADDS, use_mlp=int(model is not None),
+                       pull=args.pull if model is not None else 0)
     if args.check > 0:
         ok = d.tobytes() == orc.dijkstra(g.rowptr, g.col, g.weights, int(srcs[0])).tobytes()
         log(f"routing grid source {srcs[0]}: dist bit-exact={ok}")
@@ -372,6 +374,13 @@ def main() -> int:
                          "sinks are settled (0: off)")
     ap.add_argument("--routing-batch", type=int, default=1,
                     help="0: the routing-grid legs stop after the single-source one")
+    ap.add_argument("--pull", type=int, default=1,
+                    help="the learned policy's runs pull a finished bucket's heavy edges when the "
+                         "engine estimates pulling cheaper (1) or always push (0); plain ADDS, the "
+                         "reference's algorithm, always pushes")
+    ap.add_argument("--ablation", type=int, default=1,
+                    help="also time the learned policy without pulls and plain ADDS with pulls on "
+                         "the timed sources")
     ap.add_argument("--learned", type=int, default=1,
                     help="0: --policy auto falls back to plain ADDS even when a model is shipped")
     ap.add_argument("--routing", type=int, default=1,
@@ -445,7 +454,8 @@ def main() -> int:
         pol = policy if pol is None else pol
         kw = {}
         if name == "adds":
-            kw = {"use_mlp": 1} if pol == "learned" else {"dynamic_delta": int(pol == "feedback")}
+            kw = ({"use_mlp": 1, "pull": args.pull} if pol == "learned"
+                  else {"dynamic_delta": int(pol == "feedback")})
         return dg.options(algorithm_id(name), delta=args.delta, **kw), kw
 
     alg = algorithm_id(args.algorithm)
@@ -459,6 +469,7 @@ def main() -> int:
         mr[i], nr[i] = st.reached_edges, st.reached
         if i == 0:
             policy_used = int(st.policy)
+            pulled_first = int(st.pulled_phases)
             log(f"source {s}: iterations {st.iterations_used} rounds {st.rounds} "
                 f"delta {st.final_delta:.4g} reached {st.reached} m_r {st.reached_edges}")
 
@@ -535,6 +546,19 @@ def main() -> int:
             plain["all_sources_speedup_of_timed_policy"] = aw * 1e3 / len(sources) / all_sources["ms_per_sssp"]
         log(f"plain adds: {json.dumps(plain)}")
 
+    # what each half of the learned configuration gives alone: the network
+    # without pulls, and plain ADDS (heuristic width) with the pull rule
+    ablation = None
+    if args.algorithm == "adds" and policy == "learned" and args.ablation and args.pull:
+        ablation = {}
+        for name, kwa in (("learned_push_only", {"use_mlp": 1, "pull": 0}),
+                          ("plain_with_pulls", {"pull": 1})):
+            ao = dg.options(M.ALG_ADDS, delta=args.delta, **kwa)
+            aw, _, ak = timed(ao, idx, 2)
+            ablation[name] = {"ms_per_sssp": aw * 1e3 / len(idx), "kernel_ms_per_sssp": ak / len(idx),
+                              "timed_policy_speedup_over_it": aw / wall}
+        log(f"ablation: {json.dumps(ablation)}")
+
     others = {}
     if args.compare_steps > 0:
         cidx = step_sources(args.compare_steps)
@@ -587,7 +611,9 @@ def main() -> int:
         "data": "synthetic (seeded R-MAT generator)",
         "config": {"workload": f"rmat_s{args.scale}_ef{args.edgefactor}_sym_f32_sssp",
                    "algorithm": args.algorithm,
-                   "policy": {"learned": "learned delta network (shipped model)",
+                   "policy": {"learned": "learned delta network (shipped model)"
+                                         + (", finished buckets pulled when the engine estimates it "
+                                            "cheaper" if args.pull else ""),
                               "plain": "plain fixed-heuristic delta",
                               "feedback": "feedback delta"}[policy] if args.algorithm == "adds" else None,
                    "n": g.n, "m": g.m,
@@ -602,7 +628,9 @@ def main() -> int:
                    "device_ms_per_sssp": dev_ms / args.steps,
                    "kernel_ms_per_sssp": kernel_ms / args.steps,
                    "policy_used": policy_used if args.algorithm == "adds" else None,
+                   "pulled_phases_first_source": pulled_first if args.algorithm == "adds" else None,
                    "plain_adds": plain,
+                   "ablation": ablation,
                    "others": others,
                    "traffic_random_calibrated": traffic_random,
                    "algorithmic_bytes_per_sssp": bytes_alg / len(idx),

@@ -94,7 +94,7 @@ constexpr uint32_t kClaim = 2 * kAB;  // most items per workgroup claim (two pas
 constexpr uint32_t kBigClaimAvail = 65536;  // slot size above which claims take kClaim
 constexpr int kHubChunk = 8192;     // edges per hub chunk
 constexpr int kHubDegree = 4096;  // rows with more edges become hub chunks
-static_assert(8 + kAddsSlots <= 64,
+static_assert(9 + kAddsSlots <= 64,
               "claim probe: lanes 0-3, the window's semaphores, the worker share, the pull words");
 // vertices per pull chunk (four passes of kAB)
 constexpr uint32_t kPullChunk = 4 * 512;
@@ -116,8 +116,6 @@ constexpr uint32_t kErrRing = 1u, kErrHub = 2u, kErrStall = 4u;
 #define HV_HEAD(c, s) ((c)->hv_head[(s) * kPad])
 #define HV_TAIL(c, s) ((c)->hv_tail[(s) * kPad])
 #define HV_AVAIL(c, s) ((c)->hv_avail[(s) * kPad])
-#define HV_EDGES(c, s) ((c)->hv_edges[(s) * (kPad / 2)])
-#define HV_KEY(c, s) ((c)->hv_key[(s) * kPad])
 
 // Wait until this wave's outstanding vector memory operations have completed
 // (s_waitcnt vmcnt(0)).  Device-scope atomics, and loads and stores with the
@@ -187,7 +185,7 @@ struct AddsArgs {
   const int* trow;
   const uint2* tedges;
   const int* tsplit;
-  unsigned long long m_edges;  // edges of the graph (the pull work estimate)
+  uint32_t max_degree;  // largest out-degree (bounds the push work of a heavy list)
   uint32_t n;
   int live_edges;     // keep the live edge counter (phase traces read it)
   // Upper bound on claims by one workgroup: far above any real run; reaching
@@ -229,7 +227,6 @@ constexpr uint32_t kLocal = 4096;
 constexpr uint32_t kLocalKeep = 512;
 constexpr uint32_t kNoLocal = 0xFFFFFFFFu;
 constexpr uint32_t kTakenLocal = 0xFFFFu;  // stage entry moved to the local queue
-constexpr uint32_t kFirstReach = 0x80000000u;  // stage entry: the vertex's first finite label
 static_assert((kLocal & (kLocal - 1)) == 0, "local queue is a power-of-two ring");
 struct LocalQueue {
   int* q;          // kLocal entries
@@ -300,25 +297,6 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if constexpr (P) {
-    // the out-degree of every newly reached vertex goes into the reached-edge
-    // total that the pull decision weighs (each lane reads back only the
-    // entries it flags and clears)
-    uint32_t fr = 0;
-    for (uint32_t i = lane; i < st.n; i += kWave) {
-      const uint32_t sl = st.slot[i];
-      if (sl & kFirstReach) {
-        st.slot[i] = sl & ~kFirstReach;
-        const int v = st.v[i];
-        fr += (uint32_t)(a.rowptr[v + 1] - a.rowptr[v]);
-      }
-    }
-    if (__ballot(fr != 0)) {
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) fr += __shfl_xor(fr, off, 64);
-      if (lane == 0) atomicAdd(&c->reached_edges, (unsigned long long)fr);
-    }
-  }
   const uint32_t n_local = st.lq.slot != kNoLocal ? stage_to_local(st) : 0u;
   if (n_local == st.n) {
     st.n = 0;
@@ -381,11 +359,10 @@ __device__ inline bool claim_stamp(const AddsArgs& a, bool imp, uint32_t v, uint
   return __hip_atomic_fetch_min(a.qstamp + v, bkt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > bkt;
 }
 
-// Stage the lanes that claimed a stamp (P, pulls on: first = the push follows
-// the vertex's first finite label).
+// Stage the lanes that claimed a stamp.
 template <bool P>
 __device__ inline void stage_wanted(const AddsArgs& a, PushStage& st, bool want, uint32_t v,
-                                    uint32_t bkt, unsigned long long& pushed, bool first = false) {
+                                    uint32_t bkt, unsigned long long& pushed) {
   const unsigned long long mask = __ballot(want);
   if (mask == 0) return;
   const uint32_t k = (uint32_t)__popcll(mask);
@@ -393,7 +370,7 @@ __device__ inline void stage_wanted(const AddsArgs& a, PushStage& st, bool want,
   if (want) {
     const uint32_t i = st.n + prefix_count(mask);
     st.v[i] = (int)v;
-    st.slot[i] = (bkt % kAddsSlots) | (P && first ? kFirstReach : 0u);
+    st.slot[i] = bkt % kAddsSlots;
   }
   st.n += k;
   pushed += k;
@@ -436,11 +413,9 @@ __device__ inline void relax_quad(const AddsArgs& a, PushStage& st, const uint32
     if (cand[p] < cur[p]) old[p] = atomicMin(a.labels + v[p], cand[p]);
   }
   uint32_t bk[4], q[4];
-  bool first[4];
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     imp[p] = cand[p] < cur[p] && ck[p] < (uint32_t)(old[p] >> 32);
-    first[p] = P && imp[p] && (uint32_t)(old[p] >> 32) == W::kInfKey;
     bk[p] = imp[p] ? map_bucket(W::bucket_value(ck[p]), base, tbase, inv, lo[p]) : 0u;
   }
 #pragma unroll
@@ -451,7 +426,7 @@ __device__ inline void relax_quad(const AddsArgs& a, PushStage& st, const uint32
   // keep the four claims in flight: the comparisons stay after the last issue
   asm volatile("" : "+v"(q[0]), "+v"(q[1]), "+v"(q[2]), "+v"(q[3]));
 #pragma unroll
-  for (int p = 0; p < 4; ++p) stage_wanted<P>(a, st, q[p] > bk[p], v[p], bk[p], pushed, first[p]);
+  for (int p = 0; p < 4; ++p) stage_wanted<P>(a, st, q[p] > bk[p], v[p], bk[p], pushed);
 }
 
 // Publish a wave's staged heavy-list entries (all of one bucket, list slot
@@ -529,7 +504,7 @@ __global__ void k_adds_first_phase(AddsArgs a) {
 
 // Wave-level phase advance; true when this wave moved the base.  P: pulls on
 // (the boundary decides push or pull, and the next advance waits for both).
-template <bool P>
+template <class W, bool P>
 __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   AddsCtrl* c = a.c;
   const unsigned lane = lane_id();
@@ -542,9 +517,11 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   bool ready = false;
   if (lane == 0) {
     ready = true;
-    if constexpr (P)
-      ready = ld_rlx(&c->decided) == base &&
-              (ld_rlx(&c->pull_base) != base || ld_rlx(&c->pull_done) >= ld_rlx(&c->pull_chunks));
+    if constexpr (P) {
+      // one load when the boundary registered no pull
+      const uint32_t dw = ld_rlx(&c->decided);
+      ready = (dw >> 1) == base && (!(dw & 1u) || ld_rlx(&c->pull_done) >= ld_rlx(&c->pull_chunks));
+    }
   }
   if (ready) {
     // Only holders of a claim (or hub chunk) of bucket `base` push into it,
@@ -587,6 +564,81 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   uint32_t window_items = occ;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) window_items += __shfl_xor(window_items, off, 64);
+  // Heavy edges of the bucket just finished: pushed from its heavy list, or
+  // pulled by the vertices they could still improve when the estimated push
+  // work exceeds 2^margin x the estimated pull work.  Push work: the list's
+  // length times the mean heavy degree of 256 entries spread over it.  Pull
+  // work: the edges of the vertices not reached yet (read whole; n / 256 times
+  // the degrees of the unreached ones among 256 evenly spaced vertices), one
+  // label read per vertex (n / 8 in edge reads) and about two row reads per
+  // queued vertex.  The bucket's claims are all finished (the base moved), so
+  // its list is complete; nothing of it is claimed before `decided`.  Decided
+  // first, before the policy: the heavy list waits for it.
+  bool pulled = false;
+  if constexpr (P) {
+    const uint32_t fs = base % kAddsSlots;
+    uint32_t lh = 0, lt = 0, mode_on = 0;
+    float margin = 0.0f;
+    if (lane == 0) {
+      lh = ld_rlx(&HV_HEAD(c, fs));
+      lt = ld_rlx(&HV_TAIL(c, fs));
+      mode_on = ld_rlx(&c->pull_mode);
+      margin = exp2f(ld_rlx(&c->pull_margin));
+    }
+    lh = __shfl(lh, 0, 64);
+    lt = __shfl(lt, 0, 64);
+    mode_on = __shfl(mode_on, 0, 64);
+    margin = __shfl(margin, 0, 64);
+    const uint32_t count = lt - lh;
+    // the pull estimate is at least n / 8: no sampling when even rows of the
+    // largest degree could not outweigh it
+    if (mode_on && count > 0 && (float)count * (float)a.max_degree > margin * 0.125f * (float)a.n) {
+      const uint32_t ns = min(count, 256u);
+      const int* ring = a.hring + (size_t)fs * (a.cap_mask + 1u);
+      float heavy = 0.0f, unreached = 0.0f;
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t i = lane * 4u + j;
+        if (i < ns) {
+          const uint32_t pos = lh + (uint32_t)(((unsigned long long)i * count) / ns);
+          const int u = ld_rlx(ring + (pos & a.cap_mask));
+          if (u >= 0 && (uint32_t)u < a.n)
+            heavy += (float)(a.rowptr[u + 1] - a.rowptr[u] - a.split[u]);
+        }
+        const uint32_t v = (uint32_t)((((unsigned long long)(2u * i + 1u)) * a.n) / 512u);
+        if ((uint32_t)(ld_rlx(a.labels + v) >> 32) == W::kInfKey) unreached += (float)(a.rowptr[v + 1] - a.rowptr[v]);
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        heavy += __shfl_xor(heavy, off, 64);
+        unreached += __shfl_xor(unreached, off, 64);
+      }
+      const float push_work = (float)count * heavy / (float)ns;
+      const float pull_work = unreached * ((float)a.n / 256.0f) + 0.125f * (float)a.n + 2.0f * (float)window_items;
+      pulled = push_work > margin * pull_work;
+    }
+    if (lane == 0) {
+      if (pulled) {
+        // the pull holds the new phase open (pending, and pull_done before the
+        // next advance); its chunks are offered once registered and claimed
+        // once every list entry's key is folded into pull_key
+        const uint32_t chunks = (a.n + kPullChunk - 1) / kPullChunk;
+        st_rlx(&c->pull_count, count);
+        st_rlx(&c->pull_key, kNoStamp);
+        st_rlx(&c->pull_keyed, 0u);
+        st_rlx(&c->pull_chunks, chunks);
+        st_rlx(&c->pull_head, 0u);
+        st_rlx(&c->pull_done, 0u);
+        st_rlx(&c->pull_base, nb);
+        atomicAdd(&c->pending, (unsigned long long)chunks);
+        atomicAdd(&c->pulls, 1u);
+        vm_wait();
+        atomicAdd(&c->pull_avail, (int)chunks);
+        vm_wait();
+      }
+      st_rlx(&c->decided, (nb << 1) | (pulled ? 1u : 0u));
+    }
+  }
   float delta = ld_rlx(&c->delta);
   uint32_t active = ld_rlx(&c->active);
   uint32_t workers_now = ld_rlx(&c->active_workers);
@@ -618,51 +670,13 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
       t[36] = (float)ld_rlx(&c->pushes);
       t[37] = (float)workers_now;
       t[38] = (float)(t_adv - (long long)ld_rlx(&c->diag[15])) * 0.01f;  // 100 MHz clock -> us
+      t[39] = pulled ? 1.0f : 0.0f;
     }
   }
   if (lane == 0) {
     __hip_atomic_store(&c->delta, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&c->active, active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&c->active_workers, workers_now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if constexpr (P) {
-      // Heavy edges of the bucket just finished: pushed from its heavy list, or
-      // pulled by the vertices they could still improve when the estimated push
-      // work (the listed heavy edges) exceeds 2^margin x the estimated pull work
-      // (the edges of the vertices not reached yet, which are read whole, one
-      // label read per vertex, and about two row reads per queued vertex).  Its
-      // claims are all finished (the base moved), so the slot's statistics are
-      // final; they are reset for the bucket that reuses the slot.
-      const uint32_t fs = base % kAddsSlots;
-      const unsigned long long heavy_edges = ld_rlx(&HV_EDGES(c, fs));
-      const uint32_t kmin = ld_rlx(&HV_KEY(c, fs));
-      st_rlx(&HV_EDGES(c, fs), 0ull);
-      st_rlx(&HV_KEY(c, fs), kNoStamp);
-      bool pull = false;
-      if (ld_rlx(&c->pull_mode) != 0u && heavy_edges > 0 && kmin != kNoStamp) {
-        const unsigned long long reached = ld_rlx(&c->reached_edges);
-        const float unreached = a.m_edges > reached ? (float)(a.m_edges - reached) : 0.0f;
-        const float est = unreached + 0.125f * (float)a.n + 2.0f * (float)window_items;
-        pull = (float)heavy_edges > exp2f(ld_rlx(&c->pull_margin)) * est;
-      }
-      if (pull) {
-        // the pull holds the new phase open (pending, and pull_done before the
-        // next advance); its chunks are offered last
-        const uint32_t chunks = (a.n + kPullChunk - 1) / kPullChunk;
-        st_rlx(&c->pull_key, kmin);
-        st_rlx(&c->pull_chunks, chunks);
-        st_rlx(&c->pull_head, 0u);
-        st_rlx(&c->pull_done, 0u);
-        st_rlx(&c->pull_base, nb);
-        atomicAdd(&c->pending, (unsigned long long)chunks);
-        atomicAdd(&c->pulls, 1u);
-        vm_wait();
-        atomicAdd(&c->pull_avail, (int)chunks);
-      }
-      if (a.policy && a.policy->trace && ph < a.policy->trace_cap)
-        a.policy->trace[(size_t)ph * kTraceStride + 39] = pull ? 1.0f : 0.0f;
-      vm_wait();
-      st_rlx(&c->decided, nb);
-    }
     atomicAdd(&c->diag[14], (unsigned long long)(wall_clock64() - t_adv));
   }
   return true;
@@ -812,16 +826,15 @@ __device__ inline void pull_pass(const AddsArgs& a, PushStage& st, int* s_incl, 
   }
   __syncthreads();
   const unsigned long long best = s_best[threadIdx.x];
-  bool imp = false, first = false;
+  bool imp = false;
   uint32_t bk = 0;
   if (has && best < lab) {
     const unsigned long long old = atomicMin(a.labels + v, best);
     const uint32_t bkey = (uint32_t)(best >> 32);
     imp = bkey < (uint32_t)(old >> 32);
-    first = imp && (uint32_t)(old >> 32) == W::kInfKey;
     if (imp) bk = map_bucket(W::bucket_value(bkey), base, tbase, inv, base);
   }
-  stage_wanted<true>(a, st, claim_stamp(a, imp, v, bk), v, bk, pushed, first);
+  stage_wanted<true>(a, st, claim_stamp(a, imp, v, bk), v, bk, pushed);
   __syncthreads();  // the LDS arrays are refilled by the next pass
 }
 
@@ -901,10 +914,6 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
   // first kAB x 8 bytes for the per-vertex best candidates
   __shared__ __align__(16) int s_lq[kLocal];
   __shared__ uint32_t s_lfree, s_lhead, s_ltail, s_lnum, s_lbeg;
-  // a claim's heavy-list statistics (pulls on): heavy edges listed, smallest
-  // key of a processed vertex with heavy edges
-  __shared__ unsigned long long s_hve;
-  __shared__ uint32_t s_hvk;
   static_assert(kLocal * sizeof(int) >= kAB * sizeof(unsigned long long), "pull candidates fit the local queue");
   AddsCtrl* c = a.c;
   const unsigned lane = threadIdx.x & 63;
@@ -956,8 +965,9 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
         else if (lane >= 4 && lane < 4u + kAddsSlots) probe = ld_rlx(&AVAIL(c, (base0 + lane - 4) % kAddsSlots));
         else if (lane == 4u + kAddsSlots) probe = (int)ld_rlx(&c->active_workers);
         else if (kPull && lane == 5u + kAddsSlots) probe = (int)ld_rlx(&c->decided);
-        else if (kPull && lane == 6u + kAddsSlots) probe = (int)ld_rlx(&c->pull_base);
-        else if (kPull && lane == 7u + kAddsSlots) probe = ld_rlx(&c->pull_avail);
+        else if (kPull && lane == 6u + kAddsSlots) probe = ld_rlx(&c->pull_avail);
+        else if (kPull && lane == 7u + kAddsSlots) probe = (int)ld_rlx(&c->pull_keyed);
+        else if (kPull && lane == 8u + kAddsSlots) probe = (int)ld_rlx(&c->pull_count);
         done = (uint32_t)__shfl(probe, 0, 64);
         const int hub_n = __shfl(probe, 1, 64), hv_n0 = __shfl(probe, 2, 64);
         const uint32_t A = max(1u, min((uint32_t)__shfl(probe, 3, 64), (uint32_t)kAddsSlots));
@@ -975,9 +985,12 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
         // the heavy list of bucket base0 - 1 is claimed once the boundary
         // decided whether it is pushed or pulled (a read of an older word only
         // delays the claim, or pushes edges a pull also covers)
-        const bool decided = !kPull || (uint32_t)__shfl(probe, 5 + kAddsSlots, 64) == base0;
-        const bool pulled = kPull && decided && (uint32_t)__shfl(probe, 6 + kAddsSlots, 64) == base0;
-        const int pull_n = kPull ? __shfl(probe, 7 + kAddsSlots, 64) : 0;
+        const uint32_t dw = kPull ? (uint32_t)__shfl(probe, 5 + kAddsSlots, 64) : 0u;
+        const bool decided = !kPull || (dw >> 1) == base0;
+        const bool pulled = kPull && decided && (dw & 1u);
+        // chunks once every entry of the dropped list has its key folded in
+        const int pull_n = kPull && (uint32_t)__shfl(probe, 7 + kAddsSlots, 64) >= (uint32_t)__shfl(probe, 8 + kAddsSlots, 64)
+                               ? __shfl(probe, 6 + kAddsSlots, 64) : 0;
         const bool claimer = blockIdx.x < max(share, 1u);
         dormant = !claimer;
         if (lane == 0 && !done && claimer) {
@@ -1050,7 +1063,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
             done = 1;
           } else {
             const unsigned long long bw = __shfl(lane == 0 ? ld_rlx(&c->base_word) : 0ull, 0, 64);
-            if (try_advance<kPull>(a, bw)) {
+            if (try_advance<W, kPull>(a, bw)) {
               idle = 0;
               backoff = 1;
             } else {
@@ -1073,8 +1086,6 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       if (lane == 0) {
         s_done = done;
         s_count = got;
-        s_hve = 0;
-        s_hvk = kNoStamp;
         s_slot = slot;
         s_qslot = qslot;
         s_begin = begin;
@@ -1268,7 +1279,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       // row bounds and light-prefix length of u depend only on u: issued now,
       // they arrive while the stamp exchange and the label read complete
       int r_beg = 0, r_end = 0, r_split = 0, pu = u;
-      uint32_t hv_k = kNoStamp, hv_e = 0;  // pull statistics of this item
+      uint32_t drop_key = kNoStamp;  // a dropped entry's current key (pull bound)
       if (has && entry_ok) {
         r_beg = a.rowptr[u];
         r_end = a.rowptr[u + 1];
@@ -1276,16 +1287,18 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (a.ids) pu = a.ids[u];
       }
       if (has && entry_ok) {
-        if (heavy) {
+        if (drop) {
+          // a dropped entry keeps its list stamp (bucket base - 1, finished:
+          // no later pass compares equal to it); its key joins the pull bound
+          drop_key = atomic_load_key(a.labels + u);
+        } else if (heavy) {
           // Clearing the stamp first means a later light pass of u (after an
           // improvement) lists u again, or this pass reads the label it set.
           __hip_atomic_exchange(a.sstamp + u, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (!drop) {
-            vm_wait();  // the label is read after the stamp is cleared
-            key = atomic_load_key(a.labels + u);
-            bk = base;
-            DIAG(++d_heavy);
-          }
+          vm_wait();  // the label is read after the stamp is cleared
+          key = atomic_load_key(a.labels + u);
+          bk = base;
+          DIAG(++d_heavy);
         } else {
           // Whoever takes the stamp processes the vertex.  (Putting back a
           // stamp that belongs to an entry in another ring would race with that
@@ -1316,12 +1329,9 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
             beg = split;
           } else {
             // light prefix now; the heavy suffix once bucket bk is finished
-            if (split < end) {
+            if (split < end)
               to_heavy = __hip_atomic_exchange(a.sstamp + u, bk + 1u, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT) != bk + 1u;
-              hv_k = key;
-              if (to_heavy) hv_e = (uint32_t)(end - split);
-            }
             end = split;
           }
           deg = end - beg;
@@ -1341,18 +1351,15 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
           hv_n += k;
         }
       }
-      if (kPull && __ballot(hv_k != kNoStamp)) {
-        unsigned long long e = hv_e;
-        uint32_t k = hv_k;
+      if (drop) {
+        // the pull bound: every heavy edge the list would have pushed starts
+        // at one of these keys or higher (a vertex improved since is processed
+        // again in a later bucket)
+        uint32_t k = drop_key;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          e += __shfl_xor(e, off, 64);
-          k = min(k, (uint32_t)__shfl_xor(k, off, 64));
-        }
-        if (lane == 0) {
-          atomicAdd(&s_hve, e);
-          atomicMin(&s_hvk, k);
-        }
+        for (int off = 32; off > 0; off >>= 1) k = min(k, (uint32_t)__shfl_xor(k, off, 64));
+        if (lane == 0 && k != kNoStamp) atomicMin(&c->pull_key, k);
+        vm_wait();
       }
       // re-queued items go through the stage like any push
       push_improved<kPull>(a, st, requeue, (uint32_t)u, want, pushed_local);
@@ -1368,12 +1375,8 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
     st.lq.slot = kNoLocal;
     __syncthreads();
     if (threadIdx.x == 0) {
-      // the statistics land before the claim's release, so the wave that
-      // finishes the bucket reads them whole
-      if (kPull && s_hvk != kNoStamp) {
-        atomicAdd(&HV_EDGES(c, slot), s_hve);
-        atomicMin(&HV_KEY(c, slot), s_hvk);
-      }
+      // every wave's fold has completed (each waited after its atomicMin)
+      if (drop) atomicAdd(&c->pull_keyed, count);
       atomic_add_done(&INFL(c, slot), -1);
       atomicAdd(&c->pending, 0ull - (unsigned long long)count);
     }
@@ -1469,7 +1472,6 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   h.pull_mode = (uint32_t)req.pull;
   h.pull_margin = req.pull_margin;
   h.pull_base = kNoStamp;
-  for (int i = 0; i < kAddsSlots; ++i) h.hv_key[i * kPad] = kNoStamp;
   h.active = (uint32_t)std::min(std::max(req.active_buckets, 1), kAddsSlots);
   h.workers = (uint32_t)workers;
   h.active_workers = (uint32_t)workers;
@@ -1502,7 +1504,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.tedges = nullptr;
   a.tsplit = nullptr;
   if (req.pull) pull_rows(g, &a.trow, &a.tedges, s);
-  a.m_edges = (unsigned long long)g.m;
+  a.max_degree = (uint32_t)std::max<int32_t>(g.info.max_degree, 0);
   a.n = (uint32_t)n;
   a.live_edges = req.trace ? 1 : 0;
   a.claim_limit = 16ull * (unsigned long long)(n + g.m) + (1ull << 20);

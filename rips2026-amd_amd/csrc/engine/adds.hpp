@@ -58,15 +58,16 @@ struct AddsCtrl {
   // Pulls.  At a phase boundary the heavy edges of the bucket just finished
   // are either pushed from its vertices (its heavy list) or pulled by every
   // vertex that an edge from it could still improve; the pull runs as
-  // kPullChunk-vertex chunks while base == pull_base.  `decided` is the last
-  // base whose boundary work (policy, pull registration) is published: the
-  // base does not move past it before, and the heavy list is not claimed.
+  // kPullChunk-vertex chunks while base == pull_base.  `decided` is
+  // (b << 1) | pulled for the last base b whose boundary decision is
+  // published: the base does not move past b before, nor is the heavy list of
+  // bucket b - 1 claimed (pushed, or dropped when pulled).
   uint32_t decided;
   uint32_t pull_mode;            // 0: always push; 1: pull when estimated cheaper
   float pull_margin;             // log2(push / pull work estimate) above which to pull
-  uint32_t pull_base;            // base during which the registered pull runs (kNoStamp: none)
+  uint32_t pull_base;            // base during which the registered pull runs
   uint32_t pull_chunks;          // chunks of the registered pull
-  uint32_t pull_key;             // smallest key a vertex of the pulled bucket was processed with
+  uint32_t pull_count;           // heavy-list entries of the pulled bucket
   uint32_t pulls;                // buckets pulled
   uint32_t pad8[kPad - 7];
   int pull_avail;                // unclaimed chunks of the registered pull (may dip below 0)
@@ -75,12 +76,11 @@ struct AddsCtrl {
   uint32_t pad10[kPad - 1];
   uint32_t pull_done;            // finished chunks
   uint32_t pad11[kPad - 1];
-  unsigned long long reached_edges;  // out-degree sum of the vertices reached so far
+  // smallest current key over the pulled bucket's heavy list, folded in by
+  // the claims that drop its entries; the chunks start once all are in
+  uint32_t pull_key;
+  uint32_t pull_keyed;           // entries folded in
   uint32_t pad12[kPad - 2];
-  // per bucket slot: heavy edges listed, smallest key of a processed vertex
-  // with heavy edges (read and reset when the bucket finishes)
-  unsigned long long hv_edges[kAddsSlots * (kPad / 2)];
-  uint32_t hv_key[kAddsSlots * kPad];
   uint32_t head[kAddsSlots * kPad];  // pop counter of slot s at [s * kPad] (monotone)
   uint32_t tail[kAddsSlots * kPad];  // push counter of slot s
   int inflight[kAddsSlots * kPad];   // claims of slot s still being processed

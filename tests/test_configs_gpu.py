@@ -86,6 +86,12 @@ def test_c1_rmat22_learned_adds(c1):
     try:
         for s in srcs[[1, 2]]:
             assert _check(g, dg, s, M.ALG_ADDS, use_mlp=1).policy == 2
+            # the shipped configuration: finished buckets pulled when cheaper
+            st = _check(g, dg, s, M.ALG_ADDS, use_mlp=1, pull=1)
+            assert st.policy == 2 and st.pulled_phases > 0
+        # plain ADDS with the pull rule, and every bucket pulled
+        _check(g, dg, srcs[3], M.ALG_ADDS, pull=1)
+        _check(g, dg, srcs[3], M.ALG_ADDS, use_mlp=1, pull=1, pull_margin=-60.0)
     finally:
         L.clear_delta_model()
 
@@ -111,6 +117,7 @@ def test_c2_routing_grid_learned_adds(c2):
     try:
         for s in srcs[[0, 9]]:
             _check(g, dg, s, M.ALG_ADDS, use_mlp=1)
+        _check(g, dg, srcs[5], M.ALG_ADDS, use_mlp=1, pull=1)
     finally:
         L.clear_delta_model()
 
@@ -196,5 +203,7 @@ def test_c4_learned_adds_held_out_seeds(kind):
         for s in srcs:
             st = _check(g, dg, s, M.ALG_ADDS, use_mlp=1)
             assert st.policy == 2  # the network ran (the graph lies in its range)
+            st = _check(g, dg, s, M.ALG_ADDS, use_mlp=1, pull=1)
+            assert st.policy == 2
     finally:
         L.clear_delta_model()
