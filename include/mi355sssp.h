@@ -149,7 +149,11 @@ typedef struct {
                                network's margin for the phase) */
   float pull_margin;        /* ADDS, pull = 1: log2 of the push / pull work ratio above which
                                a bucket is pulled (0: whenever pulling is estimated cheaper) */
-  int32_t reserved[4];
+  float light_scale;        /* ADDS: light edges (relaxed when a vertex is processed; the rest
+                               when its bucket is finished) are those of weight <= light_scale x
+                               the first bucket width; 0: 1, or under use_mlp = 1 the network's
+                               choice */
+  int32_t reserved[3];
 } mi355_sssp_options;
 
 typedef struct {
@@ -508,16 +512,18 @@ int mi355_batch_autotune(mi355_graph* g, const mi355_query_batch* qb, int32_t sa
 /* ---- learned delta policy ---------------------------------------------
  * No reference counterpart: the network replaces the fixed per-phase delta
  * heuristics (JAS/hip_adds_sssp.cpp:683-727, ONE/sssp_hip.cpp:564-571,
- * ADDSF/wl.h:349-410).  32 phase features -> 64 hidden (ReLU) -> 3 outputs:
+ * ADDSF/wl.h:349-410).  32 phase features -> 64 hidden (ReLU) -> 4 outputs:
  * log2 of the bucket width relative to the heuristic delta0, the number of
- * buckets served at once, and log2 of the share of the persistent workgroups
- * that claim work.  The engine evaluates bf16-rounded weights with
+ * buckets served at once, log2 of the share of the persistent workgroups
+ * that claim work, and (read once, before the first claim) log2 of the light
+ * edges' width relative to the first bucket width.  The engine evaluates
+ * bf16-rounded weights with
  * fp32 accumulation; training keeps fp32 master weights. */
 typedef struct {
   float w1[64 * 32];       /* hidden x input, row-major */
   float b1[64];
-  float w2[3 * 64];        /* output x hidden, row-major */
-  float b2[3];
+  float w2[4 * 64];        /* output x hidden, row-major */
+  float b2[4];
   float feat_mean[32];     /* inputs are normalised (x - mean) * inv_std; not trained */
   float feat_inv_std[32];
   /* Range of the 14 graph features (inputs 0..13) over the training graphs.
@@ -537,10 +543,10 @@ typedef struct {
 } mi355_mlp_train_options;
 
 void mi355_mlp_train_options_default(mi355_mlp_train_options* o);
-/* x: n x 32 features (host), out: n x 3 outputs (host). */
+/* x: n x 32 features (host), out: n x 4 outputs (host). */
 int mi355_mlp_forward(const mi355_mlp_params* p, const float* x, int64_t n, float* out);
 /* Mean loss (1/n) sum w_s |o_s - y_s|^2 and its gradient (normalisation
- * fields of grad are zero).  y: n x 3 targets, w: n sample weights or NULL. */
+ * fields of grad are zero).  y: n x 4 targets, w: n sample weights or NULL. */
 int mi355_mlp_loss_grad(const mi355_mlp_params* p, const float* x, const float* y, const float* w,
                         int64_t n, float* loss, mi355_mlp_params* grad);
 /* Adam on the device; p is updated in place; epoch_loss (host, may be NULL)

@@ -1,7 +1,7 @@
 """CPU restatement of the learned delta policy's network (test infrastructure
 only: loaded by tests, never by the product).
 
-The network the ADDS scheduler evaluates (32 -> 64 ReLU -> 3) has no
+The network the ADDS scheduler evaluates (32 -> 64 ReLU -> 4) has no
 counterpart in the reference; it replaces the fixed per-phase heuristics of
 JAS/hip_adds_sssp.cpp:683-727 and ONE/sssp_hip.cpp:564-571.  This module
 restates its forward pass and the gradient of the training loss in float64
@@ -22,8 +22,8 @@ def bf16_round(a) -> np.ndarray:
     return u.astype(np.uint32).view(np.float32)
 
 
-N_OUT = 3
-N_PARAMS = 64 * 32 + 64 + N_OUT * 64 + N_OUT  # 2307
+N_OUT = 4
+N_PARAMS = 64 * 32 + 64 + N_OUT * 64 + N_OUT  # 2372
 
 
 def unpack(flat):
@@ -36,7 +36,7 @@ def unpack(flat):
 
 
 def forward(flat, mean, inv_std, x, round_weights=True):
-    """Outputs (n x 3) and hidden activations (n x 64)."""
+    """Outputs (n x 4) and hidden activations (n x 64)."""
     w1, b1, w2, b2 = unpack(flat)
     if round_weights:
         w1 = bf16_round(w1).astype(np.float64)
@@ -54,7 +54,7 @@ def loss(flat, mean, inv_std, x, y, w=None, round_weights=True) -> float:
 
 
 def loss_grad(flat, mean, inv_std, x, y, w=None, round_weights=True):
-    """(loss, gradient w.r.t. the 2307 trained parameters).  With rounded
+    """(loss, gradient w.r.t. the 2372 trained parameters).  With rounded
     weights the gradient is taken through the rounding unchanged
     (straight-through), as the device does."""
     w1, b1, w2, b2 = unpack(flat)

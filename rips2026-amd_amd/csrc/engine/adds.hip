@@ -460,11 +460,15 @@ __device__ inline void heavy_publish(const AddsArgs& a, int* stage, uint32_t& n,
 
 // The network's decision for the next phase: delta = delta0 * 2^clamp(o0, -4, 4)
 // (within the run's delta bounds), round(o1) active buckets in [1, 8], and
-// the share 2^clamp(o2, -5, 0) of the workgroups that claim (at least 16).
+// the share 2^clamp(o2, -5, 0) of the workgroups that claim (at least 16);
+// before the first claim also the light width, 2^clamp(o3, -5, 0) times the
+// first bucket width (light_scale).
 __device__ inline void mlp_decide(const AddsPolicy* p, const AddsCtrl* c, float feat, unsigned lane,
-                                  float& delta, uint32_t& active, uint32_t& workers_now) {
+                                  float& delta, uint32_t& active, uint32_t& workers_now,
+                                  float* light_scale = nullptr) {
   float o[kMlpOut];
   mlp_forward_wave(p, feat, lane, o);
+  if (light_scale) *light_scale = exp2f(fminf(fmaxf(o[3], -5.0f), 0.0f));
   const float share = exp2f(fminf(fmaxf(o[2], -5.0f), 0.0f));
   workers_now = (uint32_t)fmaxf((float)c->workers * share, 16.0f);
   workers_now = min(workers_now, c->workers);
@@ -491,14 +495,15 @@ __global__ void k_adds_first_phase(AddsArgs a) {
   AddsCtrl* c = a.c;
   float feat = 0.0f, raw = 0.0f;
   mlp_features(a.policy, c, lane == 0 ? 1u : 0u, 1u, c->delta, lane, &feat, &raw);
-  float delta;
+  float delta, light = 1.0f;
   uint32_t active, workers_now;
   if (a.policy->mode == kPolicySchedule) sched_decide(a.policy, c, 0u, delta, active, workers_now);
-  else mlp_decide(a.policy, c, feat, lane, delta, active, workers_now);
+  else mlp_decide(a.policy, c, feat, lane, delta, active, workers_now, &light);
   if (lane == 0) {
     c->delta = delta;
     c->active = active;
     c->active_workers = workers_now;
+    c->light_scale = light;
   }
 }
 
@@ -569,9 +574,11 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   // work exceeds 2^margin x the estimated pull work.  Push work: the list's
   // length times the mean heavy degree of 256 entries spread over it.  Pull
   // work: the edges of the vertices not reached yet (read whole; n / 256 times
-  // the degrees of the unreached ones among 256 evenly spaced vertices), one
-  // label read per vertex (n / 8 in edge reads) and about two row reads per
-  // queued vertex.  The bucket's claims are all finished (the base moved), so
+  // the degrees of the unreached ones among 256 evenly spaced vertices), an
+  // edge read's worth per vertex (its label and, at or above the bucket, a row
+  // search: counted at an eighth, the rule pulled buckets of routing grids
+  // whose pushes were far cheaper) and about two row reads per queued vertex.
+  // The bucket's claims are all finished (the base moved), so
   // its list is complete; nothing of it is claimed before `decided`.  Decided
   // first, before the policy: the heavy list waits for it.
   bool pulled = false;
@@ -590,9 +597,9 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
     mode_on = __shfl(mode_on, 0, 64);
     margin = __shfl(margin, 0, 64);
     const uint32_t count = lt - lh;
-    // the pull estimate is at least n / 8: no sampling when even rows of the
+    // the pull estimate is at least n: no sampling when even rows of the
     // largest degree could not outweigh it
-    if (mode_on && count > 0 && (float)count * (float)a.max_degree > margin * 0.125f * (float)a.n) {
+    if (mode_on && count > 0 && (float)count * (float)a.max_degree > margin * (float)a.n) {
       const uint32_t ns = min(count, 256u);
       const int* ring = a.hring + (size_t)fs * (a.cap_mask + 1u);
       float heavy = 0.0f, unreached = 0.0f;
@@ -614,7 +621,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
         unreached += __shfl_xor(unreached, off, 64);
       }
       const float push_work = (float)count * heavy / (float)ns;
-      const float pull_work = unreached * ((float)a.n / 256.0f) + 0.125f * (float)a.n + 2.0f * (float)window_items;
+      const float pull_work = unreached * ((float)a.n / 256.0f) + (float)a.n + 2.0f * (float)window_items;
       pulled = push_work > margin * pull_work;
     }
     if (lane == 0) {
@@ -1468,10 +1475,13 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   h.shrink = 0.75f;
   if (req.pull != 0 && req.pull != 1) fail(MI355_EINVAL, "pull must be 0 (push) or 1 (pull when cheaper)");
   if (!std::isfinite(req.pull_margin)) fail(MI355_EINVAL, "pull_margin must be finite");
+  if (!(req.light_scale >= 0.0f) || !std::isfinite(req.light_scale))
+    fail(MI355_EINVAL, "light_scale must be finite and nonnegative");
   h.decided = 0;
   h.pull_mode = (uint32_t)req.pull;
   h.pull_margin = req.pull_margin;
   h.pull_base = kNoStamp;
+  h.light_scale = 1.0f;
   h.active = (uint32_t)std::min(std::max(req.active_buckets, 1), kAddsSlots);
   h.workers = (uint32_t)workers;
   h.active_workers = (uint32_t)workers;
@@ -1493,8 +1503,10 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.local_keep = a.ring_limit < cap ? 0u : kLocalKeep;
   // light/heavy split at the starting bucket width (MLP mode: at the
   // network's first width, set below; one cached split per graph either way)
-  a.split = req.use_mlp ? nullptr : light_split(g, delta, s);
-  float split_delta = delta;
+  // light edges: weight <= light_scale x the first bucket width
+  const float lscale = req.light_scale > 0.0f ? req.light_scale : 1.0f;
+  a.split = req.use_mlp ? nullptr : light_split(g, delta * lscale, s);
+  float split_delta = delta * lscale;
   a.hub = ws.adds_hub.p;
   a.hub_mask = hub_cap - 1;
   a.c = c;
@@ -1530,12 +1542,17 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
     // The light prefix of each (weight-sorted) row follows the network's
     // first width, not the heuristic one: a split at a wider delta makes the
     // light pass relax edges of up to that width eagerly from vertices that
-    // are not settled yet (on R-MAT scale 22, 1.4x the relaxations).  The
-    // split is cached per width, so runs on one graph compute it once.
-    float first_delta = delta;
+    // are not settled yet (on R-MAT scale 22, 1.4x the relaxations); the
+    // network also scales it (its light width output), unless the caller
+    // fixed light_scale.  The split is cached per width, so runs on one graph
+    // compute it once.
+    float first_delta = delta, net_light = 1.0f;
     MI_HIP(hipMemcpyAsync(&first_delta, &c->delta, sizeof(float), hipMemcpyDeviceToHost, s));
+    MI_HIP(hipMemcpyAsync(&net_light, &c->light_scale, sizeof(float), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
-    split_delta = first_delta > 0.0f && std::isfinite(first_delta) ? first_delta : delta;
+    if (!(net_light > 0.0f && net_light <= 1.0f)) net_light = 1.0f;
+    split_delta = (first_delta > 0.0f && std::isfinite(first_delta) ? first_delta : delta) *
+                  (req.light_scale > 0.0f ? req.light_scale : net_light);
     a.split = light_split(g, split_delta, s);
   }
   // the pull reads the heavy in-edges at the same split width

@@ -9,7 +9,8 @@ namespace mi355 {
 constexpr int kAddsSlots = 32;     // circular bucket window
 constexpr int kMlpIn = 32;         // phase features
 constexpr int kMlpHidden = 64;
-constexpr int kMlpOut = 3;         // log2 delta scale, active buckets, log2 worker share
+// log2 delta scale, active buckets, log2 worker share, log2 light width (first phase)
+constexpr int kMlpOut = 4;
 constexpr int kTraceStride = 40;   // floats per traced phase
 
 enum PolicyMode { kPolicyFixed = 0, kPolicyHeuristic = 1, kPolicyMlp = 2, kPolicySchedule = 3 };
@@ -30,7 +31,8 @@ struct AddsCtrl {
   uint32_t phases;
   uint32_t err;
   uint32_t active_workers;       // workgroups that claim work (the rest only idle)
-  uint32_t pad0[kPad - 13];
+  float light_scale;             // network's light width / first bucket width (first phase)
+  uint32_t pad0[kPad - 14];
   unsigned long long base_word;  // (T_base float bits << 32) | base bucket
   uint32_t pad1[kPad - 2];
   unsigned long long pending;    // items queued or being processed (two's-complement adds)
@@ -137,7 +139,7 @@ __device__ inline void mlp_features(const AddsPolicy* p, const AddsCtrl* c, uint
   *feat = lane < kMlpIn ? x : 0.0f;
 }
 
-// 32 -> 64 (ReLU) -> 3 on one wave: lane j owns hidden unit j.
+// 32 -> 64 (ReLU) -> 4 on one wave: lane j owns hidden unit j.
 __device__ inline void mlp_forward_wave(const AddsPolicy* p, float feat, unsigned lane, float* out) {
   float h = p->b1[lane];
 #pragma unroll

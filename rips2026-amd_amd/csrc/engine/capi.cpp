@@ -205,6 +205,7 @@ static mi355::RunRequest make_request(const mi355_sssp_options* opt, bool want_w
     r.active_buckets = opt->active_buckets;
     r.pull = opt->pull;
     r.pull_margin = opt->pull_margin;
+    r.light_scale = opt->light_scale;
   } else {
     r.algorithm = MI355_ALG_DELTA;
   }

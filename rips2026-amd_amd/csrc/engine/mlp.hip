@@ -1,7 +1,8 @@
 // The learned delta policy's network, trained and evaluated on the device.
 //
-// 32 phase features -> 64 hidden (ReLU) -> 3 outputs (log2 of the bucket-width
-// scale, active buckets, log2 of the claiming worker share).  Weights are kept as an fp32 master copy for
+// 32 phase features -> 64 hidden (ReLU) -> 4 outputs (log2 of the bucket-width
+// scale, active buckets, log2 of the claiming worker share, log2 of the light
+// width relative to the first bucket width).  Weights are kept as an fp32 master copy for
 // training; every evaluation (here and inside the ADDS kernel) uses the
 // weights rounded to bf16 (round to nearest even) with fp32 accumulation, in
 // the same operation order, so a trained model behaves on the worklist
@@ -26,7 +27,7 @@ namespace mi355 {
 
 namespace {
 
-constexpr int kP = kMlpHidden * kMlpIn + kMlpHidden + kMlpOut * kMlpHidden + kMlpOut;  // 2307
+constexpr int kP = kMlpHidden * kMlpIn + kMlpHidden + kMlpOut * kMlpHidden + kMlpOut;  // 2372
 constexpr int kOffB1 = kMlpHidden * kMlpIn;
 constexpr int kOffW2 = kOffB1 + kMlpHidden;
 constexpr int kOffB2 = kOffW2 + kMlpOut * kMlpHidden;

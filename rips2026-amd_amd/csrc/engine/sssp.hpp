@@ -29,6 +29,9 @@ struct RunRequest {
   // the push work estimate exceeds 2^pull_margin x the pull work estimate
   int pull = 0;
   float pull_margin = 0.0f;
+  // light edges: weight <= light_scale x the first bucket width (0: 1, or the
+  // network's choice under the MLP policy)
+  float light_scale = 0.0f;
 };
 
 struct RunResult {

@@ -47,7 +47,7 @@ class SsspOptions(C.Structure):
                 ("dynamic_delta", C.c_int32), ("use_mlp", C.c_int32),
                 ("worker_blocks", C.c_int32), ("rounds_per_sync", C.c_int32),
                 ("active_buckets", C.c_int32), ("pull", C.c_int32), ("pull_margin", C.c_float),
-                ("reserved", C.c_int32 * 4)]
+                ("light_scale", C.c_float), ("reserved", C.c_int32 * 3)]
 
 
 class SsspStats(C.Structure):
@@ -61,8 +61,8 @@ class SsspStats(C.Structure):
 
 
 class MlpParams(C.Structure):
-    _fields_ = [("w1", C.c_float * (64 * 32)), ("b1", C.c_float * 64), ("w2", C.c_float * (3 * 64)),
-                ("b2", C.c_float * 3), ("feat_mean", C.c_float * 32), ("feat_inv_std", C.c_float * 32),
+    _fields_ = [("w1", C.c_float * (64 * 32)), ("b1", C.c_float * 64), ("w2", C.c_float * (4 * 64)),
+                ("b2", C.c_float * 4), ("feat_mean", C.c_float * 32), ("feat_inv_std", C.c_float * 32),
                 ("graph_feat_lo", C.c_float * 14), ("graph_feat_hi", C.c_float * 14)]
 
 

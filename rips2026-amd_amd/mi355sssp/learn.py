@@ -31,7 +31,7 @@ import numpy as np
 from ._abi import (ADDS_TRACE_STRIDE, ALG_ADDS, MlpParams, MlpTrainOptions, SsspOptions, SsspStats,
                    check, engine)
 
-N_IN, N_HID, N_OUT = 32, 64, 3
+N_IN, N_HID, N_OUT = 32, 64, 4
 N_GRAPH = 14  # graph features: inputs 0..13
 _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(_HERE, "data", "delta_mlp.json")
@@ -57,7 +57,7 @@ def _f32(a, shape=None) -> np.ndarray:
 
 
 class DeltaModel:
-    """fp32 master weights of the 32 -> 64 -> 3 network plus the input
+    """fp32 master weights of the 32 -> 64 -> 4 network plus the input
     normalisation (not trained)."""
 
     def __init__(self, w1, b1, w2, b2, mean=None, inv_std=None, feat_lo=None, feat_hi=None):
@@ -77,7 +77,7 @@ class DeltaModel:
         rng = np.random.default_rng(seed)
         w1 = rng.normal(0.0, math.sqrt(2.0 / N_IN), (N_HID, N_IN))
         w2 = rng.normal(0.0, 0.1 * math.sqrt(1.0 / N_HID), (N_OUT, N_HID))
-        return cls(w1, np.zeros(N_HID), w2, np.array([0.0, 1.0, 0.0]))
+        return cls(w1, np.zeros(N_HID), w2, np.array([0.0, 1.0, 0.0, 0.0]))
 
     # -- conversions --------------------------------------------------------
     def to_c(self) -> MlpParams:
@@ -117,7 +117,13 @@ class DeltaModel:
     def load(cls, path: str) -> "DeltaModel":
         with open(path) as f:
             d = json.load(f)
-        m = cls(d["w1"], d["b1"], d["w2"], d["b2"], d["feat_mean"], d["feat_inv_std"],
+        w2, b2 = np.asarray(d["w2"], np.float32), np.asarray(d["b2"], np.float32)
+        if w2.shape[0] == N_OUT - 1:
+            # a model without the light-width output: the light edges follow
+            # the first bucket width (output 0: log2 1)
+            w2 = np.vstack([w2, np.zeros((1, N_HID), np.float32)])
+            b2 = np.append(b2, np.float32(0.0))
+        m = cls(d["w1"], d["b1"], w2, b2, d["feat_mean"], d["feat_inv_std"],
                 d.get("graph_feat_lo"), d.get("graph_feat_hi"))
         m.meta = d.get("meta", {})
         return m
@@ -295,7 +301,8 @@ def collect_graph(dg, source: int, reps: int = 2, max_records: int = 24):
     if (1.0, 1, 1.0) not in costs:
         run(1.0, 1, 1.0)
     best = min(costs, key=lambda k: costs[k][0])
-    target = np.array([math.log2(best[0]), float(best[1]), math.log2(best[2])], np.float32)
+    # light edges at the first width in these searches (light-width output 0)
+    target = np.array([math.log2(best[0]), float(best[1]), math.log2(best[2]), 0.0], np.float32)
     xs = []
     for (s, a, w) in costs:
         rec, _ = trace(dg, source, cap=4096, delta=s * delta0, active_buckets=a,
@@ -353,7 +360,7 @@ def collect_head(dg, source: int, graph_feat, static, delta0: float, reps: int =
         head, tail = (best[0], 1, best[3]), best[1:]
 
     def target(s):
-        return np.array([math.log2(s[0]), float(s[1]), math.log2(s[2])], np.float32)
+        return np.array([math.log2(s[0]), float(s[1]), math.log2(s[2]), 0.0], np.float32)
 
     meta = {"head": list(head), "tail": list(tail), "head_ms": costs[best], "static_ms": costs["static"],
             "head_costs": {"x".join(str(v) for v in k) if k != "static" else k: c for k, c in costs.items()}}
@@ -443,6 +450,84 @@ def collect_heads(metas, x, y, w, log=print, reps: int = 2, time_budget_s: float
             np.array(HI, np.int64), HM)
 
 
+# light widths searched per graph (relative to the first bucket width), widest
+# first; a search stops once a width runs at more than twice the best time
+LIGHT_SCALES = (1.0, 0.5, 0.25, 0.125, 0.0625)
+
+
+def collect_light(specs, teacher: "DeltaModel", log=print, reps: int = 2, max_records: int = 24,
+                  time_budget_s: float | None = None):
+    """Data for the network's light-width output under pulls.  For each graph
+    the shipped configuration (the teacher network's widths, active buckets
+    and worker share, buckets pulled when cheaper) is timed at every light
+    width of LIGHT_SCALES; the fastest width is the graph's target for output
+    3, and one traced run at that width gives the phase records, whose targets
+    for outputs 0-2 are the teacher's own outputs (so the new network keeps
+    the teacher's phase decisions).  The record the engine evaluates before
+    the first claim (first_phase_features) is added with the same targets;
+    output 3 is read only there.  Returns (x, y, w, metas): each graph's
+    records weigh 1 in total, half of it on the first-phase record."""
+    from .device import DeviceGraph
+
+    X, Y, W, metas = [], [], [], []
+    t0 = time.time()
+    teacher.install()
+    for i, spec in enumerate(specs):
+        if time_budget_s is not None and time.time() - t0 > time_budget_s:
+            log(f"light: time budget reached after {i} graphs")
+            break
+        g = make_graph(spec)
+        dg = DeviceGraph(g)
+        src = int(g.pick_sources(1, seed=spec["seed"] & 0xFFFFFFFF)[0])
+        costs = {}
+        for sc in LIGHT_SCALES:
+            o = dg.options(ALG_ADDS, use_mlp=1, pull=1, light_scale=sc)
+            costs[sc] = min(dg.sssp_labels(src, o).kernel_ms for _ in range(reps))
+            if costs[sc] > 2.0 * min(costs.values()):
+                break
+        best = min(costs, key=costs.get)
+        rec, st = trace(dg, src, use_mlp=1, pull=1, light_scale=best)
+        if len(rec) > max_records:
+            rec = rec[np.linspace(0, len(rec) - 1, max_records).round().astype(int)]
+        x = np.ascontiguousarray(rec[:, :N_IN])
+        x = np.vstack([first_phase_features(x[0, :N_GRAPH]), x])
+        y = teacher.forward(x)
+        y[:, 3] = math.log2(best)
+        w = np.full(len(x), 0.5 / max(len(x) - 1, 1), np.float32)
+        w[0] = 0.5
+        meta = dict(spec, n=g.n, m=g.m, source=src, light=best, light_costs={str(k): v for k, v in costs.items()},
+                    records=int(len(x)), policy=int(st.policy))
+        X.append(x)
+        Y.append(y)
+        W.append(w)
+        metas.append(meta)
+        if i % 10 == 0:
+            log(f"light {i + 1}/{len(specs)} {spec['kind']} n={g.n} best={best} "
+                f"gain={costs[1.0] / costs[best]:.2f} ({time.time() - t0:.0f}s)")
+        del dg, g
+    x = np.concatenate(X) if X else np.zeros((0, N_IN), np.float32)
+    y = np.concatenate(Y) if Y else np.zeros((0, N_OUT), np.float32)
+    w = np.concatenate(W) if W else np.zeros(0, np.float32)
+    if len(w):
+        w *= len(w) / w.sum()
+    return x, y, w, metas
+
+
+def fit_light(teacher: "DeltaModel", x, y, w=None, epochs: int = 40, batch: int = 512, lr: float = 1e-3,
+              seed: int = 2026):
+    """The teacher's network with the light-width output trained in: starts
+    from the teacher's weights (output 3 at the mean target) and its input
+    normalisation and graph-feature range, then Adam on the device over all
+    outputs (outputs 0-2 are the teacher's own, so their error starts at
+    zero)."""
+    m = DeltaModel(teacher.w1.copy(), teacher.b1.copy(), teacher.w2.copy(), teacher.b2.copy(),
+                   teacher.mean.copy(), teacher.inv_std.copy(), teacher.feat_lo.copy(), teacher.feat_hi.copy())
+    m.w2[3, :] = 0.0
+    m.b2[3] = float(np.average(y[:, 3], weights=w)) if len(y) else 0.0
+    losses = m.train(x, y, w, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return m, losses
+
+
 def record_counts(metas, w) -> np.ndarray:
     """Phase-boundary records per graph of a dataset, in the order of metas:
     from the metadata (records) where collect() stored it, else from the
@@ -501,7 +586,7 @@ def class_weights(w, metas, class_of=graph_class):
     c = float(w.astype(np.float64).sum()) / max(len(metas), 1)
     i = 0
     for m in metas:
-        k = int(round(c / float(w[i])))
+        k = int(m["records"]) if "records" in m else int(round(c / float(w[i])))
         out[i:i + k] = w[i:i + k] / counts[class_of(m)]
         i += k
     if i != len(w):

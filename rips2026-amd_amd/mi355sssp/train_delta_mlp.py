@@ -8,6 +8,12 @@ each); --train-only trains on the shards given with --shards.  --heads-for
 adds the first-phase records (learn.collect_heads) to a shard collected
 without them.  Training graphs come from learn.training_specs(seed);
 evaluation graphs use other seeds.
+
+The light-width stage trains the network's fourth output into a shipped
+model (the teacher) under pulls:
+
+    python -m mi355sssp.train_delta_mlp --light-from mi355sssp/data/delta_mlp.json \
+        --graphs 240 --large-grids 8 --out mi355sssp/data/delta_mlp.json
 """
 from __future__ import annotations
 
@@ -58,6 +64,12 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=None, help="model JSON to write")
+    ap.add_argument("--light-from", default=None,
+                    help="teacher model: collect light-width data on --graphs training specs (and "
+                         "--large-grids large grids) with pulls on, train its light-width output, "
+                         "write --out (learn.collect_light, learn.fit_light)")
+    ap.add_argument("--light-epochs", type=int, default=40)
+    ap.add_argument("--light-lr", type=float, default=1e-3)
     ap.add_argument("--graph-weight", choices=["class", "time", "sqrt-time", "equal"], default="class",
                     help="weight of a graph's records: every size class (R-MAT scale, grid kind x "
                          "decade of n) the same, its plain-ADDS time (the time at stake), its square "
@@ -65,6 +77,8 @@ def main() -> int:
     args = ap.parse_args()
 
     shards = list(args.shards)
+    if args.light_from:
+        return light_stage(args)
     if args.heads_for:
         z = np.load(args.heads_for, allow_pickle=False)
         metas = json.loads(str(z["meta"]))
@@ -159,6 +173,54 @@ def main() -> int:
     }
     model.save(args.out, meta=summary)
     log(f"wrote {args.out}: {json.dumps(summary)}")
+    return 0
+
+
+def light_stage(args) -> int:
+    """Collect light-width data under the teacher (or read --shards), train
+    the fourth output, write --out and, with --shard-out, the data."""
+    teacher = L.DeltaModel.load(args.light_from)
+    if args.shards:
+        parts = [np.load(p, allow_pickle=False) for p in args.shards]
+        x = np.concatenate([z["x"] for z in parts])
+        y = np.concatenate([z["y"] for z in parts])
+        w = np.concatenate([z["w"] for z in parts])
+        metas = [m for z in parts for m in json.loads(str(z["meta"]))]
+    else:
+        specs = L.training_specs(args.total, args.seed)[args.first:args.first + args.graphs]
+        if args.max_rmat_scale is not None:
+            specs = [s for s in specs if s["kind"] != "rmat" or s["scale"] <= args.max_rmat_scale]
+        if args.large_grids > 0:
+            specs += L.large_grid_specs(args.large_grids, args.seed)
+        t0 = time.time()
+        x, y, w, metas = L.collect_light(specs, teacher, log=log, reps=args.reps, time_budget_s=args.budget_s)
+        log(f"light: {len(metas)} graphs, {len(x)} records in {time.time() - t0:.0f}s")
+        if args.shard_out:
+            np.savez_compressed(args.shard_out, x=x, y=y, w=w, meta=np.array(json.dumps(metas)))
+    if args.out is None or len(x) == 0:
+        return 0
+    if args.graph_weight == "class":
+        w = L.class_weights(w, metas)
+    w = (w * (len(w) / w.sum())).astype(np.float32)
+    t0 = time.time()
+    model, losses = L.fit_light(teacher, x, y, w, epochs=args.light_epochs, lr=args.light_lr)
+    pred = model.forward(x)
+    first = np.concatenate([[0], np.cumsum([m["records"] for m in metas])[:-1]]).astype(int)
+    chosen = np.exp2(np.clip(pred[first, 3], -5.0, 0.0))
+    summary = dict(teacher.meta or {})
+    summary["light_stage"] = {
+        "graphs": len(metas), "samples": int(len(x)), "epochs": args.light_epochs, "lr": args.light_lr,
+        "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
+        "light_scales": list(L.LIGHT_SCALES),
+        "targets": {str(k): int(sum(1 for m in metas if m["light"] == k)) for k in L.LIGHT_SCALES},
+        "mean_abs_err_log2_light_first_phase": float(np.abs(pred[first, 3] - y[first, 3]).mean()),
+        "mean_abs_dev_teacher_outputs": float(np.abs(pred[:, :3] - y[:, :3]).mean()),
+        "oracle_light_gain_mean": float(np.mean([m["light_costs"]["1.0"] / min(m["light_costs"].values())
+                                                 for m in metas])),
+        "network_light_choice_mean": float(chosen.mean()),
+    }
+    model.save(args.out, meta=summary)
+    log(f"trained in {time.time() - t0:.1f}s, wrote {args.out}: {json.dumps(summary['light_stage'])}")
     return 0
 
 
