@@ -109,14 +109,20 @@ __global__ void k_gather_transposed(const uint32_t* __restrict__ order,
   }
 }
 
-// rowptr from sorted keys: rowptr[u] = first i with key[i] >= u.
+// rowptr from sorted keys: rowptr[u] = first i with key[i] >= u (a thread per
+// row and a binary search, so a long run of empty rows is not one thread's
+// loop).
 __global__ void k_offsets_from_sorted(const uint32_t* __restrict__ keys, int* __restrict__ rowptr,
                                       int64_t m, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= m; i += stride) {
-    const int64_t lo = i == 0 ? -1 : (int64_t)keys[i - 1];
-    const int64_t hi = i == m ? n : (int64_t)keys[i];
-    for (int64_t u = lo + 1; u <= hi; ++u) rowptr[u] = (int)i;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u <= n; u += stride) {
+    int64_t lo = 0, hi = m;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)keys[mid] < u) lo = mid + 1;
+      else hi = mid;
+    }
+    rowptr[u] = (int)lo;
   }
 }
 
@@ -251,7 +257,7 @@ void transpose_sorted(const int* rowptr, const uint2* edges, int64_t n, int64_t 
     MI_HIP(hipStreamSynchronize(stream));
   }
   k_key_rows<<<grid_for(m), kBlock, 0, stream>>>(keys_out.p, rowid.p, m);
-  k_offsets_from_sorted<<<grid_for(m + 1), kBlock, 0, stream>>>(rowid.p, out_rp.p, m, n);
+  k_offsets_from_sorted<<<grid_for(n + 1), kBlock, 0, stream>>>(rowid.p, out_rp.p, m, n);
   k_row_weight_gather<<<grid_for(m), kBlock, 0, stream>>>(keys_out.p, vals_out.p, wtype, out_e.p, m);
   MI_LAUNCH_CHECK();
   MI_HIP(hipStreamSynchronize(stream));
@@ -553,7 +559,7 @@ mi355_graph* build_graph(const mi355_csr_desc& d, int allow_negative, hipStream_
                                        order_out.p, (size_t)m, 0, bits, stream));
       k_gather_transposed<<<grid_for(m), kBlock, 0, stream>>>(order_out.p, rowid.p, w,
                                                               g->edges.p, m);
-      k_offsets_from_sorted<<<grid_for(m + 1), kBlock, 0, stream>>>(keys_out.p, g->rowptr.p, m, n);
+      k_offsets_from_sorted<<<grid_for(n + 1), kBlock, 0, stream>>>(keys_out.p, g->rowptr.p, m, n);
       MI_LAUNCH_CHECK();
       MI_HIP(hipStreamSynchronize(stream));
     }
