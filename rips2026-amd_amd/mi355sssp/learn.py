@@ -513,6 +513,87 @@ def collect_light(specs, teacher: "DeltaModel", log=print, reps: int = 2, max_re
     return x, y, w, metas
 
 
+# the pulled-configuration search on graphs with a giant bucket: width scale x
+# light width at one bucket and half the workgroups, then the share and the
+# active buckets at the best pair
+PULL_WIDTHS = (0.125, 0.25, 0.5, 1.0)
+PULL_LIGHTS = (0.25, 0.125, 0.0625)
+
+
+def collect_pulled(specs, teacher: "DeltaModel", log=print, reps: int = 2, max_records: int = 24,
+                   time_budget_s: float | None = None):
+    """Targets for the shipped configuration (pulls on) where it differs from
+    the push-only training: on each R-MAT graph, static settings (width x
+    delta0, active buckets, worker share, light width) are timed with pulls
+    (PULL_WIDTHS x PULL_LIGHTS at one bucket and half the workgroups, then
+    shares {1, 1/2, 1/4} and 1 or 2 buckets at the best pair), the fastest
+    becomes the target of all four outputs for every record of the graph (and
+    the first-phase record), and the records come from one traced run of that
+    setting.  Other graphs go through collect_light (the teacher's decisions
+    kept, the light width searched).  Returns (x, y, w, metas) as collect_light."""
+    from .device import DeviceGraph
+
+    rmat = [sp for sp in specs if sp["kind"] == "rmat"]
+    other = [sp for sp in specs if sp["kind"] != "rmat"]
+    X, Y, W, metas = [], [], [], []
+    t0 = time.time()
+    teacher.install()
+    for i, spec in enumerate(rmat):
+        if time_budget_s is not None and time.time() - t0 > time_budget_s:
+            log(f"pulled: time budget reached after {i} graphs")
+            break
+        g = make_graph(spec)
+        dg = DeviceGraph(g)
+        src = int(g.pick_sources(1, seed=spec["seed"] & 0xFFFFFFFF)[0])
+        delta0 = dg.sssp_labels(src, dg.options(ALG_ADDS)).initial_delta
+
+        def run(sc, act, sh, lt):
+            o = dg.options(ALG_ADDS, delta=sc * delta0, active_buckets=act, worker_blocks=workers_for(sh),
+                           pull=1, light_scale=lt)
+            return min(dg.sssp_labels(src, o).kernel_ms for _ in range(reps))
+
+        costs = {(sc, 1, 0.5, lt): run(sc, 1, 0.5, lt) for sc in PULL_WIDTHS for lt in PULL_LIGHTS}
+        sc, _, _, lt = min(costs, key=costs.get)
+        for act in (1, 2):
+            for sh in (1.0, 0.5, 0.25):
+                if (sc, act, sh, lt) not in costs:
+                    costs[(sc, act, sh, lt)] = run(sc, act, sh, lt)
+        best = min(costs, key=costs.get)
+        o = dg.options(ALG_ADDS, use_mlp=1, pull=1)
+        learned = min(dg.sssp_labels(src, o).kernel_ms for _ in range(reps))
+        plain = min(dg.sssp_labels(src, dg.options(ALG_ADDS)).kernel_ms for _ in range(reps))
+        rec, _ = trace(dg, src, delta=best[0] * delta0, active_buckets=best[1],
+                       worker_blocks=workers_for(best[2]), pull=1, light_scale=best[3])
+        if len(rec) > max_records:
+            rec = rec[np.linspace(0, len(rec) - 1, max_records).round().astype(int)]
+        x = np.ascontiguousarray(rec[:, :N_IN])
+        x = np.vstack([first_phase_features(x[0, :N_GRAPH]), x])
+        y = np.repeat(np.array([[math.log2(best[0]), float(best[1]), math.log2(best[2]), math.log2(best[3])]],
+                               np.float32), len(x), axis=0)
+        w = np.full(len(x), 0.5 / max(len(x) - 1, 1), np.float32)
+        w[0] = 0.5
+        meta = dict(spec, n=g.n, m=g.m, source=src, best=list(best), best_ms=costs[best],
+                    teacher_ms=learned, plain_ms=plain, records=int(len(x)), light=best[3],
+                    light_costs={"1.0": learned})
+        X.append(x)
+        Y.append(y)
+        W.append(w)
+        metas.append(meta)
+        if i % 10 == 0:
+            log(f"pulled {i + 1}/{len(rmat)} scale {spec['scale']} best={best} {costs[best]:.3f} ms "
+                f"(network {learned:.3f}, plain {plain:.3f}) ({time.time() - t0:.0f}s)")
+        del dg, g
+    xo, yo, wo, mo = collect_light(other, teacher, log=log, reps=reps, max_records=max_records,
+                                   time_budget_s=None if time_budget_s is None else
+                                   max(0.0, time_budget_s - (time.time() - t0)))
+    x = np.concatenate(X + [xo]) if X else xo
+    y = np.concatenate(Y + [yo]) if Y else yo
+    w = np.concatenate(W + [wo]) if W else wo
+    if len(w):
+        w = w * (len(w) / w.sum())
+    return x, y, w, metas + mo
+
+
 def fit_light(teacher: "DeltaModel", x, y, w=None, epochs: int = 40, batch: int = 512, lr: float = 1e-3,
               seed: int = 2026):
     """The teacher's network with the light-width output trained in: starts

@@ -68,6 +68,9 @@ def main() -> int:
                     help="teacher model: collect light-width data on --graphs training specs (and "
                          "--large-grids large grids) with pulls on, train its light-width output, "
                          "write --out (learn.collect_light, learn.fit_light)")
+    ap.add_argument("--pulled", action="store_true",
+                    help="with --light-from: on R-MAT graphs search static settings with pulls "
+                         "(learn.collect_pulled) and train all four outputs there")
     ap.add_argument("--light-epochs", type=int, default=40)
     ap.add_argument("--light-lr", type=float, default=1e-3)
     ap.add_argument("--graph-weight", choices=["class", "time", "sqrt-time", "equal"], default="class",
@@ -193,7 +196,8 @@ def light_stage(args) -> int:
         if args.large_grids > 0:
             specs += L.large_grid_specs(args.large_grids, args.seed)
         t0 = time.time()
-        x, y, w, metas = L.collect_light(specs, teacher, log=log, reps=args.reps, time_budget_s=args.budget_s)
+        collect = L.collect_pulled if args.pulled else L.collect_light
+        x, y, w, metas = collect(specs, teacher, log=log, reps=args.reps, time_budget_s=args.budget_s)
         log(f"light: {len(metas)} graphs, {len(x)} records in {time.time() - t0:.0f}s")
         if args.shard_out:
             np.savez_compressed(args.shard_out, x=x, y=y, w=w, meta=np.array(json.dumps(metas)))
@@ -212,11 +216,17 @@ def light_stage(args) -> int:
         "graphs": len(metas), "samples": int(len(x)), "epochs": args.light_epochs, "lr": args.light_lr,
         "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
         "light_scales": list(L.LIGHT_SCALES),
-        "targets": {str(k): int(sum(1 for m in metas if m["light"] == k)) for k in L.LIGHT_SCALES},
+        "targets": {str(k): int(sum(1 for m in metas if m["light"] == k))
+                    for k in sorted(set(L.LIGHT_SCALES) | set(L.PULL_LIGHTS), reverse=True)},
         "mean_abs_err_log2_light_first_phase": float(np.abs(pred[first, 3] - y[first, 3]).mean()),
         "mean_abs_dev_teacher_outputs": float(np.abs(pred[:, :3] - y[:, :3]).mean()),
         "oracle_light_gain_mean": float(np.mean([m["light_costs"]["1.0"] / min(m["light_costs"].values())
-                                                 for m in metas])),
+                                                 for m in metas if "best" not in m] or [1.0])),
+        "pulled_graphs": int(sum(1 for m in metas if "best" in m)),
+        "pulled_static_over_teacher_mean": float(np.mean([m["teacher_ms"] / m["best_ms"]
+                                                          for m in metas if "best" in m] or [1.0])),
+        "pulled_static_over_plain_mean": float(np.mean([m["plain_ms"] / m["best_ms"]
+                                                        for m in metas if "best" in m] or [1.0])),
         "network_light_choice_mean": float(chosen.mean()),
     }
     model.save(args.out, meta=summary)
