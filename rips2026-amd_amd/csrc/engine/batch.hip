@@ -42,6 +42,7 @@
 #include <numeric>
 #include <vector>
 
+#include "fastdiv.hpp"
 #include "frontier_kernels.hpp"
 #include "sssp.hpp"
 
@@ -136,6 +137,7 @@ struct BatchArgs {
   uint32_t qoff[kParts + 1];  // part p's segment of every queue / pile: [qoff[p], qoff[p + 1])
   BatchCtrl* ctrl;
   int32_t cols;       // grid width: vertex v = row * cols + col
+  FastDiv cdiv;       // v / cols without a divide instruction
   int32_t rounds_per_graph;
   int want_edges;
   float delta;        // near-far bucket width (value domain); <= 0: Bellman-Ford rounds
@@ -269,6 +271,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
   __shared__ uint32_t s_key[kWavesPerBlock][kWave];
   __shared__ int s_u[kWavesPerBlock][kWave];
   __shared__ int s_q[kWavesPerBlock][kWave];
+  // each entry's query rectangle (origin, size) and label base, read once
+  // per entry instead of once per edge
+  __shared__ int4 s_rect[kWavesPerBlock][kWave];
+  __shared__ int64_t s_qbase[kWavesPerBlock][kWave];
   __shared__ uint2 s_near[kWavesPerBlock][kNearStage];
   __shared__ uint2 s_far[kWavesPerBlock][kFarStage];
   __shared__ uint2 s_loc[kWavesPerBlock][kLocalQ];
@@ -338,10 +344,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
       const bool has = from_local ? lane < take : i < n_in;
       int u = 0, beg = 0, deg = 0, qid = 0;
       uint32_t key = W::kInfKey;
+      QueryInfo qi{};
       if (has) {
         const uint2 it = from_local ? loc[(lhead + lane) & (kLocalQ - 1u)] : qin[i];
         qid = (int)it.x;
-        const QueryInfo qi = a.qinfo[qid];
+        qi = a.qinfo[qid];
         uint32_t r, col;
         tiled_cell(it.y, (uint32_t)qi.bh, (uint32_t)qi.bw, r, col);
         u = (qi.r0 + (int)r) * a.cols + qi.c0 + (int)col;
@@ -364,6 +371,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
       s_key[wv][lane] = key;
       s_u[wv][lane] = u;
       s_q[wv][lane] = qid;
+      s_rect[wv][lane] = make_int4(qi.r0, qi.c0, qi.bh, qi.bw);
+      s_qbase[wv][lane] = qi.base;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -389,12 +398,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
             if (s_incl[wv][pos[p] + s2 - 1] <= t + p * kWave + (int)lane) pos[p] += s2;
         }
         uint2 e[4];
-        QueryInfo qi[4];
+        int4 rect[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           qq[p] = ok[p] ? s_q[wv][pos[p]] : 0;
           e[p] = a.edges[ok[p] ? s_beg[wv][pos[p]] + t + p * kWave + (int)lane : 0];
-          qi[p] = a.qinfo[qq[p]];
+          rect[p] = s_rect[wv][pos[p]];
         }
         constexpr unsigned long long kNone = ~0ull;
         int64_t slot[4];
@@ -404,11 +413,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           // target inside the query's rectangle -> its local slot
-          const int rv = (int)e[p].x / a.cols, cv = (int)e[p].x - rv * a.cols;
-          const uint32_t dr = (uint32_t)(rv - qi[p].r0), dc = (uint32_t)(cv - qi[p].c0);
-          const bool in = ok[p] && dr < (uint32_t)qi[p].bh && dc < (uint32_t)qi[p].bw;
-          lv[p] = in ? tiled_slot(dr, dc, (uint32_t)qi[p].bh, (uint32_t)qi[p].bw) : 0u;
-          slot[p] = in ? qi[p].base + lv[p] : 0;
+          const int rv = (int)fastdiv(e[p].x, a.cdiv), cv = (int)e[p].x - rv * a.cols;
+          const uint32_t dr = (uint32_t)(rv - rect[p].x), dc = (uint32_t)(cv - rect[p].y);
+          const bool in = ok[p] && dr < (uint32_t)rect[p].z && dc < (uint32_t)rect[p].w;
+          lv[p] = in ? tiled_slot(dr, dc, (uint32_t)rect[p].z, (uint32_t)rect[p].w) : 0u;
+          slot[p] = in ? s_qbase[wv][pos[p]] + lv[p] : 0;
           ck[p] = in ? W::relax(s_key[wv][pos[p]], e[p].y) : W::kInfKey;
           cand[p] = ck[p] != W::kInfKey ? ((unsigned long long)ck[p] << 32) | (uint32_t)s_u[wv][pos[p]] : kNone;
           // label probe and both stamps (the stamp reads are only filters:
@@ -910,6 +919,7 @@ static void run_batch_typed(mi355_graph& g, const mi355_query_batch& qb, const m
   };
   a.ctrl = ctrl;
   a.cols = qb.grid_cols;
+  a.cdiv = make_fastdiv((uint32_t)qb.grid_cols);
   a.rounds_per_graph = K;
   a.want_edges = o.count_edges ? 1 : 0;
 
