@@ -378,11 +378,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       edge_visits += (unsigned long long)total;
       // Four edges per lane per step (j, j + 64, j + 128, j + 192): four
-      // rounds of independent accesses (edges + query records; labels + both
-      // stamps; atomicMins; stamp exchanges), each issued for all four edges
-      // before any result is used.  Lanes past the end read edge 0 / query 0
-      // / slot 0 and discard them (guarded loads would each wait inside
-      // their branch).
+      // rounds of independent accesses (edges; labels; atomicMins; stamp
+      // exchanges of the improving edges), each issued for all four edges
+      // before any result is used.  Lanes past the end read edge 0 / slot 0
+      // and discard them (guarded loads would each wait inside their
+      // branch).
       for (int t = 0; t < total; t += 4 * kWave) {
         int pos[4], qq[4];
         bool ok[4];
@@ -409,7 +409,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
         int64_t slot[4];
         uint32_t lv[4], ck[4];
         unsigned long long cand[4], cur[4], old[4];
-        uint32_t so[4], fo[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           // target inside the query's rectangle -> its local slot
@@ -420,16 +419,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
           slot[p] = in ? s_qbase[wv][pos[p]] + lv[p] : 0;
           ck[p] = in ? W::relax(s_key[wv][pos[p]], e[p].y) : W::kInfKey;
           cand[p] = ck[p] != W::kInfKey ? ((unsigned long long)ck[p] << 32) | (uint32_t)s_u[wv][pos[p]] : kNone;
-          // label probe and both stamps (the stamp reads are only filters:
-          // the exchanges below decide)
+          // label probe only: the stamps (another line of HBM) are touched by
+          // the exchanges of the improving edges alone, a minority
           cur[p] = a.labels[slot[p]];
-          const uint2 sf = a.stamps[slot[p]];
-          so[p] = sf.x;
-          fo[p] = sf.y;
         }
         asm volatile("" : "+v"(cur[0]), "+v"(cur[1]), "+v"(cur[2]), "+v"(cur[3]));
-        asm volatile("" : "+v"(so[0]), "+v"(so[1]), "+v"(so[2]), "+v"(so[3]), "+v"(fo[0]), "+v"(fo[1]),
-                     "+v"(fo[2]), "+v"(fo[3]));
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           old[p] = kNone;
@@ -465,8 +459,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_work(BatchArgs a, int k) {
         }
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-          if (nr[p] && !lc[p] && so[p] != stamp) x[p] = atomicExch(&a.stamps[slot[p]].x, stamp);
-          if (fr[p] && fo[p] != fst) y[p] = atomicExch(&a.stamps[slot[p]].y, fst);
+          if (nr[p] && !lc[p]) x[p] = atomicExch(&a.stamps[slot[p]].x, stamp);
+          if (fr[p]) y[p] = atomicExch(&a.stamps[slot[p]].y, fst);
         }
         asm volatile("" : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(y[0]), "+v"(y[1]),
                      "+v"(y[2]), "+v"(y[3]));
