@@ -50,6 +50,8 @@ def main() -> int:
                     help="leave out R-MAT specs above this scale (21: the training set of scales 16-21 "
                          "and grids of 1e5-2e6 vertices, evaluated on held-out scale 22 / 24M graphs)")
     ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--no-heads", action="store_true",
+                    help="collect without the two-phase schedule search (first-phase records)")
     ap.add_argument("--budget-s", type=float, default=None, help="stop collecting after this long")
     ap.add_argument("--shard-out", default=None, help="write the collected shard here (.npz)")
     ap.add_argument("--shards", nargs="*", default=[], help="train on these shards")
@@ -107,7 +109,8 @@ def main() -> int:
         if args.max_rmat_scale is not None:
             specs = [s for s in specs if s["kind"] != "rmat" or s["scale"] <= args.max_rmat_scale]
         t0 = time.time()
-        x, y, w, metas, hx, hy, hidx = L.collect(specs, log=log, reps=args.reps, time_budget_s=args.budget_s)
+        x, y, w, metas, hx, hy, hidx = L.collect(specs, log=log, reps=args.reps, time_budget_s=args.budget_s,
+                                                  heads=not args.no_heads)
         log(f"collected {len(metas)} graphs, {len(x)} samples in {time.time() - t0:.0f}s")
         if args.shard_out:
             np.savez_compressed(args.shard_out, x=x, y=y, w=w, hx=hx, hy=hy, hidx=hidx,
