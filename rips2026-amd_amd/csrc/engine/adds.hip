@@ -341,12 +341,15 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
     const uint32_t s = so >> 16;
     ring_store(a, s, st.hist[s] + (so & 0xFFFFu), st.v[i]);
   }
-  // entries written before they are offered to claimers (a claimer that is
-  // early waits on the sentinel anyway)
+  // pending (and the entries: a claimer that is early waits on the sentinel
+  // anyway) before the offer: a claimer could otherwise finish the entries
+  // and take their count off pending before it was added, and pending could
+  // read zero while this claim still works.  The offer itself is not waited
+  // for here: every wave waits for its outstanding operations once, before
+  // its claim is released.
   vm_wait();
   __builtin_amdgcn_wave_barrier();
   if (k) atomicAdd(&AVAIL(c, lane), (int)k);
-  vm_wait();  // complete before the producer's claim is released
   st.n = 0;
 }
 
@@ -451,10 +454,11 @@ __device__ inline void heavy_publish(const AddsArgs& a, int* stage, uint32_t& n,
   pos0 = __shfl(pos0, 0, 64);
   int* ring = a.hring + (size_t)hs * (a.cap_mask + 1u);
   for (uint32_t i = lane; i < n; i += kWave) ring_put(ring + ((pos0 + i) & a.cap_mask), stage[i], &c->err, &c->done);
+  // pending and the entries before the offer, as in stage_publish; the
+  // offer is waited for before the claim's release
   vm_wait();
   __builtin_amdgcn_wave_barrier();
   if (lane == 0) atomicAdd(&HV_AVAIL(c, hs), (int)n);
-  vm_wait();
   n = 0;
 }
 
@@ -1157,6 +1161,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       if (a.live_edges && wv == 0 && lane == 0) atomicAdd(&c->edges, (unsigned long long)hcnt);
       edges_local += hcnt;
       DIAG(d_hub += threadIdx.x == 0);
+      vm_wait();  // every wave's publishes complete before the release
       __syncthreads();
       if (threadIdx.x == 0) {
         atomic_add_done(&INFL(c, hslot), -1);
@@ -1208,9 +1213,10 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
       }
       stage_publish<true>(a, st);
+      vm_wait();  // every wave's publishes complete before the release
       __syncthreads();
       if (threadIdx.x == 0) {
-        // the chunk's pushes have completed (each wave waited in its publish)
+        // the chunk's pushes have completed (each wave waited above)
         atomicAdd(&c->pull_done, 1u);
         atomic_add_done(&INFL(c, s_slot), -1);
         atomicAdd(&c->pending, ~0ull);
@@ -1380,6 +1386,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       DIAG(t_relax += t_r1 - t_r0; t_pub += wall_clock64() - t_r1);
     }
     st.lq.slot = kNoLocal;
+    vm_wait();  // every wave's publishes complete before the release
     __syncthreads();
     if (threadIdx.x == 0) {
       // every wave's fold has completed (each waited after its atomicMin)
