@@ -335,11 +335,25 @@ __device__ inline void stage_publish(const AddsArgs& a, PushStage& st) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (uint32_t i = lane; i < st.n; i += kWave) {
-    const uint32_t so = st.slot[i];
-    if (so == kTakenLocal) continue;
-    const uint32_t s = so >> 16;
-    ring_store(a, s, st.hist[s] + (so & 0xFFFFu), st.v[i]);
+  // four entries per lane in flight: the exchanges' old values (the overwrite
+  // check of ring_put) are examined after all four are issued
+  for (uint32_t i0 = 0; i0 < st.n; i0 += 4 * kWave) {
+    int old[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t i = i0 + (uint32_t)j * kWave + lane;
+      old[j] = kEmptyEntry;
+      const uint32_t so = i < st.n ? st.slot[i] : kTakenLocal;
+      if (so != kTakenLocal) {
+        const uint32_t s = so >> 16;
+        int* cell = a.ring + (size_t)s * (a.cap_mask + 1u) + ((st.hist[s] + (so & 0xFFFFu)) & a.cap_mask);
+        old[j] = __hip_atomic_exchange(cell, st.v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (old[0] != kEmptyEntry || old[1] != kEmptyEntry || old[2] != kEmptyEntry || old[3] != kEmptyEntry) {
+      atomicOr(&c->err, kErrRing);
+      atomicExch(&c->done, 1u);
+    }
   }
   // pending (and the entries: a claimer that is early waits on the sentinel
   // anyway) before the offer: a claimer could otherwise finish the entries
