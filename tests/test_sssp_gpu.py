@@ -311,3 +311,27 @@ def test_hip_adds_warm_start_on_a_renumbered_graph():
     want = orc.dijkstra(rp, g.col[keep], g.weights[keep], src)
     want[v] = init[v]
     assert dist.tobytes() == want.tobytes()
+
+
+@pytest.mark.parametrize("alg", [M.ALG_BF, M.ALG_BF_JACOBI, M.ALG_DELTA, M.ALG_ADDS])
+def test_inf_mode_flt_max_marks_unreached(alg):
+    """MI355_INF_FLT_MAX writes FLT_MAX (bits 0x7F7FFFFF) for an unreached
+    vertex, the reference ADDS driver's convention; MI355_INF_IEEE writes +inf
+    (0x7F800000).  Finite distances are the same bits either way."""
+    rng = np.random.default_rng(41)
+    n = 600
+    src = rng.integers(0, 400, 3000)
+    dst = rng.integers(0, 400, 3000)  # vertices 400.. have no in-edges: unreachable
+    keep = src != dst
+    w = rng.uniform(0.01, 1.0, keep.sum()).astype(np.float32)
+    g = M.from_edges(n, src[keep], dst[keep], w, M.W_F32)
+    dg = M.DeviceGraph(g)
+    want = orc.dijkstra(g.rowptr, g.col, g.weights, 0)
+    unreached = ~np.isfinite(want)
+    assert unreached.sum() >= 200
+    for mode, bits in ((M.INF_FLT_MAX, 0x7F7FFFFF), (M.INF_IEEE, 0x7F800000)):
+        dist, pred, _ = dg.sssp(0, alg, inf_mode=mode)
+        u = dist.view(np.uint32)
+        assert np.all(u[unreached] == bits), (mode, np.unique(u[unreached])[:4])
+        assert np.array_equal(u[~unreached], want.view(np.uint32)[~unreached])
+        assert np.all(pred[unreached] == -1)
