@@ -187,6 +187,7 @@ struct AddsArgs {
   const int* tsplit;
   uint32_t max_degree;  // largest out-degree (bounds the push work of a heavy list)
   uint32_t n;
+  uint32_t pull_n;      // 1 + the last vertex with an in-edge: pulls cover [0, pull_n)
   int live_edges;     // keep the live edge counter (phase traces read it)
   // Upper bound on claims by one workgroup: far above any real run; reaching
   // it means a livelock and aborts the run with an error.
@@ -617,7 +618,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
     const uint32_t count = lt - lh;
     // the pull estimate is at least n: no sampling when even rows of the
     // largest degree could not outweigh it
-    if (mode_on && count > 0 && (float)count * (float)a.max_degree > margin * (float)a.n) {
+    if (mode_on && count > 0 && a.pull_n > 0 && (float)count * (float)a.max_degree > margin * (float)a.pull_n) {
       const uint32_t ns = min(count, 256u);
       const int* ring = a.hring + (size_t)fs * (a.cap_mask + 1u);
       float heavy = 0.0f, unreached = 0.0f;
@@ -630,7 +631,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
           if (u >= 0 && (uint32_t)u < a.n)
             heavy += (float)(a.rowptr[u + 1] - a.rowptr[u] - a.split[u]);
         }
-        const uint32_t v = (uint32_t)((((unsigned long long)(2u * i + 1u)) * a.n) / 512u);
+        const uint32_t v = (uint32_t)((((unsigned long long)(2u * i + 1u)) * a.pull_n) / 512u);
         if ((uint32_t)(ld_rlx(a.labels + v) >> 32) == W::kInfKey) unreached += (float)(a.rowptr[v + 1] - a.rowptr[v]);
       }
 #pragma unroll
@@ -639,7 +640,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
         unreached += __shfl_xor(unreached, off, 64);
       }
       const float push_work = (float)count * heavy / (float)ns;
-      const float pull_work = unreached * ((float)a.n / 256.0f) + (float)a.n + 2.0f * (float)window_items;
+      const float pull_work = unreached * ((float)a.pull_n / 256.0f) + (float)a.pull_n + 2.0f * (float)window_items;
       pulled = push_work > margin * pull_work;
     }
     if (lane == 0) {
@@ -647,7 +648,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
         // the pull holds the new phase open (pending, and pull_done before the
         // next advance); its chunks are offered once registered and claimed
         // once every list entry's key is folded into pull_key
-        const uint32_t chunks = (a.n + kPullChunk - 1) / kPullChunk;
+        const uint32_t chunks = (a.pull_n + kPullChunk - 1) / kPullChunk;
         st_rlx(&c->pull_count, count);
         st_rlx(&c->pull_key, kNoStamp);
         st_rlx(&c->pull_keyed, 0u);
@@ -1196,7 +1197,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       const uint32_t q = s_begin;
       unsigned long long* s_best = reinterpret_cast<unsigned long long*>(s_lq);
       if (q < ld_rlx(&c->pull_chunks)) {
-        const uint32_t v0 = q * kPullChunk, v1 = min(a.n, v0 + kPullChunk);
+        const uint32_t v0 = q * kPullChunk, v1 = min(a.pull_n, v0 + kPullChunk);
         for (uint32_t p0 = v0; p0 < v1; p0 += kAB) {
           const uint32_t v = p0 + threadIdx.x;
           const bool has = v < v1;
@@ -1208,10 +1209,16 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
             int lo = rb;
             if (key_v == W::kInfKey) {
               lo = re;
-            } else if (rb < re && W::relax(pkey, a.tedges[rb].y) <= key_v) {
-              // the first heavy in-edge can reach: search for the end
-              int hi = re;
-              ++lo;
+            } else if (rb < re) {
+              // the first four heavy in-edges in one round trip (most prefixes
+              // end there; the weights ascend, so the reaching ones lead)
+              uint32_t w4[4];
+#pragma unroll
+              for (int p = 0; p < 4; ++p) w4[p] = a.tedges[min(rb + p, re - 1)].y;
+#pragma unroll
+              for (int p = 0; p < 4; ++p) lo += (rb + p < re && W::relax(pkey, w4[p]) <= key_v) ? 1 : 0;
+              // all four reach: search the rest for the end
+              int hi = lo == rb + 4 ? re : lo;
               while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
                 if (W::relax(pkey, a.tedges[mid].y) <= key_v) lo = mid + 1;
@@ -1536,7 +1543,10 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.trow = nullptr;
   a.tedges = nullptr;
   a.tsplit = nullptr;
-  if (req.pull) pull_rows(g, &a.trow, &a.tedges, s);
+  if (req.pull) {
+    pull_rows(g, &a.trow, &a.tedges, s);
+    a.pull_n = (uint32_t)g.pull_n;
+  }
   a.max_degree = (uint32_t)std::max<int32_t>(g.info.max_degree, 0);
   a.n = (uint32_t)n;
   a.live_edges = req.trace ? 1 : 0;

@@ -246,6 +246,14 @@ void run_sssp(mi355_graph& g, uint32_t input_source, const RunRequest& req, RunR
     case MI355_ALG_BF_JACOBI:
       if (g.weight_type == MI355_W_F32) run_bf_typed<WF32>(g, source, req, res, stream, false);
       else run_bf_typed<WU32>(g, source, req, res, stream, false);
+      if (g.log_dst) {
+        // the caller's round log (mi355_sssp_round_log): |F_0| .. |F_rounds|,
+        // for this run only
+        const size_t k = std::min(res.frontier_log.size(), (size_t)std::max<int32_t>(g.log_cap, 0));
+        for (size_t i = 0; i < k; ++i) g.log_dst[i] = (int32_t)res.frontier_log[i];
+        g.log_dst = nullptr;
+        g.log_cap = 0;
+      }
       break;
     case MI355_ALG_DELTA:
       run_delta_stepping(g, source, req, res, stream);

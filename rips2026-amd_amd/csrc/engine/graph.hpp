@@ -102,6 +102,9 @@ struct mi355_graph {
   // orientations) and its own rows serve; 2: t_rowptr / t_edges hold the
   // transpose.
   int pull_state = 0;
+  // 1 + the last vertex with an in-edge: pulls skip the vertices past it (in
+  // degree order, the isolated vertices at the end)
+  int64_t pull_n = 0;
   mi355::DevBuf<int> t_rowptr;
   mi355::DevBuf<uint2> t_edges;
   mi355::DevBuf<int> t_split;  // light in-edges per row of the transpose (for t_split_bits)

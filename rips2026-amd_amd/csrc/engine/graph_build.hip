@@ -428,6 +428,15 @@ void input_order_arrays(mi355_graph& g, const int** rowptr, const uint2** edges,
 // exactly when the two are equal, and then its own rows are its in-edge rows
 // (a pull reads a row by weight only, so the order of equal weights in it does
 // not matter).  Otherwise T is kept.
+// 1 + the last row with an entry (0 when every row is empty).
+__global__ void k_last_nonempty_row(const int* __restrict__ rowptr, int64_t n, unsigned long long* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long best = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+    if (rowptr[v + 1] > rowptr[v]) best = (unsigned long long)(v + 1);
+  if (best) atomicMax(out, best);
+}
+
 void pull_rows(mi355_graph& g, const int** rowptr, const uint2** edges, hipStream_t stream) {
   if (g.pull_state == 0) {
     DevBuf<int> t_rp, tt_rp;
@@ -449,6 +458,15 @@ void pull_rows(mi355_graph& g, const int** rowptr, const uint2** edges, hipStrea
     } else {
       g.pull_state = 1;
     }
+    DevBuf<unsigned long long> last(1);
+    MI_HIP(hipMemsetAsync(last.p, 0, 8, stream));
+    k_last_nonempty_row<<<grid_for(g.n), kBlock, 0, stream>>>(g.pull_state == 2 ? g.t_rowptr.p : g.rowptr.p, g.n,
+                                                              last.p);
+    MI_LAUNCH_CHECK();
+    unsigned long long h_last = 0;
+    MI_HIP(hipMemcpyAsync(&h_last, last.p, 8, hipMemcpyDeviceToHost, stream));
+    MI_HIP(hipStreamSynchronize(stream));
+    g.pull_n = (int64_t)h_last;
   }
   *rowptr = g.pull_state == 2 ? g.t_rowptr.p : g.rowptr.p;
   *edges = g.pull_state == 2 ? g.t_edges.p : g.edges.p;

@@ -157,6 +157,7 @@ _ENGINE_SIGS = {
     "mi355_sssp": (C.c_int, [P, I32, C.POINTER(SsspOptions), P, P, C.POINTER(SsspStats), P]),
     "mi355_sssp_labels": (C.c_int, [P, I32, C.POINTER(SsspOptions), C.POINTER(SsspStats), P]),
     "mi355_sssp_labels_to_output": (C.c_int, [P, I32, I32, P, P, P]),
+    "mi355_sssp_round_log": (C.c_int, [P, P, I32]),
     "mi355_bf_csr_f32": (C.c_int, [C.POINTER(CsrDesc), C.c_int, C.c_int, C.c_int, P,
                                    C.POINTER(BfCsrStats), PROGRESS_CB, P, P]),
     "mi355_delta_stepping_csr_f32": (C.c_int, [C.POINTER(CsrDesc), C.c_int, C.c_float, C.c_int, P,

@@ -116,6 +116,14 @@ class DeviceGraph:
                                   C.byref(st) if st is not None else None, None))
         return st
 
+    def round_log(self, cap: int) -> np.ndarray:
+        """Array that the next BF / BF_JACOBI run on this graph fills with its
+        per-round frontier sizes |F_0| .. |F_rounds| (at most cap entries; the
+        rest stay -1).  Keep it alive until that run returns."""
+        log = np.full(cap, -1, np.int32)
+        check(engine().mi355_sssp_round_log(self._h, log.ctypes.data, int(cap)))
+        return log
+
     def sssp_labels(self, source: int, opts: SsspOptions) -> SsspStats:
         st = SsspStats()
         check(engine().mi355_sssp_labels(self._h, source, C.byref(opts), C.byref(st), None))

@@ -335,3 +335,31 @@ def test_inf_mode_flt_max_marks_unreached(alg):
         assert np.all(u[unreached] == bits), (mode, np.unique(u[unreached])[:4])
         assert np.array_equal(u[~unreached], want.view(np.uint32)[~unreached])
         assert np.all(pred[unreached] == -1)
+
+
+@pytest.mark.gpu
+def test_round_log_matches_the_jacobi_frontiers():
+    # mi355_sssp_round_log: the next BF_JACOBI run writes |F_0| .. |F_rounds|,
+    # where F_k = {v : d_k[v] < d_(k-1)[v]} of the Jacobi iterates (F_0 = the
+    # source); the oracle's iterates after k rounds give every size.
+    print("round log: R-MAT scale 9, Jacobi BF", flush=True)
+    g = M.rmat(9, 16, seed=77)
+    dg = M.DeviceGraph(g)
+    src = int(g.pick_sources(1, 5)[0])
+    log = dg.round_log(4096)
+    dist, _, st = dg.sssp(src, M.ALG_BF_JACOBI)
+    rounds = st.rounds
+    assert log[rounds + 1] == -1 and (log[: rounds + 1] >= 0).all()
+    prev = np.full(g.n, np.inf, np.float32)
+    prev[src] = 0.0
+    want = [1]
+    for k in range(1, rounds + 1):
+        d_k, _, _ = orc.bf_jacobi_in(g.rowptr, g.col, g.weights, src, k, 1)
+        want.append(int((d_k < prev).sum()))
+        prev = d_k
+    assert log[: rounds + 1].tolist() == want
+    assert want[-1] == 0 and dist.tobytes() == prev.tobytes()
+    # the log is for one run only: the next run leaves the array alone
+    log2 = log.copy()
+    dg.sssp(src, M.ALG_BF_JACOBI)
+    assert (log == log2).all()
