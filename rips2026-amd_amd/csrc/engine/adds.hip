@@ -958,7 +958,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
   unsigned long long claims = 0;
   unsigned long long d_cas = 0, d_idle = 0, d_spin = 0, d_hub = 0, d_drop = 0, d_req = 0, d_heavy = 0;
   unsigned long long t_claim = 0, t_idle = 0, t_items = 0, t_hubs = 0, t_pub = 0, t_relax = 0;
-  unsigned long long t_pull = 0, d_psearch = 0;
+  unsigned long long t_pull = 0, d_psearch = 0, d_punreached = 0;
   // A workgroup beyond the policy's worker share is dormant: it sleeps and
   // reads only the end flag and the share, so it adds no traffic on the
   // control words that the claimers update (wave 0's copy is the one used).
@@ -1207,8 +1207,31 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
           if (has && key_v >= pkey) {
             const int rb = a.trow[v] + a.tsplit[v], re = a.trow[v + 1];
             int lo = rb;
-            if (key_v == W::kInfKey) {
+            // the in-edges that can still lower or tie v: pkey (+) w <= bound,
+            // bound = v's key.  An unreached vertex bounds its row by the best
+            // candidate of its first four heavy in-edges instead: that
+            // candidate is a real path length, and an edge with
+            // pkey (+) w > bound gives a candidate above it, which can
+            // neither lower nor tie the label the pass leaves (its first
+            // edge lies inside the bound, so the pass applies the candidate).
+            uint32_t bound = key_v;
+            if (key_v == W::kInfKey && rb < re) {
+              uint2 e4[4];
+#pragma unroll
+              for (int p = 0; p < 4; ++p) e4[p] = a.tedges[min(rb + p, re - 1)];
+              unsigned long long l4[4];
+#pragma unroll
+              for (int p = 0; p < 4; ++p)
+                l4[p] = __hip_atomic_load(a.labels + e4[p].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+              for (int p = 0; p < 4; ++p) {
+                const uint32_t ku = (uint32_t)(l4[p] >> 32);
+                if (rb + p < re && ku != W::kInfKey) bound = min(bound, W::relax(ku, e4[p].y));
+              }
+            }
+            if (bound == W::kInfKey) {
               lo = re;
+              DIAG(d_punreached += (unsigned long long)(re - rb));
             } else if (rb < re) {
               // the first four heavy in-edges in one round trip (most prefixes
               // end there; the weights ascend, so the reaching ones lead)
@@ -1216,14 +1239,15 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
               for (int p = 0; p < 4; ++p) w4[p] = a.tedges[min(rb + p, re - 1)].y;
 #pragma unroll
-              for (int p = 0; p < 4; ++p) lo += (rb + p < re && W::relax(pkey, w4[p]) <= key_v) ? 1 : 0;
+              for (int p = 0; p < 4; ++p) lo += (rb + p < re && W::relax(pkey, w4[p]) <= bound) ? 1 : 0;
               // all four reach: search the rest for the end
               int hi = lo == rb + 4 ? re : lo;
               while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
-                if (W::relax(pkey, a.tedges[mid].y) <= key_v) lo = mid + 1;
+                if (W::relax(pkey, a.tedges[mid].y) <= bound) lo = mid + 1;
                 else hi = mid;
               }
+              DIAG(if (key_v == W::kInfKey) d_punreached += (unsigned long long)(lo - rb));
             }
             beg = rb;
             len = lo - rb;
@@ -1436,6 +1460,7 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (d_heavy) atomicAdd(&c->diag[7], d_heavy);
     if (threadIdx.x == 0 && t_pull) atomicAdd(&c->pdiag[0], t_pull);
     if (d_psearch) atomicAdd(&c->pdiag[2], d_psearch);
+    if (d_punreached) atomicAdd(&c->pdiag[3], d_punreached);
   }
   if (lane == 0) {
     if (!a.live_edges && edges_local) atomicAdd(&c->edges, edges_local);
@@ -1613,12 +1638,12 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
                  "adds: %.3f ms phases %u pulls %u items %llu edges %llu pushes %llu | claims %llu cas_fail %llu "
                  "idle %llu ring_wait %llu hub %llu dropped %llu requeued %llu heavy %llu\n"
                  "  per-workgroup ms: claim %.2f idle %.2f items %.2f hubs %.2f pulls %.2f (publish %.2f relax %.2f)"
-                 " | phase advances %.3f ms in total | pulled in-edges %llu searched vertices %llu\n",
+                 " | phase advances %.3f ms in total | pulled in-edges %llu (of unreached vertices %llu) searched vertices %llu\n",
                  res.device_ms, h.phases, h.pulls, h.items, h.edges, h.pushes, h.diag[0], h.diag[1], h.diag[2],
                  h.diag[3], h.diag[4], h.diag[5], h.diag[6], h.diag[7], h.diag[8] * 1e-5 / workers,
                  h.diag[9] * 1e-5 / workers, h.diag[10] * 1e-5 / workers, h.diag[11] * 1e-5 / workers,
                  h.pdiag[0] * 1e-5 / workers, h.diag[12] * 1e-5 / workers, h.diag[13] * 1e-5 / workers,
-                 h.diag[14] * 1e-5, h.pdiag[1], h.pdiag[2]);
+                 h.diag[14] * 1e-5, h.pdiag[1], h.pdiag[3], h.pdiag[2]);
   if (h.err & kErrRing)
     fail(MI355_ECAPACITY, a.ring_limit < cap ? "ADDS bucket overflow: a bucket exceeded the requested per-bucket capacity"
                                              : "ADDS bucket queue overflow");
