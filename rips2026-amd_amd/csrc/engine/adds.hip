@@ -98,6 +98,9 @@ static_assert(9 + kAddsSlots <= 64,
               "claim probe: lanes 0-3, the window's semaphores, the worker share, the pull words");
 // vertices per pull chunk (four passes of kAB)
 constexpr uint32_t kPullChunk = 4 * 512;
+// edge reads a pull charges an unreached vertex: four probed in-edges, their
+// labels, and the prefix their best candidate bounds
+constexpr float kPullUnreachedReads = 10.0f;
 constexpr int kEmptyEntry = -1;     // ring sentinel: reserved, not yet written
 constexpr uint32_t kPushStage = 512;  // staged pushes per wave
 constexpr uint32_t kHeavyStage = 256; // staged heavy-list entries per wave
@@ -592,11 +595,12 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   // pulled by the vertices they could still improve when the estimated push
   // work exceeds 2^margin x the estimated pull work.  Push work: the list's
   // length times the mean heavy degree of 256 entries spread over it.  Pull
-  // work: the edges of the vertices not reached yet (read whole; n / 256 times
-  // the degrees of the unreached ones among 256 evenly spaced vertices), an
-  // edge read's worth per vertex (its label and, at or above the bucket, a row
-  // search: counted at an eighth, the rule pulled buckets of routing grids
-  // whose pushes were far cheaper) and about two row reads per queued vertex.
+  // work, over the pull range [0, pull_n): kPullUnreachedReads edge reads per
+  // vertex not reached yet (its first four heavy in-edges, their labels and
+  // the prefix they bound; pull_n / 256 times the unreached ones with edges
+  // among 256 evenly spaced vertices), an edge read's worth per vertex (its
+  // label and, at or above the bucket, a row search) and about two row reads
+  // per queued vertex.
   // The bucket's claims are all finished (the base moved), so
   // its list is complete; nothing of it is claimed before `decided`.  Decided
   // first, before the policy: the heavy list waits for it.
@@ -632,7 +636,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
             heavy += (float)(a.rowptr[u + 1] - a.rowptr[u] - a.split[u]);
         }
         const uint32_t v = (uint32_t)((((unsigned long long)(2u * i + 1u)) * a.pull_n) / 512u);
-        if ((uint32_t)(ld_rlx(a.labels + v) >> 32) == W::kInfKey) unreached += (float)(a.rowptr[v + 1] - a.rowptr[v]);
+        if ((uint32_t)(ld_rlx(a.labels + v) >> 32) == W::kInfKey && a.rowptr[v + 1] > a.rowptr[v]) unreached += 1.0f;
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
@@ -640,7 +644,8 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
         unreached += __shfl_xor(unreached, off, 64);
       }
       const float push_work = (float)count * heavy / (float)ns;
-      const float pull_work = unreached * ((float)a.pull_n / 256.0f) + (float)a.pull_n + 2.0f * (float)window_items;
+      const float pull_work = kPullUnreachedReads * unreached * ((float)a.pull_n / 256.0f) + (float)a.pull_n +
+                              2.0f * (float)window_items;
       pulled = push_work > margin * pull_work;
     }
     if (lane == 0) {
