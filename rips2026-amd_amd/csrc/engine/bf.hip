@@ -53,7 +53,10 @@ __global__ void k_init_bf(unsigned long long* __restrict__ labels, int64_t n, ui
 
 // Output element x (input numbering) comes from internal vertex inv[x]
 // (inv == nullptr: the same id); parents are stored as input ids.  src is the
-// internal id of the source.
+// internal id of the source.  The reached counts walk the internal ids in
+// order (label and row offsets of internal vertex x: coalesced reads; every
+// vertex is counted once either way), not inv[x] (two more random lines per
+// vertex).
 template <class W>
 __global__ void k_labels_to_output(const unsigned long long* __restrict__ labels, int64_t n,
                                    uint32_t src, int inf_mode, uint32_t* __restrict__ dist,
@@ -79,9 +82,12 @@ __global__ void k_labels_to_output(const unsigned long long* __restrict__ labels
       const uint32_t p = (uint32_t)l;
       pred[x] = (v == (int64_t)src || !reached || p == kNoPred) ? -1 : (int32_t)p;
     }
-    if (reached) {
-      ++r;
-      e += (unsigned long long)(rowptr[v + 1] - rowptr[v]);
+    if (reach) {
+      const bool rx = inv ? (uint32_t)(labels[x] >> 32) != W::kInfKey : reached;
+      if (rx) {
+        ++r;
+        e += (unsigned long long)(rowptr[x + 1] - rowptr[x]);
+      }
     }
   }
   if (!reach) return;

@@ -520,8 +520,87 @@ PULL_WIDTHS = (0.125, 0.25, 0.5, 1.0)
 PULL_LIGHTS = (0.25, 0.125, 0.0625)
 
 
+# the static search on grid graphs (high diameter, no bucket ever pulled):
+# width x active buckets at a quarter of the workgroups, then the share at the
+# best pair, then the light width
+GRID_WIDTHS = (1.0, 2.0, 3.0, 4.0)
+GRID_ACTIVE = (1, 2, 4)
+GRID_SHARES = (0.5, 0.25, 0.125)
+GRID_LIGHTS = (1.0, 0.5)
+
+
+def collect_grid_static(specs, teacher: "DeltaModel", log=print, reps: int = 2, max_records: int = 24,
+                        time_budget_s: float | None = None):
+    """Targets of all four outputs on grid graphs from the current kernel: on
+    each graph, static settings (width x delta0, active buckets, worker share,
+    light width; pulls on, as shipped) are timed (GRID_WIDTHS x GRID_ACTIVE at
+    a quarter of the workgroups, then GRID_SHARES at the best pair, then
+    GRID_LIGHTS), the fastest becomes the target of every record of one traced
+    run of that setting and of the first-phase record.  Returns (x, y, w,
+    metas) as collect_light."""
+    from .device import DeviceGraph
+
+    X, Y, W, metas = [], [], [], []
+    t0 = time.time()
+    teacher.install()
+    for i, spec in enumerate(specs):
+        if time_budget_s is not None and time.time() - t0 > time_budget_s:
+            log(f"grid static: time budget reached after {i} graphs")
+            break
+        g = make_graph(spec)
+        dg = DeviceGraph(g)
+        src = int(g.pick_sources(1, seed=spec["seed"] & 0xFFFFFFFF)[0])
+        delta0 = dg.sssp_labels(src, dg.options(ALG_ADDS)).initial_delta
+
+        def run(sc, act, sh, lt):
+            o = dg.options(ALG_ADDS, delta=sc * delta0, active_buckets=act, worker_blocks=workers_for(sh),
+                           pull=1, light_scale=lt)
+            return min(dg.sssp_labels(src, o).kernel_ms for _ in range(reps))
+
+        costs = {(sc, act, 0.25, 1.0): run(sc, act, 0.25, 1.0) for sc in GRID_WIDTHS for act in GRID_ACTIVE}
+        sc, act, _, _ = min(costs, key=costs.get)
+        for sh in GRID_SHARES:
+            if (sc, act, sh, 1.0) not in costs:
+                costs[(sc, act, sh, 1.0)] = run(sc, act, sh, 1.0)
+        sc, act, sh, _ = min(costs, key=costs.get)
+        for lt in GRID_LIGHTS:
+            if (sc, act, sh, lt) not in costs:
+                costs[(sc, act, sh, lt)] = run(sc, act, sh, lt)
+        best = min(costs, key=costs.get)
+        o = dg.options(ALG_ADDS, use_mlp=1, pull=1)
+        learned = min(dg.sssp_labels(src, o).kernel_ms for _ in range(reps))
+        plain = min(dg.sssp_labels(src, dg.options(ALG_ADDS)).kernel_ms for _ in range(reps))
+        rec, _ = trace(dg, src, delta=best[0] * delta0, active_buckets=best[1],
+                       worker_blocks=workers_for(best[2]), pull=1, light_scale=best[3])
+        if len(rec) > max_records:
+            rec = rec[np.linspace(0, len(rec) - 1, max_records).round().astype(int)]
+        x = np.ascontiguousarray(rec[:, :N_IN])
+        x = np.vstack([first_phase_features(x[0, :N_GRAPH]), x])
+        y = np.repeat(np.array([[math.log2(best[0]), float(best[1]), math.log2(best[2]), math.log2(best[3])]],
+                               np.float32), len(x), axis=0)
+        w = np.full(len(x), 0.5 / max(len(x) - 1, 1), np.float32)
+        w[0] = 0.5
+        meta = dict(spec, n=g.n, m=g.m, source=src, best=list(best), best_ms=costs[best],
+                    teacher_ms=learned, plain_ms=plain, records=int(len(x)), light=best[3],
+                    light_costs={"1.0": learned})
+        X.append(x)
+        Y.append(y)
+        W.append(w)
+        metas.append(meta)
+        if i % 10 == 0 or g.n > 2_000_000:
+            log(f"grid static {i + 1}/{len(specs)} {spec['kind']} n={g.n} best={best} {costs[best]:.3f} ms "
+                f"(network {learned:.3f}, plain {plain:.3f}) ({time.time() - t0:.0f}s)")
+        del dg, g
+    x = np.concatenate(X) if X else np.zeros((0, N_IN), np.float32)
+    y = np.concatenate(Y) if Y else np.zeros((0, N_OUT), np.float32)
+    w = np.concatenate(W) if W else np.zeros(0, np.float32)
+    if len(w):
+        w *= len(w) / w.sum()
+    return x, y, w, metas
+
+
 def collect_pulled(specs, teacher: "DeltaModel", log=print, reps: int = 2, max_records: int = 24,
-                   time_budget_s: float | None = None):
+                   time_budget_s: float | None = None, grid_static: bool = False):
     """Targets for the shipped configuration (pulls on) where it differs from
     the push-only training: on each R-MAT graph, static settings (width x
     delta0, active buckets, worker share, light width) are timed with pulls
@@ -530,7 +609,8 @@ def collect_pulled(specs, teacher: "DeltaModel", log=print, reps: int = 2, max_r
     becomes the target of all four outputs for every record of the graph (and
     the first-phase record), and the records come from one traced run of that
     setting.  Other graphs go through collect_light (the teacher's decisions
-    kept, the light width searched).  Returns (x, y, w, metas) as collect_light."""
+    kept, the light width searched), or with grid_static through
+    collect_grid_static.  Returns (x, y, w, metas) as collect_light."""
     from .device import DeviceGraph
 
     rmat = [sp for sp in specs if sp["kind"] == "rmat"]
@@ -583,7 +663,8 @@ def collect_pulled(specs, teacher: "DeltaModel", log=print, reps: int = 2, max_r
             log(f"pulled {i + 1}/{len(rmat)} scale {spec['scale']} best={best} {costs[best]:.3f} ms "
                 f"(network {learned:.3f}, plain {plain:.3f}) ({time.time() - t0:.0f}s)")
         del dg, g
-    xo, yo, wo, mo = collect_light(other, teacher, log=log, reps=reps, max_records=max_records,
+    collect_other = collect_grid_static if grid_static else collect_light
+    xo, yo, wo, mo = collect_other(other, teacher, log=log, reps=reps, max_records=max_records,
                                    time_budget_s=None if time_budget_s is None else
                                    max(0.0, time_budget_s - (time.time() - t0)))
     x = np.concatenate(X + [xo]) if X else xo

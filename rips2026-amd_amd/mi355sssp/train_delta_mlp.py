@@ -73,6 +73,12 @@ def main() -> int:
     ap.add_argument("--pulled", action="store_true",
                     help="with --light-from: on R-MAT graphs search static settings with pulls "
                          "(learn.collect_pulled) and train all four outputs there")
+    ap.add_argument("--grid-static", action="store_true",
+                    help="with --pulled: on grid graphs too, search static settings "
+                         "(learn.collect_grid_static) and train all four outputs there")
+    ap.add_argument("--rmat-teacher", action="store_true",
+                    help="light stage: the records of R-MAT graphs target the teacher's own outputs "
+                         "(its decisions kept there; the new targets apply to the other graphs)")
     ap.add_argument("--light-epochs", type=int, default=40)
     ap.add_argument("--light-lr", type=float, default=1e-3)
     ap.add_argument("--graph-weight", choices=["class", "time", "sqrt-time", "equal"], default="class",
@@ -199,13 +205,22 @@ def light_stage(args) -> int:
         if args.large_grids > 0:
             specs += L.large_grid_specs(args.large_grids, args.seed)
         t0 = time.time()
-        collect = L.collect_pulled if args.pulled else L.collect_light
-        x, y, w, metas = collect(specs, teacher, log=log, reps=args.reps, time_budget_s=args.budget_s)
+        if args.pulled:
+            x, y, w, metas = L.collect_pulled(specs, teacher, log=log, reps=args.reps, time_budget_s=args.budget_s,
+                                              grid_static=args.grid_static)
+        else:
+            x, y, w, metas = L.collect_light(specs, teacher, log=log, reps=args.reps, time_budget_s=args.budget_s)
         log(f"light: {len(metas)} graphs, {len(x)} records in {time.time() - t0:.0f}s")
         if args.shard_out:
             np.savez_compressed(args.shard_out, x=x, y=y, w=w, meta=np.array(json.dumps(metas)))
     if args.out is None or len(x) == 0:
         return 0
+    if args.rmat_teacher:
+        kinds = np.repeat(np.array([m["kind"] for m in metas]), [m["records"] for m in metas])
+        keep = kinds == "rmat"
+        y = y.copy()
+        y[keep] = teacher.forward(x[keep])
+        log(f"light: {int(keep.sum())} R-MAT records target the teacher's outputs")
     if args.graph_weight == "class":
         w = L.class_weights(w, metas)
     w = (w * (len(w) / w.sum())).astype(np.float32)
@@ -225,11 +240,14 @@ def light_stage(args) -> int:
         "mean_abs_dev_teacher_outputs": float(np.abs(pred[:, :3] - y[:, :3]).mean()),
         "oracle_light_gain_mean": float(np.mean([m["light_costs"]["1.0"] / min(m["light_costs"].values())
                                                  for m in metas if "best" not in m] or [1.0])),
-        "pulled_graphs": int(sum(1 for m in metas if "best" in m)),
-        "pulled_static_over_teacher_mean": float(np.mean([m["teacher_ms"] / m["best_ms"]
-                                                          for m in metas if "best" in m] or [1.0])),
-        "pulled_static_over_plain_mean": float(np.mean([m["plain_ms"] / m["best_ms"]
-                                                        for m in metas if "best" in m] or [1.0])),
+        "pulled_graphs": int(sum(1 for m in metas if "best" in m and m["kind"] == "rmat")),
+        "grid_static_graphs": int(sum(1 for m in metas if "best" in m and m["kind"] != "rmat")),
+        "grid_static_over_teacher_mean": float(np.mean([m["teacher_ms"] / m["best_ms"] for m in metas
+                                                        if "best" in m and m["kind"] != "rmat"] or [1.0])),
+        "pulled_static_over_teacher_mean": float(np.mean([m["teacher_ms"] / m["best_ms"] for m in metas
+                                                          if "best" in m and m["kind"] == "rmat"] or [1.0])),
+        "pulled_static_over_plain_mean": float(np.mean([m["plain_ms"] / m["best_ms"] for m in metas
+                                                        if "best" in m and m["kind"] == "rmat"] or [1.0])),
         "network_light_choice_mean": float(chosen.mean()),
     }
     model.save(args.out, meta=summary)
