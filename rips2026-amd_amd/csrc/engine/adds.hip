@@ -119,6 +119,7 @@ constexpr uint32_t kErrRing = 1u, kErrHub = 2u, kErrStall = 4u;
 #define HV_HEAD(c, s) ((c)->hv_head[(s) * kPad])
 #define HV_TAIL(c, s) ((c)->hv_tail[(s) * kPad])
 #define HV_AVAIL(c, s) ((c)->hv_avail[(s) * kPad])
+#define HV_MIN(c, s) ((c)->hv_min[(s) * kPad])
 
 // Wait until this wave's outstanding vector memory operations have completed
 // (s_waitcnt vmcnt(0)).  Device-scope atomics, and loads and stores with the
@@ -607,18 +608,25 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
   bool pulled = false;
   if constexpr (P) {
     const uint32_t fs = base % kAddsSlots;
-    uint32_t lh = 0, lt = 0, mode_on = 0;
+    uint32_t lh = 0, lt = 0, mode_on = 0, key_min = kNoStamp;
     float margin = 0.0f;
     if (lane == 0) {
       lh = ld_rlx(&HV_HEAD(c, fs));
       lt = ld_rlx(&HV_TAIL(c, fs));
       mode_on = ld_rlx(&c->pull_mode);
       margin = exp2f(ld_rlx(&c->pull_margin));
+      // the finished bucket's light passes are complete (each claim waited
+      // for its updates before releasing inflight); the slot is next used by
+      // bucket base + kAddsSlots, whose items are processed only once it is
+      // active, long after this reset
+      key_min = ld_rlx(&HV_MIN(c, fs));
+      st_rlx(&HV_MIN(c, fs), kNoStamp);
     }
     lh = __shfl(lh, 0, 64);
     lt = __shfl(lt, 0, 64);
     mode_on = __shfl(mode_on, 0, 64);
     margin = __shfl(margin, 0, 64);
+    key_min = __shfl(key_min, 0, 64);
     const uint32_t count = lt - lh;
     // the pull estimate is at least n: no sampling when even rows of the
     // largest degree could not outweigh it
@@ -646,7 +654,7 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
       const float push_work = (float)count * heavy / (float)ns;
       const float pull_work = kPullUnreachedReads * unreached * ((float)a.pull_n / 256.0f) + (float)a.pull_n +
                               2.0f * (float)window_items;
-      pulled = push_work > margin * pull_work;
+      pulled = push_work > margin * pull_work && key_min != kNoStamp;
     }
     if (lane == 0) {
       if (pulled) {
@@ -655,8 +663,8 @@ __device__ bool try_advance(const AddsArgs& a, unsigned long long bw) {
         // once every list entry's key is folded into pull_key
         const uint32_t chunks = (a.pull_n + kPullChunk - 1) / kPullChunk;
         st_rlx(&c->pull_count, count);
-        st_rlx(&c->pull_key, kNoStamp);
-        st_rlx(&c->pull_keyed, 0u);
+        st_rlx(&c->pull_key, key_min);
+        st_rlx(&c->pull_keyed, count);
         st_rlx(&c->pull_chunks, chunks);
         st_rlx(&c->pull_head, 0u);
         st_rlx(&c->pull_done, 0u);
@@ -1342,7 +1350,6 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       // row bounds and light-prefix length of u depend only on u: issued now,
       // they arrive while the stamp exchange and the label read complete
       int r_beg = 0, r_end = 0, r_split = 0, pu = u;
-      uint32_t drop_key = kNoStamp;  // a dropped entry's current key (pull bound)
       if (has && entry_ok) {
         r_beg = a.rowptr[u];
         r_end = a.rowptr[u + 1];
@@ -1352,8 +1359,8 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       if (has && entry_ok) {
         if (drop) {
           // a dropped entry keeps its list stamp (bucket base - 1, finished:
-          // no later pass compares equal to it); its key joins the pull bound
-          drop_key = atomic_load_key(a.labels + u);
+          // no later pass compares equal to it); the pull bound came from the
+          // bucket's light passes
         } else if (heavy) {
           // Clearing the stamp first means a later light pass of u (after an
           // improvement) lists u again, or this pass reads the label it set.
@@ -1404,6 +1411,18 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
           }
         }
       }
+      if constexpr (kPull) {
+        // the bucket's pull bound: the smallest key with which a vertex that
+        // has heavy edges is processed here (whether newly listed or already
+        // on the list: a listed vertex improved later is processed again,
+        // here or in a later bucket that lists it)
+        if (!heavy) {
+          uint32_t km = (key != W::kInfKey && r_split < r_end - r_beg) ? key : kNoStamp;
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) km = min(km, (uint32_t)__shfl_xor(km, off, 64));
+          if (lane == 0 && km != kNoStamp) atomicMin(&HV_MIN(c, held_bucket % kAddsSlots), km);
+        }
+      }
       // heavy-list entries of this claim's bucket (list slot = bucket slot)
       {
         const unsigned long long hm = __ballot(to_heavy);
@@ -1413,16 +1432,6 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
           if (to_heavy) s_hv[wv][hv_n + prefix_count(hm)] = u;
           hv_n += k;
         }
-      }
-      if (drop) {
-        // the pull bound: every heavy edge the list would have pushed starts
-        // at one of these keys or higher (a vertex improved since is processed
-        // again in a later bucket)
-        uint32_t k = drop_key;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) k = min(k, (uint32_t)__shfl_xor(k, off, 64));
-        if (lane == 0 && k != kNoStamp) atomicMin(&c->pull_key, k);
-        vm_wait();
       }
       // re-queued items go through the stage like any push
       push_improved<kPull>(a, st, requeue, (uint32_t)u, want, pushed_local);
@@ -1439,8 +1448,6 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
     vm_wait();  // every wave's publishes complete before the release
     __syncthreads();
     if (threadIdx.x == 0) {
-      // every wave's fold has completed (each waited after its atomicMin)
-      if (drop) atomicAdd(&c->pull_keyed, count);
       atomic_add_done(&INFL(c, slot), -1);
       atomicAdd(&c->pending, 0ull - (unsigned long long)count);
     }
@@ -1546,6 +1553,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   h.pending = 1;
   h.tail[0] = 1;
   h.avail[0] = 1;
+  for (int k = 0; k < kAddsSlots; ++k) h.hv_min[k * kPad] = kNoStamp;
   AddsArgs a{};
   a.rowptr = g.rowptr.p;
   a.edges = g.edges.p;

@@ -78,10 +78,11 @@ struct AddsCtrl {
   uint32_t pad10[kPad - 1];
   uint32_t pull_done;            // finished chunks
   uint32_t pad11[kPad - 1];
-  // smallest current key over the pulled bucket's heavy list, folded in by
-  // the claims that drop its entries; the chunks start once all are in
+  // the pulled bucket's bound: every heavy edge its list would have pushed
+  // starts at this key or higher (the bucket's hv_min at the boundary); the
+  // chunks start once pull_keyed reaches pull_count (set together with it)
   uint32_t pull_key;
-  uint32_t pull_keyed;           // entries folded in
+  uint32_t pull_keyed;
   uint32_t pad12[kPad - 2];
   uint32_t head[kAddsSlots * kPad];  // pop counter of slot s at [s * kPad] (monotone)
   uint32_t tail[kAddsSlots * kPad];  // push counter of slot s
@@ -92,6 +93,10 @@ struct AddsCtrl {
   uint32_t hv_head[kAddsSlots * kPad];
   uint32_t hv_tail[kAddsSlots * kPad];
   int hv_avail[kAddsSlots * kPad];
+  // smallest key with which a vertex that has heavy edges was processed in
+  // the light pass of bucket b (slot b % kAddsSlots; 0xFFFFFFFF: none), kept
+  // by the pull kernel and reset when b is finished
+  uint32_t hv_min[kAddsSlots * kPad];
 };
 
 // One 2048-edge piece of a high-degree vertex.
