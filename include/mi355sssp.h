@@ -196,7 +196,11 @@ int mi355_sssp_labels(mi355_graph* g, int32_t source, const mi355_sssp_options* 
                       mi355_sssp_stats* stats, void* stream);
 int mi355_sssp_labels_to_output(mi355_graph* g, int32_t source, int32_t inf_mode, void* d_dist,
                                 int32_t* d_pred, void* stream);
-/* Record per-round frontier sizes of the next BF / BF_JACOBI run (host). */
+/* Record per-round frontier sizes of the next BF / BF_JACOBI run on g: that
+ * run writes |F_0| .. |F_rounds| (at most cap entries) into the host array
+ * `sizes`, which must stay valid until it returns; the registration is used
+ * by that one run.  No counterpart in the reference (a diagnostic of this
+ * engine's frontier rounds). */
 int mi355_sssp_round_log(mi355_graph* g, int32_t* sizes, int32_t cap);
 
 /* ---- reference entry points ------------------------------------------- */
