@@ -72,6 +72,7 @@
 //    invalidates the whole L2 and made the worklist 10x slower.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -85,6 +86,9 @@ namespace mi355 {
 namespace {
 
 constexpr uint32_t kNoStamp = 0xFFFFFFFFu;
+// the control block's header (everything before the slot arrays): what the
+// host uploads before a run and reads back after it
+constexpr size_t kCtrlHeader = offsetof(AddsCtrl, head);
 // An ADDS workgroup is 8 waves: its claims are shared by more waves (more
 // memory accesses in flight per claimer) while the number of claimers, which
 // contend on the worklist's control words, stays that of 4-wave workgroups.
@@ -1481,6 +1485,29 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (items_local) atomicAdd(&c->items, items_local);
 }
 
+// The light width of a run under the delta network: its first bucket width
+// (c->delta after k_adds_first_phase; `fallback` when not a positive finite
+// number) times the caller's light_scale, or the network's light width when
+// the caller set none (outside (0, 1]: 1).  sel[0] / sel[2]: the width each
+// split (out-rows / transposed in-rows) holds; sel[1] / sel[3] become nonzero
+// when that split must be recounted, and the held width is updated (the
+// transposed one only when `transposed`).  The recount reads the width from
+// sel[0] (sel[2]), which this kernel leaves equal to the new width.
+__global__ void k_split_width(const AddsCtrl* __restrict__ c, float fallback, float req_light, int transposed,
+                              uint32_t* __restrict__ sel) {
+  float net_light = c->light_scale;
+  if (!(net_light > 0.0f && net_light <= 1.0f)) net_light = 1.0f;
+  const float first = c->delta;
+  const float width = (first > 0.0f && isfinite(first) ? first : fallback) * (req_light > 0.0f ? req_light : net_light);
+  const uint32_t bits = __float_as_uint(width);
+  sel[1] = sel[0] != bits ? 1u : 0u;
+  sel[0] = bits;
+  if (transposed) {
+    sel[3] = sel[2] != bits ? 1u : 0u;
+    sel[2] = bits;
+  }
+}
+
 template <class W>
 __global__ void k_adds_init(AddsArgs a, int64_t n, uint32_t src, const uint32_t* __restrict__ init,
                             const AddsCtrl* __restrict__ init_ctrl) {
@@ -1495,9 +1522,21 @@ __global__ void k_adds_init(AddsArgs a, int64_t n, uint32_t src, const uint32_t*
     a.sstamp[v] = 0u;
   }
   for (int64_t i = tid; i <= (int64_t)a.hub_mask; i += stride) a.hub[i].seq = 0u;
+  // the header comes from the host's copy; the slot arrays start empty (slot
+  // 0 holding the source: tail and avail 1) with no pull bound
   const uint32_t* src_words = reinterpret_cast<const uint32_t*>(init_ctrl);
   uint32_t* dst_words = reinterpret_cast<uint32_t*>(a.c);
-  for (int64_t i = tid; i < (int64_t)(sizeof(AddsCtrl) / 4); i += stride) dst_words[i] = src_words[i];
+  constexpr int64_t kHdr = (int64_t)(kCtrlHeader / 4);
+  constexpr int64_t kTail0 = (int64_t)(offsetof(AddsCtrl, tail) / 4);
+  constexpr int64_t kAvail0 = (int64_t)(offsetof(AddsCtrl, avail) / 4);
+  constexpr int64_t kMin0 = (int64_t)(offsetof(AddsCtrl, hv_min) / 4);
+  for (int64_t i = tid; i < (int64_t)(sizeof(AddsCtrl) / 4); i += stride) {
+    uint32_t wv = 0u;
+    if (i < kHdr) wv = src_words[i];
+    else if (i == kTail0 || i == kAvail0) wv = 1u;
+    else if (i >= kMin0 && (i - kMin0) % kPad == 0) wv = kNoStamp;
+    dst_words[i] = wv;
+  }
   if (tid == 0) a.ring[0] = (int)src;
 }
 
@@ -1551,9 +1590,8 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   h.workers = (uint32_t)workers;
   h.active_workers = (uint32_t)workers;
   h.pending = 1;
-  h.tail[0] = 1;
-  h.avail[0] = 1;
-  for (int k = 0; k < kAddsSlots; ++k) h.hv_min[k * kPad] = kNoStamp;
+  // the slot arrays (tail[0] = avail[0] = 1 for the source, hv_min empty) are
+  // set by k_adds_init; only the header travels
   AddsArgs a{};
   a.rowptr = g.rowptr.p;
   a.edges = g.edges.p;
@@ -1589,7 +1627,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   a.n = (uint32_t)n;
   a.live_edges = req.trace ? 1 : 0;
   a.claim_limit = 16ull * (unsigned long long)(n + g.m) + (1ull << 20);
-  MI_HIP(hipMemcpyAsync(c_init, hbuf.data(), sizeof(AddsCtrl), hipMemcpyHostToDevice, s));
+  MI_HIP(hipMemcpyAsync(c_init, hbuf.data(), kCtrlHeader, hipMemcpyHostToDevice, s));
   if (!ws.adds_ring_clean) {
     // every entry starts as the empty sentinel; a completed run leaves the
     // ring in that state again (all pushed entries are consumed)
@@ -1615,17 +1653,38 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
     // network also scales it (its light width output), unless the caller
     // fixed light_scale.  The split is cached per width, so runs on one graph
     // compute it once.
-    float first_delta = delta, net_light = 1.0f;
-    MI_HIP(hipMemcpyAsync(&first_delta, &c->delta, sizeof(float), hipMemcpyDeviceToHost, s));
-    MI_HIP(hipMemcpyAsync(&net_light, &c->light_scale, sizeof(float), hipMemcpyDeviceToHost, s));
-    MI_HIP(hipStreamSynchronize(s));
-    if (!(net_light > 0.0f && net_light <= 1.0f)) net_light = 1.0f;
-    split_delta = (first_delta > 0.0f && std::isfinite(first_delta) ? first_delta : delta) *
-                  (req.light_scale > 0.0f ? req.light_scale : net_light);
-    a.split = light_split(g, split_delta, s);
+    // The width is chosen on the device (k_split_width) and the splits are
+    // recounted only when it changed, so the host does not wait for the
+    // first phase.
+    ws.mlp_split.ensure((size_t)n);
+    ws.mlp_sel.ensure(4);
+    if (!ws.mlp_sel_ready) {
+      MI_HIP(hipMemsetAsync(ws.mlp_sel.p, 0xFF, 4 * sizeof(uint32_t), s));  // no width held yet
+      ws.mlp_sel_ready = true;
+    }
+    const bool transposed = a.trow && g.pull_state == 2;
+    if (transposed) ws.mlp_tsplit.ensure((size_t)n);
+    k_split_width<<<1, 1, 0, s>>>(c, delta, req.light_scale, transposed ? 1 : 0, ws.mlp_sel.p);
+    MI_LAUNCH_CHECK();
+    const unsigned lgrid = std::min<unsigned>(div_up((uint64_t)n, kBlock), 2048u);
+    k_light_count_sel<W><<<lgrid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, n, ws.mlp_sel.p, ws.mlp_split.p);
+    MI_LAUNCH_CHECK();
+    a.split = ws.mlp_split.p;
+    if (a.trow) {
+      // the pull reads the heavy in-edges at the same split width
+      if (transposed) {
+        k_light_count_sel<W><<<lgrid, kBlock, 0, s>>>(g.t_rowptr.p, g.t_edges.p, n, ws.mlp_sel.p + 2,
+                                                       ws.mlp_tsplit.p);
+        MI_LAUNCH_CHECK();
+        a.tsplit = ws.mlp_tsplit.p;
+      } else {
+        a.tsplit = ws.mlp_split.p;
+      }
+    }
+  } else if (a.trow) {
+    // the pull reads the heavy in-edges at the same split width
+    a.tsplit = pull_light_split(g, split_delta, s);
   }
-  // the pull reads the heavy in-edges at the same split width
-  if (a.trow) a.tsplit = pull_light_split(g, split_delta, s);
   MI_HIP(hipEventRecord(evk, s));
   const bool dbg = std::getenv("MI355_ADDS_DEBUG") != nullptr;
   if (a.trow) {
@@ -1643,7 +1702,7 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   (void)hipEventDestroy(evk);
-  MI_HIP(hipMemcpyAsync(hbuf.data(), c, sizeof(AddsCtrl), hipMemcpyDeviceToHost, s));
+  MI_HIP(hipMemcpyAsync(hbuf.data(), c, kCtrlHeader, hipMemcpyDeviceToHost, s));
   MI_HIP(hipStreamSynchronize(s));
   if (h.err) ws.adds_ring_clean = false;
   if (std::getenv("MI355_ADDS_DEBUG"))

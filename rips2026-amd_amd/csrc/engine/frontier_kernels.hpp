@@ -229,6 +229,26 @@ __global__ void k_light_count(const int* __restrict__ rowptr, const uint2* __res
   }
 }
 
+// The same split at a width chosen on the device: sel[0] = its float bits,
+// sel[1] = nonzero when `split` does not hold them yet (else nothing to do).
+template <class W>
+__global__ void k_light_count_sel(const int* __restrict__ rowptr, const uint2* __restrict__ edges,
+                                  int64_t n, const uint32_t* __restrict__ sel, int* __restrict__ split) {
+  if (sel[1] == 0u) return;
+  const uint32_t delta_bits = sel[0];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const int beg = rowptr[v];
+    int lo = beg, hi = rowptr[v + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (edge_selected<W>(kLightEdges, edges[mid].y, delta_bits)) lo = mid + 1;
+      else hi = mid;
+    }
+    split[v] = lo - beg;
+  }
+}
+
 // Jacobi snapshot: keys of the next frontier as they stand after the round.
 __global__ void k_snapshot_keys(const int* __restrict__ q, const uint32_t* __restrict__ cnt,
                                 const unsigned long long* __restrict__ labels,

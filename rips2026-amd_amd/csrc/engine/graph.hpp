@@ -49,6 +49,11 @@ struct Workspace {
   DevBuf<uint32_t> adds_qstamp;       // n queue stamps, then n heavy-list stamps
   DevBuf<HubChunk> adds_hub;
   DevBuf<unsigned char> adds_ctrl_mem;
+  // light splits at the delta network's first width, chosen on the device
+  // (adds.hip): out-rows, transposed in-rows, and their selection words
+  DevBuf<int> mlp_split, mlp_tsplit;
+  DevBuf<uint32_t> mlp_sel;
+  bool mlp_sel_ready = false;
   DevBuf<AddsPolicy> adds_policy;
   // batched multi-source queries (batch.hip)
   DevBuf<unsigned long long> batch_labels;
