@@ -1696,6 +1696,11 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   }
   MI_LAUNCH_CHECK();
   MI_HIP(hipEventRecord(ev1, s));
+  if (req.out_dist || req.out_pred || req.out_reach) {
+    // the caller's output conversion runs right behind the kernel
+    labels_to_output_internal(g, src, req.out_inf_mode, req.out_dist, req.out_pred, req.out_reach, s);
+    res.converted = true;
+  }
   MI_HIP(hipEventSynchronize(ev1));
   MI_HIP(hipEventElapsedTime(&res.device_ms, ev0, ev1));
   MI_HIP(hipEventElapsedTime(&res.kernel_ms, evk, ev1));

@@ -287,8 +287,12 @@ void run_bf_continue(mi355_graph& g, uint32_t input_source, const RunRequest& re
 
 void labels_to_output(mi355_graph& g, uint32_t source, int inf_mode, void* d_dist, int32_t* d_pred,
                       unsigned long long* reach, hipStream_t s) {
+  labels_to_output_internal(g, internal_vertex(g, source), inf_mode, d_dist, d_pred, reach, s);
+}
+
+void labels_to_output_internal(mi355_graph& g, uint32_t src, int inf_mode, void* d_dist, int32_t* d_pred,
+                               unsigned long long* reach, hipStream_t s) {
   const unsigned grid = std::min<unsigned>(div_up((uint64_t)g.n, kBlock), 4096u);
-  const uint32_t src = internal_vertex(g, source);
   const int* inv = g.relabeled ? g.inv.p : nullptr;
   if (g.weight_type == MI355_W_F32)
     k_labels_to_output<WF32><<<grid, kBlock, 0, s>>>(g.ws.labels.p, g.n, src, inf_mode,

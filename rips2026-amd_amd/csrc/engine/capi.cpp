@@ -238,15 +238,20 @@ int mi355_sssp(mi355_graph* g, int32_t source, const mi355_sssp_options* opt, vo
     MI_HIP(hipSetDevice(g->device));
     const hipStream_t s = as_stream(stream);
     mi355::RunResult res;
-    mi355::run_sssp(*g, (uint32_t)source, make_request(opt, stats != nullptr), res, s);
     unsigned long long* reach = nullptr;
     if (stats) {
       g->ws.reach.ensure(2);
       reach = g->ws.reach.p;
       MI_HIP(hipMemsetAsync(reach, 0, 16, s));
     }
-    mi355::labels_to_output(*g, (uint32_t)source, opt ? opt->inf_mode : MI355_INF_IEEE, d_dist,
-                            d_pred, reach, s);
+    mi355::RunRequest req = make_request(opt, stats != nullptr);
+    req.out_dist = d_dist;
+    req.out_pred = d_pred;
+    req.out_reach = reach;
+    req.out_inf_mode = opt ? opt->inf_mode : MI355_INF_IEEE;
+    mi355::run_sssp(*g, (uint32_t)source, req, res, s);
+    if (!res.converted)
+      mi355::labels_to_output(*g, (uint32_t)source, req.out_inf_mode, d_dist, d_pred, reach, s);
     unsigned long long hr[2] = {0, 0};
     if (stats) MI_HIP(hipMemcpyAsync(hr, reach, 16, hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));

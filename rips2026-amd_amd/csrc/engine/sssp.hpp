@@ -32,6 +32,15 @@ struct RunRequest {
   // light edges: weight <= light_scale x the first bucket width (0: 1, or the
   // network's choice under the MLP policy)
   float light_scale = 0.0f;
+  // Output conversion enqueued by the driver right behind its kernels, before
+  // it waits for them (ADDS does; the others leave it to the caller): labels
+  // to distances / parents in input order and the reached counts (any of the
+  // three may be null; all null: no conversion).  RunResult::converted says
+  // whether it was enqueued.
+  void* out_dist = nullptr;
+  int32_t* out_pred = nullptr;
+  unsigned long long* out_reach = nullptr;
+  int out_inf_mode = 0;
 };
 
 struct RunResult {
@@ -49,6 +58,7 @@ struct RunResult {
   int64_t pulls = 0;       // ADDS: finished buckets whose heavy edges were pulled
   bool negative_cycle = false;
   std::vector<uint32_t> frontier_log;  // BF: |F_r| for r = 0..rounds
+  bool converted = false;  // the driver enqueued the request's output conversion
 };
 
 // Runs SSSP leaving packed labels in g.ws.labels.
@@ -63,6 +73,9 @@ void run_bf_continue(mi355_graph& g, uint32_t source, const RunRequest& req, Run
 // the sum of their out-degrees when reach != nullptr (device: 2 x u64).
 void labels_to_output(mi355_graph& g, uint32_t source, int inf_mode, void* d_dist, int32_t* d_pred,
                       unsigned long long* reach, hipStream_t stream);
+// The same with the source given as its internal id.
+void labels_to_output_internal(mi355_graph& g, uint32_t src, int inf_mode, void* d_dist, int32_t* d_pred,
+                               unsigned long long* reach, hipStream_t stream);
 
 // Delta-stepping driver (delta.hip).
 // Streaming device-to-device copy (copy.hip).
