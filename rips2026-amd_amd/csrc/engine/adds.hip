@@ -75,6 +75,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "adds.hpp"
@@ -1564,7 +1565,11 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   AddsCtrl* c_init = c + 1;
   const DeviceProps p = device_props();
   const int workers = req.worker_blocks > 0 ? req.worker_blocks : p.cus * 2;
-  std::vector<unsigned char> hbuf(sizeof(AddsCtrl), 0);
+  // the host image of the control block: its header is filled here, uploaded,
+  // and read back after the run
+  std::vector<unsigned char>& hbuf = ws.adds_ctrl_host;
+  if (hbuf.size() != sizeof(AddsCtrl)) hbuf.assign(sizeof(AddsCtrl), 0);
+  else std::memset(hbuf.data(), 0, kCtrlHeader);
   AddsCtrl& h = *reinterpret_cast<AddsCtrl*>(hbuf.data());
   h.base_word = 0;  // bucket 0 starts at distance 0.0f
   h.delta = delta;
@@ -1635,10 +1640,9 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
     ws.adds_ring_clean = true;
   }
 
-  hipEvent_t ev0, ev1, evk;
-  MI_HIP(hipEventCreate(&ev0));
-  MI_HIP(hipEventCreate(&ev1));
-  MI_HIP(hipEventCreate(&evk));
+  for (hipEvent_t& e : ws.adds_ev)
+    if (!e) MI_HIP(hipEventCreate(&e));
+  hipEvent_t ev0 = ws.adds_ev[0], evk = ws.adds_ev[1], ev1 = ws.adds_ev[2];
   MI_HIP(hipEventRecord(ev0, s));
   k_adds_init<W><<<std::min<unsigned>(div_up((uint64_t)std::max<int64_t>(n, hub_cap), kBlock), 4096u),
                    kBlock, 0, s>>>(a, n, src, req.init_dist, c_init);
@@ -1704,9 +1708,6 @@ void run_adds_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResu
   MI_HIP(hipEventSynchronize(ev1));
   MI_HIP(hipEventElapsedTime(&res.device_ms, ev0, ev1));
   MI_HIP(hipEventElapsedTime(&res.kernel_ms, evk, ev1));
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
-  (void)hipEventDestroy(evk);
   MI_HIP(hipMemcpyAsync(hbuf.data(), c, kCtrlHeader, hipMemcpyDeviceToHost, s));
   MI_HIP(hipStreamSynchronize(s));
   if (h.err) ws.adds_ring_clean = false;

@@ -23,6 +23,13 @@ struct RoundCtrl {
 struct BucketCtrl;
 
 struct Workspace {
+  Workspace() = default;
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
+  ~Workspace() {
+    for (hipEvent_t& e : adds_ev)
+      if (e) (void)hipEventDestroy(e);
+  }
   DevBuf<unsigned long long> labels;  // n packed labels
   DevBuf<unsigned long long> reach;   // reached vertices / edges of the last output conversion
   DevBuf<uint32_t> stamp;             // n round stamps (frontier dedup)
@@ -55,6 +62,9 @@ struct Workspace {
   DevBuf<uint32_t> mlp_sel;
   bool mlp_sel_ready = false;
   DevBuf<AddsPolicy> adds_policy;
+  std::vector<unsigned char> adds_policy_host;  // what adds_policy holds (skip identical uploads)
+  std::vector<unsigned char> adds_ctrl_host;    // host image of the control block
+  hipEvent_t adds_ev[3] = {nullptr, nullptr, nullptr};  // run start, kernel start, kernel end
   // batched multi-source queries (batch.hip)
   DevBuf<unsigned long long> batch_labels;
   DevBuf<uint32_t> batch_stamp;   // per slot: round stamp, far stamp (interleaved)

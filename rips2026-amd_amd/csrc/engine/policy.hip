@@ -2,6 +2,7 @@
 // process-wide MLP weights, and the device policy object of a run.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -142,8 +143,15 @@ const AddsPolicy* prepare_adds_policy(mi355_graph& g, const RunRequest& req, hip
   h.trace_cap = req.trace ? req.trace_cap : 0u;
   *mode_out = mode;
   if (mode == kPolicyFixed && !h.trace) return nullptr;
+  // the same policy as the last upload (same model, graph and options): the
+  // device copy is current (the kernel only writes the trace buffer it points to)
+  std::vector<unsigned char>& last = g.ws.adds_policy_host;
+  const unsigned char* hb = reinterpret_cast<const unsigned char*>(&h);
+  if (g.ws.adds_policy.p && last.size() == sizeof(AddsPolicy) && std::memcmp(last.data(), hb, sizeof(AddsPolicy)) == 0)
+    return g.ws.adds_policy.p;
   g.ws.adds_policy.ensure(1);
   MI_HIP(hipMemcpyAsync(g.ws.adds_policy.p, &h, sizeof(AddsPolicy), hipMemcpyHostToDevice, s));
+  last.assign(hb, hb + sizeof(AddsPolicy));
   return g.ws.adds_policy.p;
 }
 
