@@ -144,3 +144,21 @@ def test_pull_under_the_learned_policy():
                 _check(g, dg, int(src), use_mlp=1, pull=1, pull_margin=ALWAYS)
     finally:
         L.clear_delta_model()
+
+
+def test_device_light_split_follows_the_width_across_runs():
+    # Under a device-side policy (here an explicit phase schedule) the light
+    # split is chosen on the device: recounted when the width changes, kept
+    # when it does not.  Alternating light widths on one handle exercises both
+    # (for a symmetric graph the split serves the out-rows and the pull's
+    # in-rows; for a directed one the transposed in-rows have their own).
+    L.set_schedule([(1.0, 1, 1.0), (0.5, 2, 0.5)])
+    try:
+        for g in (_digraph(4000, 48000, M.W_F32, seed=11), M.rmat(14, 16, seed=3)):
+            dg = M.DeviceGraph(g)
+            srcs = [int(s) for s in g.pick_sources(2, seed=4)]
+            for ls in (0.5, 0.25, 0.25, 1.0, 0.5):
+                for s in srcs:
+                    _check(g, dg, s, use_mlp=2, pull=1, pull_margin=ALWAYS, light_scale=ls)
+    finally:
+        L.set_schedule(None)
