@@ -320,6 +320,21 @@ def batched_gemm(args):
 
     once()
     eng.mi355_stream_sync(None)
+    tuning = None
+    if args.gemm_tune:
+        # tile per size class chosen on this batch (untimed): coordinate descent
+        # from the built-in rule, every class trying every tile variant
+        from mi355sssp import kernels as K
+
+        def run_sync():
+            once()
+            eng.mi355_stream_sync(None)
+
+        table, best_ms, static_ms, names = K.minplus_tune_batch(run_sync, ns)
+        classes = sorted({K.size_class(n) for n in ns})
+        tuning = {"tiles": {str(1 << c): (names[table[c]] if table[c] >= 0 else "built-in rule") for c in classes},
+                  "static_rule_ms": static_ms, "tuned_ms": best_ms}
+        log(f"min-plus batch tuning: {json.dumps(tuning)}")
     reps = 5
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -339,7 +354,8 @@ def batched_gemm(args):
     ceiling = 157.3 / 2 / 1.5 * 2
     res = {"matrices": cnt, "n_range": [min(ns), max(ns)], "ops": ops, "ms": dt * 1e3,
            "tops": ops / dt / 1e12, "valu_ceiling_tops": round(ceiling, 1),
-           "frac_of_ceiling": ops / dt / 1e12 / ceiling, "bit_exact_spot_check": bool(ok)}
+           "frac_of_ceiling": ops / dt / 1e12 / ceiling, "bit_exact_spot_check": bool(ok),
+           "tuning": tuning}
     log(f"batched min-plus GEMM: {json.dumps(res)}")
     return res
 
@@ -395,6 +411,8 @@ def main() -> int:
     ap.add_argument("--one-at-a-time", type=int, default=16,
                     help="queries sampled for the one-at-a-time plain ADDS comparison")
     ap.add_argument("--gemm", type=int, default=1, help="also time the batched min-plus GEMM workload")
+    ap.add_argument("--gemm-tune", type=int, default=1,
+                    help="choose the batch's tile per size class on the batch itself before timing it")
     ap.add_argument("--batch-graph", type=int, default=1,
                     help="0: batched queries as plain launches with fixed options, no tuning "
                          "(kernel-trace profiling runs)")

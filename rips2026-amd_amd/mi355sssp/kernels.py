@@ -3,6 +3,7 @@ points (ctypes over the C ABI)."""
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -61,6 +62,67 @@ def minplus_tuning(table=None):
         check(engine().mi355_minplus_tuning_set(t))
     check(engine().mi355_minplus_tuning_get(t))
     return list(t)
+
+
+def size_class(n: int) -> int:
+    """Tuning class of a product whose larger output side is n: ceil(log2(n))."""
+    c = 0
+    while (1 << c) < n and c < 31:
+        c += 1
+    return c
+
+
+def minplus_tune_batch(run, sizes, reps: int = 3, rounds: int = 2, margin: float = 0.01):
+    """Tile choice per size class for one batch of products, timed on the batch
+    itself (a product's best tile alone differs from its best among thousands
+    of tiles).  run() executes the batch once and waits for it; sizes are the
+    products' larger output sides.  Coordinate descent from the built-in rule:
+    each class present, largest first, tries every tile variant with the other
+    classes' choices held, keeping the fastest; `rounds` sweeps.  The built-in
+    rule is timed before and after (the GPU's clocks rise under the sustained
+    load, so a late candidate would otherwise look faster than an early one)
+    and kept unless a table beats its better time; a choice replaces the
+    current one only when faster by more than `margin` (timing noise does not
+    pick tiles).  Installs and returns (table, best_ms, static_ms, tile
+    names)."""
+    nt = engine().mi355_minplus_tile_count()
+    names = [engine().mi355_minplus_tile_name(i).decode() for i in range(nt)]
+    classes = sorted({size_class(int(n)) for n in sizes}, reverse=True)
+
+    def timed(table):
+        minplus_tuning(table)
+        run()  # the plan for this table
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            run()
+            best = min(best, (time.perf_counter() - t0) * 1e3)
+        return best
+
+    table = [-1] * 32
+    minplus_tuning(table)
+    for _ in range(60):  # clocks up before the first candidate is timed
+        run()
+    static_ms = best_ms = timed(table)
+    for _ in range(rounds):
+        changed = False
+        for c in classes:
+            for t in range(nt):
+                if table[c] == t:
+                    continue
+                trial = list(table)
+                trial[c] = t
+                ms = timed(trial)
+                if ms < best_ms * (1.0 - margin):
+                    best_ms, table, changed = ms, trial, True
+        if not changed:
+            break
+    # the built-in rule once more, now at the clocks the candidates ran at
+    static_ms = min(static_ms, timed([-1] * 32))
+    if static_ms * (1.0 - margin) <= best_ms:
+        table, best_ms = [-1] * 32, min(static_ms, best_ms)
+    minplus_tuning(table)
+    return table, best_ms, static_ms, names
 
 
 def spgemm(a_rowptr, a_col, a_val, a_cols, b_rowptr, b_col, b_val, b_cols, hash_capacity=512,

@@ -341,3 +341,41 @@ def test_minplus_signed_zero_ties():
     zero = want == 0.0
     assert zero[:8, :8].all() and zero.sum() == 64
     assert got[~zero].tobytes() == want[~zero].tobytes()
+
+
+@pytest.mark.gpu
+def test_minplus_batch_tuner_keeps_results_exact():
+    # the batch-level tuner only reorders which tile variant computes each
+    # size class; every choice it installs must give bit-exact products
+    from mi355sssp import kernels as K
+    import ctypes as C
+    rng = np.random.default_rng(8)
+    ns = [int(x) for x in rng.integers(20, 300, 24)]
+    mats = []
+    for n in ns:
+        a = rng.random((n, n), dtype=np.float32) + np.float32(0.01)
+        b = rng.random((n, n), dtype=np.float32) + np.float32(0.01)
+        mats.append((n, a, b, M.DeviceBuffer.from_array(a.ravel()), M.DeviceBuffer.from_array(b.ravel()),
+                     M.DeviceBuffer(4 * n * n)))
+    cnt = len(mats)
+    pa = (C.c_void_p * cnt)(*[m[3].ptr.value for m in mats])
+    pb = (C.c_void_p * cnt)(*[m[4].ptr.value for m in mats])
+    pc = (C.c_void_p * cnt)(*[m[5].ptr.value for m in mats])
+    dims = (C.c_int * cnt)(*ns)
+    eng = M.engine()
+
+    def run():
+        M._abi.check(eng.mi355_minplus_gemm_f32_batched(cnt, pa, pb, pc, dims, dims, dims, dims, dims, dims,
+                                                        None))
+        eng.mi355_stream_sync(None)
+
+    try:
+        table, best_ms, static_ms, names = K.minplus_tune_batch(run, ns, reps=1, rounds=1)
+        assert best_ms <= static_ms and len(names) == eng.mi355_minplus_tile_count()
+        run()
+        for n, a, b, _, _, dc in mats:
+            want = orc.minplus_gemm(a.ravel(), b.ravel(), n, n, n)
+            got = dc.to_array(np.float32, n * n)
+            assert got.tobytes() == np.ascontiguousarray(want, np.float32).ravel().tobytes(), n
+    finally:
+        K.minplus_tuning([-1] * 32)
