@@ -219,7 +219,8 @@ def light_stage(args) -> int:
         kinds = np.repeat(np.array([m["kind"] for m in metas]), [m["records"] for m in metas])
         keep = kinds == "rmat"
         y = y.copy()
-        y[keep] = teacher.forward(x[keep])
+        if keep.any():
+            y[keep] = teacher.forward(x[keep])
         log(f"light: {int(keep.sum())} R-MAT records target the teacher's outputs")
     if args.graph_weight == "class":
         w = L.class_weights(w, metas)
