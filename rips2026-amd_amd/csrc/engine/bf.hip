@@ -160,6 +160,11 @@ void run_bf_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
   int batch = req.rounds_per_sync > 0 ? req.rounds_per_sync : 1;
   uint32_t frontier = 1;
   res.syncs = 0;
+  // in-place rounds relax chains of improvements inside the round
+  // (k_relax_frontier_chain); Jacobi rounds keep their iteration semantics.
+  // Only graphs without chunked rows: the chunk queue holds each high-degree
+  // row once per round, and a chain could chunk a row again.
+  const bool chain = !jacobi && g.info.large_vertices == 0;
   while (r < max_rounds && frontier > 0) {
     const int64_t k = std::min<int64_t>(batch, max_rounds - r);
     for (int64_t i = 0; i < k; ++i, ++r) {
@@ -177,6 +182,10 @@ void run_bf_typed(mi355_graph& g, uint32_t src, const RunRequest& req, RunResult
                            r < (int64_t)ws.log.n ? ws.log.p + r : nullptr};
       if (jacobi)
         k_relax_frontier<W, true, BfSink>
+            <<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, ws.labels.p, in, cq, sink, hk, work,
+                                     input_ids(g));
+      else if (chain)
+        k_relax_frontier_chain<W>
             <<<grid, kBlock, 0, s>>>(g.rowptr.p, g.edges.p, ws.labels.p, in, cq, sink, hk, work,
                                      input_ids(g));
       else

@@ -363,3 +363,46 @@ def test_round_log_matches_the_jacobi_frontiers():
     log2 = log.copy()
     dg.sssp(src, M.ALG_BF_JACOBI)
     assert (log == log2).all()
+
+
+@pytest.mark.gpu
+def test_in_place_bf_chains_inside_rounds():
+    # In-place Bellman-Ford relaxes chains of improvements inside a round
+    # (per-wave lists, a bounded number of passes, the rest to the next
+    # round's frontier).  A bare path makes every chain hit the pass limit, a
+    # grid overflows the lists into the frontier; both stay exact and need
+    # far fewer frontier rounds than hops.  Negative weights without a cycle
+    # and graphs with chunked rows (no chaining) are exact too.
+    n = 5000
+    path = M.from_edges(n, np.arange(n - 1, dtype=np.int32), np.arange(1, n, dtype=np.int32),
+                        np.full(n - 1, 0.5, np.float32))
+    dg = M.DeviceGraph(path)
+    log = dg.round_log(2 * n)
+    _check(path, dg, 0, M.ALG_BF)
+    busy = int((log > 0).sum())
+    assert busy < (n - 1) // 4, busy  # more than four hops per round on average
+    g = M.grid4(300, 300, 1, 100, seed=12)
+    dg = M.DeviceGraph(g)
+    for src in (0, 300 * 150 + 150):
+        _check(g, dg, src, M.ALG_BF)
+    rng = np.random.default_rng(3)
+    m = 30000
+    s = rng.integers(0, 3000, m).astype(np.int32)
+    d = rng.integers(0, 3000, m).astype(np.int32)
+    # a potential-shifted weight keeps every cycle positive: w(u,v) = c + p(v) - p(u) >= 0
+    p = rng.uniform(0, 5, 3000).astype(np.float64)
+    w = (rng.uniform(0.5, 2.0, m) + p[d] - p[s]).astype(np.float32)
+    keep = (s != d) & (w + p[s] - p[d] > 0.25)
+    neg = M.from_edges(3000, s[keep], d[keep], w[keep])
+    dneg = M.DeviceGraph(neg, allow_negative=True)
+    dist, _, st = dneg.sssp(0, M.ALG_BF)
+    # every label is some walk's fp32 left fold and the fixpoint is at most
+    # every walk's (rounding is monotone), so any relaxation order reaches the
+    # oracle's Jacobi fixpoint bit for bit
+    t = neg.transpose()
+    want, _, conv = orc.bf_jacobi_in(t.rowptr, t.col, t.weights, 0, -1, 1)
+    assert conv and st.converged == 1
+    assert dist.tobytes() == want.tobytes()
+    hub = M.rmat(12, 16, seed=21)  # rows above 1024 edges: chunked, chaining off
+    dh = M.DeviceGraph(hub)
+    _check(hub, dh, int(hub.pick_sources(1, seed=2)[0]), M.ALG_BF)
