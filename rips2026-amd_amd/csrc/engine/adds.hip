@@ -108,6 +108,7 @@ constexpr uint32_t kPullChunk = 4 * 512;
 constexpr float kPullUnreachedReads = 10.0f;
 constexpr int kEmptyEntry = -1;     // ring sentinel: reserved, not yet written
 constexpr uint32_t kPushStage = 512;  // staged pushes per wave
+constexpr uint32_t kPublishAt = 64;   // staged entries of other buckets offered between passes
 constexpr uint32_t kHeavyStage = 256; // staged heavy-list entries per wave
 // Bound of every spin (ring entry, hub chunk, idle worker): far above any
 // legitimate wait; reaching it aborts the run with an error instead of
@@ -297,6 +298,36 @@ __device__ inline uint32_t stage_to_local(PushStage& st) {
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   return take;
+}
+
+// Move the staged entries of the held bucket to the local queue and drop
+// them from the stage (the rest keep their order).
+__device__ inline void stage_keep_local(PushStage& st) {
+  if (st.n == 0 || st.lq.slot == kNoLocal) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (stage_to_local(st) == 0) return;
+  const unsigned lane = lane_id();
+  uint32_t out = 0;
+  for (uint32_t i0 = 0; i0 < st.n; i0 += kWave) {
+    // an entry moves only left, below every entry this chunk reads
+    const uint32_t i = i0 + lane;
+    const bool keep = i < st.n && st.slot[i] != kTakenLocal;
+    const int v = keep ? st.v[i] : 0;
+    const uint32_t sl = keep ? st.slot[i] : 0u;
+    const unsigned long long m = __ballot(keep);
+    __builtin_amdgcn_wave_barrier();
+    if (keep) {
+      st.v[out + prefix_count(m)] = v;
+      st.slot[out + prefix_count(m)] = sl;
+    }
+    out += (uint32_t)__popcll(m);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  st.n = out;
 }
 
 template <bool P>
@@ -1445,11 +1476,20 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       relax_block<W, kPull>(a, st, &s_incl[0][0], &s_beg[0][0], &s_key[0][0], &s_u[0][0], &s_lo[0][0], s_wsum,
                      pu, beg, deg, key, bk, base, tbase, inv, edges_local, pushed_local);
       DIAG(t_r1 = wall_clock64());
-      stage_publish<kPull>(a, st);
-      heavy_publish(a, s_hv[wv], hv_n, slot);
+      // Between passes only the held bucket's entries move (to the local
+      // queue: the next pass); entries for other buckets are offered once
+      // kPublishAt of them are staged, heavy-list entries (claimable only
+      // after the bucket) at the end of the claim.  A hop of a local chain
+      // then costs its relaxation's round trips, not a publish's too.
+      stage_keep_local(st);
+      if (st.n >= kPublishAt) stage_publish<kPull>(a, st);
       DIAG(t_relax += t_r1 - t_r0; t_pub += wall_clock64() - t_r1);
     }
+    // everything staged is offered to the rings before the claim is released
+    // (the local queue is no longer read: entries of the held bucket too)
     st.lq.slot = kNoLocal;
+    stage_publish<kPull>(a, st);
+    heavy_publish(a, s_hv[wv], hv_n, slot);
     vm_wait();  // every wave's publishes complete before the release
     __syncthreads();
     if (threadIdx.x == 0) {
