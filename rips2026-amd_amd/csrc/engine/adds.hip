@@ -1326,6 +1326,11 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
     // and their list stamps cleared, their edges not relaxed
     const bool drop = kPull && kind == kClaimHeavyDrop;
     const uint32_t slot = s_slot, begin = s_begin;
+    // entries for other buckets are offered between passes once this many
+    // are staged.  A light claim in a one-bucket window holds the base bucket:
+    // nothing else is claimable until it finishes, so its other entries wait
+    // for the end of the claim (or a full stage).
+    const uint32_t publish_at = (!heavy && s_active == 1) ? kPushStage : kPublishAt;
     int* ring = (heavy ? a.hring : a.ring) + (size_t)s_qslot * (a.cap_mask + 1u);
     // improved vertices of the held bucket stay in this workgroup (item
     // claims), up to local_keep at a time; the rest go to the ring
@@ -1478,11 +1483,11 @@ __global__ __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4))) void k
       DIAG(t_r1 = wall_clock64());
       // Between passes only the held bucket's entries move (to the local
       // queue: the next pass); entries for other buckets are offered once
-      // kPublishAt of them are staged, heavy-list entries (claimable only
+      // publish_at of them are staged, heavy-list entries (claimable only
       // after the bucket) at the end of the claim.  A hop of a local chain
       // then costs its relaxation's round trips, not a publish's too.
       stage_keep_local(st);
-      if (st.n >= kPublishAt) stage_publish<kPull>(a, st);
+      if (st.n >= publish_at) stage_publish<kPull>(a, st);
       DIAG(t_relax += t_r1 - t_r0; t_pub += wall_clock64() - t_r1);
     }
     // everything staged is offered to the rings before the claim is released
