@@ -96,6 +96,29 @@ def test_c1_rmat22_learned_adds(c1):
         L.clear_delta_model()
 
 
+@pytest.mark.skipif(not HAVE_MODEL, reason="no trained delta model shipped")
+def test_c1_rmat22_many_runs_complete_and_stay_exact(c1):
+    # A claim keeps its bucket's pushes in the workgroup between passes and
+    # offers the rest at the end; nothing may reach the workgroup's local queue
+    # after its last pass (an entry left there corrupted a later claim of the
+    # same launch, which the engine reported as a bucket queue overflow in a
+    # few runs out of dozens).  32 learned runs with pulls and 8 plain ones
+    # must all finish; the next of each is checked bit for bit.
+    g, dg, srcs = c1
+    L.load_default_delta_model()
+    try:
+        o = dg.options(M.ALG_ADDS, use_mlp=1, pull=1)
+        for s in srcs[4:36]:
+            dg.sssp_labels(int(s), o)
+        _check(g, dg, srcs[36], M.ALG_ADDS, use_mlp=1, pull=1)
+    finally:
+        L.clear_delta_model()
+    o = dg.options(M.ALG_ADDS)
+    for s in srcs[40:48]:
+        dg.sssp_labels(int(s), o)
+    _check(g, dg, srcs[48], M.ALG_ADDS)
+
+
 # ---- C2 ----------------------------------------------------------------------
 @pytest.fixture(scope="module")
 def c2():
